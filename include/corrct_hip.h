@@ -1,0 +1,185 @@
+/*
+ * corrct_hip.h -- C ABI of libcorrct_hip.so, the MI355X (gfx950) projector
+ * backend for attenuation-corrected XRF tomography.
+ *
+ * Conventions
+ *   - every function returns an int status (CHIP_OK = 0); on error
+ *     chip_last_error() returns a per-thread message;
+ *   - array pointers are DEVICE pointers owned by the caller; the library
+ *     never returns views of internal memory;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *     all work is enqueued on it, nothing is synchronised unless stated;
+ *   - layouts are C-contiguous float32: volume [B][N][N] (rows = Y, cols = X),
+ *     maps [A][D][N][N] per slice, sinogram [B][D][A][N].
+ *
+ * Each entry point names the reference (corrct 2.0.0) interface it replaces,
+ * as file:line relative to the corrct source tree.
+ */
+#ifndef CORRCT_HIP_H
+#define CORRCT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chip_plan chip_plan;
+
+enum chip_status {
+  CHIP_OK = 0,
+  CHIP_EINVAL = 1, /* bad argument (shape, count, null pointer, mode) */
+  CHIP_EHIP = 2,   /* HIP runtime error */
+  CHIP_ENOMEM = 3  /* device allocation failed */
+};
+
+enum chip_bp_mode {
+  /* sum_d bp_skimage(sino[d]): iradon(filter None) discretisation, no maps,
+   * no weights -- ProjectorAttenuationXRF.bp with is_symmetric=False
+   * (src/corrct/projectors.py:619-622) and ProjectorUncorrected.bp
+   * (projectors.py:323-341 -> _projector_backends.py:335-359) */
+  CHIP_BP_SKIMAGE = 0,
+  /* sum_a sum_d w[a,d] M[a,d] * bp_skimage_a(sino[d,a]) -- is_symmetric=True
+   * (projectors.py:533-568, 609-618) */
+  CHIP_BP_SKIMAGE_SYM = 1,
+  /* exact transpose of chip_fp (bilinear weights, maps, weights); no
+   * reference counterpart: used for the adjoint (dot) test */
+  CHIP_BP_ADJOINT = 2
+};
+
+/* Library version (major*10000 + minor*100 + patch). */
+int chip_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* chip_last_error(void);
+
+/* Number of exported entry points and their names (for load checks). */
+int chip_num_symbols(void);
+const char* chip_symbol_name(int i);
+
+/*
+ * Geometry plan for an n x n slice, n_angles rotation angles (float32
+ * radians, as ProjectorBackend.initialize_geometry stores them:
+ * _projector_backends.py:145-160), n_det detectors at det_angles_rad
+ * (float64, physics/attenuation.py:97-109), detector weights w_det[n_det]
+ * (NULL = ones) and per-angle weights w_ang[n_angles][n_det] (NULL = ones)
+ * -- ProjectorAttenuationXRF.__init__ (projectors.py:439-459).
+ * Replaces ProjectorBackendSKimage.initialize_geometry
+ * (_projector_backends.py:259-299) and the weight handling above.
+ * Allocates small per-angle tables on the current device.
+ */
+int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_rad, int n_det,
+                     const double* det_angles_rad, const double* w_det, const double* w_ang);
+int chip_plan_destroy(chip_plan* plan);
+
+/*
+ * Attenuation maps maps[a][d][n][n] = E_in(theta_a) * E_out(theta_a + phi_d + pi)
+ * with E(phi) = exp(-trapezoid cumulative sum of mu along the beam), for
+ * `batch` slices (mu_* and maps advance by n*n and n_angles*n_det*n*n per
+ * slice). mu_in or mu_out may be NULL (factor 1). Replaces
+ * AttenuationVolume.compute_maps (physics/attenuation.py:122-180) over
+ * compute_attenuation (_projector_backends.py:43-99). Uses a plan-owned
+ * scratch buffer (allocated on first use, reused, freed with the plan).
+ */
+int chip_att_maps(chip_plan* plan, const float* mu_in, const float* mu_out, float* maps, int batch,
+                  void* stream);
+
+/*
+ * Forward projection sino[b][d][a][u] = w[a,d] * radon_a(vol[b] * M[a,d])[u]
+ * for `batch` slices. maps may be NULL (M = 1: ProjectorUncorrected.fp,
+ * projectors.py:302-321); maps_slice_stride is the element stride between
+ * slices' map sets (0 = all slices share one set). Replaces
+ * ProjectorAttenuationXRF.fp (projectors.py:570-596) over fp_angle
+ * (projectors.py:502-531) and ProjectorBackendSKimage.fp
+ * (_projector_backends.py:312-333).
+ */
+int chip_fp(const chip_plan* plan, const float* vol, const float* maps, int64_t maps_slice_stride,
+            float* sino, int batch, void* stream);
+
+/*
+ * Back-projection vol[b] = BP_mode(sino[b]); see enum chip_bp_mode. Replaces
+ * ProjectorAttenuationXRF.bp (projectors.py:598-622) and
+ * ProjectorBackendSKimage.bp (_projector_backends.py:335-359).
+ */
+int chip_bp(const chip_plan* plan, const float* sino, const float* maps, int64_t maps_slice_stride,
+            float* vol, int batch, int mode, void* stream);
+
+/*
+ * Single-angle variants: line[d][u] for angle a (fp), and the back-projection
+ * of line[d][u] at angle a (bp, accumulate = 0 overwrites vol, 1 adds).
+ * maps points at the full [A][D][n][n] set. Replace fp_angle / bp_angle
+ * (projectors.py:259-300, 502-568).
+ */
+int chip_fp_angle(const chip_plan* plan, const float* vol, const float* maps, float* line, int angle,
+                  void* stream);
+int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, float* vol, int angle,
+                  int mode, void* stream);
+
+/* ---- solver building blocks (solvers.py:60-117, 688-1206) ------------------
+ * n = number of elements. Optional arrays may be NULL. All elementwise
+ * updates are fused single-pass kernels; reductions are deterministic.
+ */
+
+/* out[0] = max_i |v[i]| (one float, device). Replaces np.max(np.abs(.)) in
+ * compute_diagonal_scaling (solvers.py:98-116). */
+int chip_absmax(const float* v, int64_t n, float* out, void* stream);
+
+/* v[i] = rho / fix(|v[i]| + add), fix(t) = 1 where t / max < 1e-5, with
+ * max = maxval[0] + add and maxval the device scalar chip_absmax(v) wrote.
+ * compute_diagonal_scaling (solvers.py:98-116). */
+int chip_precond_fix(float* v, int64_t n, float add, const float* maxval, float rho, void* stream);
+
+/* p = prox_l2(p + sigma * (ax + bg)) * mask, prox_l2(p) = (p - sigma*b) / (1 + sigma).
+ * DataFidelity_l2.update_dual / apply_proximal_dual (data_terms.py:217-230,
+ * 294-304) and the b_mask product (solvers.py:1172-1173). */
+int chip_pdhg_dual_l2(float* p, const float* ax, const float* sigma, const float* b, const float* bg,
+                      const float* b_mask, int64_t n, void* stream);
+
+/* p = prox_KL(p + sigma * (ax + bg)) * mask,
+ * prox_KL(p) = (1 + p - sqrt((p - 1)^2 + 4 sigma max(b, 0))) / 2.
+ * DataFidelity_KL (data_terms.py:677-692). */
+int chip_pdhg_dual_kl(float* p, const float* ax, const float* sigma, const float* b, const float* bg,
+                      const float* b_mask, int64_t n, void* stream);
+
+/* TV2D dual on `batch` n x n slices: q += 0.5 grad(x); q <- lam q / max(|q|_2, lam).
+ * q is [batch][2][n][n]. Regularizer_Grad.update_dual + DataFidelity_l12
+ * prox (regularizers.py:134-162; data_terms.py:549-555, 607-608;
+ * operators.py:796-816 edge padding). */
+int chip_tv_dual(float* q, const float* x, int n, int batch, float lam, void* stream);
+
+/* PDHG primal: upd = atp - div(q) (q may be NULL); x_new = clip(x - tau upd) * mask;
+ * xbar = x_new + (x_new - x); x = x_new. tau is elementwise. lower/upper are
+ * used when has_lower / has_upper. solvers.py:1175-1190, operators.py:818-844. */
+int chip_pdhg_primal(float* x, float* xbar, const float* atp, const float* q, const float* tau,
+                     int has_lower, float lower, int has_upper, float upper, const float* x_mask, int n,
+                     int batch, void* stream);
+
+/* SIRT residual r = (b - (ax + bg)) * mask * sigma  (written to `res`).
+ * DataFidelityBase.compute_residual (data_terms.py:136-161), solvers.py:954, 975. */
+int chip_sirt_residual(float* res, const float* b, const float* ax, const float* bg, const float* b_mask,
+                       const float* sigma, int64_t n, void* stream);
+
+/* SIRT update x = clip(x + (upd - reg_upd) * tau) * mask, reg_upd = -div(q)
+ * of a TV dual q (NULL = none). solvers.py:975-982. */
+int chip_sirt_update(float* x, const float* upd, const float* q, const float* tau, int has_lower,
+                     float lower, int has_upper, float upper, const float* x_mask, int n, int batch,
+                     void* stream);
+
+/* MLEM ratio r = b / max(ax + bg, eps_f32). solvers.py:793-797. */
+int chip_mlem_ratio(float* r, const float* b, const float* ax, const float* bg, int64_t n, void* stream);
+
+/* MLEM update x = clip(x * (upd / tau)) * mask. solvers.py:797-803. */
+int chip_mlem_update(float* x, const float* upd, const float* tau, int has_lower, float lower,
+                     int has_upper, float upper, const float* x_mask, int64_t n, void* stream);
+
+/* Residual norms (double accumulation, deterministic): out[0] = sum r^2 (l2,
+ * data_terms.py:298-299) or sum |r| (KL/l1, data_terms.py:741-742). */
+int chip_sumsq(const float* v, int64_t n, double* out, void* stream);
+int chip_sumabs(const float* v, int64_t n, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORRCT_HIP_H */

@@ -1,0 +1,158 @@
+"""CPU restatement of corrct's SIRT / MLEM / PDHG loops (float32 numpy).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Operators are plain callables ``A(x)`` and ``At(y)`` on numpy arrays, so the
+same loops run over the oracle projector, a dense matrix, or (in tests) the
+GPU projector's explicit matrix.
+
+* ``diagonal_scaling`` -- ``compute_diagonal_scaling`` (``solvers.py:60-117``)
+* ``grad`` / ``div``    -- ``TransformGradient`` edge mode (``operators.py:796-844``)
+* ``tv_dual``           -- ``Regularizer_Grad.update_dual`` + l12 prox
+                           (``regularizers.py:134-162, 260-273``; ``data_terms.py:549-555, 607-608``)
+* ``sirt``              -- ``solvers.py:867-984``
+* ``mlem``              -- ``solvers.py:688-805``
+* ``pdhg``              -- ``solvers.py:1060-1206`` with ``DataFidelity_l2``
+                           (``data_terms.py:283-331``) or ``DataFidelity_KL``
+                           (``data_terms.py:677-692``)
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+EPS = np.finfo(np.float32).eps
+
+
+def _fix(v: np.ndarray) -> np.ndarray:
+    v = v.copy()
+    v[(v / np.max(v)) < 1e-5] = 1
+    return v
+
+
+def diagonal_scaling(A, At, b, x_shape, tau_reg=0.0, rho_sigma=1.0, rho_tau=1.0, x_mask=None, b_mask=None):
+    """``sigma = rho_s / fix(|A 1_x|)``, ``tau = rho_t / fix(|At 1_b| + tau_reg)``."""
+    b_ones = np.ones_like(b, dtype=F32)
+    if b_mask is not None:
+        b_ones = b_ones * b_mask
+    tau = np.abs(At(b_ones)).astype(F32)
+    if tau_reg:
+        tau = tau + F32(tau_reg)
+    tau = _fix(tau)
+    tau = (F32(rho_tau) / tau).astype(F32)
+    x_ones = np.ones(x_shape, dtype=F32)
+    if x_mask is not None:
+        x_ones = x_ones * x_mask
+    sigma = np.abs(A(x_ones)).astype(F32)
+    sigma = _fix(sigma)
+    sigma = (F32(rho_sigma) / sigma).astype(F32)
+    return sigma, tau
+
+
+def grad(x: np.ndarray) -> np.ndarray:
+    """Forward differences along (rows, cols) with edge padding (last difference 0)."""
+    g = np.zeros((2, *x.shape), dtype=x.dtype)
+    g[0, :-1, :] = x[1:, :] - x[:-1, :]
+    g[1, :, :-1] = x[:, 1:] - x[:, :-1]
+    return g
+
+
+def div(q: np.ndarray) -> np.ndarray:
+    """Backward differences with edge padding: ``d[k] = q[k] - q[k-1]``, ``d[0] = 0``."""
+    d0 = np.zeros(q.shape[1:], dtype=q.dtype)
+    d1 = np.zeros(q.shape[1:], dtype=q.dtype)
+    d0[1:, :] = q[0, 1:, :] - q[0, :-1, :]
+    d1[:, 1:] = q[1, :, 1:] - q[1, :, :-1]
+    return d0 + d1
+
+
+def tv_dual(q: np.ndarray, xbar: np.ndarray, weight: float) -> np.ndarray:
+    """``q += 0.5 grad(xbar)``; ``q <- w q / max(|q|_2, w)``."""
+    q = q + F32(0.5) * grad(xbar)
+    nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=q.dtype))
+    q = q / np.fmax(nrm, F32(weight))
+    return q * F32(weight)
+
+
+def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=None, b_mask=None, tv_weight=None, relaxation=1.95):
+    b = np.asarray(b, dtype=F32)
+    sigma, tau = diagonal_scaling(A, At, b, x_shape, tau_reg=4.0 if tv_weight is not None else 0.0, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
+    x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
+    for _ in range(iterations):
+        res = b - A(x)
+        if b_mask is not None:
+            res = res * b_mask
+        upd = At(res * sigma)
+        if tv_weight is not None:
+            q = tv_dual(np.zeros((2, *x_shape), dtype=F32), x, tv_weight)
+            upd = upd - (-div(q))
+        x = x + upd * tau
+        if lower is not None or upper is not None:
+            x = x.clip(lower, upper)
+        if x_mask is not None:
+            x = x * x_mask
+    return x
+
+
+def mlem(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=None, b_mask=None, background=None):
+    b = np.asarray(b, dtype=F32)
+    b_ones = np.ones_like(b)
+    if b_mask is not None:
+        b_ones = b_ones * b_mask
+    tau = At(b_ones).astype(F32)
+    x = np.ones(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
+    if x_mask is not None:
+        x = x * x_mask
+    for _ in range(iterations):
+        ax = A(x)
+        if background is not None:
+            ax = ax + F32(background)
+        ax = ax.clip(EPS, None)
+        upd = At(b / ax)
+        x = x * (upd / tau)
+        if lower is not None or upper is not None:
+            x = x.clip(lower, upper)
+        if x_mask is not None:
+            x = x * x_mask
+    return x
+
+
+def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None, lower=None, upper=None,
+         x_mask=None, b_mask=None, relaxation=0.95):
+    b = np.asarray(b, dtype=F32)
+    sigma, tau = diagonal_scaling(
+        A, At, b, x_shape, tau_reg=4.0 if tv_weight is not None else 0.0, rho_sigma=relaxation, rho_tau=relaxation,
+        x_mask=x_mask, b_mask=b_mask,
+    )
+    x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
+    xr = x.copy()
+    if data_term == "l2":
+        sigma_data = sigma * b
+        sigma1 = (F32(1) / (F32(1) + sigma)).astype(F32)
+    elif data_term == "kl":
+        sigma_data = F32(4) * sigma * np.fmax(b, F32(0))
+    else:
+        raise ValueError(data_term)
+    p = np.zeros_like(b)
+    q = np.zeros((2, *x_shape), dtype=F32) if tv_weight is not None else None
+    for _ in range(iterations):
+        p = p + A(xr) * sigma
+        if data_term == "l2":
+            p = (p - sigma_data) * sigma1
+        else:
+            p = (F32(1) + p - np.sqrt((p - F32(1)) ** 2 + sigma_data)) / F32(2)
+        if b_mask is not None:
+            p = p * b_mask
+        upd = At(p)
+        if q is not None:
+            q = tv_dual(q, xr, tv_weight)
+            upd = upd + (-div(q))
+        x_new = x - upd * tau
+        if lower is not None or upper is not None:
+            x_new = x_new.clip(lower, upper)
+        if x_mask is not None:
+            x_new = x_new * x_mask
+        xr = x_new + (x_new - x)
+        x = x_new
+    return x

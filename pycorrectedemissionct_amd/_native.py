@@ -1,0 +1,127 @@
+"""ctypes binding of libcorrct_hip.so (the C ABI declared in include/corrct_hip.h).
+
+torch is imported first so the library binds to the HIP runtime torch has
+already loaded (one runtime per process). The library is never optional:
+if it is missing or fails to load, every projector/solver call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must be loaded before the HIP library)
+
+from . import build as _build
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_void_p = ctypes.c_void_p
+
+CHIP_BP_SKIMAGE = 0
+CHIP_BP_SKIMAGE_SYM = 1
+CHIP_BP_ADJOINT = 2
+
+_lock = threading.Lock()
+_lib = None
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGNATURES = {
+    "chip_version": [],
+    "chip_last_error": [],
+    "chip_num_symbols": [],
+    "chip_symbol_name": [ctypes.c_int],
+    "chip_plan_create": [
+        ctypes.POINTER(c_void_p), ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p,
+    ],
+    "chip_plan_destroy": [c_void_p],
+    "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
+    "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
+    "chip_fp_angle": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_bp_angle": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
+    "chip_absmax": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+    "chip_precond_fix": [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, ctypes.c_float, c_void_p],
+    "chip_pdhg_dual_l2": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_pdhg_dual_kl": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_tv_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, c_void_p],
+    "chip_pdhg_primal": [
+        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+        c_void_p, ctypes.c_int, ctypes.c_int, c_void_p,
+    ],
+    "chip_sirt_residual": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_sirt_update": [
+        c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
+        ctypes.c_int, ctypes.c_int, c_void_p,
+    ],
+    "chip_mlem_ratio": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_mlem_update": [
+        c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
+        ctypes.c_int64, c_void_p,
+    ],
+    "chip_sumsq": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+    "chip_sumabs": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+}
+_RESTYPES = {"chip_last_error": ctypes.c_char_p, "chip_symbol_name": ctypes.c_char_p}
+
+
+class NativeError(RuntimeError):
+    """A libcorrct_hip call returned a non-zero status."""
+
+
+def library_path() -> str:
+    return os.environ.get("CORRCT_HIP_LIB", _build.library_path())
+
+
+def lib():
+    """The loaded library (loads on first use; raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = library_path()
+        if not os.path.exists(path):
+            raise NativeError(
+                f"libcorrct_hip.so not found at {path}: build it with `python -m pycorrectedemissionct_amd.build`"
+            )
+        handle = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = handle
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().chip_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed (status {status}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)
+
+
+def exported_symbols() -> list[str]:
+    L = lib()
+    return [L.chip_symbol_name(i).decode() for i in range(L.chip_num_symbols())]
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a contiguous float32 CUDA tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def stream() -> int:
+    """The HIP stream torch is currently using on this device."""
+    return torch.cuda.current_stream().cuda_stream

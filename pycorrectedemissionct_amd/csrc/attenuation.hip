@@ -1,0 +1,178 @@
+// Attenuation maps of libcorrct_hip (gfx950).
+//
+// Restates compute_attenuation (_projector_backends.py:43-99) for every beam
+// direction of a plan and composes AttenuationVolume.compute_maps
+// (physics/attenuation.py:122-180):
+//   mu_p   = mu zero-padded by e on each side (side Np = N + 2e)
+//   R      = rotate(mu_p, -deg)                 (order-1 warp, 0 outside)
+//   C[r]   = sum_{k<=r} (R[k] + R[k-1 mod Np]) / 2   (float32, sequential in r)
+//   E      = exp(-rotate(C, +deg))[e:-e, e:-e]
+//   M[a,d] = E_in(theta_a) * E_out(theta_a + phi_d + pi)
+//
+// Kernel 1 (att_scan_kernel): one thread per (beam, column x of the rotated
+// frame) walks the column top to bottom, sampling mu_p at the rotated
+// position and carrying the trapezoid prefix sum; consecutive threads write
+// consecutive columns, so every row store is coalesced. The running sum is
+// the same float32 sequence numpy's cumsum performs.
+// Kernel 2 (att_compose_kernel): one thread per map pixel samples the
+// beams' prefix-sum images at the back-rotated position, takes exp and
+// multiplies in-beam by out-beam.
+//
+// The order-1 blend follows scikit-image 0.18.3's compiled warp: the
+// fractions r - floor(r) are float32, the products with the right-hand
+// neighbours are float32, the rest of the blend is double, rounded to
+// float32 -- so the maps match a float32 scikit-image evaluation to the
+// last bit except where exp differs by an ulp.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace chip {
+
+__device__ __forceinline__ float warp_sample(const float* __restrict__ img, int side, int off, float xs, float ys) {
+  // img is a side x side image embedded at offset `off` inside a zero frame
+  // (the padded frame of compute_attenuation): pixel (r, c) of the frame is
+  // img[(r - off) * side + (c - off)] when inside, else 0.
+  const float c0f = floorf(xs);
+  const float r0f = floorf(ys);
+  const float dcf = xs - c0f;
+  const float drf = ys - r0f;
+  const int c0 = static_cast<int>(c0f) - off;
+  const int r0 = static_cast<int>(r0f) - off;
+  const int c1 = static_cast<int>(ceilf(xs)) - off;
+  const int r1 = static_cast<int>(ceilf(ys)) - off;
+  auto px = [&](int r, int c) -> float {
+    return (r >= 0 && r < side && c >= 0 && c < side) ? img[static_cast<size_t>(r) * side + c] : 0.f;
+  };
+  const float tl = px(r0, c0), tr = px(r0, c1), bl = px(r1, c0), br = px(r1, c1);
+  const double dc = dcf, dr = drf;
+  // explicit roundings: no FMA contraction, as in the x86-64 build of scikit-image
+  const double top = __dadd_rn(__dmul_rn(1.0 - dc, static_cast<double>(tl)), static_cast<double>(__fmul_rn(dcf, tr)));
+  const double bot = __dadd_rn(__dmul_rn(1.0 - dc, static_cast<double>(bl)), static_cast<double>(__fmul_rn(dcf, br)));
+  return static_cast<float>(__dadd_rn(__dmul_rn(1.0 - dr, top), __dmul_rn(dr, bot)));
+}
+
+__device__ __forceinline__ void affine_apply(const Affine& m, float x, float y, float& xs, float& ys) {
+  xs = __fadd_rn(__fadd_rn(__fmul_rn(m.m00, x), __fmul_rn(m.m01, y)), m.m02);
+  ys = __fadd_rn(__fadd_rn(__fmul_rn(m.m10, x), __fmul_rn(m.m11, y)), m.m12);
+}
+
+// beams [0, n_in) use mu_in with rot[rot_in0 + beam], the rest mu_out with
+// rot[rot_out0 + (beam - n_in)]; output C[beam][r][x], side np.
+__global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__ mu_in, const float* __restrict__ mu_out,
+                                                       const Affine* __restrict__ rot_in,
+                                                       const Affine* __restrict__ rot_out, int n_in, int n_beams,
+                                                       int n, int pad, int np, float* __restrict__ cum) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int beam = blockIdx.y;
+  if (x >= np || beam >= n_beams) return;
+  const bool inb = beam < n_in;
+  const float* mu = inb ? mu_in : mu_out;
+  const Affine m = inb ? rot_in[beam] : rot_out[beam - n_in];
+  const float xf = static_cast<float>(x);
+  float* out = cum + static_cast<size_t>(beam) * np * np + x;
+  float xs, ys;
+  affine_apply(m, xf, static_cast<float>(np - 1), xs, ys);
+  float prev = warp_sample(mu, n, pad, xs, ys);  // R[np-1]: the roll wraps it onto row 0
+  float run = 0.f;
+  for (int r = 0; r < np; ++r) {
+    affine_apply(m, xf, static_cast<float>(r), xs, ys);
+    const float cur = warp_sample(mu, n, pad, xs, ys);
+    const float v = (cur + prev) * 0.5f;
+    run = (r == 0) ? v : run + v;
+    out[static_cast<size_t>(r) * np] = run;
+    prev = cur;
+  }
+}
+
+// maps[a][d][i][j] for angles [a0, a0 + n_a) of the chunk whose prefix sums
+// are in cum: beam index of angle a (chunk-local la): in = la (if has_in),
+// out = n_in + la * D + d (if has_out).
+__global__ __launch_bounds__(256) void att_compose_kernel(const float* __restrict__ cum,
+                                                          const Affine* __restrict__ rot_in_back,
+                                                          const Affine* __restrict__ rot_out_back, int has_in,
+                                                          int has_out, int n_in, int a0, int n_det, int n, int pad,
+                                                          int np, float* __restrict__ maps) {
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int ad = blockIdx.z;  // chunk-local (la, d)
+  if (i >= n || j >= n) return;
+  const int la = ad / n_det;
+  const int d = ad - la * n_det;
+  const int a = a0 + la;
+  const float xf = static_cast<float>(j + pad);
+  const float yf = static_cast<float>(i + pad);
+  const size_t npp = static_cast<size_t>(np) * np;
+  float e_in = 1.f, e_out = 1.f;
+  float xs, ys;
+  if (has_in) {
+    affine_apply(rot_in_back[a], xf, yf, xs, ys);
+    e_in = expf(-warp_sample(cum + static_cast<size_t>(la) * npp, np, 0, xs, ys));
+  }
+  if (has_out) {
+    affine_apply(rot_out_back[static_cast<size_t>(a) * n_det + d], xf, yf, xs, ys);
+    e_out = expf(-warp_sample(cum + static_cast<size_t>(n_in + la * n_det + d) * npp, np, 0, xs, ys));
+  }
+  maps[(static_cast<size_t>(a) * n_det + d) * n * n + static_cast<size_t>(i) * n + j] = e_in * e_out;
+}
+
+}  // namespace chip
+
+using namespace chip;
+
+extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* mu_out, float* maps, int batch,
+                             void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr, "plan is NULL");
+  CHIP_CHECK_ARG(maps != nullptr, "maps is NULL");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
+  Plan& p = plan->p;
+  hipStream_t st = as_stream(stream);
+  const int A = p.n_angles, D = p.n_det, n = p.n, np = p.np;
+  const size_t npp = static_cast<size_t>(np) * np;
+  const int beams_per_angle = (mu_in ? 1 : 0) + (mu_out ? D : 0);
+  // angles per chunk: bounded scratch (~1 GiB) and grid.y limit
+  const size_t budget = size_t(1) << 30;
+  int chunk = static_cast<int>(std::max<size_t>(1, budget / (npp * sizeof(float) * beams_per_angle)));
+  chunk = std::min(chunk, A);
+  chunk = std::min(chunk, 65535 / std::max(beams_per_angle, D));
+  const size_t need = static_cast<size_t>(chunk) * beams_per_angle * npp * sizeof(float);
+  if (p.scratch_bytes < need) {
+    if (p.d_scratch) {
+      CHIP_HIP(hipStreamSynchronize(st));
+      CHIP_HIP(hipFree(p.d_scratch));
+      p.d_scratch = nullptr;
+      p.scratch_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p.d_scratch, need);
+    if (e != hipSuccess) {
+      set_error(std::string("chip_att_maps: scratch hipMalloc -> ") + hipGetErrorString(e));
+      return CHIP_ENOMEM;
+    }
+    p.scratch_bytes = need;
+  }
+  const Affine* rin_f = p.d_rot;
+  const Affine* rin_b = p.d_rot + A;
+  const Affine* rout_f = p.d_rot + 2 * A;
+  const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D;
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + static_cast<size_t>(b) * n * n : nullptr;
+    const float* mo = mu_out ? mu_out + static_cast<size_t>(b) * n * n : nullptr;
+    float* mb = maps + static_cast<size_t>(b) * A * D * n * n;
+    for (int a0 = 0; a0 < A; a0 += chunk) {
+      const int na = std::min(chunk, A - a0);
+      const int n_in = mi ? na : 0;
+      const int n_beams = n_in + (mo ? na * D : 0);
+      dim3 g1((np + 255) / 256, n_beams);
+      hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
+                         rout_f + static_cast<size_t>(a0) * D, n_in, n_beams, n, p.pad, np, p.d_scratch);
+      CHIP_LAUNCH_CHECK();
+      dim3 g2((n + 15) / 16, (n + 15) / 16, na * D);
+      hipLaunchKernelGGL(att_compose_kernel, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
+                         mo ? 1 : 0, n_in, a0, D, n, p.pad, np, mb);
+      CHIP_LAUNCH_CHECK();
+    }
+  }
+  return CHIP_OK;
+}

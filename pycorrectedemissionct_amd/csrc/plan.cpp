@@ -1,0 +1,178 @@
+// Geometry plan, error strings and symbol table of libcorrct_hip.
+//
+// The per-angle tables reproduce how corrct + scikit-image 0.18.3 derive
+// their float32 inverse maps from the float32 rotation angles:
+//   radon:  deg = rad2deg(float32 angle)  (_projector_backends.py:148, 299)
+//           theta = deg2rad(deg) in float32, cos/sin of theta, and
+//           R = [[c, s, -k(c + s - 1)], [-s, c, -k(c - s - 1)]], k = N // 2,
+//           evaluated in float32 (skimage radon_transform.py:104-112).
+//   rotate: tform = T(-ctr) + R(r) + T(ctr) composed in float64, ctr =
+//           (N/2 - 0.5, N/2 - 0.5), then cast to float32
+//           (skimage _warps.py:366-404, _geometric.py:705-716, 1119-1148).
+//   maps:   in-beam angles stay float32 (attenuation.py:105-106); out-beam
+//           angles are theta_a + phi_d (float64) + pi (attenuation.py:108-120,
+//           _projector_backends.py:79-80).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace chip {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+// numpy float32 rad2deg / deg2rad (multiply by a float32 constant)
+static inline float rad2deg_f32(float x) {
+  const float k = 180.0f / static_cast<float>(M_PI);
+  return x * k;
+}
+static inline float deg2rad_f32(float x) {
+  const float k = static_cast<float>(M_PI / 180.0);
+  return x * k;
+}
+
+// tform = T(-ctr) + R(r) + T(ctr) -> T1 @ (R @ T3), float64, then float32
+static Affine rotate_affine(int side, double r) {
+  const double cx = side / 2.0 - 0.5;
+  const double cy = side / 2.0 - 0.5;
+  const double c = std::cos(r), s = std::sin(r);
+  // R @ T3 with T3 = [[1,0,-cx],[0,1,-cy]]
+  const double a02 = c * (-cx) + (-s) * (-cy);
+  const double a12 = s * (-cx) + c * (-cy);
+  Affine m;
+  m.m00 = static_cast<float>(c);
+  m.m01 = static_cast<float>(-s);
+  m.m02 = static_cast<float>(a02 + cx);
+  m.m10 = static_cast<float>(s);
+  m.m11 = static_cast<float>(c);
+  m.m12 = static_cast<float>(a12 + cy);
+  return m;
+}
+
+}  // namespace chip
+
+using namespace chip;
+
+static const char* kSymbols[] = {
+    "chip_version",       "chip_last_error",   "chip_num_symbols",  "chip_symbol_name",
+    "chip_plan_create",   "chip_plan_destroy", "chip_att_maps",     "chip_fp",
+    "chip_bp",            "chip_fp_angle",     "chip_bp_angle",     "chip_absmax",
+    "chip_precond_fix",   "chip_pdhg_dual_l2", "chip_pdhg_dual_kl", "chip_tv_dual",
+    "chip_pdhg_primal",   "chip_sirt_residual", "chip_sirt_update", "chip_mlem_ratio",
+    "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",
+};
+
+extern "C" {
+
+int chip_version(void) { return 100; }
+
+const char* chip_last_error(void) { return g_last_error.c_str(); }
+
+int chip_num_symbols(void) { return static_cast<int>(sizeof(kSymbols) / sizeof(kSymbols[0])); }
+
+const char* chip_symbol_name(int i) {
+  if (i < 0 || i >= chip_num_symbols()) return nullptr;
+  return kSymbols[i];
+}
+
+int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_rad, int n_det,
+                     const double* det_angles_rad, const double* w_det, const double* w_ang) {
+  clear_error();
+  CHIP_CHECK_ARG(out != nullptr, "out is NULL");
+  *out = nullptr;
+  CHIP_CHECK_ARG(n >= 2 && n <= 16384, "slice side must be in [2, 16384]");
+  CHIP_CHECK_ARG(n_angles >= 1, "need at least one angle");
+  CHIP_CHECK_ARG(angles_rad != nullptr, "angles_rad is NULL");
+  CHIP_CHECK_ARG(n_det >= 1 && n_det <= 64, "detector count must be in [1, 64]");
+  CHIP_CHECK_ARG(det_angles_rad != nullptr, "det_angles_rad is NULL");
+
+  auto* plan = new (std::nothrow) chip_plan();
+  if (!plan) {
+    set_error("chip_plan_create: host allocation failed");
+    return CHIP_ENOMEM;
+  }
+  Plan& p = plan->p;
+  p.n = n;
+  p.n_angles = n_angles;
+  p.n_det = n_det;
+  const double min_size = std::ceil(std::sqrt(2.0 * n * static_cast<double>(n)));
+  p.pad = static_cast<int>(std::ceil((min_size - n) / 2.0));
+  p.np = n + 2 * p.pad;
+  (void)hipGetDevice(&p.device);
+
+  const int k = n / 2;
+  p.h_fp_geom.resize(static_cast<size_t>(n_angles) * 4);
+  p.h_w.resize(static_cast<size_t>(n_angles) * n_det);
+  p.h_rot_in_fwd.resize(n_angles);
+  p.h_rot_in_back.resize(n_angles);
+  p.h_rot_out_fwd.resize(static_cast<size_t>(n_angles) * n_det);
+  p.h_rot_out_back.resize(static_cast<size_t>(n_angles) * n_det);
+  for (int a = 0; a < n_angles; ++a) {
+    const float deg = rad2deg_f32(angles_rad[a]);
+    const float th = deg2rad_f32(deg);
+    const float c = static_cast<float>(std::cos(static_cast<double>(th)));
+    const float s = static_cast<float>(std::sin(static_cast<double>(th)));
+    const float kf = static_cast<float>(-k);
+    p.h_fp_geom[4 * a + 0] = c;
+    p.h_fp_geom[4 * a + 1] = s;
+    p.h_fp_geom[4 * a + 2] = kf * ((c + s) - 1.0f);
+    p.h_fp_geom[4 * a + 3] = kf * ((c - s) - 1.0f);
+    for (int d = 0; d < n_det; ++d) {
+      const double wd = w_det ? w_det[d] : 1.0;
+      const double wa = w_ang ? w_ang[static_cast<size_t>(a) * n_det + d] : 1.0;
+      p.h_w[static_cast<size_t>(a) * n_det + d] = static_cast<float>(wd * wa);
+    }
+    // in-beam: float32 angle through float32 rad2deg / deg2rad
+    p.h_rot_in_fwd[a] = rotate_affine(p.np, static_cast<double>(deg2rad_f32(-deg)));
+    p.h_rot_in_back[a] = rotate_affine(p.np, static_cast<double>(deg2rad_f32(deg)));
+    // out-beam: float64 angle
+    for (int d = 0; d < n_det; ++d) {
+      const double phi = static_cast<double>(angles_rad[a]) + det_angles_rad[d] + M_PI;
+      const double dg = phi * (180.0 / M_PI);
+      p.h_rot_out_fwd[static_cast<size_t>(a) * n_det + d] = rotate_affine(p.np, (-dg) * (M_PI / 180.0));
+      p.h_rot_out_back[static_cast<size_t>(a) * n_det + d] = rotate_affine(p.np, dg * (M_PI / 180.0));
+    }
+  }
+
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string("chip_plan_create: ") + what + " -> " + hipGetErrorString(e));
+    chip_plan_destroy(plan);
+    return e == hipErrorOutOfMemory ? CHIP_ENOMEM : CHIP_EHIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&p.d_fp_geom, sizeof(float4) * n_angles)) != hipSuccess) return fail(e, "hipMalloc geom");
+  if ((e = hipMalloc(&p.d_w, sizeof(float) * p.h_w.size())) != hipSuccess) return fail(e, "hipMalloc w");
+  const size_t n_rot = 2 * static_cast<size_t>(n_angles) + 2 * static_cast<size_t>(n_angles) * n_det;
+  if ((e = hipMalloc(&p.d_rot, sizeof(Affine) * n_rot)) != hipSuccess) return fail(e, "hipMalloc rot");
+  std::vector<Affine> rot;
+  rot.reserve(n_rot);
+  rot.insert(rot.end(), p.h_rot_in_fwd.begin(), p.h_rot_in_fwd.end());
+  rot.insert(rot.end(), p.h_rot_in_back.begin(), p.h_rot_in_back.end());
+  rot.insert(rot.end(), p.h_rot_out_fwd.begin(), p.h_rot_out_fwd.end());
+  rot.insert(rot.end(), p.h_rot_out_back.begin(), p.h_rot_out_back.end());
+  if ((e = hipMemcpy(p.d_fp_geom, p.h_fp_geom.data(), sizeof(float4) * n_angles, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy geom");
+  if ((e = hipMemcpy(p.d_w, p.h_w.data(), sizeof(float) * p.h_w.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy w");
+  if ((e = hipMemcpy(p.d_rot, rot.data(), sizeof(Affine) * n_rot, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy rot");
+  *out = plan;
+  return CHIP_OK;
+}
+
+int chip_plan_destroy(chip_plan* plan) {
+  if (!plan) return CHIP_OK;
+  Plan& p = plan->p;
+  if (p.d_fp_geom) (void)hipFree(p.d_fp_geom);
+  if (p.d_w) (void)hipFree(p.d_w);
+  if (p.d_rot) (void)hipFree(p.d_rot);
+  if (p.d_scratch) (void)hipFree(p.d_scratch);
+  delete plan;
+  return CHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,397 @@
+"""SIRT / MLEM / PDHG on the device (corrct ``solvers.py`` mirror).
+
+Same classes, constructor arguments, ``__call__`` signature and return value
+``(x, SolutionInfo)`` as ``solvers.py:161-1206``. All iterates live in device
+memory; each iteration is one projector ``fp`` + one ``bp`` plus one or two
+fused HIP kernels for the elementwise work (dual prox, TV dual, primal
+update), so nothing returns to the host unless a tolerance or a test mask
+asks for residual norms.
+
+* ``compute_diagonal_scaling`` -- ``solvers.py:60-117``
+* ``SIRT``  -- ``solvers.py:808-984``
+* ``MLEM``  -- ``solvers.py:634-805``
+* ``PDHG``  -- ``solvers.py:987-1206`` (preconditioned; l2 or KL data term,
+  optional TV2D regularizer)
+
+``b`` may be a NumPy array (the result comes back as NumPy) or a device
+tensor (the result stays on the device). The operator ``A`` is any
+``BaseTransform``; it receives and returns device tensors.
+"""
+
+from __future__ import annotations
+
+import copy as cp
+from collections.abc import Sequence
+
+import numpy as np
+import torch
+
+from . import _arrays, _native
+from . import data_terms, regularizers
+
+eps = np.finfo(np.float32).eps
+
+
+def _dev(x, like: torch.Tensor | None = None):
+    if x is None:
+        return None
+    t = _arrays.to_dev(x)[0]
+    if like is not None and t.shape != like.shape:
+        t = torch.broadcast_to(t, like.shape).contiguous()
+    return t
+
+
+def _scalar_limit(v):
+    if v is None:
+        return 0, 0.0
+    if np.ndim(v) != 0:
+        raise NotImplementedError("array-valued clipping limits are not supported by the fused kernels")
+    return 1, float(v)
+
+
+def compute_diagonal_scaling(A_abs, At_abs, b: torch.Tensor, regs: Sequence, relaxation_sigma: float = 1.0,
+                             relaxation_tau: float = 1.0, x_mask=None, b_mask=None):
+    """``sigma = rho_s / fix(|A 1|)``, ``tau = rho_t / fix(|A^T 1| + sum reg_tau)`` (``solvers.py:60-117``)."""
+    st = _native.stream()
+    b_ones = torch.ones_like(b)
+    if b_mask is not None:
+        b_ones = b_ones * b_mask
+    tau = _arrays.to_dev(At_abs(b_ones))[0].clone()
+    tau_reg = 0.0
+    for reg in regs:
+        tau_reg += float(reg.initialize_sigma_tau(tau))
+    mx = torch.empty(1, dtype=torch.float32, device=tau.device)
+    _native.call("chip_absmax", tau.data_ptr(), tau.numel(), mx.data_ptr(), st)
+    _native.call("chip_precond_fix", tau.data_ptr(), tau.numel(), float(tau_reg), mx.data_ptr(), float(relaxation_tau), st)
+    x_ones = torch.ones_like(tau)
+    if x_mask is not None:
+        x_ones = x_ones * x_mask
+    sigma = _arrays.to_dev(A_abs(x_ones))[0].clone()
+    _native.call("chip_absmax", sigma.data_ptr(), sigma.numel(), mx.data_ptr(), st)
+    _native.call("chip_precond_fix", sigma.data_ptr(), sigma.numel(), 0.0, mx.data_ptr(), float(relaxation_sigma), st)
+    return sigma, tau, tuple(tau.shape), tau.dtype
+
+
+class SolutionInfo:
+    """Reconstruction info (``solvers.py:161-201``)."""
+
+    def __init__(self, method: str, max_iterations: int, tolerance, residual0: float = np.inf,
+                 residual0_cv: float = np.inf) -> None:
+        self.method = method
+        self.max_iterations = max_iterations
+        self.tolerance = tolerance
+        self.residual0 = residual0
+        self.residual0_cv = residual0_cv
+        self.residuals = np.zeros(max_iterations)
+        self.residuals_cv = np.zeros(max_iterations)
+        self.iterations = 0
+
+    @property
+    def residuals_rel(self) -> np.ndarray:
+        return self.residuals / self.residual0
+
+    @property
+    def residuals_cv_rel(self) -> np.ndarray:
+        return self.residuals_cv / self.residual0_cv
+
+
+class Solver:
+    """Base solver (``solvers.py:204-343``)."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = True, relaxation: float = 1.0, tolerance=None,
+                 data_term="l2", data_term_test=None):
+        self.verbose = verbose
+        self.leave_progress = leave_progress
+        self.relaxation = relaxation
+        self.tolerance = tolerance
+        self.data_term = self._initialize_data_fidelity_function(data_term)
+        if data_term_test is None:
+            data_term_test = self.data_term
+        else:
+            data_term_test = self._initialize_data_fidelity_function(data_term_test)
+        self.data_term_test = cp.deepcopy(data_term_test)
+
+    def info(self) -> str:
+        return type(self).__name__
+
+    def upper(self) -> str:
+        return type(self).__name__.upper()
+
+    def lower(self) -> str:
+        return type(self).__name__.lower()
+
+    @staticmethod
+    def _initialize_data_fidelity_function(data_term):
+        if isinstance(data_term, str):
+            if data_term.lower() == "l2":
+                return data_terms.DataFidelity_l2()
+            if data_term.lower() == "kl":
+                return data_terms.DataFidelity_KL()
+            raise ValueError(f"Unknown data term: '{data_term}', only accepted terms are: 'l2' | 'kl'.")
+        if isinstance(data_term, (data_terms.DataFidelity_l2, data_terms.DataFidelity_KL)):
+            return cp.deepcopy(data_term)
+        raise ValueError(f"Unsupported data term: '{data_term.info()}', only accepted terms are 'kl' and 'l2'-based.")
+
+    @staticmethod
+    def _initialize_regularizer(regularizer):
+        if regularizer is None:
+            return []
+        if isinstance(regularizer, regularizers.BaseRegularizer):
+            return [regularizer]
+        if isinstance(regularizer, (list, tuple)):
+            if not all(isinstance(r, regularizers.BaseRegularizer) for r in regularizer):
+                raise ValueError("Some regularizers are not derived from the regularizers.BaseRegularizer class")
+            return list(regularizer)
+        raise ValueError("Unknown regularizer type.")
+
+    @staticmethod
+    def _initialize_b_masks(b, b_mask, b_test_mask):
+        if b_test_mask is not None:
+            if b_mask is None:
+                b_mask = torch.ones_like(b)
+            b_mask, b_test_mask = b_mask * (1 - b_test_mask), b_test_mask * b_mask
+        return b_mask, b_test_mask
+
+    def _tv_weight(self) -> float | None:
+        if not self.regularizer:
+            return None
+        if len(self.regularizer) > 1:
+            raise NotImplementedError("one TV2D regularizer is supported by the fused HIP kernels")
+        reg = self.regularizer[0]
+        if not isinstance(reg, regularizers.Regularizer_Grad):
+            raise NotImplementedError(f"regularizer {reg.info()} is not supported by the fused HIP kernels")
+        return reg.check_fused()
+
+    def _track(self, info, ii, Ax, b, b_mask, b_test_mask) -> bool:
+        """Residual bookkeeping of one iteration; True when the tolerance is reached."""
+        res = self.data_term.compute_residual(Ax, mask=b_mask)
+        info.residuals[ii] = self.data_term.compute_residual_norm(res)
+        if b_test_mask is not None:
+            res_test = self.data_term_test.compute_residual(Ax, mask=b_test_mask)
+            info.residuals_cv[ii] = self.data_term_test.compute_residual_norm(res_test)
+        if self.tolerance is not None and self.tolerance > info.residuals[ii]:
+            if self.verbose:
+                print(f"Residual reached the desired tolerance of {self.tolerance}. Ending iterations..")
+            return True
+        return False
+
+    def _init_residuals(self, info, A, x, b, b_mask, b_test_mask, sigma=None):
+        Ax = A(x)
+        res_0 = self.data_term.compute_residual(Ax, mask=b_mask)
+        info.residual0 = self.data_term.compute_residual_norm(res_0)
+        if b_test_mask is not None:
+            if self.data_term_test.background != self.data_term.background:
+                print("WARNING - the data_term and and data_term_test should have the same background. Making them equal.")
+                self.data_term_test.background = self.data_term.background
+            self.data_term_test.assign_data(b, sigma if sigma is not None else 1.0)
+            res_test_0 = self.data_term_test.compute_residual(Ax, mask=b_test_mask)
+            info.residual0_cv = self.data_term_test.compute_residual_norm(res_test_0)
+
+
+class MLEM(Solver):
+    """MLEM (``solvers.py:634-805``)."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = True, tolerance=None, regularizer=None,
+                 data_term="kl", data_term_test=None):
+        super().__init__(verbose=verbose, leave_progress=leave_progress, tolerance=tolerance, data_term=data_term,
+                         data_term_test=data_term_test)
+        self.regularizer = self._initialize_regularizer(regularizer)
+
+    def info(self) -> str:
+        return Solver.info(self) + f"(B:{self.data_term.background:g})" if self.data_term.background is not None else ""
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, x_mask=None, b_mask=None,
+                 b_test_mask=None):
+        b_t, host = _arrays.to_dev(b)
+        b_t = b_t.clone()
+        b_mask = _dev(b_mask, b_t)
+        b_test_mask = _dev(b_test_mask, b_t)
+        b_mask, b_test_mask = self._initialize_b_masks(b_t, b_mask, b_test_mask)
+        st = _native.stream()
+        b_ones = torch.ones_like(b_t)
+        if b_mask is not None:
+            b_ones = b_ones * b_mask
+        tau = _arrays.to_dev(A.T(b_ones))[0].contiguous()
+        x_mask = _dev(x_mask, tau)
+        x = torch.ones_like(tau) if x0 is None else _dev(x0, tau).clone()
+        if x_mask is not None:
+            x = x * x_mask
+        x = x.contiguous()
+        self.data_term.assign_data(b_t)
+        bg = self.data_term.background_dev(b_t)
+        hl, lo = _scalar_limit(lower_limit)
+        hu, hi = _scalar_limit(upper_limit)
+        info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
+        track = b_test_mask is not None or self.tolerance is not None
+        if track:
+            self._init_residuals(info, A, x, b_t, b_mask, b_test_mask)
+        ratio = torch.empty_like(b_t)
+        for ii in range(iterations):
+            info.iterations += 1
+            Ax = _arrays.to_dev(A(x))[0]
+            if track and self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
+                break
+            _native.call("chip_mlem_ratio", ratio.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
+                         b_t.numel(), st)
+            upd = _arrays.to_dev(A.T(ratio))[0]
+            _native.call("chip_mlem_update", x.data_ptr(), upd.data_ptr(), tau.data_ptr(), hl, lo, hu, hi,
+                         _native.ptr(x_mask), x.numel(), st)
+        return _arrays.from_dev(x, host), info
+
+
+class SIRT(Solver):
+    """SIRT (``solvers.py:808-984``)."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = True, relaxation: float = 1.95, tolerance=None,
+                 regularizer=None, data_term="l2", data_term_test=None):
+        super().__init__(verbose=verbose, leave_progress=leave_progress, relaxation=relaxation, tolerance=tolerance,
+                         data_term=data_term, data_term_test=data_term_test)
+        self.regularizer = self._initialize_regularizer(regularizer)
+
+    def info(self) -> str:
+        reg_info = "".join(["-" + r.info().upper() for r in self.regularizer])
+        return Solver.info(self) + "-" + self.data_term.info() + reg_info
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, x_mask=None, b_mask=None,
+                 b_test_mask=None):
+        b_t, host = _arrays.to_dev(b)
+        b_t = b_t.clone()
+        b_mask = _dev(b_mask, b_t)
+        b_test_mask = _dev(b_test_mask, b_t)
+        b_mask, b_test_mask = self._initialize_b_masks(b_t, b_mask, b_test_mask)
+        tv_w = self._tv_weight()
+        try:
+            At_abs = A.T.absolute()
+            A_abs = A.absolute()
+        except AttributeError:
+            print("WARNING: operator does not support absolute(); Using the operator itself.")
+            A_abs, At_abs = A, A.T
+        x_mask_t = None
+        if x_mask is not None:
+            x_mask_t = _arrays.to_dev(x_mask)[0]
+        sigma, tau, x_shape, _ = compute_diagonal_scaling(A_abs, At_abs, b_t, regs=self.regularizer,
+                                                          relaxation_tau=self.relaxation, x_mask=x_mask_t,
+                                                          b_mask=b_mask)
+        x_mask_t = _dev(x_mask_t, tau)
+        x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
+        self.data_term.assign_data(b_t, sigma)
+        bg = self.data_term.background_dev(b_t)
+        hl, lo = _scalar_limit(lower_limit)
+        hu, hi = _scalar_limit(upper_limit)
+        info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
+        track = b_test_mask is not None or self.tolerance is not None
+        if track:
+            self._init_residuals(info, A, x, b_t, b_mask, b_test_mask, sigma)
+        st = _native.stream()
+        res = torch.empty_like(b_t)
+        n = int(x.shape[-1])
+        batch = x.numel() // (n * n) if x.ndim >= 2 else 1
+        q = torch.empty((batch, 2, n, n), dtype=torch.float32, device=x.device) if tv_w is not None else None
+        for ii in range(iterations):
+            info.iterations += 1
+            Ax = _arrays.to_dev(A(x))[0]
+            if track and self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
+                break
+            _native.call("chip_sirt_residual", res.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
+                         _native.ptr(b_mask), sigma.data_ptr(), b_t.numel(), st)
+            upd = _arrays.to_dev(A.T(res))[0]
+            if q is not None:
+                q.zero_()
+                _native.call("chip_tv_dual", q.data_ptr(), x.data_ptr(), n, batch, tv_w, st)
+            _native.call("chip_sirt_update", x.data_ptr(), upd.data_ptr(), _native.ptr(q), tau.data_ptr(), hl, lo, hu,
+                         hi, _native.ptr(x_mask_t), n, batch, st)
+        return _arrays.from_dev(x, host), info
+
+
+class PDHG(Solver):
+    """Preconditioned primal-dual hybrid gradient (Chambolle-Pock), ``solvers.py:987-1206``."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = True, tolerance=None, relaxation: float = 0.95,
+                 regularizer=None, data_term="l2", data_term_test=None):
+        super().__init__(verbose=verbose, leave_progress=leave_progress, relaxation=relaxation, tolerance=tolerance,
+                         data_term=data_term, data_term_test=data_term_test)
+        self.regularizer = self._initialize_regularizer(regularizer)
+
+    def info(self) -> str:
+        reg_info = "".join(["-" + r.info().upper() for r in self.regularizer])
+        return Solver.info(self) + "-" + self.data_term.info() + reg_info
+
+    @staticmethod
+    def _initialize_data_fidelity_function(data_term):
+        if isinstance(data_term, str):
+            if data_term.lower() == "l2":
+                return data_terms.DataFidelity_l2()
+            if data_term.lower() == "kl":
+                return data_terms.DataFidelity_KL()
+            if data_term.lower() == "l1":
+                raise NotImplementedError("the l1 data term is not supported by the fused HIP kernels")
+            raise ValueError(f'Unknown data term: "{data_term}", accepted terms are: "l2" | "l1" | "kl".')
+        return cp.deepcopy(data_term)
+
+    def prepare(self, A, b, x0=None, x_mask=None, b_mask=None, b_test_mask=None):
+        """Device state of a solve: dict with b, masks, sigma, tau, x, x_relax, p, q, background."""
+        b_t, host = _arrays.to_dev(b)
+        b_t = b_t.clone()
+        b_mask = _dev(b_mask, b_t)
+        b_test_mask = _dev(b_test_mask, b_t)
+        b_mask, b_test_mask = self._initialize_b_masks(b_t, b_mask, b_test_mask)
+        tv_w = self._tv_weight()
+        if not isinstance(self.data_term, (data_terms.DataFidelity_l2, data_terms.DataFidelity_KL)):
+            raise NotImplementedError(f"data term {self.data_term.info()} is not supported by the fused HIP kernels")
+        try:
+            At_abs = A.T.absolute()
+            A_abs = A.absolute()
+        except AttributeError:
+            raise NotImplementedError("PDHG without diagonal preconditioning is not supported") from None
+        x_mask_t = None if x_mask is None else _arrays.to_dev(x_mask)[0]
+        sigma, tau, x_shape, _ = compute_diagonal_scaling(A_abs, At_abs, b_t, regs=self.regularizer,
+                                                          relaxation_tau=self.relaxation,
+                                                          relaxation_sigma=self.relaxation, x_mask=x_mask_t,
+                                                          b_mask=b_mask)
+        x_mask_t = _dev(x_mask_t, tau)
+        x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
+        n = int(x.shape[-1])
+        batch = x.numel() // (n * n)
+        self.data_term.assign_data(b_t, sigma)
+        return dict(
+            b=b_t, host=host, b_mask=b_mask, b_test_mask=b_test_mask, sigma=sigma, tau=tau, x_mask=x_mask_t, x=x,
+            x_relax=x.clone(), p=torch.zeros_like(b_t), q=None if tv_w is None else torch.zeros(
+                (batch, 2, n, n), dtype=torch.float32, device=x.device), tv_w=tv_w, n=n, batch=batch,
+            bg=self.data_term.background_dev(b_t), kl=isinstance(self.data_term, data_terms.DataFidelity_KL),
+        )
+
+    def iterate(self, A, s: dict, iterations: int, lower_limit=None, upper_limit=None, info=None):
+        """Run ``iterations`` PDHG iterations on the device state ``s`` (no host synchronisation)."""
+        st = _native.stream()
+        hl, lo = _scalar_limit(lower_limit)
+        hu, hi = _scalar_limit(upper_limit)
+        b, p, q, x, xr = s["b"], s["p"], s["q"], s["x"], s["x_relax"]
+        dual = "chip_pdhg_dual_kl" if s["kl"] else "chip_pdhg_dual_l2"
+        track = info is not None and (s["b_test_mask"] is not None or self.tolerance is not None)
+        for ii in range(iterations):
+            if info is not None:
+                info.iterations += 1
+            Ax = _arrays.to_dev(A(xr))[0]
+            _native.call(dual, p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(), _native.ptr(s["bg"]),
+                         _native.ptr(s["b_mask"]), p.numel(), st)
+            if q is not None:
+                _native.call("chip_tv_dual", q.data_ptr(), xr.data_ptr(), s["n"], s["batch"], s["tv_w"], st)
+            upd = _arrays.to_dev(A.T(p))[0]
+            _native.call("chip_pdhg_primal", x.data_ptr(), xr.data_ptr(), upd.data_ptr(), _native.ptr(q),
+                         s["tau"].data_ptr(), hl, lo, hu, hi, _native.ptr(s["x_mask"]), s["n"], s["batch"], st)
+            if track:
+                Axn = _arrays.to_dev(A(x))[0]
+                if self._track(info, ii, Axn, b, s["b_mask"], s["b_test_mask"]):
+                    break
+        return s
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, x_mask=None, b_mask=None,
+                 b_test_mask=None, precondition: bool = True):
+        if not precondition:
+            raise NotImplementedError("PDHG without diagonal preconditioning is not supported")
+        s = self.prepare(A, b, x0=x0, x_mask=x_mask, b_mask=b_mask, b_test_mask=b_test_mask)
+        info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
+        if s["b_test_mask"] is not None or self.tolerance is not None:
+            self._init_residuals(info, A, s["x"], s["b"], s["b_mask"], s["b_test_mask"], s["sigma"])
+        self.iterate(A, s, iterations, lower_limit, upper_limit, info)
+        return _arrays.from_dev(s["x"], s["host"]), info

@@ -1,0 +1,199 @@
+"""GPU parity of the HIP projector against the oracle (run with -m gpu).
+
+Bar (north star): fp/bp agree with the FP32 restatement of corrct's skimage
+backend to 1e-5 relative (max|ref - cmp| / max|ref|); the exact-adjoint bp
+passes <A x, y> = <x, A^T y> to 1e-5 relative; the FP64 evaluation error of
+the GPU result is printed beside the FP32 one.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import attenuation as oatt
+from oracle import xrf as oxrf
+from tests.conftest import rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pycorrectedemissionct_amd as p
+
+    return p
+
+
+def _phantom(n, rng):
+    img = rng.random((n, n)).astype(np.float32)
+    yy, xx = np.mgrid[:n, :n] - n // 2
+    img[(yy**2 + xx**2) > (n // 2) ** 2] = 0
+    return img
+
+
+def _angles(a):
+    return np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,a", [(64, 48), (45, 33)])
+def test_fp_uncorrected_parity(pkg, n, a):
+    rng = np.random.default_rng(100 + n)
+    vol = _phantom(n, rng)
+    ang = _angles(a)
+    prj = pkg.ProjectorUncorrected([n, n], ang)
+    got = prj.fp(vol)
+    ref32 = oxrf.fp(vol, ang)[0]
+    ref64 = oxrf.fp(vol, ang, fp32=False)[0]
+    e32, e64 = rel_max(ref32, got), rel_max(ref64, got)
+    print(f"fp uncorrected n={n} a={a}: rel_max vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e}")
+    assert got.shape == (a, n)
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("n,a", [(64, 40), (33, 24)])
+def test_bp_skimage_parity(pkg, n, a):
+    rng = np.random.default_rng(200 + n)
+    ang = _angles(a)
+    sino = rng.random((a, n)).astype(np.float32)
+    prj = pkg.ProjectorUncorrected([n, n], ang)
+    got = prj.bp(sino)
+    ref32 = oxrf.bp_skimage(sino[None], ang, n)
+    ref64 = oxrf.bp_skimage(sino[None], ang, n, fp32=False)
+    e32, e64 = rel_max(ref32, got), rel_max(ref64, got)
+    print(f"bp skimage n={n} a={a}: rel_max vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e}")
+    assert e32 < TOL
+
+
+def _xrf_case(pkg, n, a, rng, symmetric=False, bp_mode="skimage"):
+    ang = _angles(a)
+    maps = rng.uniform(0.2, 1.0, (a, 2, n, n)).astype(np.float32)
+    w_det = np.array([1.0, 0.6])
+    w_ang = rng.uniform(0.5, 1.5, (a, 2))
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=[np.pi / 2, -np.pi / 2],
+                                      weights_detectors=w_det, weights_angles=w_ang, is_symmetric=symmetric,
+                                      bp_mode=bp_mode)
+    return prj, ang, maps, w_det, w_ang
+
+
+def test_xrf_fp_parity(pkg):
+    n, a = 48, 36
+    rng = np.random.default_rng(300)
+    prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng)
+    vol = _phantom(n, rng)
+    got = prj.fp(vol)
+    ref = oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang)
+    ref64 = oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang, fp32=False)
+    print(f"xrf fp: rel_max {rel_max(ref, got):.2e} (FP64 {rel_max(ref64, got):.2e}), rel_l2 {rel_l2(ref, got):.2e}")
+    assert got.shape == (2, a, n)
+    assert rel_max(ref, got) < TOL
+
+
+def test_xrf_bp_nonsymmetric_parity(pkg):
+    n, a = 48, 36
+    rng = np.random.default_rng(301)
+    prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng)
+    sino = rng.random((2, a, n)).astype(np.float32)
+    got = prj.bp(sino)
+    ref = oxrf.bp_skimage(sino, ang, n)  # ignores maps and weights, as the reference does
+    print(f"xrf bp (non-symmetric): rel_max {rel_max(ref, got):.2e}")
+    assert rel_max(ref, got) < TOL
+
+
+def test_xrf_bp_symmetric_parity(pkg):
+    n, a = 40, 30
+    rng = np.random.default_rng(302)
+    prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, symmetric=True)
+    sino = rng.random((2, a, n)).astype(np.float32)
+    got = prj.bp(sino)
+    ref = oxrf.bp_skimage(sino, ang, n, maps=maps, w_det=w_det, w_ang=w_ang, symmetric=True)
+    print(f"xrf bp (symmetric): rel_max {rel_max(ref, got):.2e}")
+    assert rel_max(ref, got) < TOL
+
+
+def test_adjoint_dot_test_and_oracle(pkg):
+    n, a = 48, 40
+    rng = np.random.default_rng(303)
+    prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, bp_mode="adjoint")
+    x = rng.random((n, n)).astype(np.float32)
+    y = rng.random((2, a, n)).astype(np.float32)
+    ax = prj.fp(x).astype(np.float64)
+    aty = prj.bp(y).astype(np.float64)
+    lhs = float(np.sum(ax * y))
+    rhs = float(np.sum(x * aty))
+    print(f"dot test: <Ax,y>={lhs:.9e} <x,A^T y>={rhs:.9e} rel={abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) / abs(lhs) < TOL
+    ref = oxrf.bp_adjoint(y, ang, n, maps, w_det=w_det, w_ang=w_ang)
+    assert rel_max(ref, aty) < TOL
+
+
+def test_fp_angle_bp_angle_consistent(pkg):
+    n, a = 32, 12
+    rng = np.random.default_rng(304)
+    prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, symmetric=True)
+    vol = _phantom(n, rng)
+    full = prj.fp(vol)
+    for ia in (0, 5, 11):
+        line = prj.fp_angle(vol, ia)
+        assert np.allclose(line, full[:, ia, :], rtol=0, atol=1e-6 * np.abs(full).max())
+    sino = rng.random((2, a, n)).astype(np.float32)
+    acc = sum(prj.bp_angle(sino, ia) for ia in range(a))
+    assert rel_max(prj.bp(sino), acc) < TOL
+
+
+@pytest.mark.parametrize("n", [32, 37])
+def test_att_maps_parity(pkg, n):
+    rng = np.random.default_rng(400 + n)
+    a = 16
+    ang = _angles(a)
+    det = np.array([np.pi / 2, -np.pi / 2])
+    mu_in = (rng.random((n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.08).astype(np.float32)
+    av = pkg.AttenuationVolume(mu_in, mu_out, ang, det)
+    av.compute_maps()
+    got = av.get_maps()
+    ref = oatt.compute_maps(mu_in, mu_out, ang, det)
+    mism = int(np.sum(got != ref))
+    print(f"maps n={n}: rel_max {rel_max(ref, got):.2e}, {mism} of {ref.size} entries differ from FP32 oracle")
+    assert got.shape == ref.shape
+    assert rel_max(ref, got) < TOL
+
+
+def test_compute_attenuation_free_function(pkg):
+    rng = np.random.default_rng(405)
+    mu = (rng.random((24, 24)) * 0.1).astype(np.float32)
+    a = np.float32(0.83)
+    got = pkg.compute_attenuation(mu, a)
+    assert rel_max(oatt.compute_attenuation(mu, a), got) < TOL
+    got = pkg.compute_attenuation(mu, 0.83, invert=True)
+    assert rel_max(oatt.compute_attenuation(mu, np.float64(0.83), invert=True), got) < TOL
+
+
+def test_full_size_properties(pkg):
+    """512 x 512, 720 angles, detectors at +-90 deg: zero mu gives unit maps; fp is linear; adjoint dot test."""
+    import torch
+
+    n, a = 512, 720
+    rng = np.random.default_rng(500)
+    ang = _angles(a)
+    det = [np.pi / 2, -np.pi / 2]
+    av = pkg.AttenuationVolume(np.zeros((n, n), np.float32), np.zeros((n, n), np.float32), ang, det)
+    av.compute_maps()
+    assert bool(torch.all(av.maps_device == 1.0))
+    f = pkg.testing.workload_pdhg_512()
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_in=f["mu_in"], att_out=f["mu_out"], angles_detectors_rad=det,
+                                      bp_mode="adjoint")
+    x1 = torch.as_tensor(rng.random((n, n)), dtype=torch.float32, device="cuda")
+    x2 = torch.as_tensor(rng.random((n, n)), dtype=torch.float32, device="cuda")
+    s = prj.fp(x1 + 2 * x2)
+    s12 = prj.fp(x1) + 2 * prj.fp(x2)
+    assert float(torch.max(torch.abs(s - s12)) / torch.max(torch.abs(s12))) < 1e-5
+    y = torch.as_tensor(rng.random((2, a, n)), dtype=torch.float32, device="cuda")
+    lhs = float(torch.sum(prj.fp(x1).double() * y.double()))
+    rhs = float(torch.sum(x1.double() * prj.bp(y).double()))
+    print(f"full-size dot test rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) / abs(lhs) < TOL

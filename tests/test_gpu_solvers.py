@@ -1,0 +1,118 @@
+"""GPU parity of the solvers (run with -m gpu).
+
+Bar (north star): solver outputs after a fixed number of iterations agree with
+the oracle's float32 restatement of corrct's loops (fed by the oracle
+projector) to 1e-4 relative. Plus the reference's own convergence tests
+(tests/test_solvers.py:100-164) on the dense-matrix fake projector, seeded.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import solvers as osol
+from oracle import xrf as oxrf
+from tests.conftest import rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pycorrectedemissionct_amd as p
+
+    return p
+
+
+@pytest.fixture(scope="module")
+def xrf_problem(pkg):
+    n, a = 32, 24
+    rng = np.random.default_rng(600)
+    f = pkg.testing.phantom_ellipses(n, 4, rng, semi_axes=(3, 8))
+    mu_in, mu_out = pkg.testing.xrf_attenuation(f)
+    ang = np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+    det = [np.pi / 2, -np.pi / 2]
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, angles_detectors_rad=det)
+    maps = prj.att_vol_angles.cpu().numpy()
+    sino = oxrf.fp(f, ang, maps)
+    b = pkg.testing.poisson_data(sino, 1e3, rng)
+    A = lambda x: oxrf.fp(x, ang, maps)  # noqa: E731
+    At = lambda y: oxrf.bp_skimage(y, ang, n)  # noqa: E731
+    return dict(prj=prj, b=b, A=A, At=At, shape=(n, n), f=f)
+
+
+def test_sirt_parity(pkg, xrf_problem):
+    p = xrf_problem
+    x = pkg.SIRT()(p["prj"], p["b"], iterations=30)[0]
+    ref = osol.sirt(p["A"], p["At"], p["b"], p["shape"], 30)
+    print(f"SIRT 30 it: rel_max {rel_max(ref, x):.2e} rel_l2 {rel_l2(ref, x):.2e}")
+    assert rel_max(ref, x) < TOL
+
+
+def test_mlem_parity(pkg, xrf_problem):
+    p = xrf_problem
+    x = pkg.MLEM()(p["prj"], p["b"], iterations=30)[0]
+    ref = osol.mlem(p["A"], p["At"], p["b"], p["shape"], 30)
+    print(f"MLEM 30 it: rel_max {rel_max(ref, x):.2e} rel_l2 {rel_l2(ref, x):.2e}")
+    assert rel_max(ref, x) < TOL
+
+
+@pytest.mark.parametrize("data_term", ["l2", "kl"])
+def test_pdhg_tv_parity(pkg, xrf_problem, data_term):
+    p = xrf_problem
+    reg = pkg.regularizers.Regularizer_TV2D(1e-2)
+    x = pkg.PDHG(data_term=data_term, regularizer=reg)(p["prj"], p["b"], iterations=30)[0]
+    ref = osol.pdhg(p["A"], p["At"], p["b"], p["shape"], 30, data_term=data_term, tv_weight=1e-2)
+    print(f"PDHG-{data_term}-TV 30 it: rel_max {rel_max(ref, x):.2e} rel_l2 {rel_l2(ref, x):.2e}")
+    assert rel_max(ref, x) < TOL
+
+
+def test_pdhg_limits_and_masks(pkg, xrf_problem):
+    p = xrf_problem
+    n = p["shape"][0]
+    x_mask = np.zeros(p["shape"], np.float32)
+    x_mask[2:-2, 2:-2] = 1
+    b_mask = (np.random.default_rng(7).random(p["b"].shape) > 0.1).astype(np.float32)
+    x = pkg.PDHG(data_term="kl")(p["prj"], p["b"], iterations=20, lower_limit=0, upper_limit=0.8, x_mask=x_mask,
+                                 b_mask=b_mask)[0]
+    ref = osol.pdhg(p["A"], p["At"], p["b"], p["shape"], 20, data_term="kl", lower=0, upper=0.8, x_mask=x_mask,
+                    b_mask=b_mask)
+    assert rel_max(ref, x) < TOL
+    assert x.max() <= 0.8 and x.min() >= 0 and np.all(x[:2] == 0) and n == 32
+
+
+# --- the reference's convergence tests on the dense fake projector ---------------
+
+
+@pytest.fixture(scope="module")
+def rand_2d():
+    rng = np.random.default_rng(9)
+    vol = np.fmin(rng.random((10, 10)) + np.finfo(np.float32).eps, 1).astype(np.float32)
+    mat = (rng.random((500, 100)) > 0.5).astype(np.float32)
+    data = (mat @ vol.ravel()).reshape(5, 100).astype(np.float32)
+    return mat, vol, data
+
+
+@pytest.mark.parametrize("algo,its", [("SIRT", 1000), ("PDHG", 500), ("MLEM", 10000)])
+def test_algo_rand(pkg, rand_2d, algo, its):
+    mat, vol, data = rand_2d
+    prj = pkg.ProjectorMatrix(mat, vol.shape, data.shape)
+    sol, _ = getattr(pkg, algo)()(prj, data, iterations=its)
+    assert np.all(np.isclose(sol, vol, atol=1e-2))
+
+
+def test_algo_flat_tv(pkg):
+    rng = np.random.default_rng(11)
+    yy, xx = np.mgrid[:10, :10] - 4.5
+    vol = ((yy**2 + xx**2) <= 9).astype(np.float32)
+    mat = (rng.random((500, 100)) > 0.5).astype(np.float32)
+    data = ((mat @ vol.ravel()).reshape(5, 100) + rng.normal(size=(5, 100)) * 1e-3).astype(np.float32)
+    prj = pkg.ProjectorMatrix(mat, vol.shape, data.shape)
+    for algo, its, kw in ((pkg.SIRT, 2500, {}), (pkg.PDHG, 500, {}), (pkg.PDHG, 500, dict(lower_limit=0, upper_limit=1))):
+        sol, _ = algo(regularizer=pkg.regularizers.Regularizer_TV2D(1e-4))(prj, data, iterations=its, **kw)
+        assert np.all(np.isclose(sol, vol, atol=1e-2)), algo
