@@ -116,6 +116,14 @@ int chip_fp_angle(const chip_plan* plan, const float* vol, const float* maps, fl
 int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, float* vol, int angle,
                   int mode, void* stream);
 
+/*
+ * Kernel selection / tile parameters of a plan, set by the autotuner or by
+ * hand; no reference counterpart (the reference has one CPU code path).
+ * Keys: "fp_kernel" = 0 (per-sample gather through the caches) or
+ * 1 (LDS-staged whole-sinogram kernel, the default).
+ */
+int chip_plan_tune(chip_plan* plan, const char* key, int value);
+
 /* ---- solver building blocks (solvers.py:60-117, 688-1206) ------------------
  * n = number of elements. Optional arrays may be NULL. All elementwise
  * updates are fused single-pass kernels; reductions are deterministic.

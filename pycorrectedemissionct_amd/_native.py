@@ -61,6 +61,7 @@ _SIGNATURES = {
         ctypes.c_int64, c_void_p,
     ],
     "chip_sumsq": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+    "chip_plan_tune": [c_void_p, ctypes.c_char_p, ctypes.c_int],
     "chip_sumabs": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
 }
 _RESTYPES = {"chip_last_error": ctypes.c_char_p, "chip_symbol_name": ctypes.c_char_p}

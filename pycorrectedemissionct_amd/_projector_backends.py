@@ -68,6 +68,10 @@ class HipPlan:
         except Exception:
             pass
 
+    def tune(self, key: str, value: int) -> None:
+        """Kernel selection / tile parameter (``chip_plan_tune``)."""
+        _native.call("chip_plan_tune", self._h, key.encode(), int(value))
+
     # ---- launches (device tensors in / out) -------------------------------
     def fp(self, vol: torch.Tensor, maps: torch.Tensor | None, out: torch.Tensor, batch: int, maps_stride: int = 0):
         _native.call("chip_fp", self._h, _native.ptr(vol), _native.ptr(maps), int(maps_stride), _native.ptr(out),

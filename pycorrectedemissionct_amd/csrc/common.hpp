@@ -64,6 +64,8 @@ struct Plan {
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   int device = 0;
+  // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged
+  int fp_variant = 1;
 };
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }

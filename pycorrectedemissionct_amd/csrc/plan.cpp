@@ -63,7 +63,7 @@ static const char* kSymbols[] = {
     "chip_bp",            "chip_fp_angle",     "chip_bp_angle",     "chip_absmax",
     "chip_precond_fix",   "chip_pdhg_dual_l2", "chip_pdhg_dual_kl", "chip_tv_dual",
     "chip_pdhg_primal",   "chip_sirt_residual", "chip_sirt_update", "chip_mlem_ratio",
-    "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",
+    "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",       "chip_plan_tune",
 };
 
 extern "C" {

@@ -135,8 +135,189 @@ static void launch_fp_group(const Plan& p, const float* vol, const float* maps, 
                      p.n_angles, p.n_det, det0, angle0, line_mode);
 }
 
+// ---------------------------------------------------------------------------
+// fp, LDS-staged (whole-sinogram path)
+// ---------------------------------------------------------------------------
+//
+// One workgroup = one angle x 64 consecutive detector bins, walking the full
+// ray length in chunks of 32 sample rows. Per chunk the axis-aligned
+// bounding box of the chunk's bilinear footprint is loaded once, with
+// coalesced row segments, as T_d = vol * M[a, d] into LDS (zero outside the
+// slice), and the 64 x 32 samples gather their four taps from LDS. Thread
+// (bin = tid % 64, group = tid / 64) accumulates 8 consecutive rows of each
+// chunk in order; the four groups are summed in a fixed order at the end, so
+// results are bitwise reproducible. The sample map is monotone in (u, y) in
+// float32, so the chunk's footprint lies inside the box spanned by its four
+// corner samples (+1 for the right/bottom taps): every tap index stays in the
+// tile without per-tap bounds checks.
+//
+// Block order is XCD-aware: workgroup p runs on XCD p % 8, so logical block
+// ids are assigned such that the 8 bin-blocks of one angle land on the same
+// XCD one after another and share that XCD's L2 copy of the angle's maps.
+
+constexpr int kLdsBins = 64;
+constexpr int kLdsRows = 32;        // sample rows per chunk
+constexpr int kLdsRowsPerThread = 8;
+constexpr int kLdsTile = 76;        // >= 63|c| + 31|s| + 3 over all angles
+constexpr int kLdsPitch = kLdsTile + 1;
+
+__device__ __forceinline__ int xcd_logical_block(int p, int g) {
+  const int q = g >> 3, r = g & 7;
+  const int x = p & 7, k = p >> 3;
+  return x * q + min(x, r) + k;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+                                                     int64_t maps_slice_stride, float* __restrict__ sino,
+                                                     const float4* __restrict__ geom, const float* __restrict__ w,
+                                                     int n, int n_angles, int n_det_total, int det0, int n_ublk) {
+  extern __shared__ float lds[];  // [D][kLdsTile * kLdsPitch]
+  const int g = gridDim.x;
+  const int lb = xcd_logical_block(blockIdx.x, g);
+  const int a = lb / n_ublk;
+  const int u0 = (lb - a * n_ublk) * kLdsBins;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int grp = tid >> 6;
+  const int u = u0 + lane;
+
+  const float4 gm = geom[a];
+  const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* v = vol + static_cast<size_t>(b) * nn;
+  const float* mbase = maps ? maps + static_cast<size_t>(b) * maps_slice_stride +
+                                  (static_cast<size_t>(a) * n_det_total + det0) * nn
+                            : nullptr;
+  const float fn = static_cast<float>(n);
+
+  // per-thread constant parts of the sample map
+  const float uf = static_cast<float>(u);
+  const float xu = __fmul_rn(cs, uf);
+  const float yu = __fmul_rn(-sn, uf);
+  // corner columns of the block (clamped to real bins)
+  const int u_last = min(u0 + kLdsBins, n) - 1;
+  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+
+  float acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = 0.f;
+
+  for (int yc = 0; yc < n; yc += kLdsRows) {
+    const int y_last = min(yc + kLdsRows, n) - 1;
+    // footprint box from the four corner samples
+    float xmin, xmax, ymin, ymax;
+    {
+      const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
+      const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
+      const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
+      const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
+      const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
+      const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
+      const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+      xmin = fminf(fminf(x0, x1), fminf(x2, x3));
+      xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+      ymin = fminf(fminf(y0, y1), fminf(y2, y3));
+      ymax = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+    }
+    // taps span [floor(min), floor(max) + 1]; only [-1, n] can be non-trivial
+    const int clo = max(static_cast<int>(floorf(fmaxf(xmin, -2.f))), -1);
+    const int chi = min(static_cast<int>(floorf(fminf(xmax, fn + 1.f))) + 1, n);
+    const int rlo = max(static_cast<int>(floorf(fmaxf(ymin, -2.f))), -1);
+    const int rhi = min(static_cast<int>(floorf(fminf(ymax, fn + 1.f))) + 1, n);
+    if (chi < 0 || clo > n - 1 || rhi < 0 || rlo > n - 1) continue;  // chunk misses the slice (uniform)
+    const int tw = chi - clo + 1;
+    const int th = rhi - rlo + 1;
+
+    __syncthreads();  // previous chunk's gathers are done with the tile
+    // stage T_d = vol * M_d over the box; rows spread over the 4 waves, 64 lanes per row segment
+    for (int rr = grp; rr < th; rr += 4) {
+      const int r = rlo + rr;
+      const bool rin = r >= 0 && r < n;
+      for (int cc = lane; cc < tw; cc += 64) {
+        const int c = clo + cc;
+        const bool in = rin && c >= 0 && c < n;
+        const size_t off = static_cast<size_t>(r) * n + c;
+        const float fv = in ? v[off] : 0.f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float mv = (in && mbase) ? mbase[static_cast<size_t>(d) * nn + off] : 1.f;
+          lds[d * (kLdsTile * kLdsPitch) + rr * kLdsPitch + cc] = fv * mv;
+        }
+      }
+    }
+    __syncthreads();
+
+    if (u < n) {
+      const int yb0 = yc + grp * kLdsRowsPerThread;
+#pragma unroll
+      for (int k = 0; k < kLdsRowsPerThread; ++k) {
+        const int y = yb0 + k;
+        if (y >= n) break;
+        const float yf = static_cast<float>(y);
+        const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
+        const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+        if (!(xs > -1.f && xs < fn && ys > -1.f && ys < fn)) continue;
+        const float c0f = floorf(xs);
+        const float r0f = floorf(ys);
+        const float dc = xs - c0f;
+        const float dr = ys - r0f;
+        const int li = (static_cast<int>(r0f) - rlo) * kLdsPitch + (static_cast<int>(c0f) - clo);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float* t = lds + d * (kLdsTile * kLdsPitch) + li;
+          const float t00 = t[0], t01 = t[1], t10 = t[kLdsPitch], t11 = t[kLdsPitch + 1];
+          const float top = fmaf(dc, t01 - t00, t00);
+          const float bot = fmaf(dc, t11 - t10, t10);
+          acc[d] += fmaf(dr, bot - top, top);
+        }
+      }
+    }
+  }
+
+  // fixed-order reduction of the 4 row groups
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) lds[(d * 4 + grp) * 64 + lane] = acc[d];
+  __syncthreads();
+  if (grp == 0 && u < n) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float s = ((lds[(d * 4 + 0) * 64 + lane] + lds[(d * 4 + 1) * 64 + lane]) + lds[(d * 4 + 2) * 64 + lane]) +
+                      lds[(d * 4 + 3) * 64 + lane];
+      const int dd = det0 + d;
+      sino[((static_cast<size_t>(b) * n_det_total + dd) * n_angles + a) * n + u] =
+          s * w[static_cast<size_t>(a) * n_det_total + dd];
+    }
+  }
+}
+
+template <int D>
+static void launch_fp_lds_group(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino,
+                                int batch, int det0, hipStream_t st) {
+  const int n_ublk = (p.n + kLdsBins - 1) / kLdsBins;
+  dim3 grid(n_ublk * p.n_angles, batch);
+  const size_t shmem = sizeof(float) * D * kLdsTile * kLdsPitch;
+  hipLaunchKernelGGL(fp_lds_kernel<D>, grid, dim3(256), shmem, st, vol, maps, stride, sino, p.d_fp_geom, p.d_w, p.n,
+                     p.n_angles, p.n_det, det0, n_ublk);
+}
+
 static void launch_fp(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
                       int angle0, int n_ang, int line_mode, hipStream_t st) {
+  const bool lds_path = p.fp_variant == 1 && !line_mode && angle0 == 0 && n_ang == p.n_angles;
+  if (lds_path) {
+    // two detectors per pass keep the staged tiles within 64 KiB of LDS
+    for (int det0 = 0; det0 < p.n_det; det0 += 2) {
+      if (p.n_det - det0 >= 2) {
+        launch_fp_lds_group<2>(p, vol, maps, stride, sino, batch, det0, st);
+      } else {
+        launch_fp_lds_group<1>(p, vol, maps, stride, sino, batch, det0, st);
+      }
+    }
+    return;
+  }
   for (int det0 = 0; det0 < p.n_det; det0 += kMaxDetPerPass) {
     const int nd = std::min(kMaxDetPerPass, p.n_det - det0);
     switch (nd) {
@@ -178,8 +359,19 @@ __global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict
   for (int aa = 0; aa < n_ang; ++aa) {
     const int a = angle0 + aa;
     const float4 g = geom[a];
-    const float t = fmaf(jc, g.x, -(ic * g.y));
-    if (!(t >= tlo && t <= thi)) continue;
+    float t = fmaf(jc, g.x, -(ic * g.y));
+    // np.interp is discontinuous at the ends of the line (0 outside): decide
+    // membership with the reference's double-precision t (exact products of
+    // the integer offsets with the float32 cos/sin, one rounding) whenever
+    // the float32 t is within its error bound of an end
+    if (fabsf(t - tlo) < 1e-3f || fabsf(t - thi) < 1e-3f) {
+      const double td = fma(static_cast<double>(jc), static_cast<double>(g.x),
+                            -(static_cast<double>(ic) * static_cast<double>(g.y)));
+      if (!(td >= static_cast<double>(tlo) && td <= static_cast<double>(thi))) continue;
+      t = fminf(fmaxf(t, tlo), thi);
+    } else if (!(t >= tlo && t <= thi)) {
+      continue;
+    }
     const float tf = floorf(t);
     const float fr = t - tf;
     const int kk = static_cast<int>(tf) + k;  // in [0, n-1]
@@ -330,6 +522,19 @@ int chip_bp(const chip_plan* plan, const float* sino, const float* maps, int64_t
   launch_bp(p, sino, maps, maps_slice_stride, vol, batch, mode, 0, p.n_angles, 0, 0, as_stream(stream));
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
+}
+
+int chip_plan_tune(chip_plan* plan, const char* key, int value) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
+  const std::string k(key);
+  if (k == "fp_kernel") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_kernel must be 0 (gather) or 1 (LDS-staged)");
+    plan->p.fp_variant = value;
+    return CHIP_OK;
+  }
+  set_error("chip_plan_tune: unknown key '" + k + "'");
+  return CHIP_EINVAL;
 }
 
 int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, float* vol, int angle, int mode,

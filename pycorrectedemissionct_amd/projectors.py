@@ -36,20 +36,23 @@ astra_available = False
 class ProjectorMatrix(operators.ProjectorOperator):
     """Projector that uses an explicit projection matrix (``projectors.py:24-100``)."""
 
-    def __init__(self, A, vol_shape, prj_shape) -> None:
-        self.vol_shape = vol_shape
-        self.prj_shape = prj_shape
+    def __init__(self, A, vol_shape, prj_shape, _dev_mat: torch.Tensor | None = None) -> None:
+        self.vol_shape = np.array(vol_shape, ndmin=1, dtype=int)
+        self.prj_shape = np.array(prj_shape, ndmin=1, dtype=int)
         self.A = A
-        self._At = None
+        self._dev = _dev_mat
         super().__init__()
 
     def _mat(self) -> torch.Tensor:
-        if self._At is None:
-            self._At = _arrays.to_dev(np.asarray(self.A, dtype=np.float32))[0]
-        return self._At
+        if self._dev is None:
+            self._dev = _arrays.to_dev(np.asarray(self.A, dtype=np.float32))[0]
+        return self._dev
+
+    def _transpose(self) -> "ProjectorMatrix":
+        return ProjectorMatrix(np.asarray(self.A).transpose(), self.prj_shape, self.vol_shape, self._mat().t())
 
     def absolute(self) -> "ProjectorMatrix":
-        return ProjectorMatrix(np.abs(np.asarray(self.A)), self.vol_shape, self.prj_shape)
+        return ProjectorMatrix(np.abs(np.asarray(self.A)), self.vol_shape, self.prj_shape, torch.abs(self._mat()))
 
     def fp(self, x):
         t, host = _arrays.to_dev(x)

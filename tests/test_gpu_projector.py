@@ -80,17 +80,31 @@ def _xrf_case(pkg, n, a, rng, symmetric=False, bp_mode="skimage"):
     return prj, ang, maps, w_det, w_ang
 
 
-def test_xrf_fp_parity(pkg):
-    n, a = 48, 36
-    rng = np.random.default_rng(300)
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("n,a", [(48, 36), (130, 20)])
+def test_xrf_fp_parity(pkg, kernel, n, a):
+    rng = np.random.default_rng(300 + n)
     prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng)
-    vol = _phantom(n, rng)
+    prj._plan().tune("fp_kernel", kernel)
+    vol = rng.random((n, n)).astype(np.float32)  # also non-zero outside the inscribed circle
     got = prj.fp(vol)
     ref = oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang)
     ref64 = oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang, fp32=False)
-    print(f"xrf fp: rel_max {rel_max(ref, got):.2e} (FP64 {rel_max(ref64, got):.2e}), rel_l2 {rel_l2(ref, got):.2e}")
+    print(f"xrf fp (kernel {kernel}, n={n}): rel_max {rel_max(ref, got):.2e} (FP64 {rel_max(ref64, got):.2e}), "
+          f"rel_l2 {rel_l2(ref, got):.2e}")
     assert got.shape == (2, a, n)
     assert rel_max(ref, got) < TOL
+
+
+def test_fp_three_detectors_lds_path(pkg):
+    """Three detectors run as a two-detector and a one-detector LDS pass."""
+    n, a = 40, 16
+    rng = np.random.default_rng(310)
+    ang = _angles(a)
+    maps = rng.uniform(0.2, 1.0, (a, 3, n, n)).astype(np.float32)
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=[0.5, 1.5, 2.5])
+    vol = _phantom(n, rng)
+    assert rel_max(oxrf.fp(vol, ang, maps), prj.fp(vol)) < TOL
 
 
 def test_xrf_bp_nonsymmetric_parity(pkg):
