@@ -58,9 +58,13 @@ class HipPlan:
         return self._h
 
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            _native.lib().chip_plan_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h is not None and h.value:
+            try:
+                _native.lib().chip_plan_destroy(h)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
 
     def __del__(self):
         try:

@@ -1,0 +1,158 @@
+// Exact adjoint of the forward projection as a scatter-free gather (gfx950).
+//
+// The fp (fp_stream.hip) blends four taps of vol * M[a, d] at every sample
+// (y, u) of angle a: pixel (i, j) receives the weight
+//   hat(Px - j) * hat(Py - i),  hat(t) = max(0, 1 - |t|),
+//   (Px, Py) = (c u + s y + m02, -s u + c y + m12),
+// from each sample whose footprint covers it. The adjoint therefore is
+//   out[i, j] = sum_a sum_d w[a, d] M[a, d][i, j] sum_u K_a(u; i, j) line[d, a, u],
+//   K_a(u; i, j) = sum_y hat(Px(u, y) - j) hat(Py(u, y) - i).
+// In lattice coordinates the covering samples lie within |u - u*| < |c| + |s|
+// <= sqrt(2) (same for y) of the pixel's preimage (u*, y*), i.e. in a 3 x 3
+// block of lattice points; each pixel evaluates those nine weights directly
+// (no scatter, no atomics, deterministic). Samples outside [0, N) in u or y do
+// not exist in the fp and get weight 0.
+//
+// A workgroup owns a 32 x 8 pixel tile (rows of 128 contiguous bytes for the
+// map reads) and walks all angles; per block of 32 angles the line segments
+// the tile needs (<= 40 bins per angle and detector) are staged in LDS.
+#include "common.hpp"
+
+namespace chip {
+namespace {
+
+constexpr int kTileW = 32;
+constexpr int kTileH = 8;
+constexpr int kAngBlk = 32;
+constexpr int kWin = 40;  // >= 31 |c| + 7 |s| + 2 (|c| + |s|) + 2
+
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __restrict__ sino,
+                                                              const float* __restrict__ maps,
+                                                              int64_t maps_slice_stride, float* __restrict__ vol,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ w, int n, int n_angles,
+                                                              int n_det_total, int det0, int accumulate) {
+  __shared__ float win[kAngBlk][D][kWin];
+  __shared__ int wlo_s[kAngBlk];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x * kTileW + (tid & (kTileW - 1));
+  const int i = blockIdx.y * kTileH + (tid / kTileW);
+  const int b = blockIdx.z;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* sb = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
+  const float* mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  const bool inside = i < n && j < n;
+  const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
+  const float fj = static_cast<float>(j), fi = static_cast<float>(i);
+  // tile corners for the per-angle bin window
+  const float tj0 = static_cast<float>(blockIdx.x * kTileW), tj1 = tj0 + (kTileW - 1);
+  const float ti0 = static_cast<float>(blockIdx.y * kTileH), ti1 = ti0 + (kTileH - 1);
+
+  float out = 0.f;
+  for (int a0 = 0; a0 < n_angles; a0 += kAngBlk) {
+    const int na = min(kAngBlk, n_angles - a0);
+    __syncthreads();
+    if (tid < na) {
+      const float4 g = geom[a0 + tid];
+      const float cs = g.x, sn = g.y;
+      const float r = fabsf(cs) + fabsf(sn);
+      // u* = c (x - m02) - s (y - m12) over the tile corners
+      const float u00 = cs * (tj0 - g.z) - sn * (ti0 - g.w), u01 = cs * (tj1 - g.z) - sn * (ti0 - g.w);
+      const float u10 = cs * (tj0 - g.z) - sn * (ti1 - g.w), u11 = cs * (tj1 - g.z) - sn * (ti1 - g.w);
+      const float umin = fminf(fminf(u00, u01), fminf(u10, u11));
+      wlo_s[tid] = static_cast<int>(floorf(umin - r)) - 1;
+    }
+    __syncthreads();
+    for (int k = tid; k < na * D * kWin; k += 256) {
+      const int al = k / (D * kWin);
+      const int rem = k - al * (D * kWin);
+      const int d = rem / kWin;
+      const int q = rem - d * kWin;
+      const int u = wlo_s[al] + q;
+      win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] : 0.f;
+    }
+    __syncthreads();
+    if (inside) {
+      for (int al = 0; al < na; ++al) {
+        const int a = a0 + al;
+        const float4 g = geom[a];
+        const float cs = g.x, sn = g.y, m02 = g.z, m12 = g.w;
+        const float r = fabsf(cs) + fabsf(sn);
+        const float px = fj - m02, py = fi - m12;
+        const float us = cs * px - sn * py;  // preimage of the pixel centre
+        const float ys = sn * px + cs * py;
+        const float fu0 = floorf(us - r) + 1.f;  // first of <= 3 candidate bins / rows
+        const float fy0 = floorf(ys - r) + 1.f;
+        const int iu0 = static_cast<int>(fu0);
+        const int iy0 = static_cast<int>(fy0);
+        // candidate sample coordinates with the fp's float32 rounding sequence
+        float xu[3], yu[3], sy[3], cy[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float uf = fu0 + static_cast<float>(k);
+          const float yf = fy0 + static_cast<float>(k);
+          xu[k] = __fmul_rn(cs, uf);
+          yu[k] = __fmul_rn(-sn, uf);
+          sy[k] = (static_cast<unsigned>(iy0 + k) < static_cast<unsigned>(n)) ? __fmul_rn(sn, yf) : 1e30f;
+          cy[k] = __fmul_rn(cs, yf);
+        }
+        float K[3];
+#pragma unroll
+        for (int ku = 0; ku < 3; ++ku) {
+          float ksum = 0.f;
+#pragma unroll
+          for (int ky = 0; ky < 3; ++ky) {
+            // rows outside [0, N) carry a huge x offset and weigh 0
+            const float xs = __fadd_rn(__fadd_rn(xu[ku], sy[ky]), m02);
+            const float ysc = __fadd_rn(__fadd_rn(yu[ku], cy[ky]), m12);
+            const float wx = fmaxf(1.f - fabsf(xs - fj), 0.f);
+            const float wy = fmaxf(1.f - fabsf(ysc - fi), 0.f);
+            ksum = fmaf(wx, wy, ksum);
+          }
+          K[ku] = ksum;
+        }
+        const int q0 = iu0 - wlo_s[al];  // bins outside [0, N) read the zero padding
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
+          float f = w[static_cast<size_t>(a) * n_det_total + det0 + d];
+          if (MAPS) f *= mb[(static_cast<size_t>(a) * n_det_total + det0 + d) * nn + pix];
+          out = fmaf(f, gd, out);
+        }
+      }
+    }
+  }
+  if (inside) {
+    float* o = vol + static_cast<size_t>(b) * nn + pix;
+    *o = accumulate ? *o + out : out;
+  }
+}
+
+template <int D, bool MAPS>
+void launch(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch, int det0,
+            int accumulate, hipStream_t st) {
+  dim3 grid((p.n + kTileW - 1) / kTileW, (p.n + kTileH - 1) / kTileH, batch);
+  hipLaunchKernelGGL((bp_adjoint_tile_kernel<D, MAPS>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, accumulate);
+}
+
+}  // namespace
+
+void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
+                             int batch, hipStream_t st) {
+  // detectors in passes of two; the first pass overwrites, later passes add
+  for (int det0 = 0; det0 < p.n_det; det0 += 2) {
+    const bool two = p.n_det - det0 >= 2;
+    const int acc = det0 > 0 ? 1 : 0;
+    if (maps) {
+      if (two) launch<2, true>(p, sino, maps, stride, vol, batch, det0, acc, st);
+      else launch<1, true>(p, sino, maps, stride, vol, batch, det0, acc, st);
+    } else {
+      if (two) launch<2, false>(p, sino, nullptr, stride, vol, batch, det0, acc, st);
+      else launch<1, false>(p, sino, nullptr, stride, vol, batch, det0, acc, st);
+    }
+  }
+}
+
+}  // namespace chip

@@ -58,17 +58,29 @@ struct Plan {
   std::vector<Affine> h_rot_in_fwd, h_rot_in_back;    // [A]
   std::vector<Affine> h_rot_out_fwd, h_rot_out_back;  // [A][D]
   float4* d_fp_geom = nullptr;
+  float2* d_ytab = nullptr;  // [A][N] (fl(sin * y), fl(cos * y)) row terms of the fp map
   float* d_w = nullptr;
   Affine* d_rot = nullptr;  // [A] in-fwd, [A] in-back, [A*D] out-fwd, [A*D] out-back
   // scratch for the attenuation builder (rotated-frame cumulative sums)
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   int device = 0;
-  // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged
-  int fp_variant = 1;
+  // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged,
+  // 2 = LDS-staged with register prefetch, 3 = prefetched and lean
+  // (2 and 3 fall back to 1 when N % 4 != 0)
+  int fp_variant = 3;
 };
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// whole-sinogram fp over detectors [det0, det0 + nd), nd in {1, 2}; needs
+// N % 4 == 0 and 16-byte aligned vol / maps (fp_stream.hip)
+void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
+                      int det0, int nd, hipStream_t st);
+
+// whole-sinogram exact adjoint of the fp, all detectors (bp_adjoint.hip)
+void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
+                             int batch, hipStream_t st);
 
 constexpr int kMaxDetPerPass = 4;
 

@@ -1,0 +1,294 @@
+// Whole-sinogram forward projection streaming the attenuation maps (gfx950).
+//
+// Discretisation: scikit-image 0.18.3 radon as called by corrct's skimage
+// backend (_projector_backends.py:312-333) on vol * M[a, d]
+// (projectors.py:519-525): sample (y, u) of angle a reads
+//   x' = (c u + s y) + m02,  y' = (-s u + c y) + m12       (float32, no FMA)
+// with an order-1 blend, 0 outside the slice; the line is the sum over y,
+// times w[a, d].
+//
+// Decomposition: a workgroup owns one angle and 64 consecutive bins and walks
+// the rays in chunks of 32 sample rows. For each chunk the bounding box of
+// the chunk's bilinear footprint (monotone float32 map => spanned by the four
+// corner samples, +1 for the right/bottom taps) is staged in LDS as
+// T_d = vol * M[a, d] for the two detectors of the pass; the 2048 samples then
+// gather their taps from LDS. Staging is software-pipelined: the next
+// chunk's box is fetched into registers as float4 (box columns start on a
+// multiple of 4; with N % 4 == 0 each float4 lies wholly inside or outside
+// the slice) while the current chunk is gathered. Each slot of the box has a
+// per-thread constant offset, so interior chunks cost one add per slot;
+// chunks touching the slice border take a checked path that also writes
+// the zero padding.
+//
+// Thread (bin = tid % 64, group = tid / 64) accumulates 8 consecutive rows
+// of each chunk in order and the four groups are added in a fixed order, so
+// the result is bitwise reproducible.
+//
+// Workgroup p runs on XCD p % 8; logical block ids are permuted so that the
+// bin-blocks of one angle run back to back on one XCD and share its L2.
+#include "common.hpp"
+
+namespace chip {
+namespace {
+
+constexpr int kBins = 64;         // detector bins per workgroup
+constexpr int kRows = 32;         // sample rows per chunk
+constexpr int kRowsPerGroup = 8;  // rows per wave per chunk
+constexpr int kBoxRows = 76;      // >= 63|s| + 31|c| + 3
+constexpr int kSlots = 20;        // float4 slots per box row (<= 76 + 3 alignment columns)
+constexpr int kPitch = 81;        // odd pitch: row-strided gathers spread over banks
+constexpr int kTile = kBoxRows * kPitch;
+constexpr int kPerThread = (kBoxRows * kSlots + 255) / 256;  // 6
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+struct Box {
+  int clo, chi, rlo, rhi;  // inclusive; clo is a multiple of 4 (or -4)
+  int all_valid;           // every sample of the chunk lies in (-1, N)^2
+  int empty;               // the chunk misses the slice
+  int border;              // the box reaches outside the slice
+};
+
+__device__ __forceinline__ int logical_block(int p, int g) {
+  const int q = g >> 3, r = g & 7;
+  const int x = p & 7, k = p >> 3;
+  return x * q + min(x, r) + k;
+}
+
+__device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, float yu_a, float xu_b, float yu_b,
+                                         float cs, float sn, float m02, float m12) {
+  const int y_last = min(yc + kRows, n) - 1;
+  const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
+  const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
+  const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
+  const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
+  const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
+  const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
+  const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+  const float xmin = fminf(fminf(x0, x1), fminf(x2, x3));
+  const float xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+  const float ymin = fminf(fminf(y0, y1), fminf(y2, y3));
+  const float ymax = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+  Box bx;
+  bx.all_valid = (xmin > -1.f && xmax < fn && ymin > -1.f && ymax < fn) ? 1 : 0;
+  const int clo = max(static_cast<int>(floorf(fmaxf(xmin, -2.f))), -1);
+  bx.chi = min(static_cast<int>(floorf(fminf(xmax, fn + 1.f))) + 1, n);
+  bx.rlo = max(static_cast<int>(floorf(fmaxf(ymin, -2.f))), -1);
+  bx.rhi = min(static_cast<int>(floorf(fminf(ymax, fn + 1.f))) + 1, n);
+  bx.clo = clo >= 0 ? (clo & ~3) : -4;
+  bx.empty = (bx.chi < 0 || clo > n - 1 || bx.rhi < 0 || bx.rlo > n - 1) ? 1 : 0;
+  bx.border = (bx.rlo < 0 || bx.rhi > n - 1 || bx.clo < 0 || bx.chi > n - 1) ? 1 : 0;
+  // border chunks clamp sample coordinates to [-1, N]; a sample clamped to N
+  // reads taps N and N + 1, so the box then carries one more zero row/column
+  if (bx.chi == n) bx.chi = n + 1;
+  if (bx.rhi == n) bx.rhi = n + 1;
+  return bx;
+}
+
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+                                                        int64_t maps_slice_stride, float* __restrict__ sino,
+                                                        const float4* __restrict__ geom, const float* __restrict__ w,
+                                                        int n, int n_angles, int n_det_total, int det0, int n_ublk) {
+  extern __shared__ float lds[];  // [D][kTile]
+  const int lb = logical_block(blockIdx.x, gridDim.x);
+  const int a = lb / n_ublk;
+  const int u0 = (lb - a * n_ublk) * kBins;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int grp = tid >> 6;
+  const int u = u0 + lane;
+
+  const float4 gm = geom[a];
+  const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* v = vol + static_cast<size_t>(b) * nn;
+  const float* mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride +
+                               (static_cast<size_t>(a) * n_det_total + det0) * nn
+                         : nullptr;
+  const float fn = static_cast<float>(n);
+  const float uf = static_cast<float>(u);
+  const float xu = __fmul_rn(cs, uf);
+  const float yu = __fmul_rn(-sn, uf);
+  const int u_last = min(u0 + kBins, n) - 1;
+  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+
+  // staging slots of this thread: box row, first column, offset from the box origin
+  int s_row[kPerThread], s_col[kPerThread], s_off[kPerThread];
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    const int s = tid + 256 * i;
+    s_row[i] = s / kSlots;
+    s_col[i] = 4 * (s - s_row[i] * kSlots);
+    s_off[i] = s_row[i] * n + s_col[i];
+  }
+
+  float4 fr[kPerThread];
+  float4 mr[D][kPerThread];
+  unsigned inmask = 0u;
+
+  auto issue = [&](const Box& bx) {
+    const int th = bx.rhi - bx.rlo + 1;
+    const int cw = bx.chi - bx.clo + 1;
+    const int base = bx.rlo * n + bx.clo;
+    inmask = 0u;
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      bool in = s_row[i] < th && s_col[i] < cw;
+      if (bx.border) {
+        const int r = bx.rlo + s_row[i];
+        const int c = bx.clo + s_col[i];
+        in = in && static_cast<unsigned>(r) < static_cast<unsigned>(n) &&
+             static_cast<unsigned>(c) < static_cast<unsigned>(n);
+      }
+      if (in) {
+        inmask |= 1u << i;
+        // uniform base + 32-bit byte offset: scalar-base addressing, no 64-bit temporaries
+        const unsigned off = static_cast<unsigned>(base + s_off[i]) * 4u;
+        fr[i] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(v) + off);
+        if (MAPS) {
+          // regular (L2-cached) loads: neighbouring bin-blocks re-read the overlap of their boxes
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            mr[d][i] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(mb + d * nn) + off);
+        }
+      }
+    }
+  };
+  auto stage = [&](const Box& bx) {
+    const int th = bx.rhi - bx.rlo + 1;
+    const int cw = bx.chi - bx.clo + 1;
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const int o = s_row[i] * kPitch + s_col[i];
+      if (inmask & (1u << i)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          float* t = lds + d * kTile + o;
+          t[0] = MAPS ? fr[i].x * mr[d][i].x : fr[i].x;
+          t[1] = MAPS ? fr[i].y * mr[d][i].y : fr[i].y;
+          t[2] = MAPS ? fr[i].z * mr[d][i].z : fr[i].z;
+          t[3] = MAPS ? fr[i].w * mr[d][i].w : fr[i].w;
+        }
+      } else if (bx.border && s_row[i] < th && s_col[i] < cw) {  // zero padding outside the slice
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          float* t = lds + d * kTile + o;
+          t[0] = 0.f;
+          t[1] = 0.f;
+          t[2] = 0.f;
+          t[3] = 0.f;
+        }
+      }
+    }
+  };
+
+  float acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = 0.f;
+
+  int yc = 0;
+  Box cur;
+  for (;; yc += kRows) {
+    if (yc >= n) break;
+    cur = chunk_box(yc, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+    if (!cur.empty) break;
+  }
+  if (yc < n) issue(cur);
+  while (yc < n) {
+    __syncthreads();  // the previous chunk's gathers are done with the tile
+    stage(cur);
+    __syncthreads();
+    int yn = yc + kRows;  // next chunk meeting the slice; its loads overlap this chunk's gathers
+    Box nxt;
+    for (; yn < n; yn += kRows) {
+      nxt = chunk_box(yn, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+      if (!nxt.empty) break;
+    }
+    if (yn < n) issue(nxt);
+
+    if (u < n) {
+      const int yb0 = yc + grp * kRowsPerGroup;  // wave-uniform
+      const int base = cur.rlo * kPitch + cur.clo;
+      // one sample: skimage inverse map, taps from LDS, order-1 blend
+      auto sample = [&](int y, bool clamp) {
+        const float yf = static_cast<float>(y);
+        float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
+        float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+        if (clamp) {
+          // outside (-1, N) every tap is padding: clamping keeps the taps on
+          // the zero rows/columns of the box and the sample at exactly 0
+          xs = fminf(fmaxf(xs, -1.f), fn);
+          ys = fminf(fmaxf(ys, -1.f), fn);
+        }
+        const float c0f = floorf(xs);
+        const float r0f = floorf(ys);
+        const float dc = xs - c0f;
+        const float dr = ys - r0f;
+        const int li = static_cast<int>(fmaf(r0f, static_cast<float>(kPitch), c0f)) - base;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float* t = lds + d * kTile + li;
+          const float t00 = t[0], t01 = t[1], t10 = t[kPitch], t11 = t[kPitch + 1];
+          const float top = fmaf(dc, t01 - t00, t00);
+          const float bot = fmaf(dc, t11 - t10, t10);
+          acc[d] += fmaf(dr, bot - top, top);
+        }
+      };
+      if (!cur.border) {
+#pragma unroll
+        for (int k = 0; k < kRowsPerGroup; ++k)
+          if (yb0 + k < n) sample(yb0 + k, false);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kRowsPerGroup; ++k)
+          if (yb0 + k < n) sample(yb0 + k, true);
+      }
+    }
+    yc = yn;
+    cur = nxt;
+  }
+
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) lds[(d * 4 + grp) * 64 + lane] = acc[d];
+  __syncthreads();
+  if (grp == 0 && u < n) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float s = ((lds[(d * 4 + 0) * 64 + lane] + lds[(d * 4 + 1) * 64 + lane]) + lds[(d * 4 + 2) * 64 + lane]) +
+                      lds[(d * 4 + 3) * 64 + lane];
+      const int dd = det0 + d;
+      sino[((static_cast<size_t>(b) * n_det_total + dd) * n_angles + a) * n + u] =
+          s * w[static_cast<size_t>(a) * n_det_total + dd];
+    }
+  }
+}
+
+template <int D, bool MAPS>
+void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch, int det0,
+            hipStream_t st) {
+  const int n_ublk = (p.n + kBins - 1) / kBins;
+  dim3 grid(n_ublk * p.n_angles, batch);
+  const size_t shmem = sizeof(float) * D * kTile;
+  hipLaunchKernelGGL((fp_stream_kernel<D, MAPS>), grid, dim3(256), shmem, st, vol, maps, stride, sino, p.d_fp_geom,
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk);
+}
+
+}  // namespace
+
+void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
+                      int det0, int nd, hipStream_t st) {
+  if (maps) {
+    if (nd == 2) launch<2, true>(p, vol, maps, stride, sino, batch, det0, st);
+    else launch<1, true>(p, vol, maps, stride, sino, batch, det0, st);
+  } else {
+    // without maps every detector sees the same slice
+    if (nd == 2) launch<2, false>(p, vol, nullptr, stride, sino, batch, det0, st);
+    else launch<1, false>(p, vol, nullptr, stride, sino, batch, det0, st);
+  }
+}
+
+}  // namespace chip

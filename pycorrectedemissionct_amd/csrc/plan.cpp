@@ -146,6 +146,20 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
   hipError_t e;
   if ((e = hipMalloc(&p.d_fp_geom, sizeof(float4) * n_angles)) != hipSuccess) return fail(e, "hipMalloc geom");
   if ((e = hipMalloc(&p.d_w, sizeof(float) * p.h_w.size())) != hipSuccess) return fail(e, "hipMalloc w");
+  {
+    // row terms of the fp inverse map, one float32 product each (as the kernels form them)
+    std::vector<float2> ytab(static_cast<size_t>(n_angles) * n);
+    for (int a = 0; a < n_angles; ++a) {
+      const float c = p.h_fp_geom[4 * a + 0], s = p.h_fp_geom[4 * a + 1];
+      for (int y = 0; y < n; ++y) {
+        const float yf = static_cast<float>(y);
+        ytab[static_cast<size_t>(a) * n + y] = make_float2(s * yf, c * yf);
+      }
+    }
+    if ((e = hipMalloc(&p.d_ytab, sizeof(float2) * ytab.size())) != hipSuccess) return fail(e, "hipMalloc ytab");
+    if ((e = hipMemcpy(p.d_ytab, ytab.data(), sizeof(float2) * ytab.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "hipMemcpy ytab");
+  }
   const size_t n_rot = 2 * static_cast<size_t>(n_angles) + 2 * static_cast<size_t>(n_angles) * n_det;
   if ((e = hipMalloc(&p.d_rot, sizeof(Affine) * n_rot)) != hipSuccess) return fail(e, "hipMalloc rot");
   std::vector<Affine> rot;
@@ -169,6 +183,7 @@ int chip_plan_destroy(chip_plan* plan) {
   Plan& p = plan->p;
   if (p.d_fp_geom) (void)hipFree(p.d_fp_geom);
   if (p.d_w) (void)hipFree(p.d_w);
+  if (p.d_ytab) (void)hipFree(p.d_ytab);
   if (p.d_rot) (void)hipFree(p.d_rot);
   if (p.d_scratch) (void)hipFree(p.d_scratch);
   delete plan;

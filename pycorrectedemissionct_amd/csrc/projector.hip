@@ -294,6 +294,233 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ v
   }
 }
 
+// ---------------------------------------------------------------------------
+// fp, LDS-staged with register prefetch (N % 4 == 0)
+// ---------------------------------------------------------------------------
+//
+// Same decomposition as fp_lds_kernel, with the staging software-pipelined:
+// the next chunk's footprint box is fetched into registers as float4 (its
+// columns start at a multiple of 4, so with N % 4 == 0 every float4 is
+// either wholly inside the slice or wholly outside) while the current chunk
+// is gathered from LDS. Chunks whose box misses the slice are skipped; in
+// chunks whose samples all fall inside (-1, N) the per-sample test is
+// skipped too.
+
+constexpr int kP2Rows = 76;    // box rows  <= 63|s| + 31|c| + 3
+constexpr int kP2Slots = 20;   // float4 slots per box row (box cols <= 76 + 3 alignment)
+constexpr int kP2Pitch = 81;   // odd pitch: row-strided gathers hit distinct banks
+constexpr int kP2PerThread = (kP2Rows * kP2Slots + 255) / 256;  // 6
+
+struct FpBox {
+  int clo, chi, rlo, rhi;  // box (clo is 4-aligned), inclusive
+  int valid_all;           // every sample of the chunk lies in (-1, N)^2
+  int empty;
+};
+
+__device__ __forceinline__ FpBox fp_chunk_box(int yc, int n, float fn, float xu_a, float yu_a, float xu_b, float yu_b,
+                                              float cs, float sn, float m02, float m12) {
+  const int y_last = min(yc + kLdsRows, n) - 1;
+  const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
+  const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
+  const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
+  const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
+  const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
+  const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
+  const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+  const float xmin = fminf(fminf(x0, x1), fminf(x2, x3));
+  const float xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+  const float ymin = fminf(fminf(y0, y1), fminf(y2, y3));
+  const float ymax = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+  FpBox bx;
+  bx.valid_all = (xmin > -1.f && xmax < fn && ymin > -1.f && ymax < fn) ? 1 : 0;
+  const int clo = max(static_cast<int>(floorf(fmaxf(xmin, -2.f))), -1);
+  bx.chi = min(static_cast<int>(floorf(fminf(xmax, fn + 1.f))) + 1, n);
+  bx.rlo = max(static_cast<int>(floorf(fmaxf(ymin, -2.f))), -1);
+  bx.rhi = min(static_cast<int>(floorf(fminf(ymax, fn + 1.f))) + 1, n);
+  bx.clo = clo >= 0 ? (clo & ~3) : -4;
+  bx.empty = (bx.chi < 0 || clo > n - 1 || bx.rhi < 0 || bx.rlo > n - 1) ? 1 : 0;
+  return bx;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void fp_lds2_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+                                                      int64_t maps_slice_stride, float* __restrict__ sino,
+                                                      const float4* __restrict__ geom, const float* __restrict__ w,
+                                                      int n, int n_angles, int n_det_total, int det0, int n_ublk) {
+  extern __shared__ float lds[];  // [D][kP2Rows * kP2Pitch]
+  constexpr int kMapStride = kP2Rows * kP2Pitch;
+  const int g = gridDim.x;
+  const int lb = xcd_logical_block(blockIdx.x, g);
+  const int a = lb / n_ublk;
+  const int u0 = (lb - a * n_ublk) * kLdsBins;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int grp = tid >> 6;
+  const int u = u0 + lane;
+
+  const float4 gm = geom[a];
+  const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* v = vol + static_cast<size_t>(b) * nn;
+  const float* mbase = maps ? maps + static_cast<size_t>(b) * maps_slice_stride +
+                                  (static_cast<size_t>(a) * n_det_total + det0) * nn
+                            : nullptr;
+  const float fn = static_cast<float>(n);
+  const float uf = static_cast<float>(u);
+  const float xu = __fmul_rn(cs, uf);
+  const float yu = __fmul_rn(-sn, uf);
+  const int u_last = min(u0 + kLdsBins, n) - 1;
+  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+
+  float4 fr[kP2PerThread];
+  float4 mr[D][kP2PerThread];
+
+  auto issue = [&](const FpBox& bx) {
+    const int th = bx.rhi - bx.rlo + 1;
+    const int cw = bx.chi - bx.clo + 1;
+#pragma unroll
+    for (int i = 0; i < kP2PerThread; ++i) {
+      const int s = tid + 256 * i;
+      const int row = s / kP2Slots;
+      const int c4 = s - row * kP2Slots;
+      const int r = bx.rlo + row;
+      const int c = bx.clo + 4 * c4;
+      const bool in = row < th && 4 * c4 < cw && r >= 0 && r < n && c >= 0 && c < n;
+      const size_t off = in ? static_cast<size_t>(r) * n + c : 0;
+      fr[i] = in ? *reinterpret_cast<const float4*>(v + off) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        mr[d][i] = (in && mbase) ? *reinterpret_cast<const float4*>(mbase + static_cast<size_t>(d) * nn + off)
+                                 : make_float4(1.f, 1.f, 1.f, 1.f);
+      }
+    }
+  };
+  auto stage = [&](const FpBox& bx) {
+    const int th = bx.rhi - bx.rlo + 1;
+    const int cw = bx.chi - bx.clo + 1;
+#pragma unroll
+    for (int i = 0; i < kP2PerThread; ++i) {
+      const int s = tid + 256 * i;
+      const int row = s / kP2Slots;
+      const int c4 = s - row * kP2Slots;
+      if (row < th && 4 * c4 < cw) {
+        const int o = row * kP2Pitch + 4 * c4;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          float* t = lds + d * kMapStride + o;
+          t[0] = fr[i].x * mr[d][i].x;
+          t[1] = fr[i].y * mr[d][i].y;
+          t[2] = fr[i].z * mr[d][i].z;
+          t[3] = fr[i].w * mr[d][i].w;
+        }
+      }
+    }
+  };
+
+  float acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = 0.f;
+
+  int yc = 0;
+  FpBox cur;
+  for (;; yc += kLdsRows) {  // first chunk that meets the slice
+    if (yc >= n) break;
+    cur = fp_chunk_box(yc, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+    if (!cur.empty) break;
+  }
+  if (yc < n) issue(cur);
+  while (yc < n) {
+    __syncthreads();  // the previous chunk's gathers are done with the tile
+    stage(cur);
+    __syncthreads();
+    // look ahead: next chunk that meets the slice; its loads overlap this chunk's gathers
+    int yn = yc + kLdsRows;
+    FpBox nxt;
+    for (; yn < n; yn += kLdsRows) {
+      nxt = fp_chunk_box(yn, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+      if (!nxt.empty) break;
+    }
+    if (yn < n) issue(nxt);
+
+    if (u < n) {
+      const int yb0 = yc + grp * kLdsRowsPerThread;
+      const int base = cur.rlo * kP2Pitch + cur.clo;
+      if (cur.valid_all && yb0 + kLdsRowsPerThread <= n) {
+#pragma unroll
+        for (int k = 0; k < kLdsRowsPerThread; ++k) {
+          const float yf = static_cast<float>(yb0 + k);
+          const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
+          const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+          const float c0f = floorf(xs);
+          const float r0f = floorf(ys);
+          const float dc = xs - c0f;
+          const float dr = ys - r0f;
+          const int li = static_cast<int>(r0f) * kP2Pitch + static_cast<int>(c0f) - base;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const float* t = lds + d * kMapStride + li;
+            const float t00 = t[0], t01 = t[1], t10 = t[kP2Pitch], t11 = t[kP2Pitch + 1];
+            const float top = fmaf(dc, t01 - t00, t00);
+            const float bot = fmaf(dc, t11 - t10, t10);
+            acc[d] += fmaf(dr, bot - top, top);
+          }
+        }
+      } else {
+        for (int k = 0; k < kLdsRowsPerThread; ++k) {
+          const int y = yb0 + k;
+          if (y >= n) break;
+          const float yf = static_cast<float>(y);
+          const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
+          const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+          if (!(xs > -1.f && xs < fn && ys > -1.f && ys < fn)) continue;
+          const float c0f = floorf(xs);
+          const float r0f = floorf(ys);
+          const float dc = xs - c0f;
+          const float dr = ys - r0f;
+          const int li = static_cast<int>(r0f) * kP2Pitch + static_cast<int>(c0f) - base;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const float* t = lds + d * kMapStride + li;
+            const float t00 = t[0], t01 = t[1], t10 = t[kP2Pitch], t11 = t[kP2Pitch + 1];
+            const float top = fmaf(dc, t01 - t00, t00);
+            const float bot = fmaf(dc, t11 - t10, t10);
+            acc[d] += fmaf(dr, bot - top, top);
+          }
+        }
+      }
+    }
+    yc = yn;
+    cur = nxt;
+  }
+
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) lds[(d * 4 + grp) * 64 + lane] = acc[d];
+  __syncthreads();
+  if (grp == 0 && u < n) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float s = ((lds[(d * 4 + 0) * 64 + lane] + lds[(d * 4 + 1) * 64 + lane]) + lds[(d * 4 + 2) * 64 + lane]) +
+                      lds[(d * 4 + 3) * 64 + lane];
+      const int dd = det0 + d;
+      sino[((static_cast<size_t>(b) * n_det_total + dd) * n_angles + a) * n + u] =
+          s * w[static_cast<size_t>(a) * n_det_total + dd];
+    }
+  }
+}
+
+template <int D>
+static void launch_fp_lds2_group(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino,
+                                 int batch, int det0, hipStream_t st) {
+  const int n_ublk = (p.n + kLdsBins - 1) / kLdsBins;
+  dim3 grid(n_ublk * p.n_angles, batch);
+  const size_t shmem = sizeof(float) * D * kP2Rows * kP2Pitch;
+  hipLaunchKernelGGL(fp_lds2_kernel<D>, grid, dim3(256), shmem, st, vol, maps, stride, sino, p.d_fp_geom, p.d_w, p.n,
+                     p.n_angles, p.n_det, det0, n_ublk);
+}
+
 template <int D>
 static void launch_fp_lds_group(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino,
                                 int batch, int det0, hipStream_t st) {
@@ -306,14 +533,23 @@ static void launch_fp_lds_group(const Plan& p, const float* vol, const float* ma
 
 static void launch_fp(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
                       int angle0, int n_ang, int line_mode, hipStream_t st) {
-  const bool lds_path = p.fp_variant == 1 && !line_mode && angle0 == 0 && n_ang == p.n_angles;
-  if (lds_path) {
+  const bool whole = !line_mode && angle0 == 0 && n_ang == p.n_angles;
+  // the prefetching kernel reads float4: needs N % 4 == 0 and 16-byte aligned slices / maps
+  const bool vec_ok = (p.n % 4 == 0) && (stride % 4 == 0) && (reinterpret_cast<uintptr_t>(vol) % 16 == 0) &&
+                      (reinterpret_cast<uintptr_t>(maps) % 16 == 0);
+  const int variant = (p.fp_variant >= 2 && !vec_ok) ? 1 : p.fp_variant;
+  if (whole && variant >= 1) {
     // two detectors per pass keep the staged tiles within 64 KiB of LDS
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
-      if (p.n_det - det0 >= 2) {
-        launch_fp_lds_group<2>(p, vol, maps, stride, sino, batch, det0, st);
+      const bool two = p.n_det - det0 >= 2;
+      if (variant == 3) {
+        launch_fp_stream(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, st);
+      } else if (variant == 2) {
+        if (two) launch_fp_lds2_group<2>(p, vol, maps, stride, sino, batch, det0, st);
+        else launch_fp_lds2_group<1>(p, vol, maps, stride, sino, batch, det0, st);
       } else {
-        launch_fp_lds_group<1>(p, vol, maps, stride, sino, batch, det0, st);
+        if (two) launch_fp_lds_group<2>(p, vol, maps, stride, sino, batch, det0, st);
+        else launch_fp_lds_group<1>(p, vol, maps, stride, sino, batch, det0, st);
       }
     }
     return;
@@ -472,6 +708,8 @@ static void launch_bp(const Plan& p, const float* sino, const float* maps, int64
   } else if (mode == CHIP_BP_SKIMAGE_SYM) {
     hipLaunchKernelGGL(bp_skimage_kernel<1>, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w,
                        p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
+  } else if (!line_mode && angle0 == 0 && n_ang == p.n_angles && !accumulate) {
+    launch_bp_adjoint_tiles(p, sino, maps, stride, vol, batch, st);
   } else {
     hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w, p.n,
                        p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
@@ -529,7 +767,7 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_kernel must be 0 (gather) or 1 (LDS-staged)");
+    CHIP_CHECK_ARG(value >= 0 && value <= 3, "fp_kernel must be in 0..3");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }

@@ -55,7 +55,9 @@ def phantom_ellipses(n: int, n_ellipses: int, rng: np.random.Generator, semi_axe
             dist = rng.uniform(0, max(r_max - 3 * sg, 0))
             ang = rng.uniform(0, 2 * np.pi)
             cy, cx = c + dist * np.sin(ang), c + dist * np.cos(ang)
-            img += rng.uniform(*intensities) * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg**2))
+            r2 = (yy - cy) ** 2 + (xx - cx) ** 2
+            # truncated at 3 sigma, so the blob's support (and its attenuation) stays local
+            img += np.where(r2 <= 9 * sg**2, rng.uniform(*intensities) * np.exp(-r2 / (2 * sg**2)), 0.0)
     yy, xx = np.mgrid[:n, :n] - c
     img[(yy**2 + xx**2) > (n // 2) ** 2] = 0
     return img.astype(np.float32)

@@ -80,7 +80,7 @@ def _xrf_case(pkg, n, a, rng, symmetric=False, bp_mode="skimage"):
     return prj, ang, maps, w_det, w_ang
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,a", [(48, 36), (130, 20)])
 def test_xrf_fp_parity(pkg, kernel, n, a):
     rng = np.random.default_rng(300 + n)
@@ -143,6 +143,31 @@ def test_adjoint_dot_test_and_oracle(pkg):
     assert abs(lhs - rhs) / abs(lhs) < TOL
     ref = oxrf.bp_adjoint(y, ang, n, maps, w_det=w_det, w_ang=w_ang)
     assert rel_max(ref, aty) < TOL
+
+
+def test_adjoint_ragged_three_detectors(pkg):
+    """Tiled adjoint on a ragged slice (70 = 2 x 32 + 6 columns), 50 angles (one full and one
+    partial block of 32) and three detectors (a two- and a one-detector pass) vs the scatter
+    oracle, the dot test, and the sum of the per-angle adjoint."""
+    n, a = 70, 50
+    rng = np.random.default_rng(306)
+    ang = _angles(a)
+    det = [0.5, 1.5, 2.5]
+    maps = rng.uniform(0.2, 1.0, (a, 3, n, n)).astype(np.float32)
+    w_det = np.array([1.0, 0.6, 0.8])
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=det, weights_detectors=w_det,
+                                      bp_mode="adjoint")
+    y = rng.random((3, a, n)).astype(np.float32)
+    x = rng.random((n, n)).astype(np.float32)
+    aty = prj.bp(y)
+    ref = oxrf.bp_adjoint(y, ang, n, maps, w_det=w_det)
+    print(f"tiled adjoint n={n} a={a} d=3: rel_max vs oracle {rel_max(ref, aty):.2e}")
+    assert rel_max(ref, aty) < TOL
+    lhs = float(np.sum(prj.fp(x).astype(np.float64) * y))
+    rhs = float(np.sum(x * aty.astype(np.float64)))
+    assert abs(lhs - rhs) / abs(lhs) < TOL
+    acc = sum(prj.bp_angle(y, ia) for ia in range(a))
+    assert rel_max(aty, acc) < TOL
 
 
 def test_fp_angle_bp_angle_consistent(pkg):
