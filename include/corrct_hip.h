@@ -120,8 +120,9 @@ int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, f
  * Kernel selection / tile parameters of a plan, set by the autotuner or by
  * hand; no reference counterpart (the reference has one CPU code path).
  * Keys: "fp_kernel" = 0 (per-sample gather through the caches),
- * 1 (LDS-staged whole-sinogram kernel) or 2 (LDS-staged with register
- * prefetch, the default; used when N % 4 == 0, else 1).
+ * 1 (LDS-staged whole-sinogram kernel), 2 (LDS-staged with register
+ * prefetch) or 3 (streamed per-detector LDS tiles, the default); 2 and 3
+ * need N % 4 == 0 and 16-byte aligned vol / maps, else 1 runs.
  */
 int chip_plan_tune(chip_plan* plan, const char* key, int value);
 
