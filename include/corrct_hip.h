@@ -117,6 +117,24 @@ int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, f
                   int mode, void* stream);
 
 /*
+ * On-the-fly attenuation: fp / bp with the maps evaluated from mu_in and
+ * mu_out (batch slices of n*n each, either may be NULL) instead of read from
+ * a precomputed [A][D][n][n] set. Angles run in blocks; each block's maps are
+ * built by the same kernels as chip_att_maps into plan-owned scratch bounded
+ * by the "otf_budget_mib" tune key (default 1024), then projected, so the
+ * results equal chip_fp / chip_bp over chip_att_maps' output. For
+ * configurations whose maps do not fit in memory (the fused on-the-fly mode
+ * of AttenuationVolume.compute_maps, physics/attenuation.py:122-180, with
+ * ProjectorAttenuationXRF.fp / bp, projectors.py:570-622). chip_bp_otf takes
+ * mode CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT (CHIP_BP_SKIMAGE reads no maps:
+ * use chip_bp). Calls on one plan must not overlap: they share its scratch.
+ */
+int chip_fp_otf(chip_plan* plan, const float* vol, const float* mu_in, const float* mu_out, float* sino,
+                int batch, void* stream);
+int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const float* mu_out, float* vol,
+                int batch, int mode, void* stream);
+
+/*
  * Kernel selection / tile parameters of a plan, set by the autotuner or by
  * hand; no reference counterpart (the reference has one CPU code path).
  * Keys: "fp_kernel" = 0 (per-sample gather through the caches),

@@ -63,6 +63,8 @@ _SIGNATURES = {
     "chip_sumsq": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
     "chip_plan_tune": [c_void_p, ctypes.c_char_p, ctypes.c_int],
     "chip_sumabs": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+    "chip_fp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_bp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
 }
 _RESTYPES = {"chip_last_error": ctypes.c_char_p, "chip_symbol_name": ctypes.c_char_p}
 

@@ -103,6 +103,18 @@ class HipPlan:
                      int(batch), _native.stream())
         return maps
 
+    def fp_otf(self, vol, mu_in, mu_out, out, batch: int = 1):
+        """fp with the maps built per angle block from mu_in / mu_out (``chip_fp_otf``)."""
+        _native.call("chip_fp_otf", self._h, _native.ptr(vol), _native.ptr(mu_in), _native.ptr(mu_out),
+                     _native.ptr(out), int(batch), _native.stream())
+        return out
+
+    def bp_otf(self, sino, mu_in, mu_out, out, batch: int, mode: int):
+        """bp (symmetric or exact adjoint) with the maps built per angle block (``chip_bp_otf``)."""
+        _native.call("chip_bp_otf", self._h, _native.ptr(sino), _native.ptr(mu_in), _native.ptr(mu_out),
+                     _native.ptr(out), int(batch), int(mode), _native.stream())
+        return out
+
 
 class ProjectorBackend(ABC):
     """Projector backend interface (``_projector_backends.py:102-248``)."""

@@ -85,14 +85,14 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
   }
 }
 
-// maps[a][d][i][j] for angles [a0, a0 + n_a) of the chunk whose prefix sums
-// are in cum: beam index of angle a (chunk-local la): in = la (if has_in),
-// out = n_in + la * D + d (if has_out).
+// maps[a - out_a0][d][i][j] for angles [a0, a0 + n_a) of the chunk whose
+// prefix sums are in cum: beam index of angle a (chunk-local la): in = la
+// (if has_in), out = n_in + la * D + d (if has_out).
 __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restrict__ cum,
                                                           const Affine* __restrict__ rot_in_back,
                                                           const Affine* __restrict__ rot_out_back, int has_in,
                                                           int has_out, int n_in, int a0, int n_det, int n, int pad,
-                                                          int np, float* __restrict__ maps) {
+                                                          int np, float* __restrict__ maps, int out_a0) {
   const int j = blockIdx.x * 16 + (threadIdx.x & 15);
   const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
   const int ad = blockIdx.z;  // chunk-local (la, d)
@@ -113,7 +113,62 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
     affine_apply(rot_out_back[static_cast<size_t>(a) * n_det + d], xf, yf, xs, ys);
     e_out = expf(-warp_sample(cum + static_cast<size_t>(n_in + la * n_det + d) * npp, np, 0, xs, ys));
   }
-  maps[(static_cast<size_t>(a) * n_det + d) * n * n + static_cast<size_t>(i) * n + j] = e_in * e_out;
+  maps[(static_cast<size_t>(a - out_a0) * n_det + d) * n * n + static_cast<size_t>(i) * n + j] = e_in * e_out;
+}
+
+// Grows a plan-owned device buffer to `need` bytes. Work already queued on
+// `st` may still read the old buffer, so the stream is drained first.
+static int ensure_buffer(float*& buf, size_t& bytes, size_t need, hipStream_t st, const char* what) {
+  if (bytes >= need) return CHIP_OK;
+  if (buf) {
+    CHIP_HIP(hipStreamSynchronize(st));
+    CHIP_HIP(hipFree(buf));
+    buf = nullptr;
+    bytes = 0;
+  }
+  hipError_t e = hipMalloc(&buf, need);
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": scratch hipMalloc of " + std::to_string(need) + " bytes -> " +
+              hipGetErrorString(e));
+    return CHIP_ENOMEM;
+  }
+  bytes = need;
+  return CHIP_OK;
+}
+
+// Angles per chunk: prefix sums of `beams` images of Np^2 per angle (plus a
+// D x N^2 maps block per angle when `with_block`) within `budget` bytes, and
+// within the launch-grid limit of 65535 in y / z.
+static int chunk_angles(const Plan& p, int beams, bool with_block, size_t budget) {
+  const size_t npp = static_cast<size_t>(p.np) * p.np;
+  const size_t per_angle = npp * sizeof(float) * beams +
+                           (with_block ? static_cast<size_t>(p.n_det) * p.n * p.n * sizeof(float) : 0);
+  int chunk = static_cast<int>(std::max<size_t>(1, budget / per_angle));
+  chunk = std::min(chunk, p.n_angles);
+  chunk = std::min(chunk, 65535 / std::max(beams, p.n_det));
+  return chunk;
+}
+
+// Maps of angles [a0, a0 + na) of one slice into out[(a - out_a0)][d][n][n],
+// through the prefix-sum scratch p.d_scratch (>= na beams' images).
+static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int a0, int na, float* out, int out_a0,
+                            hipStream_t st) {
+  const int A = p.n_angles, D = p.n_det;
+  const Affine* rin_f = p.d_rot;
+  const Affine* rin_b = p.d_rot + A;
+  const Affine* rout_f = p.d_rot + 2 * A;
+  const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D;
+  const int n_in = mi ? na : 0;
+  const int n_beams = n_in + (mo ? na * D : 0);
+  dim3 g1((p.np + 255) / 256, n_beams);
+  hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0, rout_f + static_cast<size_t>(a0) * D,
+                     n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+  CHIP_LAUNCH_CHECK();
+  dim3 g2((p.n + 15) / 16, (p.n + 15) / 16, na * D);
+  hipLaunchKernelGGL(att_compose_kernel, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0, mo ? 1 : 0,
+                     n_in, a0, D, p.n, p.pad, p.np, out, out_a0);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
 }
 
 }  // namespace chip
@@ -129,48 +184,89 @@ extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* m
   CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
   Plan& p = plan->p;
   hipStream_t st = as_stream(stream);
-  const int A = p.n_angles, D = p.n_det, n = p.n, np = p.np;
-  const size_t npp = static_cast<size_t>(np) * np;
-  const int beams_per_angle = (mu_in ? 1 : 0) + (mu_out ? D : 0);
-  // angles per chunk: bounded scratch (~1 GiB) and grid.y limit
-  const size_t budget = size_t(1) << 30;
-  int chunk = static_cast<int>(std::max<size_t>(1, budget / (npp * sizeof(float) * beams_per_angle)));
-  chunk = std::min(chunk, A);
-  chunk = std::min(chunk, 65535 / std::max(beams_per_angle, D));
-  const size_t need = static_cast<size_t>(chunk) * beams_per_angle * npp * sizeof(float);
-  if (p.scratch_bytes < need) {
-    if (p.d_scratch) {
-      CHIP_HIP(hipStreamSynchronize(st));
-      CHIP_HIP(hipFree(p.d_scratch));
-      p.d_scratch = nullptr;
-      p.scratch_bytes = 0;
-    }
-    hipError_t e = hipMalloc(&p.d_scratch, need);
-    if (e != hipSuccess) {
-      set_error(std::string("chip_att_maps: scratch hipMalloc -> ") + hipGetErrorString(e));
-      return CHIP_ENOMEM;
-    }
-    p.scratch_bytes = need;
-  }
-  const Affine* rin_f = p.d_rot;
-  const Affine* rin_b = p.d_rot + A;
-  const Affine* rout_f = p.d_rot + 2 * A;
-  const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D;
+  const int A = p.n_angles, D = p.n_det, n = p.n;
+  const int beams = (mu_in ? 1 : 0) + (mu_out ? D : 0);
+  const int chunk = chunk_angles(p, beams, false, size_t(1) << 30);
+  const size_t need = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
+  int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need, st, "chip_att_maps");
+  if (rc != CHIP_OK) return rc;
   for (int b = 0; b < batch; ++b) {
     const float* mi = mu_in ? mu_in + static_cast<size_t>(b) * n * n : nullptr;
     const float* mo = mu_out ? mu_out + static_cast<size_t>(b) * n * n : nullptr;
     float* mb = maps + static_cast<size_t>(b) * A * D * n * n;
     for (int a0 = 0; a0 < A; a0 += chunk) {
-      const int na = std::min(chunk, A - a0);
-      const int n_in = mi ? na : 0;
-      const int n_beams = n_in + (mo ? na * D : 0);
-      dim3 g1((np + 255) / 256, n_beams);
-      hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
-                         rout_f + static_cast<size_t>(a0) * D, n_in, n_beams, n, p.pad, np, p.d_scratch);
+      rc = build_maps_chunk(p, mi, mo, a0, std::min(chunk, A - a0), mb, 0, st);
+      if (rc != CHIP_OK) return rc;
+    }
+  }
+  return CHIP_OK;
+}
+
+// Shared set-up of the on-the-fly projectors: chunk size and both scratch buffers.
+static int otf_prepare(Plan& p, const float* mu_in, const float* mu_out, hipStream_t st, int& chunk) {
+  const int beams = (mu_in ? 1 : 0) + (mu_out ? p.n_det : 0);
+  chunk = chunk_angles(p, beams, true, p.otf_budget);
+  const size_t need_c = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
+  const size_t need_m = static_cast<size_t>(chunk) * p.n_det * p.n * p.n * sizeof(float);
+  int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need_c, st, "otf prefix sums");
+  if (rc != CHIP_OK) return rc;
+  return ensure_buffer(p.d_mblock, p.mblock_bytes, need_m, st, "otf maps block");
+}
+
+extern "C" int chip_fp_otf(chip_plan* plan, const float* vol, const float* mu_in, const float* mu_out, float* sino,
+                           int batch, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr, "plan is NULL");
+  CHIP_CHECK_ARG(vol != nullptr && sino != nullptr, "vol/sino is NULL");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
+  Plan& p = plan->p;
+  hipStream_t st = as_stream(stream);
+  int chunk = 0;
+  int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + b * nn : nullptr;
+    const float* mo = mu_out ? mu_out + b * nn : nullptr;
+    const float* v = vol + b * nn;
+    float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
+    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
+      const int na = std::min(chunk, p.n_angles - a0);
+      rc = build_maps_chunk(p, mi, mo, a0, na, p.d_mblock, a0, st);
+      if (rc != CHIP_OK) return rc;
+      launch_fp_range(p, v, p.d_mblock, s, a0, na, a0, st);
       CHIP_LAUNCH_CHECK();
-      dim3 g2((n + 15) / 16, (n + 15) / 16, na * D);
-      hipLaunchKernelGGL(att_compose_kernel, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
-                         mo ? 1 : 0, n_in, a0, D, n, p.pad, np, mb);
+    }
+  }
+  return CHIP_OK;
+}
+
+extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const float* mu_out, float* vol,
+                           int batch, int mode, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr, "plan is NULL");
+  CHIP_CHECK_ARG(vol != nullptr && sino != nullptr, "vol/sino is NULL");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
+  CHIP_CHECK_ARG(mode == CHIP_BP_SKIMAGE_SYM || mode == CHIP_BP_ADJOINT,
+                 "mode must be CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT (CHIP_BP_SKIMAGE uses no maps: call chip_bp)");
+  Plan& p = plan->p;
+  hipStream_t st = as_stream(stream);
+  int chunk = 0;
+  int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + b * nn : nullptr;
+    const float* mo = mu_out ? mu_out + b * nn : nullptr;
+    const float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
+    float* v = vol + b * nn;
+    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
+      const int na = std::min(chunk, p.n_angles - a0);
+      rc = build_maps_chunk(p, mi, mo, a0, na, p.d_mblock, a0, st);
+      if (rc != CHIP_OK) return rc;
+      launch_bp_range(p, s, p.d_mblock, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st);
       CHIP_LAUNCH_CHECK();
     }
   }

@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
                                                               int64_t maps_slice_stride, float* __restrict__ vol,
                                                               const float4* __restrict__ geom,
                                                               const float* __restrict__ w, int n, int n_angles,
-                                                              int n_det_total, int det0, int accumulate) {
+                                                              int n_det_total, int det0, int angle0, int n_ang,
+                                                              int maps_a0, int accumulate) {
   __shared__ float win[kAngBlk][D][kWin];
   __shared__ int wlo_s[kAngBlk];
   const int tid = threadIdx.x;
@@ -50,8 +51,8 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
   const float ti0 = static_cast<float>(blockIdx.y * kTileH), ti1 = ti0 + (kTileH - 1);
 
   float out = 0.f;
-  for (int a0 = 0; a0 < n_angles; a0 += kAngBlk) {
-    const int na = min(kAngBlk, n_angles - a0);
+  for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kAngBlk) {
+    const int na = min(kAngBlk, angle0 + n_ang - a0);
     __syncthreads();
     if (tid < na) {
       const float4 g = geom[a0 + tid];
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
         for (int d = 0; d < D; ++d) {
           const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
           float f = w[static_cast<size_t>(a) * n_det_total + det0 + d];
-          if (MAPS) f *= mb[(static_cast<size_t>(a) * n_det_total + det0 + d) * nn + pix];
+          if (MAPS) f *= mb[(static_cast<size_t>(a - maps_a0) * n_det_total + det0 + d) * nn + pix];
           out = fmaf(f, gd, out);
         }
       }
@@ -131,26 +132,26 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
 
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch, int det0,
-            int accumulate, hipStream_t st) {
+            int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
   dim3 grid((p.n + kTileW - 1) / kTileW, (p.n + kTileH - 1) / kTileH, batch);
   hipLaunchKernelGGL((bp_adjoint_tile_kernel<D, MAPS>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, accumulate);
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate);
 }
 
 }  // namespace
 
 void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
-                             int batch, hipStream_t st) {
-  // detectors in passes of two; the first pass overwrites, later passes add
+                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+  // detectors in passes of two; later passes add to the first
   for (int det0 = 0; det0 < p.n_det; det0 += 2) {
     const bool two = p.n_det - det0 >= 2;
-    const int acc = det0 > 0 ? 1 : 0;
+    const int acc = (det0 > 0 || accumulate) ? 1 : 0;
     if (maps) {
-      if (two) launch<2, true>(p, sino, maps, stride, vol, batch, det0, acc, st);
-      else launch<1, true>(p, sino, maps, stride, vol, batch, det0, acc, st);
+      if (two) launch<2, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
+      else launch<1, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
     } else {
-      if (two) launch<2, false>(p, sino, nullptr, stride, vol, batch, det0, acc, st);
-      else launch<1, false>(p, sino, nullptr, stride, vol, batch, det0, acc, st);
+      if (two) launch<2, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st);
+      else launch<1, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st);
     }
   }
 }

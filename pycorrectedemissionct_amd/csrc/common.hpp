@@ -64,6 +64,11 @@ struct Plan {
   // scratch for the attenuation builder (rotated-frame cumulative sums)
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  // maps of one angle block for the on-the-fly projectors
+  float* d_mblock = nullptr;
+  size_t mblock_bytes = 0;
+  // bound on d_scratch + d_mblock (chip_plan_tune "otf_budget_mib")
+  size_t otf_budget = size_t(1) << 30;
   int device = 0;
   // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged,
   // 2 = LDS-staged with register prefetch, 3 = prefetched and lean
@@ -73,14 +78,23 @@ struct Plan {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-// whole-sinogram fp over detectors [det0, det0 + nd), nd in {1, 2}; needs
-// N % 4 == 0 and 16-byte aligned vol / maps (fp_stream.hip)
+// fp over angles [angle0, angle0 + n_ang) and detectors [det0, det0 + nd),
+// nd in {1, 2}; maps[0] holds angle maps_a0. Needs N % 4 == 0 and 16-byte
+// aligned vol / maps (fp_stream.hip)
 void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
-                      int det0, int nd, hipStream_t st);
+                      int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st);
 
-// whole-sinogram exact adjoint of the fp, all detectors (bp_adjoint.hip)
+// exact adjoint of the fp over angles [angle0, angle0 + n_ang), all detectors;
+// maps[0] holds angle maps_a0; accumulate = 1 adds to vol (bp_adjoint.hip)
 void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
-                             int batch, hipStream_t st);
+                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st);
+
+// fp / bp (mode CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT) over an angle range
+// with a maps block whose first angle is maps_a0 (projector.hip)
+void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
+                     int maps_a0, hipStream_t st);
+void launch_bp_range(const Plan& p, const float* sino, const float* maps, float* vol, int mode, int angle0,
+                     int n_ang, int maps_a0, int accumulate, hipStream_t st);
 
 constexpr int kMaxDetPerPass = 4;
 

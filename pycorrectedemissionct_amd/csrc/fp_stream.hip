@@ -89,11 +89,13 @@ template <int D, bool MAPS>
 __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
                                                         int64_t maps_slice_stride, float* __restrict__ sino,
                                                         const float4* __restrict__ geom, const float* __restrict__ w,
-                                                        int n, int n_angles, int n_det_total, int det0, int n_ublk) {
+                                                        int n, int n_angles, int n_det_total, int det0, int n_ublk,
+                                                        int angle0, int maps_a0) {
   extern __shared__ float lds[];  // [D][kTile]
   const int lb = logical_block(blockIdx.x, gridDim.x);
-  const int a = lb / n_ublk;
-  const int u0 = (lb - a * n_ublk) * kBins;
+  const int la = lb / n_ublk;
+  const int a = angle0 + la;
+  const int u0 = (lb - la * n_ublk) * kBins;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
   const size_t nn = static_cast<size_t>(n) * n;
   const float* v = vol + static_cast<size_t>(b) * nn;
   const float* mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride +
-                               (static_cast<size_t>(a) * n_det_total + det0) * nn
+                               (static_cast<size_t>(a - maps_a0) * n_det_total + det0) * nn
                          : nullptr;
   const float fn = static_cast<float>(n);
   const float uf = static_cast<float>(u);
@@ -269,25 +271,25 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
 
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch, int det0,
-            hipStream_t st) {
+            int angle0, int n_ang, int maps_a0, hipStream_t st) {
   const int n_ublk = (p.n + kBins - 1) / kBins;
-  dim3 grid(n_ublk * p.n_angles, batch);
+  dim3 grid(n_ublk * n_ang, batch);
   const size_t shmem = sizeof(float) * D * kTile;
   hipLaunchKernelGGL((fp_stream_kernel<D, MAPS>), grid, dim3(256), shmem, st, vol, maps, stride, sino, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk);
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
 }
 
 }  // namespace
 
 void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
-                      int det0, int nd, hipStream_t st) {
+                      int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st) {
   if (maps) {
-    if (nd == 2) launch<2, true>(p, vol, maps, stride, sino, batch, det0, st);
-    else launch<1, true>(p, vol, maps, stride, sino, batch, det0, st);
+    if (nd == 2) launch<2, true>(p, vol, maps, stride, sino, batch, det0, angle0, n_ang, maps_a0, st);
+    else launch<1, true>(p, vol, maps, stride, sino, batch, det0, angle0, n_ang, maps_a0, st);
   } else {
     // without maps every detector sees the same slice
-    if (nd == 2) launch<2, false>(p, vol, nullptr, stride, sino, batch, det0, st);
-    else launch<1, false>(p, vol, nullptr, stride, sino, batch, det0, st);
+    if (nd == 2) launch<2, false>(p, vol, nullptr, stride, sino, batch, det0, angle0, n_ang, 0, st);
+    else launch<1, false>(p, vol, nullptr, stride, sino, batch, det0, angle0, n_ang, 0, st);
   }
 }
 

@@ -64,6 +64,7 @@ static const char* kSymbols[] = {
     "chip_precond_fix",   "chip_pdhg_dual_l2", "chip_pdhg_dual_kl", "chip_tv_dual",
     "chip_pdhg_primal",   "chip_sirt_residual", "chip_sirt_update", "chip_mlem_ratio",
     "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",       "chip_plan_tune",
+    "chip_fp_otf",        "chip_bp_otf",
 };
 
 extern "C" {
@@ -186,6 +187,7 @@ int chip_plan_destroy(chip_plan* plan) {
   if (p.d_ytab) (void)hipFree(p.d_ytab);
   if (p.d_rot) (void)hipFree(p.d_rot);
   if (p.d_scratch) (void)hipFree(p.d_scratch);
+  if (p.d_mblock) (void)hipFree(p.d_mblock);
   delete plan;
   return CHIP_OK;
 }

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void fp_kernel(const float* __restrict__ vol, 
                                                  int64_t maps_slice_stride, float* __restrict__ sino,
                                                  const float4* __restrict__ geom, const float* __restrict__ w,
                                                  int n, int n_angles, int n_det_total, int det0, int angle0,
-                                                 int line_mode) {
+                                                 int line_mode, int maps_a0) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void fp_kernel(const float* __restrict__ vol, 
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     mp[d] = maps ? maps + static_cast<size_t>(b) * maps_slice_stride +
-                       (static_cast<size_t>(a) * n_det_total + det0 + d) * nn
+                       (static_cast<size_t>(a - maps_a0) * n_det_total + det0 + d) * nn
                  : nullptr;
   }
 
@@ -129,10 +129,11 @@ __global__ __launch_bounds__(256) void fp_kernel(const float* __restrict__ vol, 
 
 template <int D>
 static void launch_fp_group(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino,
-                            int batch, int det0, int angle0, int n_ang, int line_mode, hipStream_t st) {
+                            int batch, int det0, int angle0, int n_ang, int line_mode, hipStream_t st,
+                            int maps_a0 = 0) {
   dim3 grid((p.n + kFpBinsPerBlock - 1) / kFpBinsPerBlock, n_ang, batch);
   hipLaunchKernelGGL(fp_kernel<D>, grid, dim3(256), 0, st, vol, maps, stride, sino, p.d_fp_geom, p.d_w, p.n,
-                     p.n_angles, p.n_det, det0, angle0, line_mode);
+                     p.n_angles, p.n_det, det0, angle0, line_mode, maps_a0);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,7 +544,7 @@ static void launch_fp(const Plan& p, const float* vol, const float* maps, int64_
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const bool two = p.n_det - det0 >= 2;
       if (variant == 3) {
-        launch_fp_stream(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, st);
+        launch_fp_stream(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, 0, p.n_angles, 0, st);
       } else if (variant == 2) {
         if (two) launch_fp_lds2_group<2>(p, vol, maps, stride, sino, batch, det0, st);
         else launch_fp_lds2_group<1>(p, vol, maps, stride, sino, batch, det0, st);
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict
                                                          const float4* __restrict__ geom,
                                                          const float* __restrict__ w, int n, int n_angles,
                                                          int n_det, int angle0, int n_ang, int line_mode,
-                                                         int accumulate) {
+                                                         int accumulate, int maps_a0) {
   const int j = blockIdx.x * kBpTile + (threadIdx.x & (kBpTile - 1));
   const int i = blockIdx.y * kBpTile + (threadIdx.x / kBpTile);
   const int b = blockIdx.z;
@@ -620,7 +621,10 @@ __global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict
       float val = fmaf(fr, s1 - s0, s0);
       if (SYM) {
         float f = w[static_cast<size_t>(a) * n_det + d];
-        if (maps) f *= maps[static_cast<size_t>(b) * maps_slice_stride + (static_cast<size_t>(a) * n_det + d) * nn + pix];
+        if (maps) {
+          f *= maps[static_cast<size_t>(b) * maps_slice_stride + (static_cast<size_t>(a - maps_a0) * n_det + d) * nn +
+                    pix];
+        }
         val *= f;
       }
       acc += val;
@@ -704,16 +708,49 @@ static void launch_bp(const Plan& p, const float* sino, const float* maps, int64
   dim3 grid((p.n + kBpTile - 1) / kBpTile, (p.n + kBpTile - 1) / kBpTile, batch);
   if (mode == CHIP_BP_SKIMAGE) {
     hipLaunchKernelGGL(bp_skimage_kernel<0>, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w,
-                       p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
+                       p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate, 0);
   } else if (mode == CHIP_BP_SKIMAGE_SYM) {
     hipLaunchKernelGGL(bp_skimage_kernel<1>, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w,
-                       p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
+                       p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate, 0);
   } else if (!line_mode && angle0 == 0 && n_ang == p.n_angles && !accumulate) {
-    launch_bp_adjoint_tiles(p, sino, maps, stride, vol, batch, st);
+    launch_bp_adjoint_tiles(p, sino, maps, stride, vol, batch, 0, p.n_angles, 0, 0, st);
   } else {
     hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w, p.n,
                        p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
   }
+}
+
+void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
+                     int maps_a0, hipStream_t st) {
+  const bool vec_ok = (p.n % 4 == 0) && (reinterpret_cast<uintptr_t>(vol) % 16 == 0) &&
+                      (reinterpret_cast<uintptr_t>(maps) % 16 == 0);
+  if (vec_ok && p.fp_variant == 3) {
+    for (int det0 = 0; det0 < p.n_det; det0 += 2) {
+      const int nd = p.n_det - det0 >= 2 ? 2 : 1;
+      launch_fp_stream(p, vol, maps, 0, sino, 1, det0, nd, angle0, n_ang, maps_a0, st);
+    }
+    return;
+  }
+  for (int det0 = 0; det0 < p.n_det; det0 += kMaxDetPerPass) {
+    const int nd = std::min(kMaxDetPerPass, p.n_det - det0);
+    switch (nd) {
+      case 1: launch_fp_group<1>(p, vol, maps, 0, sino, 1, det0, angle0, n_ang, 0, st, maps_a0); break;
+      case 2: launch_fp_group<2>(p, vol, maps, 0, sino, 1, det0, angle0, n_ang, 0, st, maps_a0); break;
+      case 3: launch_fp_group<3>(p, vol, maps, 0, sino, 1, det0, angle0, n_ang, 0, st, maps_a0); break;
+      default: launch_fp_group<4>(p, vol, maps, 0, sino, 1, det0, angle0, n_ang, 0, st, maps_a0); break;
+    }
+  }
+}
+
+void launch_bp_range(const Plan& p, const float* sino, const float* maps, float* vol, int mode, int angle0,
+                     int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+  if (mode == CHIP_BP_ADJOINT) {
+    launch_bp_adjoint_tiles(p, sino, maps, 0, vol, 1, angle0, n_ang, maps_a0, accumulate, st);
+    return;
+  }
+  dim3 grid((p.n + kBpTile - 1) / kBpTile, (p.n + kBpTile - 1) / kBpTile, 1);
+  hipLaunchKernelGGL(bp_skimage_kernel<1>, grid, dim3(256), 0, st, sino, maps, 0, vol, p.d_fp_geom, p.d_w, p.n,
+                     p.n_angles, p.n_det, angle0, n_ang, 0, accumulate, maps_a0);
 }
 
 }  // namespace chip
@@ -769,6 +806,11 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   if (k == "fp_kernel") {
     CHIP_CHECK_ARG(value >= 0 && value <= 3, "fp_kernel must be in 0..3");
     plan->p.fp_variant = value;
+    return CHIP_OK;
+  }
+  if (k == "otf_budget_mib") {
+    CHIP_CHECK_ARG(value >= 16, "otf_budget_mib must be >= 16");
+    plan->p.otf_budget = static_cast<size_t>(value) << 20;
     return CHIP_OK;
   }
   set_error("chip_plan_tune: unknown key '" + k + "'");

@@ -161,7 +161,7 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
                  backend="hip", att_maps=None, att_in=None, att_out=None, angles_detectors_rad=(np.pi / 2),
                  weights_detectors=None, psf=None, is_symmetric: bool = False, weights_angles=None,
                  use_multithreading: bool = True, data_type=np.float32, verbose: bool = True,
-                 bp_mode: str = "skimage"):
+                 bp_mode: str = "skimage", maps_mode: str = "precomputed"):
         ProjectorUncorrected.__init__(
             self, vol_geom=vol_geom, angles_rot_rad=angles_rot_rad, rot_axis_shift_pix=rot_axis_shift_pix,
             prj_geom=prj_geom, psf=psf, prj_intensities=prj_intensities, backend=backend, create_single_projs=True,
@@ -170,7 +170,10 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             raise NotImplementedError("the attenuation-corrected HIP projector handles 2D slices")
         if bp_mode not in ("skimage", "adjoint"):
             raise ValueError(f"Unknown bp_mode: {bp_mode}")
+        if maps_mode not in ("precomputed", "otf"):
+            raise ValueError(f"Unknown maps_mode: {maps_mode}")
         self.bp_mode = bp_mode
+        self.maps_mode = maps_mode
         self.data_type = data_type
         self.use_multithreading = use_multithreading
         self.verbose = verbose
@@ -196,7 +199,21 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             weights_angles = np.array(weights_angles)
         self.weights_angles = weights_angles
 
-        if att_maps is None:
+        self.mu_in_dev = self.mu_out_dev = None
+        if maps_mode == "otf":
+            # maps are rebuilt per angle block inside every fp / bp (chip_fp_otf, chip_bp_otf)
+            if att_maps is not None:
+                raise ValueError("maps_mode='otf' builds the maps from att_in / att_out: do not pass att_maps")
+            if att_in is None and att_out is None:
+                raise ValueError("No attenuation volumes were given.")
+            n = int(self.vol_shape[-1])
+            for mu in (att_in, att_out):
+                if mu is not None and tuple(np.shape(mu)) != (n, n):
+                    raise ValueError(f"Mismatching attenuation volume shape {np.shape(mu)} with vol_shape {self.vol_shape}")
+            self.mu_in_dev = None if att_in is None else _arrays.to_dev(att_in)[0].contiguous()
+            self.mu_out_dev = None if att_out is None else _arrays.to_dev(att_out)[0].contiguous()
+            self.att_vol_angles = None
+        elif att_maps is None:
             m = AttenuationVolume(att_in, att_out, self.angles_rot_rad, self.angles_det_rad)
             m.compute_maps(use_multithreading=self.use_multithreading, verbose=self.verbose)
             self.att_vol_angles = m.maps_device
@@ -228,8 +245,13 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             m = m.to(device)
         return m.contiguous()
 
+    def _require_maps(self, what: str) -> None:
+        if self.maps_mode == "otf":
+            raise NotImplementedError(f"{what} needs precomputed maps (maps_mode='precomputed')")
+
     def collapse_detectors(self) -> None:
         """Convert multi-detector configurations into single-detector (``projectors.py:492-500``)."""
+        self._require_maps("collapse_detectors")
         weights = np.reshape(self.weights_det, [1, -1, 1, 1]) / np.sum(self.weights_det)
         w_dev = torch.as_tensor(weights, dtype=torch.float32, device=self.att_vol_angles.device)
         self.att_vol_angles = torch.sum(self.att_vol_angles * w_dev, dim=1, keepdim=True)
@@ -239,6 +261,7 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         self.weights_det = np.sum(self.weights_det, keepdims=True)
 
     def fp_angle(self, vol, angle_ind: int):
+        self._require_maps("fp_angle")
         t, host = _arrays.to_dev(vol)
         n = int(self.vol_shape[-1])
         d = len(self.angles_det_rad)
@@ -249,6 +272,7 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         return _arrays.from_dev(line, host)
 
     def bp_angle(self, sino, angle_ind: int, single_line: bool = False):
+        self._require_maps("bp_angle")
         t, host = _arrays.to_dev(sino)
         n = int(self.vol_shape[-1])
         d = len(self.weights_det)
@@ -270,7 +294,10 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         if t.numel() != n * n:
             raise ValueError(f"Expected a volume of shape {tuple(self.vol_shape)}, got {tuple(t.shape)}")
         out = torch.empty((d, a, n), dtype=torch.float32, device=t.device)
-        self._plan().fp(t, self._maps(t.device), out, 1)
+        if self.maps_mode == "otf":
+            self._plan().fp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, 1)
+        else:
+            self._plan().fp(t, self._maps(t.device), out, 1)
         if d == 1:
             out = out[0]
         return _arrays.from_dev(out, host)
@@ -283,10 +310,11 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         if t.numel() != d * a * n:
             raise ValueError(f"Expected a sinogram with {d * a * n} elements, got {tuple(t.shape)}")
         out = torch.empty((n, n), dtype=torch.float32, device=t.device)
-        if self.bp_mode == "adjoint":
-            self._plan().bp(t, self._maps(t.device), out, 1, _native.CHIP_BP_ADJOINT)
-        elif self.is_symmetric:
-            self._plan().bp(t, self._maps(t.device), out, 1, _native.CHIP_BP_SKIMAGE_SYM)
+        mode = _native.CHIP_BP_ADJOINT if self.bp_mode == "adjoint" else _native.CHIP_BP_SKIMAGE_SYM
+        if self.maps_mode == "otf" and (self.bp_mode == "adjoint" or self.is_symmetric):
+            self._plan().bp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, 1, mode)
+        elif self.bp_mode == "adjoint" or self.is_symmetric:
+            self._plan().bp(t, self._maps(t.device), out, 1, mode)
         else:
             if self.prj_intensities is not None:
                 t = (t.reshape(d, a, n) * self._intensities_dev(t.device)[None, :, None]).contiguous()
