@@ -26,6 +26,7 @@ extern "C" {
 #endif
 
 typedef struct chip_plan chip_plan;
+typedef struct chip_batch_plan chip_batch_plan;
 
 enum chip_status {
   CHIP_OK = 0,
@@ -205,6 +206,41 @@ int chip_mlem_update(float* x, const float* upd, const float* tau, int has_lower
  * data_terms.py:298-299) or sum |r| (KL/l1, data_terms.py:741-742). */
 int chip_sumsq(const float* v, int64_t n, double* out, void* stream);
 int chip_sumabs(const float* v, int64_t n, double* out, void* stream);
+
+/* ---- batches of slices of different sizes ----------------------------------
+ * corrct reconstructs a stack slice by slice (ProjectorAttenuationXRF over a
+ * 3-D volume, _projector_backends.py:86-92 for the maps); a batch plan runs
+ * many independent slices, each of its own size N_s (N_s % 4 == 0), as one
+ * persistent launch per pass. Buffers hold the slices back to back in plan
+ * order: volumes [N_s][N_s], sinograms [D][A][N_s], maps [A][D][N_s][N_s],
+ * TV duals [2][N_s][N_s] (offsets = running sums; chip_batch_info gives the
+ * totals). Angles, detectors and weights are shared by all slices, with the
+ * meaning of chip_plan_create.
+ */
+int chip_batch_plan_create(chip_batch_plan** out, int n_slices, const int* sizes, int n_angles,
+                           const float* angles_rad, int n_det, const double* det_angles_rad, const double* w_det,
+                           const double* w_ang);
+int chip_batch_plan_destroy(chip_batch_plan* plan);
+/* Total elements of the packed volume, sinogram and maps buffers (NULL outputs are skipped). */
+int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* sino_elems, int64_t* maps_elems);
+/* Persistent-grid sizes: "fp_wg_per_cu" (1..8, default 2) and "bp_wg_per_cu"
+ * (1..16, default 4) workgroups per CU; set by the autotuner. */
+int chip_batch_tune(chip_batch_plan* plan, const char* key, int value);
+/* Maps of every slice (chip_att_maps per slice; mu_* packed like volumes). */
+int chip_batch_att_maps(chip_batch_plan* plan, const float* mu_in, const float* mu_out, float* maps, void* stream);
+/* fp / bp of every slice (chip_fp / chip_bp semantics per slice); maps may be NULL. */
+int chip_batch_fp(const chip_batch_plan* plan, const float* vol, const float* maps, float* sino, void* stream);
+int chip_batch_bp(const chip_batch_plan* plan, const float* sino, const float* maps, float* vol, int mode,
+                  void* stream);
+/* Spatial solver updates per slice (chip_tv_dual, chip_pdhg_primal,
+ * chip_sirt_update semantics); x, tau, masks packed like volumes. */
+int chip_batch_tv_dual(const chip_batch_plan* plan, float* q, const float* x, float lam, void* stream);
+int chip_batch_pdhg_primal(const chip_batch_plan* plan, float* x, float* xbar, const float* atp, const float* q,
+                           const float* tau, int has_lower, float lower, int has_upper, float upper,
+                           const float* x_mask, void* stream);
+int chip_batch_sirt_update(const chip_batch_plan* plan, float* x, const float* upd, const float* q, const float* tau,
+                           int has_lower, float lower, int has_upper, float upper, const float* x_mask,
+                           void* stream);
 
 #ifdef __cplusplus
 }

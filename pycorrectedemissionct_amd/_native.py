@@ -65,6 +65,25 @@ _SIGNATURES = {
     "chip_sumabs": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
     "chip_fp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
+    "chip_batch_plan_create": [
+        ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p,
+        c_void_p,
+    ],
+    "chip_batch_plan_destroy": [c_void_p],
+    "chip_batch_info": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "chip_batch_tune": [c_void_p, ctypes.c_char_p, ctypes.c_int],
+    "chip_batch_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "chip_batch_fp": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "chip_batch_bp": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_batch_tv_dual": [c_void_p, c_void_p, c_void_p, ctypes.c_float, c_void_p],
+    "chip_batch_pdhg_primal": [
+        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+        ctypes.c_float, c_void_p, c_void_p,
+    ],
+    "chip_batch_sirt_update": [
+        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+        c_void_p, c_void_p,
+    ],
 }
 _RESTYPES = {"chip_last_error": ctypes.c_char_p, "chip_symbol_name": ctypes.c_char_p}
 

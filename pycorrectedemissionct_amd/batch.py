@@ -1,0 +1,369 @@
+"""Many independent XRF slices of different sizes as one operator.
+
+corrct reconstructs a stack one slice at a time (``ProjectorAttenuationXRF``
+over a 3-D volume; ``_projector_backends.py:86-92`` loops the slices for the
+maps). Here every slice of a batch -- its own size, phantom and attenuation --
+lives back to back in packed device buffers, and each fp / bp is one
+persistent launch over all slices (``chip_batch_fp`` / ``chip_batch_bp``).
+
+* ``BatchPlan``            -- the ``chip_batch_plan`` handle.
+* ``BatchedProjectorXRF``  -- maps, fp, bp and pack / unpack of per-slice arrays.
+* ``BatchedSIRT``, ``BatchedPDHG`` -- the SIRT (``solvers.py:867-984``) and
+  PDHG-TV (``solvers.py:1060-1206``) updates of every slice at once, with the
+  iterations captured in a HIP graph and replayed; results equal per-slice
+  ``SIRT`` / ``PDHG`` runs.
+* ``autotune``             -- times the persistent-grid sizes of the fp and bp
+  on the actual batch and keeps the fastest.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _arrays, _native
+
+_BP_MODES = {"skimage": _native.CHIP_BP_SKIMAGE, "symmetric": _native.CHIP_BP_SKIMAGE_SYM,
+             "adjoint": _native.CHIP_BP_ADJOINT}
+
+
+class BatchPlan:
+    """A ``chip_batch_plan`` over slices of sides ``sizes`` (each a multiple of 4)."""
+
+    def __init__(self, sizes, angles_rad_f32, det_angles_rad=(np.pi / 2,), w_det=None, w_ang=None):
+        self.sizes = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
+        self.angles = np.ascontiguousarray(np.asarray(angles_rad_f32, dtype=np.float32).reshape(-1))
+        self.det_angles = np.ascontiguousarray(np.asarray(det_angles_rad, dtype=np.float64).reshape(-1))
+        self.n_slices, self.n_angles, self.n_det = len(self.sizes), len(self.angles), len(self.det_angles)
+        self.w_det = None if w_det is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(w_det, dtype=np.float64).reshape(-1), (self.n_det,)))
+        self.w_ang = None if w_ang is None else np.ascontiguousarray(
+            np.asarray(w_ang, dtype=np.float64).reshape(self.n_angles, self.n_det))
+        _arrays.device()
+        h = ctypes.c_void_p()
+        _native.call("chip_batch_plan_create", ctypes.byref(h), self.n_slices, self.sizes.ctypes.data, self.n_angles,
+                     self.angles.ctypes.data, self.n_det, self.det_angles.ctypes.data,
+                     None if self.w_det is None else self.w_det.ctypes.data,
+                     None if self.w_ang is None else self.w_ang.ctypes.data)
+        self._h = h
+        tot = (ctypes.c_int64 * 3)()
+        _native.call("chip_batch_info", h, ctypes.addressof(tot), ctypes.addressof(tot) + 8,
+                     ctypes.addressof(tot) + 16)
+        self.vol_elems, self.sino_elems, self.maps_elems = int(tot[0]), int(tot[1]), int(tot[2])
+        n2 = self.sizes.astype(np.int64) ** 2
+        self.vol_start = np.concatenate([[0], np.cumsum(n2)])
+        self.sino_start = np.concatenate([[0], np.cumsum(self.n_det * self.n_angles * self.sizes.astype(np.int64))])
+        self.maps_start = np.concatenate([[0], np.cumsum(self.n_angles * self.n_det * n2)])
+        self.tuning: dict[str, int] = {}
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h is not None and h.value:
+            try:
+                _native.lib().chip_batch_plan_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, key: str, value: int) -> None:
+        _native.call("chip_batch_tune", self._h, key.encode(), int(value))
+        self.tuning[key] = int(value)
+
+    def att_maps(self, mu_in, mu_out, maps):
+        _native.call("chip_batch_att_maps", self._h, _native.ptr(mu_in), _native.ptr(mu_out), _native.ptr(maps),
+                     _native.stream())
+        return maps
+
+    def fp(self, vol, maps, sino):
+        _native.call("chip_batch_fp", self._h, _native.ptr(vol), _native.ptr(maps), _native.ptr(sino),
+                     _native.stream())
+        return sino
+
+    def bp(self, sino, maps, vol, mode: int):
+        _native.call("chip_batch_bp", self._h, _native.ptr(sino), _native.ptr(maps), _native.ptr(vol), int(mode),
+                     _native.stream())
+        return vol
+
+
+class BatchedProjectorXRF:
+    """Attenuation-corrected XRF projector over a batch of slices of sizes ``sizes``.
+
+    ``att_in`` / ``att_out`` are per-slice lists of [N_s, N_s] arrays (or
+    None); ``att_maps`` may instead give the packed maps directly. Slices share
+    the angles, detectors and weights. ``bp_mode``: "skimage" (corrct's
+    default, maps and weights ignored), "symmetric" (``is_symmetric=True``) or
+    "adjoint" (exact transpose of the fp).
+    """
+
+    def __init__(self, sizes, angles_rot_rad, att_in=None, att_out=None, *, att_maps=None,
+                 angles_detectors_rad=(np.pi / 2,), weights_detectors=None, weights_angles=None,
+                 bp_mode: str = "adjoint"):
+        if bp_mode not in _BP_MODES:
+            raise ValueError(f"Unknown bp_mode: {bp_mode}")
+        self.bp_mode = bp_mode
+        self.plan = BatchPlan(sizes, np.asarray(angles_rot_rad, dtype=np.float32), angles_detectors_rad,
+                              weights_detectors, weights_angles)
+        self.sizes = [int(s) for s in self.plan.sizes]
+        self.n_angles, self.n_det = self.plan.n_angles, self.plan.n_det
+        self.device = _arrays.device()
+        if att_maps is not None:
+            self.maps = _arrays.to_dev(att_maps)[0].reshape(-1).contiguous()
+            if self.maps.numel() != self.plan.maps_elems:
+                raise ValueError(f"att_maps holds {self.maps.numel()} elements, the batch needs {self.plan.maps_elems}")
+        elif att_in is not None or att_out is not None:
+            mu_in = None if att_in is None else self.pack_vol(att_in)
+            mu_out = None if att_out is None else self.pack_vol(att_out)
+            self.maps = torch.empty(self.plan.maps_elems, dtype=torch.float32, device=self.device)
+            self.plan.att_maps(mu_in, mu_out, self.maps)
+        else:
+            self.maps = None
+
+    # ---- packing -----------------------------------------------------------------------
+    def _pack(self, arrays, start, shape_of) -> torch.Tensor:
+        out = torch.empty(int(start[-1]), dtype=torch.float32, device=self.device)
+        for s, a in enumerate(arrays):
+            t = _arrays.to_dev(a)[0].reshape(-1)
+            if t.numel() != int(start[s + 1] - start[s]):
+                raise ValueError(f"slice {s}: expected shape {shape_of(s)}, got {tuple(np.shape(a))}")
+            out[int(start[s]):int(start[s + 1])] = t
+        return out
+
+    def pack_vol(self, arrays) -> torch.Tensor:
+        return self._pack(arrays, self.plan.vol_start, lambda s: (self.sizes[s],) * 2)
+
+    def pack_sino(self, arrays) -> torch.Tensor:
+        return self._pack(arrays, self.plan.sino_start, lambda s: (self.n_det, self.n_angles, self.sizes[s]))
+
+    def unpack_vol(self, flat: torch.Tensor) -> list[torch.Tensor]:
+        st = self.plan.vol_start
+        return [flat[int(st[s]):int(st[s + 1])].view(n, n) for s, n in enumerate(self.sizes)]
+
+    def unpack_sino(self, flat: torch.Tensor) -> list[torch.Tensor]:
+        st = self.plan.sino_start
+        return [flat[int(st[s]):int(st[s + 1])].view(self.n_det, self.n_angles, n) for s, n in enumerate(self.sizes)]
+
+    def slice_maps(self, s: int) -> torch.Tensor:
+        st = self.plan.maps_start
+        n = self.sizes[s]
+        return self.maps[int(st[s]):int(st[s + 1])].view(self.n_angles, self.n_det, n, n)
+
+    # ---- operator ----------------------------------------------------------------------
+    @property
+    def vol_elems(self) -> int:
+        return self.plan.vol_elems
+
+    @property
+    def sino_elems(self) -> int:
+        return self.plan.sino_elems
+
+    def fp(self, vol: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(self.plan.sino_elems, dtype=torch.float32, device=self.device)
+        return self.plan.fp(vol, self.maps, out)
+
+    def bp(self, sino: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(self.plan.vol_elems, dtype=torch.float32, device=self.device)
+        maps = None if self.bp_mode == "skimage" else self.maps
+        return self.plan.bp(sino, maps, out, _BP_MODES[self.bp_mode])
+
+
+def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(1, 2, 3), bp_candidates=(2, 4, 8)) -> dict:
+    """Pick the persistent-grid sizes of the batched fp and bp by timing them on ``A``'s batch.
+
+    Returns ``{"fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``; the
+    chosen values are applied to ``A.plan``. Results do not depend on the grid size (every
+    work item is computed the same way), only the time does.
+    """
+    x = torch.rand(A.vol_elems, dtype=torch.float32, device=A.device)
+    y = torch.empty(A.sino_elems, dtype=torch.float32, device=A.device)
+    v = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn) -> float:
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / reps
+
+    fp_us, bp_us = {}, {}
+    for k in fp_candidates:
+        A.plan.tune("fp_wg_per_cu", k)
+        fp_us[k] = timed(lambda: A.fp(x, y))
+    for k in bp_candidates:
+        A.plan.tune("bp_wg_per_cu", k)
+        bp_us[k] = timed(lambda: A.bp(y, v))
+    best_fp = min(fp_us, key=fp_us.get)
+    best_bp = min(bp_us, key=bp_us.get)
+    A.plan.tune("fp_wg_per_cu", best_fp)
+    A.plan.tune("bp_wg_per_cu", best_bp)
+    return {"fp_wg_per_cu": best_fp, "bp_wg_per_cu": best_bp, "fp_us": fp_us, "bp_us": bp_us}
+
+
+def _precondition(A: BatchedProjectorXRF, vec: torch.Tensor, start, add: float, rho: float) -> torch.Tensor:
+    """Per slice: vec = rho / fix(|vec| + add), fix against the slice's own maximum (``solvers.py:98-116``)."""
+    st = _native.stream()
+    mx = torch.empty(len(start) - 1, dtype=torch.float32, device=vec.device)
+    for s in range(len(start) - 1):
+        a, b = int(start[s]), int(start[s + 1])
+        seg = vec[a:b]
+        _native.call("chip_absmax", seg.data_ptr(), b - a, mx[s:s + 1].data_ptr(), st)
+        _native.call("chip_precond_fix", seg.data_ptr(), b - a, float(add), mx[s:s + 1].data_ptr(), float(rho), st)
+    return vec
+
+
+def _limit(v):
+    return (0, 0.0) if v is None else (1, float(v))
+
+
+class _GraphedLoop:
+    """Runs ``step`` ``iterations`` times, replaying a HIP graph of ``chunk`` captured steps.
+
+    The captured launches read and write only the solver's persistent buffers, so each
+    replay continues where the previous one stopped. The first call runs one step eagerly
+    (every kernel is loaded before the capture), captures ``chunk`` steps (a capture does
+    not execute them), replays; the remainder runs eagerly.
+    """
+
+    def __init__(self, step, chunk: int, use_graph: bool):
+        self.step, self.chunk, self.use_graph = step, int(chunk), bool(use_graph)
+        self.graph = None
+
+    def run(self, iterations: int) -> None:
+        done = 0
+        if self.use_graph and self.graph is None and iterations > self.chunk:
+            self.step()
+            done = 1
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=side):
+                for _ in range(self.chunk):
+                    self.step()
+            torch.cuda.current_stream().wait_stream(side)
+            self.graph = g
+        left = iterations - done
+        if self.graph is not None:
+            full, left = divmod(left, self.chunk)
+            for _ in range(full):
+                self.graph.replay()
+        for _ in range(left):
+            self.step()
+
+
+class BatchedSIRT:
+    """SIRT (``solvers.py:867-984``) on every slice of a batch: x += tau A^T(sigma (b - A x))."""
+
+    def __init__(self, relaxation: float = 1.95, graph_chunk: int = 10, use_graph: bool = True):
+        self.relaxation = relaxation
+        self.graph_chunk = graph_chunk
+        self.use_graph = use_graph
+
+    def prepare(self, A: BatchedProjectorXRF, b: torch.Tensor, x0=None) -> dict:
+        b = _arrays.to_dev(b)[0].reshape(-1).contiguous()
+        tau = _precondition(A, A.bp(torch.ones_like(b)).clone(), A.plan.vol_start, 0.0, self.relaxation)
+        sigma = _precondition(A, A.fp(torch.ones_like(tau)).clone(), A.plan.sino_start, 0.0, 1.0)
+        x = torch.zeros_like(tau) if x0 is None else _arrays.to_dev(x0)[0].reshape(-1).clone()
+        return dict(b=b, tau=tau, sigma=sigma, x=x, ax=torch.empty_like(b), res=torch.empty_like(b),
+                    upd=torch.empty_like(tau))
+
+    def iterate(self, A: BatchedProjectorXRF, s: dict, iterations: int, lower_limit=None, upper_limit=None):
+        hl, lo = _limit(lower_limit)
+        hu, hi = _limit(upper_limit)
+
+        def step():
+            st = _native.stream()
+            A.fp(s["x"], s["ax"])
+            _native.call("chip_sirt_residual", s["res"].data_ptr(), s["b"].data_ptr(), s["ax"].data_ptr(), None,
+                         None, s["sigma"].data_ptr(), s["b"].numel(), st)
+            A.bp(s["res"], s["upd"])
+            _native.call("chip_batch_sirt_update", A.plan._h, s["x"].data_ptr(), s["upd"].data_ptr(), None,
+                         s["tau"].data_ptr(), hl, lo, hu, hi, None, st)
+
+        key = ("sirt", hl, lo, hu, hi)
+        loop = s.get(key)
+        if loop is None:
+            loop = s[key] = _GraphedLoop(step, self.graph_chunk, self.use_graph)
+        loop.run(iterations)
+        return s
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None):
+        s = self.prepare(A, b, x0)
+        self.iterate(A, s, iterations, lower_limit, upper_limit)
+        return s["x"]
+
+
+class BatchedPDHG:
+    """PDHG with an l2 or KL data term and isotropic TV(lambda) (``solvers.py:1060-1206``) on every slice."""
+
+    def __init__(self, data_term: str = "l2", tv_lambda: float | None = None, relaxation: float = 0.95,
+                 graph_chunk: int = 10, use_graph: bool = True):
+        if data_term not in ("l2", "kl"):
+            raise ValueError(f"Unknown data term: {data_term}")
+        self.data_term = data_term
+        self.tv_lambda = tv_lambda
+        self.relaxation = relaxation
+        self.graph_chunk = graph_chunk
+        self.use_graph = use_graph
+
+    def prepare(self, A: BatchedProjectorXRF, b: torch.Tensor, x0=None) -> dict:
+        b = _arrays.to_dev(b)[0].reshape(-1).contiguous()
+        # TV's tau contribution: 2 * ndims, unweighted for its l12 (an l1-family) norm (regularizers.py:260-273)
+        tau_reg = 4.0 if self.tv_lambda is not None else 0.0
+        tau = _precondition(A, A.bp(torch.ones_like(b)).clone(), A.plan.vol_start, tau_reg, self.relaxation)
+        sigma = _precondition(A, A.fp(torch.ones_like(tau)).clone(), A.plan.sino_start, 0.0, self.relaxation)
+        x = torch.zeros_like(tau) if x0 is None else _arrays.to_dev(x0)[0].reshape(-1).clone()
+        q = None if self.tv_lambda is None else torch.zeros(2 * A.vol_elems, dtype=torch.float32, device=x.device)
+        return dict(b=b, tau=tau, sigma=sigma, x=x, xbar=x.clone(), p=torch.zeros_like(b), q=q,
+                    ax=torch.empty_like(b), upd=torch.empty_like(tau))
+
+    def iterate(self, A: BatchedProjectorXRF, s: dict, iterations: int, lower_limit=None, upper_limit=None):
+        hl, lo = _limit(lower_limit)
+        hu, hi = _limit(upper_limit)
+        dual = "chip_pdhg_dual_kl" if self.data_term == "kl" else "chip_pdhg_dual_l2"
+
+        def step():
+            st = _native.stream()
+            A.fp(s["xbar"], s["ax"])
+            _native.call(dual, s["p"].data_ptr(), s["ax"].data_ptr(), s["sigma"].data_ptr(), s["b"].data_ptr(), None,
+                         None, s["p"].numel(), st)
+            if s["q"] is not None:
+                _native.call("chip_batch_tv_dual", A.plan._h, s["q"].data_ptr(), s["xbar"].data_ptr(),
+                             float(self.tv_lambda), st)
+            A.bp(s["p"], s["upd"])
+            _native.call("chip_batch_pdhg_primal", A.plan._h, s["x"].data_ptr(), s["xbar"].data_ptr(),
+                         s["upd"].data_ptr(), _native.ptr(s["q"]), s["tau"].data_ptr(), hl, lo, hu, hi, None, st)
+
+        key = ("pdhg", hl, lo, hu, hi)
+        loop = s.get(key)
+        if loop is None:
+            loop = s[key] = _GraphedLoop(step, self.graph_chunk, self.use_graph)
+        loop.run(iterations)
+        return s
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None):
+        s = self.prepare(A, b, x0)
+        self.iterate(A, s, iterations, lower_limit, upper_limit)
+        return s["x"]
+
+
+def time_iterations(fn, iterations: int) -> float:
+    """Seconds for ``fn(iterations)`` with the device drained on both sides."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn(iterations)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0

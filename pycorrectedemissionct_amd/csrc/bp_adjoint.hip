@@ -26,29 +26,34 @@ constexpr int kTileH = 8;
 constexpr int kAngBlk = 32;
 constexpr int kWin = 40;  // >= 31 |c| + 7 |s| + 2 (|c| + |s|) + 2
 
+// One workgroup's work: the 32 x 8 pixel tile (tx, ty) of one slice.
+struct BpItem {
+  const float* sino;  // slice sinogram [D_total][A][n]
+  const float* maps;  // slice maps [A - maps_a0 ...][D_total][n][n], or NULL
+  float* vol;         // slice [n][n]
+  const float4* geom; // [A] of this slice size
+  int n;
+  int tx, ty;
+};
+
 template <int D, bool MAPS>
-__global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __restrict__ sino,
-                                                              const float* __restrict__ maps,
-                                                              int64_t maps_slice_stride, float* __restrict__ vol,
-                                                              const float4* __restrict__ geom,
-                                                              const float* __restrict__ w, int n, int n_angles,
-                                                              int n_det_total, int det0, int angle0, int n_ang,
-                                                              int maps_a0, int accumulate) {
-  __shared__ float win[kAngBlk][D][kWin];
-  __shared__ int wlo_s[kAngBlk];
+__device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float* __restrict__ w, int n_angles,
+                                                 int n_det_total, int det0, int angle0, int n_ang, int maps_a0,
+                                                 int accumulate, float (*win)[D][kWin], int* wlo_s) {
+  const int n = itm.n;
+  const float4* __restrict__ geom = itm.geom;
   const int tid = threadIdx.x;
-  const int j = blockIdx.x * kTileW + (tid & (kTileW - 1));
-  const int i = blockIdx.y * kTileH + (tid / kTileW);
-  const int b = blockIdx.z;
+  const int j = itm.tx * kTileW + (tid & (kTileW - 1));
+  const int i = itm.ty * kTileH + (tid / kTileW);
   const size_t nn = static_cast<size_t>(n) * n;
-  const float* sb = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
-  const float* mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  const float* sb = itm.sino;
+  const float* mb = itm.maps;
   const bool inside = i < n && j < n;
   const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
   const float fj = static_cast<float>(j), fi = static_cast<float>(i);
   // tile corners for the per-angle bin window
-  const float tj0 = static_cast<float>(blockIdx.x * kTileW), tj1 = tj0 + (kTileW - 1);
-  const float ti0 = static_cast<float>(blockIdx.y * kTileH), ti1 = ti0 + (kTileH - 1);
+  const float tj0 = static_cast<float>(itm.tx * kTileW), tj1 = tj0 + (kTileW - 1);
+  const float ti0 = static_cast<float>(itm.ty * kTileH), ti1 = ti0 + (kTileH - 1);
 
   float out = 0.f;
   for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kAngBlk) {
@@ -125,8 +130,63 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
     }
   }
   if (inside) {
-    float* o = vol + static_cast<size_t>(b) * nn + pix;
+    float* o = itm.vol + pix;
     *o = accumulate ? *o + out : out;
+  }
+}
+
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __restrict__ sino,
+                                                              const float* __restrict__ maps,
+                                                              int64_t maps_slice_stride, float* __restrict__ vol,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ w, int n, int n_angles,
+                                                              int n_det_total, int det0, int angle0, int n_ang,
+                                                              int maps_a0, int accumulate) {
+  __shared__ float win[kAngBlk][D][kWin];
+  __shared__ int wlo_s[kAngBlk];
+  const int b = blockIdx.z;
+  const size_t nn = static_cast<size_t>(n) * n;
+  BpItem itm;
+  itm.sino = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
+  itm.maps = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  itm.vol = vol + static_cast<size_t>(b) * nn;
+  itm.geom = geom;
+  itm.n = n;
+  itm.tx = blockIdx.x;
+  itm.ty = blockIdx.y;
+  bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, angle0, n_ang, maps_a0, accumulate, win, wlo_s);
+}
+
+// Persistent form over a batch of slices of different sizes: item k is tile
+// (tx, ty) of slice s, s found in the tile prefix sums.
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
+                                                               const int64_t* __restrict__ item_start,
+                                                               const float* __restrict__ sino,
+                                                               const float* __restrict__ maps,
+                                                               float* __restrict__ vol,
+                                                               const float4* __restrict__ geom,
+                                                               const float* __restrict__ w, int n_angles,
+                                                               int n_det_total, int det0, int accumulate) {
+  __shared__ float win[kAngBlk][D][kWin];
+  __shared__ int wlo_s[kAngBlk];
+  const int64_t total = item_start[n_slices];
+  for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
+    const int s = find_slice(item_start, n_slices, k);
+    const BatchSlice sl = slices[s];
+    const int local = static_cast<int>(k - item_start[s]);
+    const int ntx = (sl.n + kTileW - 1) / kTileW;
+    BpItem itm;
+    itm.sino = sino + sl.sino_off;
+    itm.maps = MAPS ? maps + sl.maps_off : nullptr;
+    itm.vol = vol + sl.vol_off;
+    itm.geom = geom + static_cast<size_t>(sl.size_idx) * n_angles;
+    itm.n = sl.n;
+    itm.ty = local / ntx;
+    itm.tx = local - itm.ty * ntx;
+    bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, 0, n_angles, 0, accumulate, win, wlo_s);
+    __syncthreads();  // the next item restages the windows
   }
 }
 
@@ -138,7 +198,32 @@ void launch(const Plan& p, const float* sino, const float* maps, int64_t stride,
                      p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate);
 }
 
+template <int D, bool MAPS>
+void launch_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int det0, int accumulate,
+                  hipStream_t st) {
+  const int64_t total = bp.h_item_start_bpa.back();
+  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
+  const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
+  hipLaunchKernelGGL((bp_adjoint_batch_kernel<D, MAPS>), dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
+                     bp.d_item_start_bpa, sino, maps, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det, det0,
+                     accumulate);
+}
+
 }  // namespace
+
+void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st) {
+  for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
+    const bool two = bp.n_det - det0 >= 2;
+    const int acc = det0 > 0 ? 1 : 0;
+    if (maps) {
+      if (two) launch_batch<2, true>(bp, sino, maps, vol, det0, acc, st);
+      else launch_batch<1, true>(bp, sino, maps, vol, det0, acc, st);
+    } else {
+      if (two) launch_batch<2, false>(bp, sino, nullptr, vol, det0, acc, st);
+      else launch_batch<1, false>(bp, sino, nullptr, vol, det0, acc, st);
+    }
+  }
+}
 
 void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
                              int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {

@@ -78,6 +78,56 @@ struct Plan {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// ---------------------------------------------------------------------------
+// batches of slices of different sizes (one persistent launch per pass)
+// ---------------------------------------------------------------------------
+struct BatchSlice {
+  int n;         // slice side
+  int n_ublk;    // 64-bin blocks of the fp
+  int size_idx;  // index of the slice's size: geometry at geom + size_idx * A
+  int pad;
+  int64_t vol_off;   // element offset of [n][n] in volume-shaped buffers
+  int64_t sino_off;  // element offset of [D][A][n] in sinogram-shaped buffers
+  int64_t maps_off;  // element offset of [A][D][n][n] in the maps buffer
+};
+
+struct BatchPlan {
+  int n_slices = 0, n_angles = 0, n_det = 0;
+  int n_cu = 256;
+  int fp_wg_per_cu = 2;  // persistent grid of the fp: fp_wg_per_cu x CUs workgroups
+  int bp_wg_per_cu = 4;  // same for the bp
+  std::vector<chip_plan*> size_plans;  // one single-slice plan per distinct size
+  std::vector<BatchSlice> h_slices;
+  // prefix sums (n_slices + 1 entries): fp items, adjoint-bp tiles, skimage-bp
+  // tiles, pixels, sinogram elements, map elements
+  std::vector<int64_t> h_item_start_fp, h_item_start_bpa, h_item_start_bps, h_vol_start, h_sino_start,
+      h_maps_start;
+  BatchSlice* d_slices = nullptr;
+  int64_t* d_item_start_fp = nullptr;
+  int64_t* d_item_start_bpa = nullptr;
+  int64_t* d_item_start_bps = nullptr;
+  int64_t* d_vol_start = nullptr;
+  float4* d_geom = nullptr;   // [size][A]
+  const float* d_w = nullptr; // [A][D], owned by size_plans[0]
+  int device = 0;
+};
+
+// largest s in [0, n_slices) with start[s] <= k (start is non-decreasing)
+__device__ __forceinline__ int find_slice(const int64_t* __restrict__ start, int n_slices, int64_t k) {
+  int lo = 0, hi = n_slices - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+void launch_fp_stream_batch(const BatchPlan& bp, const float* vol, const float* maps, float* sino, hipStream_t st);
+void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st);
+void launch_bp_skimage_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
+                             hipStream_t st);
+
 // fp over angles [angle0, angle0 + n_ang) and detectors [det0, det0 + nd),
 // nd in {1, 2}; maps[0] holds angle maps_a0. Needs N % 4 == 0 and 16-byte
 // aligned vol / maps (fp_stream.hip)

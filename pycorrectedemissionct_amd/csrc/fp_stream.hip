@@ -85,30 +85,33 @@ __device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, fl
   return bx;
 }
 
+// One workgroup's work: bins [u0, u0 + 64) of one angle of one slice, for the
+// detectors [det0, det0 + D) of a pass.
+struct FpItem {
+  const float* v;         // slice [n][n]
+  const float* mb;        // maps of (angle, det0): D planes of n*n, or NULL
+  float* out;             // sinogram line of (det0, angle); detector d at out + d * det_stride
+  size_t det_stride;      // elements between detectors' lines (A * n)
+  const float* wa;        // weights of (angle, det0 .. det0 + D)
+  float4 gm;              // (cos, sin, m02, m12) of the angle
+  int n;
+  int u0;
+};
+
 template <int D, bool MAPS>
-__global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
-                                                        int64_t maps_slice_stride, float* __restrict__ sino,
-                                                        const float4* __restrict__ geom, const float* __restrict__ w,
-                                                        int n, int n_angles, int n_det_total, int det0, int n_ublk,
-                                                        int angle0, int maps_a0) {
-  extern __shared__ float lds[];  // [D][kTile]
-  const int lb = logical_block(blockIdx.x, gridDim.x);
-  const int la = lb / n_ublk;
-  const int a = angle0 + la;
-  const int u0 = (lb - la * n_ublk) * kBins;
-  const int b = blockIdx.y;
+__device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
+  const int n = it.n;
+  const int u0 = it.u0;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int grp = tid >> 6;
   const int u = u0 + lane;
 
-  const float4 gm = geom[a];
+  const float4 gm = it.gm;
   const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
   const size_t nn = static_cast<size_t>(n) * n;
-  const float* v = vol + static_cast<size_t>(b) * nn;
-  const float* mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride +
-                               (static_cast<size_t>(a - maps_a0) * n_det_total + det0) * nn
-                         : nullptr;
+  const float* v = it.v;
+  const float* mb = it.mb;
   const float fn = static_cast<float>(n);
   const float uf = static_cast<float>(u);
   const float xu = __fmul_rn(cs, uf);
@@ -262,10 +265,69 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
     for (int d = 0; d < D; ++d) {
       const float s = ((lds[(d * 4 + 0) * 64 + lane] + lds[(d * 4 + 1) * 64 + lane]) + lds[(d * 4 + 2) * 64 + lane]) +
                       lds[(d * 4 + 3) * 64 + lane];
-      const int dd = det0 + d;
-      sino[((static_cast<size_t>(b) * n_det_total + dd) * n_angles + a) * n + u] =
-          s * w[static_cast<size_t>(a) * n_det_total + dd];
+      it.out[d * it.det_stride + u] = s * it.wa[d];
     }
+  }
+}
+
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+                                                        int64_t maps_slice_stride, float* __restrict__ sino,
+                                                        const float4* __restrict__ geom, const float* __restrict__ w,
+                                                        int n, int n_angles, int n_det_total, int det0, int n_ublk,
+                                                        int angle0, int maps_a0) {
+  extern __shared__ float lds[];  // [D][kTile]
+  const int lb = logical_block(blockIdx.x, gridDim.x);
+  const int la = lb / n_ublk;
+  const int a = angle0 + la;
+  const int b = blockIdx.y;
+  const size_t nn = static_cast<size_t>(n) * n;
+  FpItem it;
+  it.n = n;
+  it.u0 = (lb - la * n_ublk) * kBins;
+  it.gm = geom[a];
+  it.v = vol + static_cast<size_t>(b) * nn;
+  it.mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride +
+                     (static_cast<size_t>(a - maps_a0) * n_det_total + det0) * nn
+               : nullptr;
+  it.out = sino + ((static_cast<size_t>(b) * n_det_total + det0) * n_angles + a) * n;
+  it.det_stride = static_cast<size_t>(n_angles) * n;
+  it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
+  fp_stream_block<D, MAPS>(it, lds);
+}
+
+// Persistent form over a batch of slices of different sizes: work item k of
+// the batch is (slice s, angle a, bin block) with s found in the item prefix
+// sums; workgroup p takes items r * G + logical_block(p, G), r = 0, 1, ...
+// (G workgroups), so each XCD walks a contiguous run of bin blocks.
+template <int D, bool MAPS>
+__global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
+                                                              const int64_t* __restrict__ item_start,
+                                                              const float* __restrict__ vol,
+                                                              const float* __restrict__ maps, float* __restrict__ sino,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ w, int n_angles,
+                                                              int n_det_total, int det0) {
+  extern __shared__ float lds[];  // [D][kTile]
+  const int64_t total = item_start[n_slices];
+  const int g = gridDim.x;
+  const int64_t lb = logical_block(blockIdx.x, g);
+  for (int64_t k = lb; k < total; k += g) {
+    const int s = find_slice(item_start, n_slices, k);
+    const BatchSlice sl = slices[s];
+    const int local = static_cast<int>(k - item_start[s]);
+    const int a = local / sl.n_ublk;
+    const size_t nn = static_cast<size_t>(sl.n) * sl.n;
+    FpItem it;
+    it.n = sl.n;
+    it.u0 = (local - a * sl.n_ublk) * kBins;
+    it.gm = geom[static_cast<size_t>(sl.size_idx) * n_angles + a];
+    it.v = vol + sl.vol_off;
+    it.mb = MAPS ? maps + sl.maps_off + (static_cast<size_t>(a) * n_det_total + det0) * nn : nullptr;
+    it.out = sino + sl.sino_off + (static_cast<size_t>(det0) * n_angles + a) * sl.n;
+    it.det_stride = static_cast<size_t>(n_angles) * sl.n;
+    it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
+    fp_stream_block<D, MAPS>(it, lds);
   }
 }
 
@@ -279,7 +341,30 @@ void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, 
                      p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
 }
 
+template <int D, bool MAPS>
+void launch_batch(const BatchPlan& bp, const float* vol, const float* maps, float* sino, int det0, hipStream_t st) {
+  const int64_t total = bp.h_item_start_fp.back();
+  const int64_t want = static_cast<int64_t>(bp.fp_wg_per_cu) * bp.n_cu;
+  const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
+  const size_t shmem = sizeof(float) * D * kTile;
+  hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS>), dim3(grid), dim3(256), shmem, st, bp.d_slices, bp.n_slices,
+                     bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det, det0);
+}
+
 }  // namespace
+
+void launch_fp_stream_batch(const BatchPlan& bp, const float* vol, const float* maps, float* sino, hipStream_t st) {
+  for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
+    const bool two = bp.n_det - det0 >= 2;
+    if (maps) {
+      if (two) launch_batch<2, true>(bp, vol, maps, sino, det0, st);
+      else launch_batch<1, true>(bp, vol, maps, sino, det0, st);
+    } else {
+      if (two) launch_batch<2, false>(bp, vol, nullptr, sino, det0, st);
+      else launch_batch<1, false>(bp, vol, nullptr, sino, det0, st);
+    }
+  }
+}
 
 void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
                       int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st) {

@@ -574,17 +574,15 @@ constexpr int kBpTile = 16;
 
 // mode 0: sum over angles and detectors of interp (no maps, no weights)
 // mode 1: sum of w[a,d] * M[a,d][i,j] * interp
+// Sum over angles [angle0, angle0 + n_ang) and detectors of the line
+// interpolated at pixel (i, j) of one slice: sino is the slice's [D][A][n]
+// sinogram (line_mode: one [D][n] line set), maps its [A - maps_a0][D][n][n]
+// block (SYM only, may be NULL).
 template <int SYM>
-__global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict__ sino, const float* __restrict__ maps,
-                                                         int64_t maps_slice_stride, float* __restrict__ vol,
-                                                         const float4* __restrict__ geom,
-                                                         const float* __restrict__ w, int n, int n_angles,
-                                                         int n_det, int angle0, int n_ang, int line_mode,
-                                                         int accumulate, int maps_a0) {
-  const int j = blockIdx.x * kBpTile + (threadIdx.x & (kBpTile - 1));
-  const int i = blockIdx.y * kBpTile + (threadIdx.x / kBpTile);
-  const int b = blockIdx.z;
-  if (i >= n || j >= n) return;
+__device__ __forceinline__ float bp_skimage_pixel(const float* __restrict__ sino, const float* __restrict__ maps,
+                                                  const float4* __restrict__ geom, const float* __restrict__ w, int n,
+                                                  int n_angles, int n_det, int angle0, int n_ang, int line_mode,
+                                                  int maps_a0, int i, int j) {
   const int k = n / 2;
   const float jc = static_cast<float>(j - k);
   const float ic = static_cast<float>(i - k);
@@ -615,23 +613,80 @@ __global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict
     const bool ok1 = kk + 1 < n;
     for (int d = 0; d < n_det; ++d) {
       const float* line = line_mode ? sino + static_cast<size_t>(d) * n
-                                     : sino + ((static_cast<size_t>(b) * n_det + d) * n_angles + a) * n;
+                                     : sino + (static_cast<size_t>(d) * n_angles + a) * n;
       const float s0 = line[kk];
       const float s1 = ok1 ? line[kk + 1] : 0.f;
       float val = fmaf(fr, s1 - s0, s0);
       if (SYM) {
         float f = w[static_cast<size_t>(a) * n_det + d];
-        if (maps) {
-          f *= maps[static_cast<size_t>(b) * maps_slice_stride + (static_cast<size_t>(a - maps_a0) * n_det + d) * nn +
-                    pix];
-        }
+        if (maps) f *= maps[(static_cast<size_t>(a - maps_a0) * n_det + d) * nn + pix];
         val *= f;
       }
       acc += val;
     }
   }
-  float* o = vol + static_cast<size_t>(b) * nn + pix;
+  return acc;
+}
+
+template <int SYM>
+__global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict__ sino, const float* __restrict__ maps,
+                                                         int64_t maps_slice_stride, float* __restrict__ vol,
+                                                         const float4* __restrict__ geom,
+                                                         const float* __restrict__ w, int n, int n_angles,
+                                                         int n_det, int angle0, int n_ang, int line_mode,
+                                                         int accumulate, int maps_a0) {
+  const int j = blockIdx.x * kBpTile + (threadIdx.x & (kBpTile - 1));
+  const int i = blockIdx.y * kBpTile + (threadIdx.x / kBpTile);
+  const int b = blockIdx.z;
+  if (i >= n || j >= n) return;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* sb = line_mode ? sino : sino + static_cast<size_t>(b) * n_det * n_angles * n;
+  const float* mb = maps ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  const float acc = bp_skimage_pixel<SYM>(sb, mb, geom, w, n, n_angles, n_det, angle0, n_ang, line_mode, maps_a0, i, j);
+  float* o = vol + static_cast<size_t>(b) * nn + static_cast<size_t>(i) * n + j;
   *o = accumulate ? *o + acc : acc;
+}
+
+// Persistent form over a batch of slices of different sizes: item k is the
+// 16 x 16 pixel tile of slice s found in the tile prefix sums.
+template <int SYM>
+__global__ __launch_bounds__(256) void bp_skimage_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
+                                                               const int64_t* __restrict__ item_start,
+                                                               const float* __restrict__ sino,
+                                                               const float* __restrict__ maps,
+                                                               float* __restrict__ vol,
+                                                               const float4* __restrict__ geom,
+                                                               const float* __restrict__ w, int n_angles, int n_det) {
+  const int64_t total = item_start[n_slices];
+  for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
+    const int s = find_slice(item_start, n_slices, k);
+    const BatchSlice sl = slices[s];
+    const int local = static_cast<int>(k - item_start[s]);
+    const int ntx = (sl.n + kBpTile - 1) / kBpTile;
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int j = tx * kBpTile + (threadIdx.x & (kBpTile - 1));
+    const int i = ty * kBpTile + (threadIdx.x / kBpTile);
+    if (i < sl.n && j < sl.n) {
+      const float acc = bp_skimage_pixel<SYM>(sino + sl.sino_off, (SYM && maps) ? maps + sl.maps_off : nullptr,
+                                              geom + static_cast<size_t>(sl.size_idx) * n_angles, w, sl.n, n_angles,
+                                              n_det, 0, n_angles, 0, 0, i, j);
+      vol[sl.vol_off + static_cast<int64_t>(i) * sl.n + j] = acc;
+    }
+  }
+}
+
+void launch_bp_skimage_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
+                             hipStream_t st) {
+  const int64_t total = bp.h_item_start_bps.back();
+  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
+  const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
+  if (sym) {
+    hipLaunchKernelGGL(bp_skimage_batch_kernel<1>, dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
+                       bp.d_item_start_bps, sino, maps, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det);
+  } else {
+    hipLaunchKernelGGL(bp_skimage_batch_kernel<0>, dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
+                       bp.d_item_start_bps, sino, nullptr, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,175 @@
+"""Batches of slices of different sizes (run with -m gpu).
+
+The persistent batched kernels compute every work item exactly as the
+single-slice kernels do, so per slice the batched maps, fp and bp are
+bit-identical to the single-slice projector (itself checked against the
+oracle to 1e-5), and the batched SIRT / PDHG match per-slice solver runs.
+HIP-graph replays are checked against eager iterations, bit for bit, over
+several replays with eager work in between.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import attenuation as oatt
+from oracle import xrf as oxrf
+from tests.conftest import rel_max
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [64, 128, 32, 96, 128, 40]
+DET = [np.pi / 2, -np.pi / 2]
+W_DET = [1.0, 0.7]
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pycorrectedemissionct_amd as p
+
+    return p
+
+
+def _angles(a):
+    return np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+
+
+def _mus(rng, sizes):
+    mu_in = [(rng.random((n, n)) * 0.02).astype(np.float32) for n in sizes]
+    mu_out = [(rng.random((n, n)) * 0.04).astype(np.float32) for n in sizes]
+    return mu_in, mu_out
+
+
+def _single(pkg, n, ang, mi, mo, bp_mode):
+    return pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mi, att_out=mo, angles_detectors_rad=DET,
+                                       weights_detectors=W_DET, bp_mode="adjoint" if bp_mode == "adjoint" else "skimage",
+                                       is_symmetric=bp_mode == "symmetric")
+
+
+@pytest.mark.parametrize("bp_mode", ["adjoint", "skimage", "symmetric"])
+def test_batched_projector_equals_single_slices(pkg, bp_mode):
+    import torch
+
+    a = 36
+    rng = np.random.default_rng(700)
+    ang = _angles(a)
+    mu_in, mu_out = _mus(rng, SIZES)
+    A = pkg.batch.BatchedProjectorXRF(SIZES, ang, mu_in, mu_out, angles_detectors_rad=DET, weights_detectors=W_DET,
+                                      bp_mode=bp_mode)
+    xs = [rng.random((n, n)).astype(np.float32) for n in SIZES]
+    ys = [rng.random((2, a, n)).astype(np.float32) for n in SIZES]
+    fx = A.unpack_sino(A.fp(A.pack_vol(xs)))
+    by = A.unpack_vol(A.bp(A.pack_sino(ys)))
+    for s, n in enumerate(SIZES):
+        single = _single(pkg, n, ang, mu_in[s], mu_out[s], bp_mode)
+        assert torch.equal(A.slice_maps(s), single.att_vol_angles), f"maps of slice {s}"
+        assert torch.equal(fx[s], single.fp(torch.as_tensor(xs[s], device="cuda"))), f"fp of slice {s}"
+        assert torch.equal(by[s], single.bp(torch.as_tensor(ys[s], device="cuda"))), f"bp of slice {s}"
+    # one slice against the oracle directly
+    s = 2
+    maps = oatt.compute_maps(mu_in[s], mu_out[s], ang, np.array(DET))
+    assert rel_max(oxrf.fp(xs[s], ang, maps, w_det=W_DET), fx[s].cpu().numpy()) < 1e-5
+
+
+def test_autotune_changes_time_not_results(pkg):
+    import torch
+
+    a = 24
+    rng = np.random.default_rng(701)
+    ang = _angles(a)
+    mu_in, mu_out = _mus(rng, SIZES)
+    A = pkg.batch.BatchedProjectorXRF(SIZES, ang, mu_in, mu_out, angles_detectors_rad=DET, bp_mode="adjoint")
+    x = A.pack_vol([rng.random((n, n)).astype(np.float32) for n in SIZES])
+    y = A.pack_sino([rng.random((2, a, n)).astype(np.float32) for n in SIZES])
+    A.plan.tune("fp_wg_per_cu", 1)
+    A.plan.tune("bp_wg_per_cu", 1)
+    f1, b1 = A.fp(x).clone(), A.bp(y).clone()
+    res = pkg.batch.autotune(A)
+    print("autotune:", res)
+    assert res["fp_wg_per_cu"] in res["fp_us"] and res["bp_wg_per_cu"] in res["bp_us"]
+    assert torch.equal(A.fp(x), f1) and torch.equal(A.bp(y), b1)
+    with pytest.raises(pkg._native.NativeError):
+        A.plan.tune("fp_wg_per_cu", 0)
+
+
+def test_batch_rejects_sizes_not_multiple_of_4(pkg):
+    with pytest.raises(pkg._native.NativeError, match="N % 4"):
+        pkg.batch.BatchPlan([64, 30], _angles(8))
+
+
+@pytest.mark.parametrize("data_term", ["l2", "kl"])
+def test_batched_pdhg_tv_equals_single_solves(pkg, data_term):
+    import torch
+
+    a, its = 32, 30
+    rng = np.random.default_rng(702)
+    ang = _angles(a)
+    sizes = [64, 32, 96]
+    mu_in, mu_out = _mus(rng, sizes)
+    A = pkg.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=DET, weights_detectors=W_DET,
+                                      bp_mode="adjoint")
+    fs = [pkg.testing.phantom_ellipses(n, 4, rng) for n in sizes]
+    b = A.fp(A.pack_vol(fs))
+    b = torch.clamp(b + 0.01 * b.max() * torch.as_tensor(rng.standard_normal(b.numel()), dtype=torch.float32,
+                                                         device="cuda"), min=0.0)
+    solver = pkg.batch.BatchedPDHG(data_term=data_term, tv_lambda=1e-2, use_graph=True)
+    xb = A.unpack_vol(solver(A, b, its))
+    bs = A.unpack_sino(b)
+    for s, n in enumerate(sizes):
+        single = _single(pkg, n, ang, mu_in[s], mu_out[s], "adjoint")
+        ref = pkg.PDHG(data_term=data_term, regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))
+        st = ref.prepare(single, bs[s].contiguous())
+        ref.iterate(single, st, its)
+        e = rel_max(st["x"].cpu().numpy(), xb[s].cpu().numpy())
+        print(f"batched PDHG-{data_term} slice {s} (n={n}): rel_max vs single {e:.2e}")
+        assert e < 1e-5
+
+
+def test_batched_sirt_equals_single_solves(pkg):
+    import torch
+
+    a, its = 32, 20
+    rng = np.random.default_rng(703)
+    ang = _angles(a)
+    sizes = [32, 64]
+    mu_in, mu_out = _mus(rng, sizes)
+    A = pkg.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=DET, weights_detectors=W_DET,
+                                      bp_mode="adjoint")
+    fs = [pkg.testing.phantom_ellipses(n, 3, rng) for n in sizes]
+    b = A.fp(A.pack_vol(fs))
+    xb = A.unpack_vol(pkg.batch.BatchedSIRT(use_graph=True)(A, b, its))
+    bs = A.unpack_sino(b)
+    for s, n in enumerate(sizes):
+        single = _single(pkg, n, ang, mu_in[s], mu_out[s], "adjoint")
+        x, _ = pkg.SIRT()(single, bs[s].contiguous(), its)
+        e = rel_max(x.cpu().numpy(), xb[s].cpu().numpy())
+        print(f"batched SIRT slice {s} (n={n}): rel_max vs single {e:.2e}")
+        assert e < 1e-5
+
+
+def test_graph_replays_equal_eager(pkg):
+    """25 + 20 iterations: one eager warm-up step, a 10-step graph replayed 2 + 2 times and
+    eager remainders in between, against 45 eager iterations; bit-identical."""
+    import torch
+
+    a = 24
+    rng = np.random.default_rng(704)
+    ang = _angles(a)
+    sizes = [64, 32]
+    mu_in, mu_out = _mus(rng, sizes)
+    A = pkg.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=DET, bp_mode="adjoint")
+    b = A.fp(A.pack_vol([pkg.testing.phantom_ellipses(n, 3, rng) for n in sizes]))
+    runs = {}
+    for use_graph in (True, False):
+        solver = pkg.batch.BatchedPDHG(data_term="l2", tv_lambda=1e-2, graph_chunk=10, use_graph=use_graph)
+        st = solver.prepare(A, b)
+        solver.iterate(A, st, 25)
+        torch.cuda.synchronize()
+        mid = st["x"].clone()
+        solver.iterate(A, st, 20)
+        runs[use_graph] = (mid, st["x"].clone())
+    assert torch.equal(runs[True][0], runs[False][0])
+    assert torch.equal(runs[True][1], runs[False][1])
