@@ -242,6 +242,53 @@ int chip_batch_sirt_update(const chip_batch_plan* plan, float* x, const float* u
                            int has_lower, float lower, int has_upper, float upper, const float* x_mask,
                            void* stream);
 
+/* ---- warm-start network (no reference counterpart) --------------------------
+ * Building blocks of the U-Net that seeds PDHG with an estimate of its
+ * solution from a short SIRT reconstruction. Activations are NHWC bfloat16
+ * (uint16_t bit patterns), conv weights [9][Cout][Cin] bfloat16 (tap =
+ * 3 (dh + 1) + (dw + 1)), biases, master weights and gradients float32; the
+ * 3x3 convolutions run on the matrix cores with float32 accumulation.
+ * Reductions use caller-provided float32 scratch `part` of `part_elems`
+ * elements and add partial sums in a fixed order.
+ */
+/* y = act(bias + conv3x3(concat(x1[c1], x2[c2]))), "same" zero padding; the
+ * first o1 output channels go to y1, the rest to y2 (NULL when o1 = cout). */
+int chip_nn_conv3x3(const uint16_t* x1, int c1, const uint16_t* x2, int c2, const uint16_t* wt, const float* bias,
+                    uint16_t* y1, int o1, uint16_t* y2, int n, int h, int w, int cout, int relu, void* stream);
+/* dw[tap][co][ci] = sum_p dy[p][co] * concat(x1, x2)[p + offset(tap)][ci] */
+int chip_nn_conv3x3_wgrad(const uint16_t* dy, const uint16_t* x1, int c1, const uint16_t* x2, int c2, int n, int h,
+                          int w, int cout, float* part, int64_t part_elems, float* dw, void* stream);
+/* db[ch] = sum_p dy[p][ch] */
+int chip_nn_bias_grad(const uint16_t* dy, int64_t n_pix, int c, float* part, int64_t part_elems, float* db,
+                      void* stream);
+/* wt_t[tap][ci][co] = wt[8 - tap][co][ci]: weights of the data-gradient convolution */
+int chip_nn_flip_weights(const uint16_t* wt, int cout, int cin, uint16_t* wt_t, void* stream);
+/* d[k] = 0 where y[k] <= 0 (gradient through a ReLU whose output is y) */
+int chip_nn_relu_mask(uint16_t* d, const uint16_t* y, int64_t count, void* stream);
+/* 2 x 2 average pooling of an n x h x w x c input (backward = 1: x is dy of
+ * the pooled shape, y is dx of the input shape) */
+int chip_nn_avgpool2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c, int backward, void* stream);
+/* 2x nearest upsampling of an n x h x w x c input (backward = 1: x is dy of the
+ * upsampled shape, y is dx of the input shape) */
+int chip_nn_upsample2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c, int backward, void* stream);
+/* out[p] = xin[p] + wb[c] + sum_ci wb[ci] feat[p][ci] (1 x 1 head on a residual, float32) */
+int chip_nn_head_fwd(const uint16_t* feat, int c, const float* wb, const float* xin, float* out, int64_t n_pix,
+                     void* stream);
+/* dfeat = dout x wb[:c]; dwb[ci] = sum_p dout[p] feat[p][ci], dwb[c] = sum_p dout[p] */
+int chip_nn_head_bwd(const float* dout, const uint16_t* feat, int c, const float* wb, uint16_t* dfeat, float* part,
+                     int64_t part_elems, float* dwb, int64_t n_pix, void* stream);
+/* loss[0] = mean (out - tgt)^2 (double), dout = 2 (out - tgt) / n_pix; part: >= 1 doubles */
+int chip_nn_mse(const float* out, const float* tgt, int64_t n_pix, float* dout, double* part, int part_elems,
+                double* loss, void* stream);
+/* Adam step `step` (>= 1) on float32 weights w with gradient g and moments m, v;
+ * w_bf16 (nullable) receives the bfloat16 copy */
+int chip_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
+                 float eps, int step, uint16_t* w_bf16, void* stream);
+/* y = bf16(y + x) (gradients of a tensor with two consumers) */
+int chip_nn_add_bf16(uint16_t* y, const uint16_t* x, int64_t n, void* stream);
+int chip_nn_to_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+int chip_nn_to_f32(const uint16_t* x, float* y, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ only carry device memory and the stream.
 
 __version__ = "0.1.0"
 
-from . import batch, models, operators, projectors, solvers, data_terms, regularizers, testing  # noqa: F401
+from . import batch, models, operators, projectors, solvers, data_terms, regularizers, testing, warmstart  # noqa: F401
 from ._projector_backends import ProjectorBackend, ProjectorBackendHIP, compute_attenuation  # noqa: F401
 from .physics.attenuation import AttenuationVolume  # noqa: F401
 from .projectors import ProjectorAttenuationXRF, ProjectorMatrix, ProjectorUncorrected  # noqa: F401

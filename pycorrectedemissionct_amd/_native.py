@@ -80,6 +80,30 @@ _SIGNATURES = {
         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int,
         ctypes.c_float, c_void_p, c_void_p,
     ],
+    "chip_nn_conv3x3": [
+        c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_void_p,
+    ],
+    "chip_nn_conv3x3_wgrad": [
+        c_void_p, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
+    ],
+    "chip_nn_bias_grad": [c_void_p, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p],
+    "chip_nn_flip_weights": [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p],
+    "chip_nn_relu_mask": [c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_nn_avgpool2": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         c_void_p],
+    "chip_nn_upsample2": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          c_void_p],
+    "chip_nn_head_fwd": [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_nn_head_bwd": [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+                         ctypes.c_int64, c_void_p],
+    "chip_nn_mse": [c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p],
+    "chip_nn_adam": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                     ctypes.c_float, ctypes.c_float, ctypes.c_int, c_void_p, c_void_p],
+    "chip_nn_to_bf16": [c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_nn_to_f32": [c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_nn_add_bf16": [c_void_p, c_void_p, ctypes.c_int64, c_void_p],
     "chip_batch_sirt_update": [
         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float,
         c_void_p, c_void_p,

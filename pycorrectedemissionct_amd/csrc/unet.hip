@@ -1,0 +1,665 @@
+// Building blocks of the warm-start U-Net (gfx950): BF16 activations and
+// weights, FP32 accumulation on the matrix cores.
+//
+// Layouts: activations NHWC bf16 (channels contiguous), conv weights
+// [9 taps][Cout][Cin] bf16 (tap = 3 * (dh + 1) + (dw + 1)), biases and master
+// weights fp32. A 3x3 convolution ("same" zero padding) is an implicit GEMM
+// with M = pixels, N = output channels, K = 9 taps x input channels; the
+// input may be the channel concatenation of two tensors (the U-Net's skip
+// connections) and the output may be split in two (the gradient of such a
+// concatenation).
+//
+// Matrix core: v_mfma_f32_16x16x16_bf16. Operand layout (lane l, element i):
+// A[m][k] at m = l % 16, k = 4 (l / 16) + i; B[k][n] at n = l % 16,
+// k = 4 (l / 16) + i; D[m][n] at n = l % 16, m = 4 (l / 16) + i.
+//
+// Every reduction (weight / bias gradients, loss) writes per-chunk partial
+// sums and adds them in a fixed order, so results are reproducible.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace chip {
+namespace {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// float -> bf16, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct Img {
+  int n, h, w;
+  __device__ __forceinline__ int64_t pixels() const { return static_cast<int64_t>(n) * h * w; }
+};
+
+// ---------------------------------------------------------------------------
+// forward 3x3 convolution
+// ---------------------------------------------------------------------------
+constexpr int kPxWave = 32;            // pixels per wave (2 MFMA row blocks)
+constexpr int kPxBlock = 4 * kPxWave;  // pixels per workgroup
+constexpr int kCoBlock = 32;           // output channels per workgroup (2 MFMA column blocks)
+
+// 4 consecutive input channels [ci, ci + 4) of pixel (n, hh, ww) of concat(x1, x2)
+__device__ __forceinline__ s4v load_in4(const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2,
+                                        int c2, int64_t pix, int ci, bool vec) {
+  s4v r = {0, 0, 0, 0};
+  const int cin = c1 + c2;
+  if (vec) {  // c1, c2 multiples of 4: the four channels lie in one tensor
+    if (ci < c1) {
+      const uint2 v = *reinterpret_cast<const uint2*>(x1 + pix * c1 + ci);
+      r = __builtin_bit_cast(s4v, v);
+    } else if (ci < cin) {
+      const uint2 v = *reinterpret_cast<const uint2*>(x2 + pix * c2 + (ci - c1));
+      r = __builtin_bit_cast(s4v, v);
+    }
+    return r;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int c = ci + i;
+    uint16_t v = 0;
+    if (c < c1) v = x1[pix * c1 + c];
+    else if (c < cin) v = x2[pix * c2 + (c - c1)];
+    r[i] = static_cast<short>(v);
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict__ x1, int c1,
+                                                      const uint16_t* __restrict__ x2, int c2,
+                                                      const uint16_t* __restrict__ wt, const float* __restrict__ bias,
+                                                      uint16_t* __restrict__ y1, int o1, uint16_t* __restrict__ y2,
+                                                      Img im, int cout, int relu) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t P = im.pixels();
+  const int64_t pbase = static_cast<int64_t>(blockIdx.x) * kPxBlock + wave * kPxWave;
+  const int cbase = blockIdx.y * kCoBlock;
+  const int cin = c1 + c2;
+  const bool vec_in = (c1 % 4 == 0) && (c2 % 4 == 0);
+  const bool vec_w = (cin % 4 == 0);
+  const int kq = 4 * (lane >> 4);  // this lane's k offset within a 16-wide k step
+
+  // the lane's A-operand pixel of each 16-pixel block
+  int ph[2], pw[2], pn[2];
+  bool pv[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int64_t p = pbase + 16 * b + (lane & 15);
+    pv[b] = p < P;
+    const int64_t pc = pv[b] ? p : 0;
+    pw[b] = static_cast<int>(pc % im.w);
+    const int64_t t = pc / im.w;
+    ph[b] = static_cast<int>(t % im.h);
+    pn[b] = static_cast<int>(t / im.h);
+  }
+  int co_l[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) co_l[c] = cbase + 16 * c + (lane & 15);
+
+  f4v acc[2][2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[b][c] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dh = tap / 3 - 1, dw = tap % 3 - 1;
+    int64_t src[2];
+    bool sv[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int hh = ph[b] + dh, ww = pw[b] + dw;
+      sv[b] = pv[b] && hh >= 0 && hh < im.h && ww >= 0 && ww < im.w;
+      src[b] = (static_cast<int64_t>(pn[b]) * im.h + (sv[b] ? hh : 0)) * im.w + (sv[b] ? ww : 0);
+    }
+    const uint16_t* wtap = wt + static_cast<size_t>(tap) * cout * cin;
+    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+      const int ci = ci0 + kq;
+      s4v a[2], bm[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        a[b] = sv[b] ? load_in4(x1, c1, x2, c2, src[b], ci, vec_in) : s4v{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        s4v r = {0, 0, 0, 0};
+        if (co_l[c] < cout) {
+          const uint16_t* wr = wtap + static_cast<size_t>(co_l[c]) * cin;
+          if (vec_w) {
+            if (ci < cin) r = __builtin_bit_cast(s4v, *reinterpret_cast<const uint2*>(wr + ci));
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) r[i] = (ci + i < cin) ? static_cast<short>(wr[ci + i]) : 0;
+          }
+        }
+        bm[c] = r;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[b], bm[c], acc[b][c], 0, 0, 0);
+    }
+  }
+
+  // epilogue: D[m][n] at n = lane % 16, m = 4 (lane / 16) + i
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int co = co_l[c];
+    if (co >= cout) continue;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t p = pbase + 16 * b + kq + i;
+        if (p >= P) continue;
+        float v = acc[b][c][i] + bb;
+        if (relu) v = fmaxf(v, 0.f);
+        if (co < o1) y1[p * o1 + co] = f2bf(v);
+        else y2[p * (cout - o1) + (co - o1)] = f2bf(v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: part[chunk][tap][co][ci] = sum over the chunk's pixels p of
+// dy[p][co] * x[p + offset(tap)][ci]   (M = co, N = ci, K = pixels)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t in_at(const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2,
+                                          int c2, int64_t pix, int c) {
+  if (c < c1) return x1[pix * c1 + c];
+  if (c < c1 + c2) return x2[pix * c2 + (c - c1)];
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const uint16_t* __restrict__ dy,
+                                                            const uint16_t* __restrict__ x1, int c1,
+                                                            const uint16_t* __restrict__ x2, int c2, Img im, int cout,
+                                                            int64_t chunk_px, float* __restrict__ part) {
+  __shared__ float red[4][64][16];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int cin = c1 + c2;
+  const int n_cob = (cout + 31) / 32, n_cib = (cin + 31) / 32;
+  int bid = blockIdx.x;
+  const int cib = bid % n_cib;
+  bid /= n_cib;
+  const int cob = bid % n_cob;
+  const int tap = bid / n_cob;
+  const int dh = tap / 3 - 1, dw = tap % 3 - 1;
+  const int chunk = blockIdx.y;
+  const int64_t P = im.pixels();
+  const int64_t p_lo = static_cast<int64_t>(chunk) * chunk_px;
+  const int64_t p_hi = min(P, p_lo + chunk_px);
+  const int kq = 4 * (lane >> 4);
+  int co_l[2], ci_l[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    co_l[b] = cob * 32 + 16 * b + (lane & 15);
+    ci_l[b] = cib * 32 + 16 * b + (lane & 15);
+  }
+  f4v acc[2][2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[b][c] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t k0 = p_lo + 16 * wave; k0 < p_hi; k0 += 64) {
+    s4v a[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, bm[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t p = k0 + kq + i;
+      if (p >= p_hi) continue;
+      const int pw = static_cast<int>(p % im.w);
+      const int64_t t = p / im.w;
+      const int ph = static_cast<int>(t % im.h);
+      const int pn = static_cast<int>(t / im.h);
+      const int hh = ph + dh, ww = pw + dw;
+      const bool sv = hh >= 0 && hh < im.h && ww >= 0 && ww < im.w;
+      const int64_t src = (static_cast<int64_t>(pn) * im.h + hh) * im.w + ww;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (co_l[b] < cout) a[b][i] = static_cast<short>(dy[p * cout + co_l[b]]);
+        if (sv && ci_l[b] < cin) bm[b][i] = static_cast<short>(in_at(x1, c1, x2, c2, src, ci_l[b]));
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[b], bm[c], acc[b][c], 0, 0, 0);
+  }
+  // sum the four waves' tiles in a fixed order
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[wave][lane][(b * 2 + c) * 4 + i] = acc[b][c][i];
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int ci = ci_l[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int co = cob * 32 + 16 * b + kq + i;
+          const int r = (b * 2 + c) * 4 + i;
+          const float s = ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+          if (co < cout && ci < cin)
+            part[((static_cast<size_t>(chunk) * 9 + tap) * cout + co) * cin + ci] = s;
+        }
+      }
+    }
+  }
+}
+
+// out[i] = sum_c part[c][i] (fixed order)
+__global__ void sum_parts_kernel(const float* __restrict__ part, int n_parts, int64_t len, float* __restrict__ out) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < n_parts; ++c) s += part[static_cast<size_t>(c) * len + i];
+    out[i] = s;
+  }
+}
+
+// part[chunk][co] = sum over the chunk's pixels of dy[p][co]
+__global__ void channel_sum_kernel(const uint16_t* __restrict__ dy, int64_t P, int c, int64_t chunk_px,
+                                   float* __restrict__ part) {
+  const int chunk = blockIdx.x;
+  const int64_t p_lo = static_cast<int64_t>(chunk) * chunk_px;
+  const int64_t p_hi = min(P, p_lo + chunk_px);
+  for (int ch = threadIdx.x; ch < c; ch += blockDim.x) {
+    float s = 0.f;
+    for (int64_t p = p_lo; p < p_hi; ++p) s += bf2f(dy[p * c + ch]);
+    part[static_cast<size_t>(chunk) * c + ch] = s;
+  }
+}
+
+// wt_t[tap][ci][co] = wt[8 - tap][co][ci]: the weights of the data-gradient convolution
+__global__ void flip_weights_kernel(const uint16_t* __restrict__ wt, int cout, int cin, uint16_t* __restrict__ wt_t) {
+  const int64_t total = 9LL * cout * cin;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int tap = static_cast<int>(k / (static_cast<int64_t>(cout) * cin));
+    const int rem = static_cast<int>(k - static_cast<int64_t>(tap) * cout * cin);
+    const int ci = rem / cout, co = rem - ci * cout;
+    wt_t[k] = wt[(static_cast<size_t>(8 - tap) * cout + co) * cin + ci];
+  }
+}
+
+__global__ void relu_mask_kernel(uint16_t* __restrict__ d, const uint16_t* __restrict__ y, int64_t count) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (!(bf2f(y[k]) > 0.f)) d[k] = 0;
+  }
+}
+
+// y[n][h][w][c] = (x[2h][2w] + x[2h][2w+1] + x[2h+1][2w] + x[2h+1][2w+1]) / 4; h, w of the input
+__global__ void avgpool2_kernel(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w, int c) {
+  const int ho = h / 2, wo = w / 2;
+  const int64_t total = static_cast<int64_t>(n) * ho * wo * c;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int ch = static_cast<int>(k % c);
+    int64_t t = k / c;
+    const int j = static_cast<int>(t % wo);
+    t /= wo;
+    const int i = static_cast<int>(t % ho);
+    const int64_t b = t / ho;
+    const int64_t r0 = ((b * h + 2 * i) * w + 2 * j) * c + ch;
+    const int64_t r1 = r0 + static_cast<int64_t>(w) * c;
+    const float s = (bf2f(x[r0]) + bf2f(x[r0 + c])) + (bf2f(x[r1]) + bf2f(x[r1 + c]));
+    y[k] = f2bf(0.25f * s);
+  }
+}
+
+// dx[2i + a][2j + b] = dy[i][j] / 4; h, w of the input (dx)
+__global__ void avgpool2_bwd_kernel(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int n, int h, int w,
+                                    int c) {
+  const int64_t total = static_cast<int64_t>(n) * h * w * c;
+  const int wo = w / 2, ho = h / 2;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int ch = static_cast<int>(k % c);
+    int64_t t = k / c;
+    const int j = static_cast<int>(t % w);
+    t /= w;
+    const int i = static_cast<int>(t % h);
+    const int64_t b = t / h;
+    dx[k] = f2bf(0.25f * bf2f(dy[((b * ho + i / 2) * wo + j / 2) * c + ch]));
+  }
+}
+
+// y[2i + a][2j + b] = x[i][j]; h, w of the input (x)
+__global__ void upsample2_kernel(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w, int c) {
+  const int ho = 2 * h, wo = 2 * w;
+  const int64_t total = static_cast<int64_t>(n) * ho * wo * c;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int ch = static_cast<int>(k % c);
+    int64_t t = k / c;
+    const int j = static_cast<int>(t % wo);
+    t /= wo;
+    const int i = static_cast<int>(t % ho);
+    const int64_t b = t / ho;
+    y[k] = x[((b * h + i / 2) * w + j / 2) * c + ch];
+  }
+}
+
+// dx[i][j] = sum of dy over the 2 x 2 block it was copied to; h, w of dx
+__global__ void upsample2_bwd_kernel(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx, int n, int h, int w,
+                                     int c) {
+  const int64_t total = static_cast<int64_t>(n) * h * w * c;
+  const int wo = 2 * w;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int ch = static_cast<int>(k % c);
+    int64_t t = k / c;
+    const int j = static_cast<int>(t % w);
+    t /= w;
+    const int i = static_cast<int>(t % h);
+    const int64_t b = t / h;
+    const int64_t r0 = ((b * 2 * h + 2 * i) * wo + 2 * j) * c + ch;
+    const int64_t r1 = r0 + static_cast<int64_t>(wo) * c;
+    const float s = (bf2f(dy[r0]) + bf2f(dy[r0 + c])) + (bf2f(dy[r1]) + bf2f(dy[r1 + c]));
+    dx[k] = f2bf(s);
+  }
+}
+
+// out[p] = xin[p] + wb[c] + sum_ci wb[ci] * feat[p][ci]   (1 x 1 head, residual on the input; fp32)
+__global__ void head_fwd_kernel(const uint16_t* __restrict__ feat, int c, const float* __restrict__ wb,
+                                const float* __restrict__ xin, float* __restrict__ out, int64_t P) {
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < P;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    float s = wb[c];
+    for (int ci = 0; ci < c; ++ci) s = fmaf(wb[ci], bf2f(feat[p * c + ci]), s);
+    out[p] = xin[p] + s;
+  }
+}
+
+// dfeat[p][ci] = dout[p] * wb[ci]; part[chunk][ci] = sum_p dout[p] feat[p][ci], part[chunk][c] = sum_p dout[p]
+__global__ void head_bwd_kernel(const float* __restrict__ dout, const uint16_t* __restrict__ feat, int c,
+                                const float* __restrict__ wb, uint16_t* __restrict__ dfeat, int64_t P,
+                                int64_t chunk_px, float* __restrict__ part) {
+  const int chunk = blockIdx.x;
+  const int64_t p_lo = static_cast<int64_t>(chunk) * chunk_px;
+  const int64_t p_hi = min(P, p_lo + chunk_px);
+  for (int64_t p = p_lo + threadIdx.x; p < p_hi; p += blockDim.x) {
+    for (int ci = 0; ci < c; ++ci) dfeat[p * c + ci] = f2bf(dout[p] * wb[ci]);
+  }
+  for (int ci = threadIdx.x; ci <= c; ci += blockDim.x) {
+    float s = 0.f;
+    for (int64_t p = p_lo; p < p_hi; ++p) s += dout[p] * (ci < c ? bf2f(feat[p * c + ci]) : 1.f);
+    part[static_cast<size_t>(chunk) * (c + 1) + ci] = s;
+  }
+}
+
+// dout[p] = 2 (out[p] - tgt[p]) / P; part[block] = sum of squared errors of the block's pixels
+__global__ void mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt, int64_t P,
+                           float* __restrict__ dout, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < P;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float e = out[p] - tgt[p];
+    dout[p] = 2.f * e / static_cast<float>(P);
+    s += static_cast<double>(e) * e;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void sum_double_kernel(const double* __restrict__ part, int n, double scale, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    out[0] = s * scale;
+  }
+}
+
+// Adam on fp32 master weights; optional bf16 copy of the result
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                            float c2, uint16_t* __restrict__ w_bf16) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    const float wk = w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    w[k] = wk;
+    if (w_bf16) w_bf16[k] = f2bf(wk);
+  }
+}
+
+__global__ void add_bf16_kernel(uint16_t* __restrict__ y, const uint16_t* __restrict__ x, int64_t n) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    y[k] = f2bf(bf2f(y[k]) + bf2f(x[k]));
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    y[k] = f2bf(x[k]);
+}
+
+__global__ void to_f32_kernel(const uint16_t* __restrict__ x, float* __restrict__ y, int64_t n) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    y[k] = bf2f(x[k]);
+}
+
+dim3 grid_for(int64_t n, int block = 256) {
+  return dim3(static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>((n + block - 1) / block, 1), 65536)));
+}
+
+// chunks of pixels for a split reduction whose partial rows hold `row` floats,
+// within `part_elems` floats of scratch
+int64_t chunk_pixels(int64_t P, int64_t row, int64_t part_elems, int64_t min_px, int& n_chunks) {
+  int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(part_elems / std::max<int64_t>(row, 1), (P + min_px - 1) / min_px));
+  chunks = std::min<int64_t>(chunks, 65535);
+  int64_t px = (P + chunks - 1) / chunks;
+  n_chunks = static_cast<int>((P + px - 1) / px);
+  return px;
+}
+
+}  // namespace
+}  // namespace chip
+
+using namespace chip;
+
+extern "C" {
+
+int chip_nn_conv3x3(const uint16_t* x1, int c1, const uint16_t* x2, int c2, const uint16_t* wt, const float* bias,
+                    uint16_t* y1, int o1, uint16_t* y2, int n, int h, int w, int cout, int relu, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x1 && wt && y1, "NULL argument");
+  CHIP_CHECK_ARG(c1 >= 1 && c2 >= 0 && (c2 == 0 || x2), "bad input channels");
+  CHIP_CHECK_ARG(cout >= 1 && o1 >= 1 && o1 <= cout && (o1 == cout || y2), "bad output channels");
+  CHIP_CHECK_ARG(n >= 1 && h >= 1 && w >= 1, "bad image shape");
+  CHIP_CHECK_ARG((c1 % 4 != 0 || reinterpret_cast<uintptr_t>(x1) % 8 == 0) &&
+                     (c2 % 4 != 0 || reinterpret_cast<uintptr_t>(x2) % 8 == 0) &&
+                     reinterpret_cast<uintptr_t>(wt) % 8 == 0,
+                 "x1 / x2 / wt must be 8-byte aligned");
+  const int64_t P = static_cast<int64_t>(n) * h * w;
+  dim3 grid(static_cast<unsigned>((P + kPxBlock - 1) / kPxBlock), (cout + kCoBlock - 1) / kCoBlock);
+  CHIP_CHECK_ARG(grid.x <= 2147483647u, "too many pixels");
+  hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1, o1, y2,
+                     Img{n, h, w}, cout, relu);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_conv3x3_wgrad(const uint16_t* dy, const uint16_t* x1, int c1, const uint16_t* x2, int c2, int n, int h,
+                          int w, int cout, float* part, int64_t part_elems, float* dw, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(dy && x1 && part && dw, "NULL argument");
+  CHIP_CHECK_ARG(c1 >= 1 && c2 >= 0 && (c2 == 0 || x2) && cout >= 1, "bad channels");
+  const int cin = c1 + c2;
+  const int64_t P = static_cast<int64_t>(n) * h * w;
+  const int64_t row = 9LL * cout * cin;
+  CHIP_CHECK_ARG(part_elems >= row, "partial-sum scratch smaller than one weight set");
+  int n_chunks = 1;
+  const int64_t px = chunk_pixels(P, row, part_elems, 256, n_chunks);
+  const int n_tiles = 9 * ((cout + 31) / 32) * ((cin + 31) / 32);
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(n_tiles, n_chunks), dim3(256), 0, st, dy, x1, c1, x2, c2,
+                     Img{n, h, w}, cout, px, part);
+  CHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_parts_kernel, grid_for(row), dim3(256), 0, st, part, n_chunks, row, dw);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_bias_grad(const uint16_t* dy, int64_t n_pix, int c, float* part, int64_t part_elems, float* db,
+                      void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(dy && part && db && c >= 1 && n_pix >= 1, "bad argument");
+  CHIP_CHECK_ARG(part_elems >= c, "partial-sum scratch too small");
+  int n_chunks = 1;
+  const int64_t px = chunk_pixels(n_pix, c, part_elems, 1024, n_chunks);
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(channel_sum_kernel, dim3(n_chunks), dim3(256), 0, st, dy, n_pix, c, px, part);
+  CHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_parts_kernel, grid_for(c), dim3(256), 0, st, part, n_chunks, static_cast<int64_t>(c), db);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_flip_weights(const uint16_t* wt, int cout, int cin, uint16_t* wt_t, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(wt && wt_t && cout >= 1 && cin >= 1, "bad argument");
+  hipLaunchKernelGGL(flip_weights_kernel, grid_for(9LL * cout * cin), dim3(256), 0, as_stream(stream), wt, cout, cin,
+                     wt_t);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_relu_mask(uint16_t* d, const uint16_t* y, int64_t count, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(d && y && count >= 0, "bad argument");
+  hipLaunchKernelGGL(relu_mask_kernel, grid_for(count), dim3(256), 0, as_stream(stream), d, y, count);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_avgpool2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c, int backward, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && n >= 1 && c >= 1 && h >= 2 && w >= 2 && h % 2 == 0 && w % 2 == 0, "bad argument");
+  const int64_t count = static_cast<int64_t>(n) * h * w * c;
+  if (backward)
+    hipLaunchKernelGGL(avgpool2_bwd_kernel, grid_for(count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  else
+    hipLaunchKernelGGL(avgpool2_kernel, grid_for(count / 4), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_upsample2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c, int backward, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && n >= 1 && c >= 1 && h >= 1 && w >= 1, "bad argument");
+  const int64_t count = static_cast<int64_t>(n) * h * w * c;
+  if (backward)
+    hipLaunchKernelGGL(upsample2_bwd_kernel, grid_for(count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  else
+    hipLaunchKernelGGL(upsample2_kernel, grid_for(4 * count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_head_fwd(const uint16_t* feat, int c, const float* wb, const float* xin, float* out, int64_t n_pix,
+                     void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(feat && wb && xin && out && c >= 1 && n_pix >= 1, "bad argument");
+  hipLaunchKernelGGL(head_fwd_kernel, grid_for(n_pix), dim3(256), 0, as_stream(stream), feat, c, wb, xin, out, n_pix);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_head_bwd(const float* dout, const uint16_t* feat, int c, const float* wb, uint16_t* dfeat,
+                     float* part, int64_t part_elems, float* dwb, int64_t n_pix, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(dout && feat && wb && dfeat && part && dwb && c >= 1 && n_pix >= 1, "bad argument");
+  CHIP_CHECK_ARG(part_elems >= c + 1, "partial-sum scratch too small");
+  int n_chunks = 1;
+  const int64_t px = chunk_pixels(n_pix, c + 1, part_elems, 4096, n_chunks);
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(head_bwd_kernel, dim3(n_chunks), dim3(256), 0, st, dout, feat, c, wb, dfeat, n_pix, px, part);
+  CHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_parts_kernel, grid_for(c + 1), dim3(256), 0, st, part, n_chunks,
+                     static_cast<int64_t>(c + 1), dwb);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_mse(const float* out, const float* tgt, int64_t n_pix, float* dout, double* part, int part_elems,
+                double* loss, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(out && tgt && dout && part && loss && n_pix >= 1 && part_elems >= 1, "bad argument");
+  const int blocks = static_cast<int>(std::min<int64_t>({static_cast<int64_t>(part_elems), (n_pix + 255) / 256, 1024}));
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(mse_kernel, dim3(blocks), dim3(256), 0, st, out, tgt, n_pix, dout, part);
+  CHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(64), 0, st, part, blocks, 1.0 / static_cast<double>(n_pix),
+                     loss);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
+                 float eps, int step, uint16_t* w_bf16, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(w && g && m && v && n >= 1 && step >= 1, "bad argument");
+  const float c1 = 1.f - powf(beta1, static_cast<float>(step));
+  const float c2 = 1.f - powf(beta2, static_cast<float>(step));
+  hipLaunchKernelGGL(adam_kernel, grid_for(n), dim3(256), 0, as_stream(stream), w, g, m, v, n, lr, beta1, beta2, eps,
+                     c1, c2, w_bf16);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_add_bf16(uint16_t* y, const uint16_t* x, int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && n >= 0, "bad argument");
+  hipLaunchKernelGGL(add_bf16_kernel, grid_for(n), dim3(256), 0, as_stream(stream), y, x, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_to_bf16(const float* x, uint16_t* y, int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && n >= 0, "bad argument");
+  hipLaunchKernelGGL(to_bf16_kernel, grid_for(n), dim3(256), 0, as_stream(stream), x, y, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_nn_to_f32(const uint16_t* x, float* y, int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && n >= 0, "bad argument");
+  hipLaunchKernelGGL(to_f32_kernel, grid_for(n), dim3(256), 0, as_stream(stream), x, y, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+}  // extern "C"
