@@ -177,6 +177,13 @@ int chip_pdhg_dual_kl(float* p, const float* ax, const float* sigma, const float
  * operators.py:796-816 edge padding). */
 int chip_tv_dual(float* q, const float* x, int n, int batch, float lam, void* stream);
 
+/* Dual variable consistent with a primal estimate x (for a warm start; no
+ * reference counterpart: corrct starts the duals at 0): the data term's
+ * gradient at ax = A x, p = (ax + bg) - b (l2) or p = 1 - b / (ax + bg) (kl = 1;
+ * 0 where ax + bg <= 0), times b_mask. */
+int chip_pdhg_dual_init(float* p, const float* ax, const float* b, const float* bg, const float* b_mask, int64_t n,
+                        int kl, void* stream);
+
 /* PDHG primal: upd = atp - div(q) (q may be NULL); x_new = clip(x - tau upd) * mask;
  * xbar = x_new + (x_new - x); x = x_new. tau is elementwise. lower/upper are
  * used when has_lower / has_upper. solvers.py:1175-1190, operators.py:818-844. */

@@ -45,6 +45,7 @@ _SIGNATURES = {
     "chip_precond_fix": [c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, ctypes.c_float, c_void_p],
     "chip_pdhg_dual_l2": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
     "chip_pdhg_dual_kl": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_pdhg_dual_init": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, c_void_p],
     "chip_tv_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, c_void_p],
     "chip_pdhg_primal": [
         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float,

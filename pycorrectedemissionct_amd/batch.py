@@ -319,7 +319,10 @@ class BatchedPDHG:
         self.graph_chunk = graph_chunk
         self.use_graph = use_graph
 
-    def prepare(self, A: BatchedProjectorXRF, b: torch.Tensor, x0=None) -> dict:
+    def prepare(self, A: BatchedProjectorXRF, b: torch.Tensor, x0=None, dual_init: bool = False) -> dict:
+        """Device state of the solve. With ``x0`` and ``dual_init``, the duals start consistent
+        with x0 (data term: its gradient at A x0; TV: the projection of grad x0 onto the
+        lambda-ball) instead of at 0, so a good primal estimate is not first undone."""
         b = _arrays.to_dev(b)[0].reshape(-1).contiguous()
         # TV's tau contribution: 2 * ndims, unweighted for its l12 (an l1-family) norm (regularizers.py:260-273)
         tau_reg = 4.0 if self.tv_lambda is not None else 0.0
@@ -327,7 +330,16 @@ class BatchedPDHG:
         sigma = _precondition(A, A.fp(torch.ones_like(tau)).clone(), A.plan.sino_start, 0.0, self.relaxation)
         x = torch.zeros_like(tau) if x0 is None else _arrays.to_dev(x0)[0].reshape(-1).clone()
         q = None if self.tv_lambda is None else torch.zeros(2 * A.vol_elems, dtype=torch.float32, device=x.device)
-        return dict(b=b, tau=tau, sigma=sigma, x=x, xbar=x.clone(), p=torch.zeros_like(b), q=q,
+        p = torch.zeros_like(b)
+        if x0 is not None and dual_init:
+            st = _native.stream()
+            ax = A.fp(x)
+            _native.call("chip_pdhg_dual_init", p.data_ptr(), ax.data_ptr(), b.data_ptr(), None, None, p.numel(),
+                         int(self.data_term == "kl"), st)
+            if q is not None:
+                for _ in range(8):  # q <- proj(q + grad x / 2) -> lambda grad x / |grad x| where grad x != 0
+                    _native.call("chip_batch_tv_dual", A.plan._h, q.data_ptr(), x.data_ptr(), float(self.tv_lambda), st)
+        return dict(b=b, tau=tau, sigma=sigma, x=x, xbar=x.clone(), p=p, q=q,
                     ax=torch.empty_like(b), upd=torch.empty_like(tau))
 
     def iterate(self, A: BatchedProjectorXRF, s: dict, iterations: int, lower_limit=None, upper_limit=None):
@@ -354,8 +366,8 @@ class BatchedPDHG:
         loop.run(iterations)
         return s
 
-    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None):
-        s = self.prepare(A, b, x0)
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, dual_init: bool = False):
+        s = self.prepare(A, b, x0, dual_init=dual_init)
         self.iterate(A, s, iterations, lower_limit, upper_limit)
         return s["x"]
 

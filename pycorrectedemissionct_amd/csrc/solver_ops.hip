@@ -119,6 +119,24 @@ __global__ __launch_bounds__(kEwBlock) void dual_kl_kernel(float* __restrict__ p
   }
 }
 
+// dual variable consistent with a primal estimate: the data term's gradient at
+// A x (the fixed point of the dual updates above): l2 p = (A x + bg) - b,
+// KL p = 1 - b / (A x + bg) (0 where A x + bg <= 0)
+__global__ __launch_bounds__(kEwBlock) void dual_init_kernel(float* __restrict__ p, const float* __restrict__ ax,
+                                                             const float* __restrict__ b,
+                                                             const float* __restrict__ bg,
+                                                             const float* __restrict__ mask, int64_t n, int kl) {
+  GRID_STRIDE(i, n) {
+    float a = ax[i];
+    if (bg) a = a + bg[i];
+    float r;
+    if (kl) r = a > 0.f ? 1.f - fmaxf(b[i], 0.f) / a : 0.f;
+    else r = a - b[i];
+    if (mask) r = r * mask[i];
+    p[i] = r;
+  }
+}
+
 // ---- TV2D -----------------------------------------------------------------------
 
 // q[b][0] += 0.5 * (x[i+1][j] - x[i][j]); q[b][1] += 0.5 * (x[i][j+1] - x[i][j]);
@@ -293,6 +311,16 @@ int chip_pdhg_dual_kl(float* p, const float* ax, const float* sigma, const float
   CHIP_CHECK_ARG(p && ax && sigma && b, "NULL pointer");
   CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
   hipLaunchKernelGGL(dual_kl_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), p, ax, sigma, b, bg, b_mask, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_pdhg_dual_init(float* p, const float* ax, const float* b, const float* bg, const float* b_mask, int64_t n,
+                        int kl, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(p && ax && b, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  hipLaunchKernelGGL(dual_init_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), p, ax, b, bg, b_mask, n, kl);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

@@ -25,8 +25,6 @@ batched PDHG.
 
 from __future__ import annotations
 
-import ctypes
-
 import numpy as np
 import torch
 
@@ -353,10 +351,11 @@ def warm_start(net: UNet, x_sirt: torch.Tensor) -> torch.Tensor:
     return net.predict((x_sirt / s).contiguous()) * s
 
 
-def iterations_to_tolerance(solver, A, b, x_ref_flat: torch.Tensor, x0, tols, max_its: int, check_every: int = 10):
+def iterations_to_tolerance(solver, A, b, x_ref_flat: torch.Tensor, x0, tols, max_its: int, check_every: int = 10,
+                            dual_init: bool = False):
     """Batched PDHG from x0: per slice, the first multiple of ``check_every`` iterations at which
     ||x - x_ref|| / ||x_ref|| <= tol, for each tol (``max_its + 1`` where not reached)."""
-    st = solver.prepare(A, b, x0=x0)
+    st = solver.prepare(A, b, x0=x0, dual_init=dual_init)
     vs = A.plan.vol_start
     n_sl = len(vs) - 1
     ref_norm = [float(torch.linalg.vector_norm(x_ref_flat[int(vs[i]):int(vs[i + 1])])) for i in range(n_sl)]
@@ -375,3 +374,111 @@ def iterations_to_tolerance(solver, A, b, x_ref_flat: torch.Tensor, x0, tols, ma
             newly = (hit[k] > max_its) & (err <= tol)
             hit[k][newly] = done
     return hit
+
+
+def evaluate(net: UNet, test: dict, tols=(1e-1, 5e-2, 2e-2, 1e-2, 5e-3, 1e-3), max_its: int = 2000,
+             check_every: int = 10) -> dict:
+    """Cold vs warm-started batched PDHG on the held-out chunk(s) of ``test``: iterations to each
+    tolerance against the reference ``test["x_ref"]``, for the warm start alone ("warm") and with
+    duals initialised from it ("warm_dual"), and the slices/s ratios counting the warm start's
+    SIRT-20 + network time."""
+    runs = {"cold": [], "warm": [], "warm_dual": []}
+    t_it = t_sirt = t_net = 0.0
+    n_sl = 0
+    for ch in test["chunks"]:
+        A, b = ch["A"], ch["b"]
+        k = len(A.sizes)
+        x_ref = A.pack_vol(list(test["x_ref"][n_sl:n_sl + k]))
+        solver = _batch.BatchedPDHG(data_term=test["data_term"], tv_lambda=test["tv_lambda"])
+        sirt = _batch.BatchedSIRT()
+        t_sirt += _batch.time_iterations(lambda its: sirt(A, b, its), 20)
+        xs = torch.stack(A.unpack_vol(sirt(A, b, 20)))
+        torch.cuda.synchronize()
+        t_net += _batch.time_iterations(lambda _: warm_start(net, xs), 1)
+        x0 = A.pack_vol(list(warm_start(net, xs)))
+        st = solver.prepare(A, b)
+        t_it += _batch.time_iterations(lambda its: solver.iterate(A, st, its), 200) / 200
+        runs["cold"].append(iterations_to_tolerance(solver, A, b, x_ref, None, tols, max_its, check_every))
+        runs["warm"].append(iterations_to_tolerance(solver, A, b, x_ref, x0, tols, max_its, check_every))
+        runs["warm_dual"].append(iterations_to_tolerance(solver, A, b, x_ref, x0, tols, max_its, check_every,
+                                                         dual_init=True))
+        n_sl += k
+    its = {key: np.concatenate(v, axis=1) for key, v in runs.items()}
+    n_ch = len(test["chunks"])
+    res = {"slices": n_sl, "ms_per_iteration_batch": 1e3 * t_it / n_ch, "ms_sirt20_batch": 1e3 * t_sirt / n_ch,
+           "ms_unet_batch": 1e3 * t_net / n_ch, "max_its": max_its, "per_tol": []}
+    for i, tol in enumerate(tols):
+        row = {"tol": tol}
+        c_med = float(np.median(its["cold"][i]))
+        row["cold_median_its"] = c_med
+        row["cold_reached"] = int(np.sum(its["cold"][i] <= max_its))
+        for key in ("warm", "warm_dual"):
+            w_med = float(np.median(its[key][i]))
+            row[f"{key}_median_its"] = w_med
+            row[f"{key}_reached"] = int(np.sum(its[key][i] <= max_its))
+            ok = c_med <= max_its and w_med <= max_its
+            row[f"{key}_speedup"] = (c_med * t_it) / (w_med * t_it + t_sirt + t_net) if ok else None
+        res["per_tol"].append(row)
+    return res
+
+
+def main() -> int:
+    import argparse
+    import json
+    import sys
+    import time
+
+    ap = argparse.ArgumentParser(description="Train the warm-start U-Net and evaluate it against plain PDHG.")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--angles", type=int, default=360)
+    ap.add_argument("--train", type=int, default=384)
+    ap.add_argument("--test", type=int, default=32)
+    ap.add_argument("--pdhg-its", type=int, default=2000, help="PDHG iterations of the training targets")
+    ap.add_argument("--ref-its", type=int, default=6000, help="PDHG iterations of the held-out references")
+    ap.add_argument("--eval-its", type=int, default=4000, help="iteration cap of the cold / warm runs")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--load", default=None, help="trained weights (.npz) instead of training")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--save", default=None)
+    args = ap.parse_args()
+
+    def log(msg):
+        print(f"[warmstart] {msg}", file=sys.stderr, flush=True)
+
+    kw = dict(n=args.n, n_angles=args.angles, batch=32)
+    t0 = time.perf_counter()
+    losses = [float("nan")]
+    if args.load:
+        net = UNet.load(args.load)
+        t1 = t2 = time.perf_counter()
+    else:
+        parts = []
+        for i0 in range(0, args.train, 32):
+            parts.append(make_pairs(min(32, args.train - i0), seed=1000 + i0, pdhg_its=args.pdhg_its, **kw))
+            log(f"training pairs {min(i0 + 32, args.train)}/{args.train} ({time.perf_counter() - t0:.0f} s)")
+        x_in = torch.cat([p["x_sirt"] for p in parts])
+        x_tgt = torch.cat([p["x_ref"] for p in parts])
+        del parts
+        t1 = time.perf_counter()
+        net = UNet(args.levels, args.base, seed=0)
+        losses = train(net, x_in, x_tgt, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if args.save:
+            net.save(args.save)
+    test = make_pairs(args.test, seed=99991, pdhg_its=args.ref_its, **kw)
+    log(f"evaluating cold vs warm start against {args.ref_its}-iteration references")
+    res = evaluate(net, test, max_its=args.eval_its)
+    res.update({"n": args.n, "angles": args.angles, "train_pairs": args.train, "test_slices": args.test,
+                "train_target_its": args.pdhg_its, "test_reference_its": args.ref_its, "epochs": args.epochs,
+                "params": net.n_params, "loss_first_last": [losses[0], losses[-1]], "s_pairs": t1 - t0,
+                "s_training": t2 - t1})
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
