@@ -12,6 +12,7 @@ GPU projector's explicit matrix.
                            (``regularizers.py:134-162, 260-273``; ``data_terms.py:549-555, 607-608``)
 * ``sirt``              -- ``solvers.py:867-984``
 * ``mlem``              -- ``solvers.py:688-805``
+* ``sart``              -- ``solvers.py:473-631`` (per-angle callables; explicit row order)
 * ``pdhg``              -- ``solvers.py:1060-1206`` with ``DataFidelity_l2``
                            (``data_terms.py:283-331``) or ``DataFidelity_KL``
                            (``data_terms.py:677-692``)
@@ -92,6 +93,40 @@ def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=
             x = x.clip(lower, upper)
         if x_mask is not None:
             x = x * x_mask
+    return x
+
+
+def sart(A_angle, At_angle, b, x_shape, n_angles, iterations, rows, x0=None, lower=None, upper=None, x_mask=None,
+         b_mask=None, relaxation=1.0):
+    """SART: per-angle diagonal scalings, then for each angle a in ``rows``:
+    x -= At_a((A_a x - b_a) * mask_a * sigma_a) * tau_a, clipped and masked (``solvers.py:565-629``).
+    b is [..., A, N]; A_angle(x, a) -> [..., N]; At_angle(line, a) -> x-shaped."""
+    b = np.asarray(b, dtype=F32)
+    b_ones = np.ones_like(b)
+    if b_mask is not None:
+        b_ones = b_ones * b_mask
+    tau = np.abs(np.stack([At_angle(b_ones[..., a, :], a) for a in range(n_angles)]))  # [A, *x_shape]
+    tau[(tau / np.max(tau)) < 1e-5] = 1
+    tau = (F32(relaxation) / tau).astype(F32)
+    x_ones = np.ones(x_shape, dtype=F32)
+    if x_mask is not None:
+        x_ones = x_ones * x_mask
+    sigma = np.abs(np.stack([A_angle(x_ones, a) for a in range(n_angles)], axis=-2))  # [..., A, N]
+    sigma[(sigma / np.max(sigma)) < 1e-5] = 1
+    sigma = (1 / sigma).astype(F32)
+    x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
+    for _ in range(iterations):
+        for a in rows:
+            res = A_angle(x, a) - b[..., a, :]
+            if b_mask is not None:
+                res = res * b_mask[..., a, :]
+            x = x - At_angle(res * sigma[..., a, :], a) * tau[a]
+            if lower is not None:
+                x = np.fmax(x, F32(lower))
+            if upper is not None:
+                x = np.fmin(x, F32(upper))
+            if x_mask is not None:
+                x = x * x_mask
     return x
 
 

@@ -188,6 +188,92 @@ class Solver:
             info.residual0_cv = self.data_term_test.compute_residual_norm(res_test_0)
 
 
+class SART(Solver):
+    """SART (``solvers.py:473-631``): one angle at a time, on the per-angle projector entry points.
+
+    ``rows_sequence`` (an extension) fixes the angle order; by default it is
+    ``np.random.permutation(A_num_rows)`` drawn once per call, as in the reference.
+    """
+
+    def compute_residual(self, A, b, x, A_num_rows: int, b_mask):
+        """Stack of per-angle residuals A_a x - b_a (``solvers.py:476-510``)."""
+        fp = torch.stack([_arrays.to_dev(A(x, ii))[0] for ii in range(A_num_rows)], dim=-2)
+        res = fp - _arrays.to_dev(b)[0]
+        if b_mask is not None:
+            res = res * _arrays.to_dev(b_mask)[0]
+        return res
+
+    def __call__(self, A, b, iterations: int, A_num_rows: int | None = None, At=None, x0=None, lower_limit=None,
+                 upper_limit=None, x_mask=None, b_mask=None, rows_sequence=None):
+        from . import projectors  # local: projectors imports this module's siblings
+
+        if isinstance(A, projectors.ProjectorUncorrected):
+            p = A
+            if not p.projector_backend.has_individual_projs:
+                raise ValueError("The projector needs to have enabled single projections.")
+            A = lambda x, ii: p.fp_angle(x, ii)  # noqa: E731
+            if isinstance(p, projectors.ProjectorAttenuationXRF):
+                At = lambda y, ii: p.bp_angle(y, ii, single_line=True)  # noqa: E731
+            else:
+                At = lambda y, ii: p.bp_angle(y, ii)  # noqa: E731
+            A_num_rows = len(p.angles_rot_rad)
+        elif At is None:
+            raise ValueError("Parameter `At` is required, if `A` is not a projector.")
+        elif A_num_rows is None:
+            raise ValueError("Parameter `A_num_rows` is required, if `A` is not a projector.")
+        st = _native.stream()
+        b_t, host = _arrays.to_dev(b)
+        b_t = b_t.contiguous()
+        # per-angle rows as contiguous [A, ..., N] blocks
+        b_rows = b_t.movedim(-2, 0).contiguous()
+        mask_rows = None if b_mask is None else _arrays.to_dev(b_mask)[0].movedim(-2, 0).contiguous()
+        ones = torch.ones_like(b_rows[0])
+        tau = torch.stack([_arrays.to_dev(At(ones if mask_rows is None else ones * mask_rows[ii], ii))[0]
+                           for ii in range(A_num_rows)]).contiguous()
+        mx = torch.empty(1, dtype=torch.float32, device=tau.device)
+        _native.call("chip_absmax", tau.data_ptr(), tau.numel(), mx.data_ptr(), st)
+        _native.call("chip_precond_fix", tau.data_ptr(), tau.numel(), 0.0, mx.data_ptr(), float(self.relaxation), st)
+        x_shape = tuple(tau.shape[1:])
+        x_ones = torch.ones(x_shape, dtype=torch.float32, device=tau.device)
+        x_mask_t = None
+        if x_mask is not None:
+            x_mask_t = _arrays.to_dev(x_mask)[0].contiguous()
+            x_ones = x_ones * x_mask_t
+        sigma = torch.stack([_arrays.to_dev(A(x_ones, ii))[0] for ii in range(A_num_rows)]).contiguous()
+        _native.call("chip_absmax", sigma.data_ptr(), sigma.numel(), mx.data_ptr(), st)
+        _native.call("chip_precond_fix", sigma.data_ptr(), sigma.numel(), 0.0, mx.data_ptr(), 1.0, st)
+        x = torch.zeros(x_shape, dtype=torch.float32, device=tau.device) if x0 is None else \
+            _arrays.to_dev(x0)[0].clone().contiguous()
+        info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
+        if self.tolerance is not None:
+            info.residual0 = float(torch.linalg.vector_norm(self.compute_residual(A, b_t, x, A_num_rows, b_mask)))
+        rows = np.random.permutation(A_num_rows) if rows_sequence is None else np.asarray(rows_sequence)
+        hl, lo = _scalar_limit(lower_limit)
+        hu, hi = _scalar_limit(upper_limit)
+        n = int(x.shape[-1])
+        batch = x.numel() // (n * n)
+        res = torch.empty_like(b_rows[0])
+        for ii in range(iterations):
+            info.iterations += 1
+            for a in rows:
+                a = int(a)
+                line = _arrays.to_dev(A(x, a))[0].contiguous()
+                # r = (b_a - A_a x) * mask_a * sigma_a = -(A_a x - b_a) * mask_a * sigma_a
+                _native.call("chip_sirt_residual", res.data_ptr(), b_rows[a].data_ptr(), line.data_ptr(), None,
+                             _native.ptr(None if mask_rows is None else mask_rows[a]), sigma[a].data_ptr(),
+                             res.numel(), st)
+                upd = _arrays.to_dev(At(res, a))[0].contiguous()
+                # x = clip(x - At_a(res) tau_a) * mask, with res of the opposite sign
+                _native.call("chip_sirt_update", x.data_ptr(), upd.data_ptr(), None, tau[a].data_ptr(), hl, lo, hu,
+                             hi, _native.ptr(x_mask_t), n, batch, st)
+            if self.tolerance is not None:
+                info.residuals[ii] = float(torch.linalg.vector_norm(
+                    self.compute_residual(A, b_t, x, A_num_rows, b_mask)))
+                if self.tolerance > info.residuals[ii]:
+                    break
+        return _arrays.from_dev(x, host), info
+
+
 class MLEM(Solver):
     """MLEM (``solvers.py:634-805``)."""
 

@@ -116,3 +116,34 @@ def test_algo_flat_tv(pkg):
     for algo, its, kw in ((pkg.SIRT, 2500, {}), (pkg.PDHG, 500, {}), (pkg.PDHG, 500, dict(lower_limit=0, upper_limit=1))):
         sol, _ = algo(regularizer=pkg.regularizers.Regularizer_TV2D(1e-4))(prj, data, iterations=its, **kw)
         assert np.all(np.isclose(sol, vol, atol=1e-2)), algo
+
+
+def test_sart_parity_xrf_and_uncorrected(pkg):
+    """SART (``solvers.py:473-631``) with a fixed angle order, 3 sweeps, against the oracle:
+    two-detector XRF projector (weights on the per-angle bp, maps on the fp) with a lower
+    limit, and the uncorrected projector."""
+    n, a = 32, 12
+    rng = np.random.default_rng(800)
+    ang = np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+    maps = rng.uniform(0.3, 1.0, (a, 2, n, n)).astype(np.float32)
+    w_det = np.array([1.0, 0.7])
+    f = pkg.testing.phantom_ellipses(n, 4, rng)
+    rows = rng.permutation(a)
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=[np.pi / 2, -np.pi / 2],
+                                      weights_detectors=w_det)
+    b = oxrf.fp(f, ang, maps, w_det=w_det)
+    x, info = pkg.SART()(prj, b, 3, rows_sequence=rows, lower_limit=0.0)
+    ref = osol.sart(lambda v, i: oxrf.fp(v, ang[i:i + 1], maps[i:i + 1], w_det=w_det)[:, 0, :],
+                    lambda y, i: oxrf.bp_skimage(y.reshape(2, 1, n), ang[i:i + 1], n, w_det=w_det, symmetric=True),
+                    b, (n, n), a, 3, rows, lower=0.0)
+    e = rel_max(ref, x)
+    print(f"SART xrf: rel_max vs oracle {e:.2e}; error vs phantom {rel_max(f, x):.2e}")
+    assert e < 1e-4 and info.iterations == 3
+    pu = pkg.ProjectorUncorrected([n, n], ang, create_single_projs=True)
+    bu = oxrf.fp(f, ang)[0]
+    xu, _ = pkg.SART()(pu, bu, 3, rows_sequence=rows)
+    refu = osol.sart(lambda v, i: oxrf.fp(v, ang[i:i + 1])[0, 0], lambda y, i: oxrf.bp_skimage(y.reshape(1, 1, n),
+                     ang[i:i + 1], n), bu, (n, n), a, 3, rows)
+    assert rel_max(refu, xu) < 1e-4
+    with pytest.raises(ValueError):
+        pkg.SART()(pkg.ProjectorUncorrected([n, n], ang), bu, 1)

@@ -294,3 +294,29 @@ def test_kl_prox_closed_form_minimises():
         res = minimize_scalar(lambda t: (t - v) ** 2 / (2 * s) - b * math.log(1 - t), bounds=(-50, 1 - 1e-12), method="bounded",
                               options={"xatol": 1e-12})
         assert abs(res.x - p) < 1e-6
+
+
+def test_oracle_sart_solves_consistent_system():
+    """SART (oracle) on a consistent non-negative system split into 8 'angles' of 12 rows:
+    converges to the true solution; a single 'angle' reduces to one preconditioned
+    Landweber step x - tau A^T(sigma (A x - b))."""
+    from oracle import solvers as osol
+
+    rng = np.random.default_rng(77)
+    n_ang, rows_per, nx = 8, 12, 30
+    M = rng.random((n_ang, rows_per, nx)).astype(np.float32)
+    x_true = rng.random(nx).astype(np.float32)
+    b = np.einsum("arx,x->ar", M, x_true).astype(np.float32)  # [A, N] layout
+
+    def A_angle(x, a):
+        return (M[a] @ x).astype(np.float32)
+
+    def At_angle(y, a):
+        return (M[a].T @ y).astype(np.float32)
+
+    x = osol.sart(A_angle, At_angle, b, (nx,), n_ang, 400, rng.permutation(n_ang))
+    assert np.max(np.abs(x - x_true)) < 1e-3
+    x1 = osol.sart(lambda x, a: M[0] @ x, lambda y, a: M[0].T @ y, b[:1], (nx,), 1, 1, [0])
+    tau = 1 / np.abs(M[0].T @ np.ones(rows_per))
+    sigma = 1 / np.abs(M[0] @ np.ones(nx))
+    assert np.allclose(x1, -tau * (M[0].T @ (sigma * (0 - b[0]))), rtol=1e-5)
