@@ -13,6 +13,8 @@ GPU projector's explicit matrix.
 * ``sirt``              -- ``solvers.py:867-984``
 * ``mlem``              -- ``solvers.py:688-805``
 * ``sart``              -- ``solvers.py:473-631`` (per-angle callables; explicit row order)
+* ``power_method``      -- ``solvers.py:26-57`` (explicit start vector)
+* ``fista``             -- ``solvers.py:1348-1513`` (l2 data term, optional l1 prox)
 * ``pdhg``              -- ``solvers.py:1060-1206`` with ``DataFidelity_l2``
                            (``data_terms.py:283-331``) or ``DataFidelity_KL``
                            (``data_terms.py:677-692``)
@@ -127,6 +129,56 @@ def sart(A_angle, At_angle, b, x_shape, n_angles, iterations, rows, x0=None, low
                 x = np.fmin(x, F32(upper))
             if x_mask is not None:
                 x = x * x_mask
+    return x
+
+
+def power_method(A, At, x_start, iterations: int = 5) -> float:
+    """||A||_2 by power iterations on A^T A from the data-space vector x_start."""
+    x = np.asarray(x_start, dtype=np.float64)
+    x = x / np.linalg.norm(x)
+    x = At(x)
+    x_norm = np.linalg.norm(x)
+    L = x_norm
+    for _ in range(iterations):
+        x = x / x_norm
+        x = At(A(x))
+        x_norm = np.linalg.norm(x)
+        L = np.sqrt(x_norm)
+    return float(L)
+
+
+def fista(A, At, b, x_shape, iterations, precondition=True, L=None, relaxation=1.0, l1=0.0, lower=None, upper=None,
+          x_mask=None, b_mask=None, restart_period=None):
+    """FISTA: y-gradient step with diagonal (SIRT-style) or 1/L scaling, soft threshold at
+    tau * l1, clip, mask, Beck-Teboulle momentum (``solvers.py:1391-1490``)."""
+    b = np.asarray(b, dtype=F32)
+    if precondition:
+        sigma, tau = diagonal_scaling(A, At, b, x_shape, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
+    else:
+        sigma, tau = F32(1.0 / L), F32(relaxation / L)
+    x = np.zeros(x_shape, dtype=F32)
+    if x_mask is not None:
+        x = x * x_mask
+    y = x.copy()
+    t = 1.0
+    for ii in range(iterations):
+        res = b - A(y)
+        if b_mask is not None:
+            res = res * b_mask
+        xn = y + tau * At(res * sigma)
+        if l1 > 0:
+            thr = tau * F32(l1)
+            xn = np.sign(xn) * np.fmax(np.abs(xn) - thr, 0)
+        if lower is not None or upper is not None:
+            xn = xn.clip(lower, upper)
+        if x_mask is not None:
+            xn = xn * x_mask
+        t_new = (1.0 + np.sqrt(1.0 + 4.0 * t * t)) / 2.0
+        if restart_period is not None and (ii + 1) % restart_period == 0:
+            t_new = 1.0
+        y = xn + F32((t - 1.0) / t_new) * (xn - x)
+        x = xn
+        t = t_new
     return x
 
 

@@ -14,4 +14,4 @@ from . import batch, models, operators, projectors, solvers, data_terms, regular
 from ._projector_backends import ProjectorBackend, ProjectorBackendHIP, compute_attenuation  # noqa: F401
 from .physics.attenuation import AttenuationVolume  # noqa: F401
 from .projectors import ProjectorAttenuationXRF, ProjectorMatrix, ProjectorUncorrected  # noqa: F401
-from .solvers import MLEM, PDHG, SART, SIRT, SolutionInfo  # noqa: F401
+from .solvers import FISTA, MLEM, PDHG, SART, SIRT, SolutionInfo  # noqa: F401

@@ -56,6 +56,10 @@ _SIGNATURES = {
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
         ctypes.c_int, ctypes.c_int, c_void_p,
     ],
+    "chip_fista_update": [
+        c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+        ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p,
+    ],
     "chip_mlem_ratio": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
     "chip_mlem_update": [
         c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,

@@ -270,6 +270,11 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
     plan->b.fp_wg_per_cu = value;
     return CHIP_OK;
   }
+  if (k == "fp_trim") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_trim must be 0 or 1");
+    plan->b.fp_trim = value;
+    return CHIP_OK;
+  }
   if (k == "bp_wg_per_cu") {
     CHIP_CHECK_ARG(value >= 1 && value <= 16, "bp_wg_per_cu must be in 1..16");
     plan->b.bp_wg_per_cu = value;

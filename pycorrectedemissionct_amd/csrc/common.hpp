@@ -71,8 +71,9 @@ struct Plan {
   size_t otf_budget = size_t(1) << 30;
   int device = 0;
   // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged,
-  // 2 = LDS-staged with register prefetch, 3 = prefetched and lean
-  // (2 and 3 fall back to 1 when N % 4 != 0)
+  // 2 = LDS-staged with register prefetch, 3 = streamed per-detector tiles,
+  // 4 = 3 staging only the footprint of interior chunks
+  // (2..4 fall back to 1 when N % 4 != 0)
   int fp_variant = 3;
 };
 
@@ -96,6 +97,7 @@ struct BatchPlan {
   int n_cu = 256;
   int fp_wg_per_cu = 2;  // persistent grid of the fp: fp_wg_per_cu x CUs workgroups
   int bp_wg_per_cu = 4;  // same for the bp
+  int fp_trim = 0;       // stage only the footprint of interior fp chunks
   std::vector<chip_plan*> size_plans;  // one single-slice plan per distinct size
   std::vector<BatchSlice> h_slices;
   // prefix sums (n_slices + 1 entries): fp items, adjoint-bp tiles, skimage-bp

@@ -36,7 +36,7 @@ constexpr int kRows = 32;         // sample rows per chunk
 constexpr int kRowsPerGroup = 8;  // rows per wave per chunk
 constexpr int kBoxRows = 76;      // >= 63|s| + 31|c| + 3
 constexpr int kSlots = 20;        // float4 slots per box row (<= 76 + 3 alignment columns)
-constexpr int kPitch = 81;        // odd pitch: row-strided gathers spread over banks
+constexpr int kPitch = 84;        // 16-byte aligned rows: one ds_write_b128 per staged slot
 constexpr int kTile = kBoxRows * kPitch;
 constexpr int kPerThread = (kBoxRows * kSlots + 255) / 256;  // 6
 
@@ -44,6 +44,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 struct Box {
   int clo, chi, rlo, rhi;  // inclusive; clo is a multiple of 4 (or -4)
+  int ylo, yhi;            // sample rows of the chunk
   int all_valid;           // every sample of the chunk lies in (-1, N)^2
   int empty;               // the chunk misses the slice
   int border;              // the box reaches outside the slice
@@ -70,6 +71,8 @@ __device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, fl
   const float ymin = fminf(fminf(y0, y1), fminf(y2, y3));
   const float ymax = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
   Box bx;
+  bx.ylo = yc;
+  bx.yhi = y_last;
   bx.all_valid = (xmin > -1.f && xmax < fn && ymin > -1.f && ymax < fn) ? 1 : 0;
   const int clo = max(static_cast<int>(floorf(fmaxf(xmin, -2.f))), -1);
   bx.chi = min(static_cast<int>(floorf(fminf(xmax, fn + 1.f))) + 1, n);
@@ -98,7 +101,13 @@ struct FpItem {
   int u0;
 };
 
-template <int D, bool MAPS>
+// TRIM: interior chunks stage only the slots whose centre lies within 3.5
+// pixels (in the rotated frame of the samples) of the chunk's sample
+// rectangle -- a tapped pixel lies within sqrt(2) of a sample and a slot
+// centre within 1.5 of its pixels -- instead of the whole axis-aligned box,
+// which at 45 degrees is twice the footprint. Border chunks (clamped samples
+// read the zero padding) keep the whole box.
+template <int D, bool MAPS, bool TRIM>
 __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   const int n = it.n;
   const int u0 = it.u0;
@@ -119,6 +128,7 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   const int u_last = min(u0 + kBins, n) - 1;
   const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
   const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+  const float uf_lo = static_cast<float>(u0) - 3.5f, uf_hi = static_cast<float>(u_last) + 3.5f;
 
   // staging slots of this thread: box row, first column, offset from the box origin
   int s_row[kPerThread], s_col[kPerThread], s_off[kPerThread];
@@ -147,6 +157,13 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
         const int c = bx.clo + s_col[i];
         in = in && static_cast<unsigned>(r) < static_cast<unsigned>(n) &&
              static_cast<unsigned>(c) < static_cast<unsigned>(n);
+      } else if (TRIM) {
+        const float px = static_cast<float>(bx.clo + s_col[i]) + (1.5f - m02);
+        const float py = static_cast<float>(bx.rlo + s_row[i]) - m12;
+        const float us = cs * px - sn * py;  // the slot centre in sample coordinates
+        const float ysm = sn * px + cs * py;
+        in = in && us >= uf_lo && us <= uf_hi && ysm >= static_cast<float>(bx.ylo) - 3.5f &&
+             ysm <= static_cast<float>(bx.yhi) + 3.5f;
       }
       if (in) {
         inmask |= 1u << i;
@@ -171,21 +188,16 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
       if (inmask & (1u << i)) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          float* t = lds + d * kTile + o;
-          t[0] = MAPS ? fr[i].x * mr[d][i].x : fr[i].x;
-          t[1] = MAPS ? fr[i].y * mr[d][i].y : fr[i].y;
-          t[2] = MAPS ? fr[i].z * mr[d][i].z : fr[i].z;
-          t[3] = MAPS ? fr[i].w * mr[d][i].w : fr[i].w;
+          float4 v4;
+          v4.x = MAPS ? fr[i].x * mr[d][i].x : fr[i].x;
+          v4.y = MAPS ? fr[i].y * mr[d][i].y : fr[i].y;
+          v4.z = MAPS ? fr[i].z * mr[d][i].z : fr[i].z;
+          v4.w = MAPS ? fr[i].w * mr[d][i].w : fr[i].w;
+          *reinterpret_cast<float4*>(lds + d * kTile + o) = v4;
         }
       } else if (bx.border && s_row[i] < th && s_col[i] < cw) {  // zero padding outside the slice
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          float* t = lds + d * kTile + o;
-          t[0] = 0.f;
-          t[1] = 0.f;
-          t[2] = 0.f;
-          t[3] = 0.f;
-        }
+        for (int d = 0; d < D; ++d) *reinterpret_cast<float4*>(lds + d * kTile + o) = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
   };
@@ -270,7 +282,7 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   }
 }
 
-template <int D, bool MAPS>
+template <int D, bool MAPS, bool TRIM>
 __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
                                                         int64_t maps_slice_stride, float* __restrict__ sino,
                                                         const float4* __restrict__ geom, const float* __restrict__ w,
@@ -293,14 +305,14 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
   it.out = sino + ((static_cast<size_t>(b) * n_det_total + det0) * n_angles + a) * n;
   it.det_stride = static_cast<size_t>(n_angles) * n;
   it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-  fp_stream_block<D, MAPS>(it, lds);
+  fp_stream_block<D, MAPS, TRIM>(it, lds);
 }
 
 // Persistent form over a batch of slices of different sizes: work item k of
 // the batch is (slice s, angle a, bin block) with s found in the item prefix
 // sums; workgroup p takes items r * G + logical_block(p, G), r = 0, 1, ...
 // (G workgroups), so each XCD walks a contiguous run of bin blocks.
-template <int D, bool MAPS>
+template <int D, bool MAPS, bool TRIM>
 __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
                                                               const int64_t* __restrict__ item_start,
                                                               const float* __restrict__ vol,
@@ -327,7 +339,7 @@ __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* 
     it.out = sino + sl.sino_off + (static_cast<size_t>(det0) * n_angles + a) * sl.n;
     it.det_stride = static_cast<size_t>(n_angles) * sl.n;
     it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-    fp_stream_block<D, MAPS>(it, lds);
+    fp_stream_block<D, MAPS, TRIM>(it, lds);
   }
 }
 
@@ -337,8 +349,13 @@ void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, 
   const int n_ublk = (p.n + kBins - 1) / kBins;
   dim3 grid(n_ublk * n_ang, batch);
   const size_t shmem = sizeof(float) * D * kTile;
-  hipLaunchKernelGGL((fp_stream_kernel<D, MAPS>), grid, dim3(256), shmem, st, vol, maps, stride, sino, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+  if (p.fp_variant == 4) {
+    hipLaunchKernelGGL((fp_stream_kernel<D, MAPS, true>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
+                       p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+  } else {
+    hipLaunchKernelGGL((fp_stream_kernel<D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
+                       p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+  }
 }
 
 template <int D, bool MAPS>
@@ -347,8 +364,15 @@ void launch_batch(const BatchPlan& bp, const float* vol, const float* maps, floa
   const int64_t want = static_cast<int64_t>(bp.fp_wg_per_cu) * bp.n_cu;
   const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
   const size_t shmem = sizeof(float) * D * kTile;
-  hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS>), dim3(grid), dim3(256), shmem, st, bp.d_slices, bp.n_slices,
-                     bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det, det0);
+  if (bp.fp_trim) {
+    hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS, true>), dim3(grid), dim3(256), shmem, st, bp.d_slices,
+                       bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det,
+                       det0);
+  } else {
+    hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS, false>), dim3(grid), dim3(256), shmem, st, bp.d_slices,
+                       bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det,
+                       det0);
+  }
 }
 
 }  // namespace

@@ -543,7 +543,7 @@ static void launch_fp(const Plan& p, const float* vol, const float* maps, int64_
     // two detectors per pass keep the staged tiles within 64 KiB of LDS
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const bool two = p.n_det - det0 >= 2;
-      if (variant == 3) {
+      if (variant >= 3) {
         launch_fp_stream(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, 0, p.n_angles, 0, st);
       } else if (variant == 2) {
         if (two) launch_fp_lds2_group<2>(p, vol, maps, stride, sino, batch, det0, st);
@@ -779,7 +779,7 @@ void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* 
                      int maps_a0, hipStream_t st) {
   const bool vec_ok = (p.n % 4 == 0) && (reinterpret_cast<uintptr_t>(vol) % 16 == 0) &&
                       (reinterpret_cast<uintptr_t>(maps) % 16 == 0);
-  if (vec_ok && p.fp_variant == 3) {
+  if (vec_ok && p.fp_variant >= 3) {
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const int nd = p.n_det - det0 >= 2 ? 2 : 1;
       launch_fp_stream(p, vol, maps, 0, sino, 1, det0, nd, angle0, n_ang, maps_a0, st);
@@ -859,7 +859,7 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 3, "fp_kernel must be in 0..3");
+    CHIP_CHECK_ARG(value >= 0 && value <= 4, "fp_kernel must be in 0..4");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }

@@ -241,6 +241,31 @@ __global__ __launch_bounds__(kEwBlock) void sirt_update_kernel(float* __restrict
   }
 }
 
+// ---- FISTA ----------------------------------------------------------------------
+
+// x_new = y + tau * grad; optional l1 soft threshold at tau * l1w; clip; mask;
+// y = x_new + momentum * (x_new - x); x = x_new   (tau per element or scalar)
+__global__ __launch_bounds__(kEwBlock) void fista_update_kernel(float* __restrict__ x, float* __restrict__ y,
+                                                                const float* __restrict__ grad,
+                                                                const float* __restrict__ tau, float tau_s, float l1w,
+                                                                int hl, float lo, int hu, float hi,
+                                                                const float* __restrict__ mask, float momentum,
+                                                                int64_t n) {
+  GRID_STRIDE(i, n) {
+    const float t = tau ? tau[i] : tau_s;
+    float xn = y[i] + t * grad[i];
+    if (l1w > 0.f) {
+      const float thr = t * l1w;
+      const float ax = fabsf(xn) - thr;
+      xn = ax > 0.f ? copysignf(ax, xn) : 0.f;
+    }
+    xn = clipf(xn, hl, lo, hu, hi);
+    if (mask) xn = xn * mask[i];
+    y[i] = xn + momentum * (xn - x[i]);
+    x[i] = xn;
+  }
+}
+
 // ---- MLEM -----------------------------------------------------------------------
 
 __global__ __launch_bounds__(kEwBlock) void mlem_ratio_kernel(float* __restrict__ r, const float* __restrict__ b,
@@ -367,6 +392,19 @@ int chip_sirt_update(float* x, const float* upd, const float* q, const float* ta
   const int64_t total = static_cast<int64_t>(n) * n * batch;
   hipLaunchKernelGGL(sirt_update_kernel, ew_grid(total), dim3(kEwBlock), 0, as_stream(stream), x, upd, q, tau,
                      has_lower, lower, has_upper, upper, x_mask, n, batch);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_fista_update(float* x, float* y, const float* grad, const float* tau, float tau_scalar, float l1_weight,
+                      int has_lower, float lower, int has_upper, float upper, const float* x_mask, float momentum,
+                      int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x && y && grad, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  CHIP_CHECK_ARG(l1_weight >= 0.f, "l1_weight must be >= 0");
+  hipLaunchKernelGGL(fista_update_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), x, y, grad, tau,
+                     tau_scalar, l1_weight, has_lower, lower, has_upper, upper, x_mask, momentum, n);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

@@ -1,5 +1,9 @@
 """Regularizers used by the solvers (corrct ``regularizers.py`` subset).
 
+``Regularizer_l1`` (``regularizers.py:531-585``): sparsity, identity transform;
+its primal proximal (soft thresholding at tau * weight) is fused into
+``chip_fista_update``.
+
 ``Regularizer_TV2D`` (``regularizers.py:308-321``) over ``Regularizer_Grad``
 (``regularizers.py:218-289``): isotropic total variation with the l12 norm,
 edge-padded forward gradient, ``sigma = 1/2`` and ``tau = 2 * ndims``. Its dual
@@ -84,3 +88,24 @@ class Regularizer_TV2D(Regularizer_Grad):
 
     def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
         super().__init__(weight=weight, ndims=2, axes=axes, pad_mode=pad_mode, norm=norm, upd_mask=upd_mask)
+
+
+class Regularizer_l1(BaseRegularizer):
+    """l1-norm regularizer: promotes sparse reconstructions (``regularizers.py:531-585``)."""
+
+    __reg_name__ = "l1"
+
+    def __init__(self, weight, upd_mask=None, norm=None):
+        super().__init__(weight=weight, norm=dt.DataFidelity_l1() if norm is None else norm, upd_mask=upd_mask)
+
+    def initialize_sigma_tau(self, primal) -> float:
+        self.op = None  # identity transform
+        return 1.0
+
+    def primal_prox_weight(self) -> float:
+        """The weight of the fused soft threshold (prox of tau * weight * ||.||_1)."""
+        if self.upd_mask is not None:
+            raise NotImplementedError("an update mask is not supported by the fused l1 proximal")
+        if np.ndim(self.weight) != 0:
+            raise NotImplementedError("the fused l1 proximal takes a scalar weight")
+        return float(self.weight)

@@ -10,6 +10,9 @@ asks for residual norms.
 * ``compute_diagonal_scaling`` -- ``solvers.py:60-117``
 * ``SIRT``  -- ``solvers.py:808-984``
 * ``MLEM``  -- ``solvers.py:634-805``
+* ``SART``  -- ``solvers.py:473-631``
+* ``FISTA`` -- ``solvers.py:1209-1513`` (l2 data term, optional l1 prox;
+  ``power_method`` / ``compute_Lipschitz_scaling``: ``solvers.py:26-158``)
 * ``PDHG``  -- ``solvers.py:987-1206`` (preconditioned; l2 or KL data term,
   optional TV2D regularizer)
 
@@ -70,6 +73,33 @@ def compute_diagonal_scaling(A_abs, At_abs, b: torch.Tensor, regs: Sequence, rel
     _native.call("chip_absmax", sigma.data_ptr(), sigma.numel(), mx.data_ptr(), st)
     _native.call("chip_precond_fix", sigma.data_ptr(), sigma.numel(), 0.0, mx.data_ptr(), float(relaxation_sigma), st)
     return sigma, tau, tuple(tau.shape), tau.dtype
+
+
+def power_method(A, b, iterations: int = 5):
+    """l2-norm of A by the power method (``solvers.py:26-57``); the start vector is
+    ``np.random.rand`` of b's shape, as in the reference. Returns (L, x_shape, x_dtype)."""
+    b_t = _arrays.to_dev(b)[0]
+    x = torch.as_tensor(np.random.rand(*b_t.shape).astype(np.float32), device=b_t.device)
+    x = x / torch.linalg.vector_norm(x)
+    x = _arrays.to_dev(A.T(x))[0]
+    x_norm = torch.linalg.vector_norm(x)
+    L = x_norm
+    for _ in range(iterations):
+        x = x / x_norm
+        x = _arrays.to_dev(A.T(_arrays.to_dev(A(x))[0]))[0]
+        x_norm = torch.linalg.vector_norm(x)
+        L = torch.sqrt(x_norm)
+    return float(L), tuple(x.shape), np.float32
+
+
+def compute_Lipschitz_scaling(A, b, regs: Sequence, relaxation_sigma: float = 1.0, relaxation_tau: float = 1.0):
+    """``sigma = rho_s / L``, ``tau = rho_t / (L + sum reg_tau)`` (``solvers.py:120-158``)."""
+    L, x_shape, x_dtype = power_method(A, b)
+    tau = L
+    dummy_x = np.empty(x_shape, dtype=x_dtype)
+    for reg in regs:
+        tau += float(reg.initialize_sigma_tau(dummy_x))
+    return relaxation_sigma / L, relaxation_tau / tau, x_shape, x_dtype
 
 
 class SolutionInfo:
@@ -270,6 +300,113 @@ class SART(Solver):
                 info.residuals[ii] = float(torch.linalg.vector_norm(
                     self.compute_residual(A, b_t, x, A_num_rows, b_mask)))
                 if self.tolerance > info.residuals[ii]:
+                    break
+        return _arrays.from_dev(x, host), info
+
+
+class FISTA(Solver):
+    """FISTA (``solvers.py:1209-1513``): accelerated proximal gradient with an l2 data term
+    and at most one regularizer with a primal proximal (``Regularizer_l1``); the gradient step,
+    proximal, clipping, mask and momentum run fused in ``chip_fista_update``."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = True, tolerance=None, relaxation: float = 1.0,
+                 regularizer=None, data_term="l2", data_term_test=None, restart_period: int | None = None):
+        super().__init__(verbose=verbose, leave_progress=leave_progress, relaxation=relaxation, tolerance=tolerance,
+                         data_term=data_term, data_term_test=data_term_test)
+        regs = self._initialize_regularizer(regularizer)
+        if len(regs) > 1:
+            raise ValueError(
+                f"FISTA supports at most one regularizer, but {len(regs)} were provided. "
+                "To combine multiple regularizers use PDHG instead."
+            )
+        for reg in regs:
+            if not isinstance(reg, regularizers.Regularizer_l1):
+                raise ValueError(
+                    f"Regularizer '{reg.info()}' does not support a primal proximal and cannot be used with FISTA. "
+                    "Switch to PDHG for this regularizer."
+                )
+        self.regularizer = list(regs)
+        self.restart_period = restart_period
+
+    def info(self) -> str:
+        reg_info = "".join(["-" + r.info().upper() for r in self.regularizer])
+        return Solver.info(self) + "-" + self.data_term.info() + reg_info
+
+    @staticmethod
+    def _initialize_data_fidelity_function(data_term):
+        if isinstance(data_term, str):
+            if data_term.lower() == "l2":
+                return data_terms.DataFidelity_l2()
+            raise ValueError(f"Unknown data term: '{data_term}'. FISTA only accepts 'l2' and derivatives.")
+        if isinstance(data_term, data_terms.DataFidelity_l2):
+            return cp.deepcopy(data_term)
+        raise ValueError(f"Unsupported data term type: {type(data_term)}. FISTA only accepts 'l2' and derivatives.")
+
+    def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, x_mask=None, b_mask=None,
+                 b_test_mask=None, precondition: bool = True):
+        st = _native.stream()
+        b_t, host = _arrays.to_dev(b)
+        b_t = b_t.clone()
+        b_mask = _dev(b_mask, b_t)
+        b_test_mask = _dev(b_test_mask, b_t)
+        if precondition:
+            try:
+                At_abs = A.T.absolute()
+                A_abs = A.absolute()
+            except AttributeError:
+                print("WARNING: operator does not support absolute(); falling back to un-preconditioned Lipschitz "
+                      "step size.")
+                precondition = False
+        b_mask, b_test_mask = self._initialize_b_masks(b_t, b_mask, b_test_mask)
+        x_mask_t = None if x_mask is None else _arrays.to_dev(x_mask)[0]
+        if precondition:
+            sigma, tau, x_shape, _ = compute_diagonal_scaling(A_abs, At_abs, b_t, regs=[],
+                                                              relaxation_tau=self.relaxation, x_mask=x_mask_t,
+                                                              b_mask=b_mask)
+            tau_ptr, tau_s = tau.data_ptr(), 0.0
+            sigma_t = sigma
+        else:
+            sig, tau_v, x_shape, _ = compute_Lipschitz_scaling(A, b_t, regs=[], relaxation_tau=self.relaxation)
+            tau_ptr, tau_s = None, float(tau_v)
+            sigma_t = torch.full_like(b_t, float(sig))
+        l1w = 0.0
+        if self.regularizer:
+            self.regularizer[0].initialize_sigma_tau(np.zeros(x_shape, np.float32))
+            l1w = self.regularizer[0].primal_prox_weight()
+        self.data_term.assign_data(b_t)
+        bg = self.data_term.background_dev(b_t)
+        dev = b_t.device
+        x = torch.zeros(x_shape, dtype=torch.float32, device=dev) if x0 is None else _dev(x0, b_t).clone()
+        x = x.reshape(x_shape).contiguous()
+        x_mask_t = _dev(x_mask_t, x)
+        if x_mask_t is not None:
+            x = x * x_mask_t
+        y = x.clone()
+        t = 1.0
+        hl, lo = _scalar_limit(lower_limit)
+        hu, hi = _scalar_limit(upper_limit)
+        info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
+        track = b_test_mask is not None or self.tolerance is not None
+        if track:
+            self._init_residuals(info, A, x, b_t, b_mask, b_test_mask, sigma_t)
+        res = torch.empty_like(b_t)
+        for ii in range(iterations):
+            info.iterations += 1
+            Ay = _arrays.to_dev(A(y))[0]
+            # residual = (b - (A y + bg)) * mask * sigma
+            _native.call("chip_sirt_residual", res.data_ptr(), b_t.data_ptr(), Ay.data_ptr(), _native.ptr(bg),
+                         _native.ptr(b_mask), sigma_t.data_ptr(), b_t.numel(), st)
+            grad = _arrays.to_dev(A.T(res))[0].contiguous()
+            t_new = float(1.0 + np.sqrt(1.0 + 4.0 * t * t)) / 2.0
+            if self.restart_period is not None and (ii + 1) % self.restart_period == 0:
+                t_new = 1.0
+            momentum = (t - 1.0) / t_new
+            _native.call("chip_fista_update", x.data_ptr(), y.data_ptr(), grad.data_ptr(), tau_ptr, tau_s, l1w, hl, lo,
+                         hu, hi, _native.ptr(x_mask_t), momentum, x.numel(), st)
+            t = t_new
+            if track:
+                Ax = _arrays.to_dev(A(x))[0]
+                if self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
                     break
         return _arrays.from_dev(x, host), info
 

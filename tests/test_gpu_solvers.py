@@ -147,3 +147,37 @@ def test_sart_parity_xrf_and_uncorrected(pkg):
     assert rel_max(refu, xu) < 1e-4
     with pytest.raises(ValueError):
         pkg.SART()(pkg.ProjectorUncorrected([n, n], ang), bu, 1)
+
+
+def test_fista_parity_and_reference_convergence(pkg, xrf_problem):
+    """FISTA (``solvers.py:1209-1513``) after 30 iterations against the oracle: diagonal
+    preconditioning with an l1 prox and a lower limit; Lipschitz step from the power method
+    (same ``np.random.rand`` start vector). Then the reference's own flat-object cases
+    (``tests/test_solvers.py:124-134``: 2000 iterations without preconditioning, 1000 with
+    limits) on a 500 x 100 random matrix, atol 1e-2."""
+    p = xrf_problem
+    x, info = pkg.FISTA(regularizer=pkg.regularizers.Regularizer_l1(1e-3))(p["prj"], p["b"], iterations=30,
+                                                                         lower_limit=0.0)
+    ref = osol.fista(p["A"], p["At"], p["b"], p["shape"], 30, l1=1e-3, lower=0.0)
+    e = rel_max(ref, x)
+    print(f"FISTA preconditioned 30 it: rel_max {e:.2e}")
+    assert e < TOL and info.iterations == 30
+    np.random.seed(5)
+    x2, _ = pkg.FISTA()(p["prj"], p["b"], iterations=30, precondition=False)
+    np.random.seed(5)
+    L = osol.power_method(p["A"], p["At"], np.random.rand(*np.shape(p["b"])).astype(np.float32))
+    ref2 = osol.fista(p["A"], p["At"], p["b"], p["shape"], 30, precondition=False, L=L)
+    e2 = rel_max(ref2, x2)
+    print(f"FISTA Lipschitz 30 it: rel_max {e2:.2e} (L = {L:.4f})")
+    assert e2 < 1e-3
+    with pytest.raises(ValueError):
+        pkg.FISTA(regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))
+    rng = np.random.default_rng(11)
+    yy, xx = np.mgrid[:10, :10] - 4.5
+    vol = ((yy**2 + xx**2) <= 9).astype(np.float32)
+    mat = (rng.random((500, 100)) > 0.5).astype(np.float32)
+    data = ((mat @ vol.ravel()).reshape(5, 100) + rng.normal(size=(5, 100)) * 1e-3).astype(np.float32)
+    prj = pkg.ProjectorMatrix(mat, vol.shape, data.shape)
+    for its, kw in ((2000, dict(precondition=False)), (1000, dict(lower_limit=0, upper_limit=1))):
+        sol, _ = pkg.FISTA()(prj, data, iterations=its, **kw)
+        assert np.all(np.isclose(sol, vol, atol=1e-2)), kw

@@ -320,3 +320,25 @@ def test_oracle_sart_solves_consistent_system():
     tau = 1 / np.abs(M[0].T @ np.ones(rows_per))
     sigma = 1 / np.abs(M[0] @ np.ones(nx))
     assert np.allclose(x1, -tau * (M[0].T @ (sigma * (0 - b[0]))), rtol=1e-5)
+
+
+def test_oracle_power_method_and_fista():
+    """power_method -> the largest singular value (numpy SVD); FISTA on a consistent system
+    converges to the solution with both step-size rules; the l1 prox is soft thresholding."""
+    from oracle import solvers as osol
+
+    rng = np.random.default_rng(78)
+    M = rng.random((60, 25)).astype(np.float64)
+    x_true = rng.random(25)
+    b = (M @ x_true).astype(np.float32)
+    L = osol.power_method(lambda x: M @ x, lambda y: M.T @ y, rng.random(60), iterations=60)
+    assert abs(L - np.linalg.svd(M, compute_uv=False)[0]) < 1e-6 * L
+    for pre in (True, False):
+        x = osol.fista(lambda v: (M @ v).astype(np.float32), lambda y: (M.T @ y).astype(np.float32), b, (25,),
+                       3000, precondition=pre, L=L)
+        assert np.max(np.abs(x - x_true)) < 2e-3, pre
+    # pure proximal step: with M = I, one step from 0 with tau = 1 is soft(b, l1)
+    e = np.eye(4, dtype=np.float32)
+    bb = np.array([0.5, -0.05, 0.2, -1.0], dtype=np.float32)
+    x1 = osol.fista(lambda v: e @ v, lambda y: e @ y, bb, (4,), 1, precondition=False, L=1.0, l1=0.1)
+    assert np.allclose(x1, np.sign(bb) * np.fmax(np.abs(bb) - 0.1, 0))
