@@ -203,6 +203,12 @@ int chip_sirt_update(float* x, const float* upd, const float* q, const float* ta
                      float lower, int has_upper, float upper, const float* x_mask, int n, int batch,
                      void* stream);
 
+/* FBP line filter (Filter.apply_filter with zero padding, filters.py:346-390):
+ * out[l][i] = sum_{j<n} in[l][j] h[(i - j) mod p] for n_lines lines of n samples,
+ * h = the real-space kernel (irfft of the half-spectrum filter, length p, a power
+ * of two with 2 n <= p <= 4096); float64 sums. */
+int chip_filter_lines(const float* in, float* out, const double* h, int64_t n_lines, int n, int p, void* stream);
+
 /* FISTA step (solvers.py:1458-1490): x_new = y + tau * grad (tau per element,
  * or tau_scalar when tau is NULL), soft-thresholded at tau * l1_weight when
  * l1_weight > 0 (Regularizer_l1.apply_proximal_primal, regularizers.py:559-585),

@@ -10,8 +10,8 @@ only carry device memory and the stream.
 
 __version__ = "0.1.0"
 
-from . import batch, models, operators, projectors, solvers, data_terms, regularizers, testing, warmstart  # noqa: F401
+from . import batch, filters, models, operators, projectors, solvers, data_terms, regularizers, testing, warmstart  # noqa: F401
 from ._projector_backends import ProjectorBackend, ProjectorBackendHIP, compute_attenuation  # noqa: F401
 from .physics.attenuation import AttenuationVolume  # noqa: F401
 from .projectors import ProjectorAttenuationXRF, ProjectorMatrix, ProjectorUncorrected  # noqa: F401
-from .solvers import FISTA, MLEM, PDHG, SART, SIRT, SolutionInfo  # noqa: F401
+from .solvers import FBP, FISTA, MLEM, PDHG, SART, SIRT, SolutionInfo  # noqa: F401

@@ -56,6 +56,7 @@ _SIGNATURES = {
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
         ctypes.c_int, ctypes.c_int, c_void_p,
     ],
+    "chip_filter_lines": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_fista_update": [
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
         ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p,

@@ -241,6 +241,29 @@ __global__ __launch_bounds__(kEwBlock) void sirt_update_kernel(float* __restrict
   }
 }
 
+// ---- FBP filtering ----------------------------------------------------------------
+
+// out[l][i] = sum_{j < n} in[l][j] h[(i - j) mod p]: the zero-padded circular
+// convolution of length p (a power of two >= 2n) that corrct's FFT filtering
+// computes; one workgroup per line, the line and h in LDS, float64 sums.
+constexpr int kFiltMaxP = 4096;
+
+__global__ __launch_bounds__(256) void filter_lines_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                           const double* __restrict__ h, int n, int p) {
+  __shared__ double hs[kFiltMaxP];
+  __shared__ float xs[kFiltMaxP / 2];
+  const int64_t l = blockIdx.x;
+  for (int k = threadIdx.x; k < p; k += blockDim.x) hs[k] = h[k];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) xs[k] = in[l * n + k];
+  __syncthreads();
+  const int mask = p - 1;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc = fma(static_cast<double>(xs[j]), hs[(i - j) & mask], acc);
+    out[l * n + i] = static_cast<float>(acc);
+  }
+}
+
 // ---- FISTA ----------------------------------------------------------------------
 
 // x_new = y + tau * grad; optional l1 soft threshold at tau * l1w; clip; mask;
@@ -392,6 +415,19 @@ int chip_sirt_update(float* x, const float* upd, const float* q, const float* ta
   const int64_t total = static_cast<int64_t>(n) * n * batch;
   hipLaunchKernelGGL(sirt_update_kernel, ew_grid(total), dim3(kEwBlock), 0, as_stream(stream), x, upd, q, tau,
                      has_lower, lower, has_upper, upper, x_mask, n, batch);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_filter_lines(const float* in, float* out, const double* h, int64_t n_lines, int n, int p, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(in && out && h, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1 && n_lines >= 1 && n_lines <= 2147483647, "bad shape");
+  CHIP_CHECK_ARG(p >= 2 * n && (p & (p - 1)) == 0 && p <= kFiltMaxP,
+                 "p must be a power of two with 2 n <= p <= " + std::to_string(kFiltMaxP));
+  CHIP_CHECK_ARG(in != out, "in and out must not alias");
+  hipLaunchKernelGGL(filter_lines_kernel, dim3(static_cast<unsigned>(n_lines)), dim3(256), 0, as_stream(stream), in,
+                     out, h, n, p);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

@@ -10,6 +10,7 @@ asks for residual norms.
 * ``compute_diagonal_scaling`` -- ``solvers.py:60-117``
 * ``SIRT``  -- ``solvers.py:808-984``
 * ``MLEM``  -- ``solvers.py:634-805``
+* ``FBP``   -- ``solvers.py:346-470`` (filters in ``filters.py``)
 * ``SART``  -- ``solvers.py:473-631``
 * ``FISTA`` -- ``solvers.py:1209-1513`` (l2 data term, optional l1 prox;
   ``power_method`` / ``compute_Lipschitz_scaling``: ``solvers.py:26-158``)
@@ -30,7 +31,7 @@ import numpy as np
 import torch
 
 from . import _arrays, _native
-from . import data_terms, regularizers
+from . import data_terms, filters, regularizers
 
 eps = np.finfo(np.float32).eps
 
@@ -216,6 +217,51 @@ class Solver:
             self.data_term_test.assign_data(b, sigma if sigma is not None else 1.0)
             res_test_0 = self.data_term_test.compute_residual(Ax, mask=b_test_mask)
             info.residual0_cv = self.data_term_test.compute_residual_norm(res_test_0)
+
+
+class FBP(Solver):
+    """Filtered back-projection (``solvers.py:346-470``): ``A.T(filter(b))``, clipped and masked."""
+
+    def __init__(self, verbose: bool = False, leave_progress: bool = False, regularizer=None, data_term="l2",
+                 fbp_filter="ramp", pad_mode: str = "constant"):
+        super().__init__(verbose=verbose)
+        if isinstance(fbp_filter, str):
+            fbp_filter = fbp_filter.lower()
+        self.fbp_filter = fbp_filter
+        self.pad_mode = pad_mode
+
+    def info(self) -> str:
+        if isinstance(self.fbp_filter, str):
+            return super().info() + "(F:" + self.fbp_filter.upper() + ")"
+        if isinstance(self.fbp_filter, np.ndarray):
+            return super().info() + "(F:" + filters.FilterCustom.__name__.upper() + ")"
+        return super().info() + "(F:" + type(self.fbp_filter).__name__.upper() + ")"
+
+    def __call__(self, A, b, iterations: int = 0, x0=None, lower_limit=None, upper_limit=None, x_mask=None,
+                 b_mask=None):
+        if len(np.shape(b)) < 2:
+            raise ValueError(f"Data should be at least 2-dimensional (b.shape = {np.shape(b)})")
+        info = SolutionInfo(self.info(), max_iterations=0, tolerance=0.0)
+        if isinstance(self.fbp_filter, str):
+            if self.fbp_filter in ("mr", "data"):
+                raise NotImplementedError("the data-driven (MR) FBP filter is not supported")
+            local_filter = filters.FilterFBP(filter_name=self.fbp_filter, pad_mode=self.pad_mode)
+        elif isinstance(self.fbp_filter, np.ndarray):
+            local_filter = filters.FilterCustom(self.fbp_filter, pad_mode=self.pad_mode)
+        else:
+            local_filter = self.fbp_filter
+            local_filter.pad_mode = self.pad_mode
+        b_t, host = _arrays.to_dev(b)
+        pre_weights = A.get_pre_weights() if hasattr(A, "get_pre_weights") else None
+        if pre_weights is not None:
+            b_t = b_t * _arrays.to_dev(pre_weights)[0]
+        b_f = local_filter(b_t)
+        x = _arrays.to_dev(A.T(b_f))[0]
+        if lower_limit is not None or upper_limit is not None:
+            x = x.clip(lower_limit, upper_limit)
+        if x_mask is not None:
+            x = x * _arrays.to_dev(x_mask)[0]
+        return _arrays.from_dev(x, host), info
 
 
 class SART(Solver):

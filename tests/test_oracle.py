@@ -12,6 +12,8 @@ The oracle restates corrct's skimage-backend path. It is pinned by
 
 import math
 
+import os
+
 import numpy as np
 import pytest
 
@@ -342,3 +344,26 @@ def test_oracle_power_method_and_fista():
     bb = np.array([0.5, -0.05, 0.2, -1.0], dtype=np.float32)
     x1 = osol.fista(lambda v: e @ v, lambda y: e @ y, bb, (4,), 1, precondition=False, L=1.0, l1=0.1)
     assert np.allclose(x1, np.sign(bb) * np.fmax(np.abs(bb) - 0.1, 0))
+
+
+def test_oracle_fbp_filters_and_iradon_golden():
+    """The oracle's FBP filters equal scikit-image's (all five names, sizes 64 and 128), and
+    ramp filter + iradon-style back-projection equals scikit-image's filtered iradon. (For the
+    windowed filters corrct multiplies the real FFT by the first half of the filter, which is not
+    Hermitian-symmetric for hann / hamming / cosine, so it deliberately differs from iradon's
+    full complex FFT there: 3 % for hann.)"""
+    from oracle import filters as ofl
+    from oracle import xrf as oxrf
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "skimage_fbp_golden.npz"))
+    for size in (64, 128):
+        for name in ofl.NAMES:
+            assert np.allclose(ofl.fourier_filter(size, name), g[f"filter_{name}_{size}"], rtol=0, atol=1e-12), name
+    assert ofl.padding_size(24) == 64 and ofl.padding_size(100) == 256
+    sino, rad = g["fbp_sino"], g["fbp_rad"]
+    n = sino.shape[-1]
+    got = ofl.fbp(lambda y: oxrf.bp_skimage(y[None].astype(np.float32), rad, n), sino, len(rad), "ramp")
+    ref = g["fbp_ramp_vol"]
+    assert np.max(np.abs(got - ref)) < 1e-5 * np.max(np.abs(ref))
+    got = ofl.fbp(lambda y: oxrf.bp_skimage(y[None].astype(np.float32), rad, n), sino, len(rad), "hann")
+    assert 1e-2 < np.max(np.abs(got - g["fbp_hann_vol"])) / np.max(np.abs(g["fbp_hann_vol"])) < 0.1
