@@ -75,6 +75,21 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
 int chip_plan_destroy(chip_plan* plan);
 
 /*
+ * Detector-surface sampling of the emitted beam: detector d's outgoing maps
+ * become the float32 mean, in order, of exp(-cumulative mu) over its
+ * n_sub_det[d] (1 <= n_sub_det[d] <= n_sub_max <= 64) angles
+ * ((theta_a + phi_d) + sub_angles[d][k]) + pi; sub_angles is
+ * [n_det][n_sub_max] float64 (entries past n_sub_det[d] unused), angles_rad
+ * and det_angles_rad as given to chip_plan_create. Replaces the sub-angle
+ * loop of AttenuationVolume._compute_attenuation_angle_out
+ * (physics/attenuation.py:110-119) with DetectorXRF.angle_range_rad
+ * (physics/xrf.py:238-246) evaluated by the caller. Synchronises the device
+ * (the previous tables may be in use) and replaces the plan's tables.
+ */
+int chip_plan_set_subangles(chip_plan* plan, int n_sub_max, const int* n_sub_det, const double* sub_angles,
+                            const float* angles_rad, const double* det_angles_rad);
+
+/*
  * Attenuation maps maps[a][d][n][n] = E_in(theta_a) * E_out(theta_a + phi_d + pi)
  * with E(phi) = exp(-trapezoid cumulative sum of mu along the beam), for
  * `batch` slices (mu_* and maps advance by n*n and n_angles*n_det*n*n per
@@ -208,6 +223,13 @@ int chip_sirt_update(float* x, const float* upd, const float* q, const float* ta
  * h = the real-space kernel (irfft of the half-spectrum filter, length p, a power
  * of two with 2 n <= p <= 4096); float64 sums. */
 int chip_filter_lines(const float* in, float* out, const double* h, int64_t n_lines, int n, int p, void* stream);
+
+/* Detector PSF on each line of n samples (TransformConvolution with edge
+ * padding, operators.py:424-496, as ProjectorUncorrected applies it after
+ * fp, projectors.py:229-244, 275-321): out[l][i] = sum_k xe[i + (K-1)/2 - k] kern[k], xe the
+ * line extended by (K-1)/2 edge copies on each side and zero beyond. */
+int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, int64_t n_lines, int n,
+                   void* stream);
 
 /* FISTA step (solvers.py:1458-1490): x_new = y + tau * grad (tau per element,
  * or tau_scalar when tau is NULL), soft-thresholded at tau * l1_weight when

@@ -3,8 +3,13 @@
 Test infrastructure only (see ``oracle/__init__.py``).
 
 * ``compute_attenuation``  -- ``_projector_backends.py:43-99``
-* ``compute_maps``         -- ``physics/attenuation.py:105-180`` (float detector
-  angles, so one sub-angle of offset 0, ``attenuation.py:111-116``)
+* ``compute_maps``         -- ``physics/attenuation.py:105-180``, with the
+  outgoing factor of each detector averaged (float32, in order) over its
+  sub-angle offsets (``attenuation.py:110-119``; one offset 0 for a point
+  detector)
+* ``detector_sub_angles``  -- the offsets: midpoints of S equal parts of
+  ``[-atan2(sqrt(surface) / 2, distance), +...]`` (``physics/xrf.py:238-246``,
+  ``attenuation.py:112-113``)
 """
 
 from __future__ import annotations
@@ -62,8 +67,12 @@ def compute_maps(
     angles_rot_rad_f32: np.ndarray,
     angles_det_rad: np.ndarray,
     fp32: bool = True,
+    sub_angles=None,
 ) -> np.ndarray:
     """``maps[a, d] = E_in(theta_a) * E_out(theta_a + phi_d + pi)``, shape [A, D, N, N].
+
+    ``sub_angles`` (one 1-D array of offsets per detector, None = a single 0)
+    replaces E_out by the mean over the offsets s of E_out(theta_a + phi_d + s + pi).
 
     ``theta_a`` are float32 (the projector's ``angles_w_rad``); the outgoing
     angle ``theta_a + phi_d`` is float64 (``attenuation.py:109``). A missing
@@ -82,5 +91,19 @@ def compute_maps(
             maps[ia] *= compute_attenuation(mu_in, a if fp32 else np.float64(a), invert=False, fp32=fp32)[None]
         if mu_out is not None:
             for idet, phi in enumerate(dets):
-                maps[ia, idet] *= compute_attenuation(mu_out, np.float64(a) + phi, invert=True, fp32=fp32)
+                subs = np.zeros(1) if sub_angles is None else np.asarray(sub_angles[idet], dtype=np.float64)
+                acc = np.zeros((n, n), dtype=dt)
+                for s in subs:
+                    acc += compute_attenuation(mu_out, (np.float64(a) + phi) + s, invert=True, fp32=fp32)
+                acc /= len(subs)
+                maps[ia, idet] *= acc
     return maps
+
+
+def detector_sub_angles(surface_mm2: float, distance_mm: float, n_sub: int) -> np.ndarray:
+    """Sub-angle offsets of a detector of the given surface and distance."""
+    if surface_mm2 <= 0.0:
+        return np.zeros(1)
+    half = math.atan2(math.sqrt(surface_mm2) / 2, distance_mm)
+    edges = np.linspace(-half, half, n_sub + 1)
+    return np.convolve(edges, np.ones(2), "valid") / 2

@@ -163,3 +163,20 @@ def explicit_fp_matrix(n: int, angles_rad_f32, maps=None, w_det=None, w_ang=None
         e[k] = 1
         cols.append(fp(e.reshape(n, n), angles_rad_f32, maps, w_det, w_ang, n_det, fp32=False).ravel())
     return np.stack(cols, axis=1)
+
+
+def psf_lines(prj, kern) -> np.ndarray:
+    """Detector PSF along the last axis: ``TransformConvolution`` with edge padding
+    (``operators.py:471-480``): out[i] = sum_k xe[i + (K-1)//2 - k] kern[k], with xe the line
+    extended by (K-1)//2 edge copies on each side and zero beyond (an even-length kernel
+    reaches one sample past the padding on the low side)."""
+    x = np.asarray(prj, dtype=np.float64)
+    kern = np.asarray(kern, dtype=np.float64).ravel()
+    n, k_len = x.shape[-1], len(kern)
+    pw = (k_len - 1) // 2
+    out = np.zeros_like(x)
+    for k in range(k_len):
+        m = np.arange(n) + pw - k
+        inside = (m >= -pw) & (m <= n - 1 + pw)
+        out += np.where(inside, x[..., np.clip(m, 0, n - 1)], 0.0) * kern[k]
+    return out

@@ -36,6 +36,7 @@ _SIGNATURES = {
         ctypes.POINTER(c_void_p), ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p,
     ],
     "chip_plan_destroy": [c_void_p],
+    "chip_plan_set_subangles": [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
@@ -57,6 +58,7 @@ _SIGNATURES = {
         ctypes.c_int, ctypes.c_int, c_void_p,
     ],
     "chip_filter_lines": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p],
+    "chip_psf_lines": [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p],
     "chip_fista_update": [
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
         ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p,

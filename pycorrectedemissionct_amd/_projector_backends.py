@@ -98,6 +98,19 @@ class HipPlan:
                      int(mode), _native.stream())
         return vol
 
+    def set_subangles(self, sub_angles) -> None:
+        """Per-detector outgoing sub-angle offsets (a sequence of 1-D arrays, ``chip_plan_set_subangles``)."""
+        subs = [np.asarray(s, dtype=np.float64).reshape(-1) for s in sub_angles]
+        if len(subs) != self.n_det:
+            raise ValueError(f"Expected {self.n_det} sub-angle sets, got {len(subs)}")
+        s_max = max(len(s) for s in subs)
+        counts = np.ascontiguousarray([len(s) for s in subs], dtype=np.int32)
+        table = np.zeros((self.n_det, s_max), dtype=np.float64)
+        for d, s in enumerate(subs):
+            table[d, : len(s)] = s
+        _native.call("chip_plan_set_subangles", self._h, s_max, counts.ctypes.data, table.ctypes.data,
+                     self.angles.ctypes.data, self.det_angles.ctypes.data)
+
     def att_maps(self, mu_in, mu_out, maps, batch: int = 1):
         _native.call("chip_att_maps", self._h, _native.ptr(mu_in), _native.ptr(mu_out), _native.ptr(maps),
                      int(batch), _native.stream())

@@ -87,12 +87,15 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
 
 // maps[a - out_a0][d][i][j] for angles [a0, a0 + n_a) of the chunk whose
 // prefix sums are in cum: beam index of angle a (chunk-local la): in = la
-// (if has_in), out = n_in + la * D + d (if has_out).
+// (if has_in), out = n_in + (la * D + d) * n_sub + k (if has_out) for the
+// detector's nsub[d] sub-angles k, whose exp(-.) are averaged in float32 in
+// order (physics/attenuation.py:110-119).
 __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restrict__ cum,
                                                           const Affine* __restrict__ rot_in_back,
                                                           const Affine* __restrict__ rot_out_back, int has_in,
                                                           int has_out, int n_in, int a0, int n_det, int n, int pad,
-                                                          int np, float* __restrict__ maps, int out_a0) {
+                                                          int np, float* __restrict__ maps, int out_a0, int n_sub,
+                                                          const int* __restrict__ nsub) {
   const int j = blockIdx.x * 16 + (threadIdx.x & 15);
   const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
   const int ad = blockIdx.z;  // chunk-local (la, d)
@@ -110,8 +113,13 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
     e_in = expf(-warp_sample(cum + static_cast<size_t>(la) * npp, np, 0, xs, ys));
   }
   if (has_out) {
-    affine_apply(rot_out_back[static_cast<size_t>(a) * n_det + d], xf, yf, xs, ys);
-    e_out = expf(-warp_sample(cum + static_cast<size_t>(n_in + la * n_det + d) * npp, np, 0, xs, ys));
+    const int cnt = nsub ? nsub[d] : 1;
+    float acc = 0.f;
+    for (int k = 0; k < cnt; ++k) {
+      affine_apply(rot_out_back[(static_cast<size_t>(a) * n_det + d) * n_sub + k], xf, yf, xs, ys);
+      acc += expf(-warp_sample(cum + static_cast<size_t>(n_in + (la * n_det + d) * n_sub + k) * npp, np, 0, xs, ys));
+    }
+    e_out = acc / static_cast<float>(cnt);
   }
   maps[(static_cast<size_t>(a - out_a0) * n_det + d) * n * n + static_cast<size_t>(i) * n + j] = e_in * e_out;
 }
@@ -140,6 +148,7 @@ static int ensure_buffer(float*& buf, size_t& bytes, size_t need, hipStream_t st
 // D x N^2 maps block per angle when `with_block`) within `budget` bytes, and
 // within the launch-grid limit of 65535 in y / z.
 static int chunk_angles(const Plan& p, int beams, bool with_block, size_t budget) {
+  // beams per angle include n_sub out-beams per detector
   const size_t npp = static_cast<size_t>(p.np) * p.np;
   const size_t per_angle = npp * sizeof(float) * beams +
                            (with_block ? static_cast<size_t>(p.n_det) * p.n * p.n * sizeof(float) : 0);
@@ -156,17 +165,18 @@ static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int
   const int A = p.n_angles, D = p.n_det;
   const Affine* rin_f = p.d_rot;
   const Affine* rin_b = p.d_rot + A;
+  const int S = p.n_sub;
   const Affine* rout_f = p.d_rot + 2 * A;
-  const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D;
+  const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D * S;
   const int n_in = mi ? na : 0;
-  const int n_beams = n_in + (mo ? na * D : 0);
+  const int n_beams = n_in + (mo ? na * D * S : 0);
   dim3 g1((p.np + 255) / 256, n_beams);
-  hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0, rout_f + static_cast<size_t>(a0) * D,
-                     n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+  hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
+                     rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
   CHIP_LAUNCH_CHECK();
   dim3 g2((p.n + 15) / 16, (p.n + 15) / 16, na * D);
   hipLaunchKernelGGL(att_compose_kernel, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0, mo ? 1 : 0,
-                     n_in, a0, D, p.n, p.pad, p.np, out, out_a0);
+                     n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -185,7 +195,7 @@ extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* m
   Plan& p = plan->p;
   hipStream_t st = as_stream(stream);
   const int A = p.n_angles, D = p.n_det, n = p.n;
-  const int beams = (mu_in ? 1 : 0) + (mu_out ? D : 0);
+  const int beams = (mu_in ? 1 : 0) + (mu_out ? D * p.n_sub : 0);
   const int chunk = chunk_angles(p, beams, false, size_t(1) << 30);
   const size_t need = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
   int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need, st, "chip_att_maps");
@@ -204,7 +214,7 @@ extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* m
 
 // Shared set-up of the on-the-fly projectors: chunk size and both scratch buffers.
 static int otf_prepare(Plan& p, const float* mu_in, const float* mu_out, hipStream_t st, int& chunk) {
-  const int beams = (mu_in ? 1 : 0) + (mu_out ? p.n_det : 0);
+  const int beams = (mu_in ? 1 : 0) + (mu_out ? p.n_det * p.n_sub : 0);
   chunk = chunk_angles(p, beams, true, p.otf_budget);
   const size_t need_c = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
   const size_t need_m = static_cast<size_t>(chunk) * p.n_det * p.n * p.n * sizeof(float);

@@ -56,11 +56,15 @@ struct Plan {
   std::vector<float> h_fp_geom;  // [A][4]
   std::vector<float> h_w;        // [A][D] = w_det[d] * w_ang[a][d]
   std::vector<Affine> h_rot_in_fwd, h_rot_in_back;    // [A]
-  std::vector<Affine> h_rot_out_fwd, h_rot_out_back;  // [A][D]
+  std::vector<Affine> h_rot_out_fwd, h_rot_out_back;  // [A][D][n_sub]
+  // outgoing-beam sub-angles per detector (detector surface sampling): the
+  // out-beam maps average exp(-.) over n_sub_det[d] <= n_sub angles
+  int n_sub = 1;
+  int* d_nsub = nullptr;  // [D]
   float4* d_fp_geom = nullptr;
   float2* d_ytab = nullptr;  // [A][N] (fl(sin * y), fl(cos * y)) row terms of the fp map
   float* d_w = nullptr;
-  Affine* d_rot = nullptr;  // [A] in-fwd, [A] in-back, [A*D] out-fwd, [A*D] out-back
+  Affine* d_rot = nullptr;  // [A] in-fwd, [A] in-back, [A*D*n_sub] out-fwd, [A*D*n_sub] out-back
   // scratch for the attenuation builder (rotated-frame cumulative sums)
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;

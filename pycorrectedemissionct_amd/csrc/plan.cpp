@@ -70,7 +70,7 @@ static const char* kSymbols[] = {
     "chip_nn_conv3x3",    "chip_nn_conv3x3_wgrad", "chip_nn_bias_grad", "chip_nn_flip_weights",
     "chip_nn_relu_mask",  "chip_nn_avgpool2",  "chip_nn_upsample2", "chip_nn_head_fwd",
     "chip_nn_head_bwd",   "chip_nn_mse",       "chip_nn_adam",      "chip_nn_to_bf16",
-    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",
+    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",  "chip_psf_lines",     "chip_plan_set_subangles",
 };
 
 extern "C" {
@@ -192,9 +192,62 @@ int chip_plan_destroy(chip_plan* plan) {
   if (p.d_w) (void)hipFree(p.d_w);
   if (p.d_ytab) (void)hipFree(p.d_ytab);
   if (p.d_rot) (void)hipFree(p.d_rot);
+  if (p.d_nsub) (void)hipFree(p.d_nsub);
   if (p.d_scratch) (void)hipFree(p.d_scratch);
   if (p.d_mblock) (void)hipFree(p.d_mblock);
   delete plan;
+  return CHIP_OK;
+}
+
+int chip_plan_set_subangles(chip_plan* plan, int n_sub_max, const int* n_sub_det, const double* sub_angles,
+                            const float* angles_rad, const double* det_angles_rad) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && n_sub_det != nullptr && sub_angles != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(angles_rad != nullptr && det_angles_rad != nullptr, "NULL angles");
+  CHIP_CHECK_ARG(n_sub_max >= 1 && n_sub_max <= 64, "n_sub_max must be in [1, 64]");
+  Plan& p = plan->p;
+  const int A = p.n_angles, D = p.n_det, S = n_sub_max;
+  for (int d = 0; d < D; ++d)
+    CHIP_CHECK_ARG(n_sub_det[d] >= 1 && n_sub_det[d] <= S, "n_sub_det entries must be in [1, n_sub_max]");
+  std::vector<Affine> fwd(static_cast<size_t>(A) * D * S), back(static_cast<size_t>(A) * D * S);
+  for (int a = 0; a < A; ++a) {
+    for (int d = 0; d < D; ++d) {
+      // the reference's order: ((theta + phi_d) + sub) + pi, float64 (physics/attenuation.py:109-118,
+      // _projector_backends.py:79-80)
+      const double base = static_cast<double>(angles_rad[a]) + det_angles_rad[d];
+      for (int k = 0; k < S; ++k) {
+        const double sub = k < n_sub_det[d] ? sub_angles[static_cast<size_t>(d) * S + k] : 0.0;
+        const double dg = ((base + sub) + M_PI) * (180.0 / M_PI);
+        const size_t idx = (static_cast<size_t>(a) * D + d) * S + k;
+        fwd[idx] = rotate_affine(p.np, (-dg) * (M_PI / 180.0));
+        back[idx] = rotate_affine(p.np, dg * (M_PI / 180.0));
+      }
+    }
+  }
+  std::vector<Affine> rot;
+  rot.reserve(2 * static_cast<size_t>(A) + 2 * fwd.size());
+  rot.insert(rot.end(), p.h_rot_in_fwd.begin(), p.h_rot_in_fwd.end());
+  rot.insert(rot.end(), p.h_rot_in_back.begin(), p.h_rot_in_back.end());
+  rot.insert(rot.end(), fwd.begin(), fwd.end());
+  rot.insert(rot.end(), back.begin(), back.end());
+  Affine* d_rot = nullptr;
+  int* d_nsub = nullptr;
+  CHIP_HIP(hipDeviceSynchronize());  // the old tables may still be read by queued work
+  CHIP_HIP(hipMalloc(&d_rot, sizeof(Affine) * rot.size()));
+  if (hipMalloc(&d_nsub, sizeof(int) * D) != hipSuccess) {
+    (void)hipFree(d_rot);
+    set_error("chip_plan_set_subangles: hipMalloc failed");
+    return CHIP_ENOMEM;
+  }
+  CHIP_HIP(hipMemcpy(d_rot, rot.data(), sizeof(Affine) * rot.size(), hipMemcpyHostToDevice));
+  CHIP_HIP(hipMemcpy(d_nsub, n_sub_det, sizeof(int) * D, hipMemcpyHostToDevice));
+  if (p.d_rot) (void)hipFree(p.d_rot);
+  if (p.d_nsub) (void)hipFree(p.d_nsub);
+  p.d_rot = d_rot;
+  p.d_nsub = d_nsub;
+  p.h_rot_out_fwd.swap(fwd);
+  p.h_rot_out_back.swap(back);
+  p.n_sub = S;
   return CHIP_OK;
 }
 

@@ -264,6 +264,30 @@ __global__ __launch_bounds__(256) void filter_lines_kernel(const float* __restri
   }
 }
 
+// ---- detector PSF -----------------------------------------------------------------
+
+// out[l][i] = sum_{k<K} xe[i + (K-1)/2 - k] kern[k]: "same" convolution of each
+// line extended by (K-1)/2 edge copies on each side and zero beyond
+__global__ __launch_bounds__(256) void psf_lines_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                        const float* __restrict__ kern, int k_len, int64_t n_lines,
+                                                        int n) {
+  const int pw = (k_len - 1) / 2;
+  const int64_t total = n_lines * n;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t l = t / n;
+    const int i = static_cast<int>(t - l * n);
+    const float* row = in + l * n;
+    float acc = 0.f;
+    for (int k = 0; k < k_len; ++k) {
+      const int m = i + pw - k;
+      if (m < -pw || m > n - 1 + pw) continue;
+      acc = fmaf(row[min(max(m, 0), n - 1)], kern[k], acc);
+    }
+    out[t] = acc;
+  }
+}
+
 // ---- FISTA ----------------------------------------------------------------------
 
 // x_new = y + tau * grad; optional l1 soft threshold at tau * l1w; clip; mask;
@@ -428,6 +452,18 @@ int chip_filter_lines(const float* in, float* out, const double* h, int64_t n_li
   CHIP_CHECK_ARG(in != out, "in and out must not alias");
   hipLaunchKernelGGL(filter_lines_kernel, dim3(static_cast<unsigned>(n_lines)), dim3(256), 0, as_stream(stream), in,
                      out, h, n, p);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, int64_t n_lines, int n,
+                   void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(in && out && kern, "NULL pointer");
+  CHIP_CHECK_ARG(in != out, "in and out must not alias");
+  CHIP_CHECK_ARG(k_len >= 1 && n >= 1 && n_lines >= 1, "bad shape");
+  hipLaunchKernelGGL(psf_lines_kernel, ew_grid(n_lines * n), dim3(kEwBlock), 0, as_stream(stream), in, out, kern,
+                     k_len, n_lines, n);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

@@ -8,9 +8,10 @@ fills ``maps[A, D, *vol]`` with ``E_in(theta_a) * E_out(theta_a + phi_d + pi)``
 angles. The maps stay in device memory; ``get_maps`` returns NumPy when the
 inputs were NumPy (as the reference does) and the device tensor otherwise.
 
-Not supported (raise): detectors with a non-zero surface (sub-angle
-averaging, ``attenuation.py:111-119``), ``roi`` extraction (also
-unimplemented in the reference) and ``binning > 1``.
+Detectors with a non-zero surface average the outgoing factor over
+``emitted_sub_sampling`` sub-angles spanning the detector's angular range
+(``attenuation.py:110-119``), on the device. Not supported (raise): ``roi``
+extraction (also unimplemented in the reference) and ``binning > 1``.
 """
 
 from __future__ import annotations
@@ -31,6 +32,13 @@ class DetectorXRF:
         self.surface_mm2 = float(surface_mm2)
         self.distance_mm = float(distance_mm)
         self.angle_rad = float(angle_rad)
+
+    @property
+    def angle_range_rad(self) -> tuple[float, float]:
+        """Half-opening angle of the detector surface seen from the sample (``physics/xrf.py:238-246``)."""
+        half_edge = np.sqrt(self.surface_mm2) / 2
+        half_angle = np.arctan2(half_edge, self.distance_mm)
+        return (-half_angle, half_angle)
 
 
 class AttenuationVolume:
@@ -71,13 +79,24 @@ class AttenuationVolume:
         num_dims = len(self.vol_shape_zyx)
         if num_dims not in (2, 3):
             raise ValueError(f"Maps can only be 2D or 3D Arrays. A {num_dims}-dimensional was passed ({self.vol_shape_zyx}).")
-        if any(d.surface_mm2 > 0.0 for d in self.detectors):
-            raise NotImplementedError("detectors with a non-zero surface (sub-angle averaging) are not supported")
         self._host = _arrays.is_host(incident_local if incident_local is not None else emitted_local)
         self.maps_device: torch.Tensor | None = None
 
     def _get_detector_angles(self) -> np.ndarray:
         return np.array([det.angle_rad for det in self.detectors], ndmin=1)
+
+    def _get_sub_angles(self) -> list[np.ndarray]:
+        """Outgoing sub-angle offsets per detector: midpoints of ``emitted_sub_sampling`` equal
+        parts of the detector's angular range, or a single 0 for a point detector
+        (``attenuation.py:111-116``)."""
+        subs = []
+        for det in self.detectors:
+            if det.surface_mm2 > 0.0:
+                edges = np.linspace(*det.angle_range_rad, self.emitted_sub_sampling + 1)
+                subs.append((edges[:-1] + edges[1:]) / 2)
+            else:
+                subs.append(np.zeros(1))
+        return subs
 
     def compute_maps(self, use_multithreading: bool = True, verbose: bool = True) -> None:
         """Fill the maps ``[A, D, *vol]`` on the device (``attenuation.py:122-180``)."""
@@ -88,6 +107,9 @@ class AttenuationVolume:
         mu_in = None if self.incident_local is None else _arrays.to_dev(self.incident_local)[0]
         mu_out = None if self.emitted_local is None else _arrays.to_dev(self.emitted_local)[0]
         plan = HipPlan(n, np.asarray(self.angles_rot_rad, dtype=np.float32), self._get_detector_angles())
+        subs = self._get_sub_angles()
+        if mu_out is not None and any(len(sa) > 1 or sa[0] != 0.0 for sa in subs):
+            plan.set_subangles(subs)
         n_a, n_d = plan.n_angles, plan.n_det
         maps = torch.empty((n_slices, n_a, n_d, n, n), dtype=torch.float32, device=_arrays.device())
         plan.att_maps(mu_in, mu_out, maps, n_slices)

@@ -101,9 +101,7 @@ class ProjectorUncorrected(operators.ProjectorOperator):
         if prj_intensities is not None:
             prj_intensities = np.array(prj_intensities, dtype=np.float64)
         self.prj_intensities = prj_intensities
-        if psf is not None:
-            raise NotImplementedError("a detector PSF is not supported by the HIP backend")
-        self.psf_vu = self.psf_vwu = None
+        self._set_psf(psf)
         self.vol_shape = np.array(self.projector_backend.get_vol_shape(), ndmin=1)
         self.prj_shape = np.array(self.projector_backend.get_prj_shape(), ndmin=1)
         super().__init__()
@@ -111,6 +109,33 @@ class ProjectorUncorrected(operators.ProjectorOperator):
     @property
     def angles_rot_rad(self) -> np.ndarray:
         return self.projector_backend.angles_w_rad
+
+    def _set_psf(self, psf) -> None:
+        """Detector PSF (``projectors.py:229-244``): a 1-D kernel convolved along u after every fp
+        with edge-replicated borders. As in the reference (``is_conv_symm=False``), its adjoint is
+        the identity, so the back-projections ignore it."""
+        if psf is None:
+            self.psf_vu = self.psf_vwu = None
+            return
+        psf = np.squeeze(np.array(psf, dtype=np.float32))
+        if psf.ndim >= len(self.vol_geom.shape_xyz) or psf.ndim != 1:
+            raise ValueError(
+                "PSF should either be 1D (for 2D and 3D reconstructions) or 2D (for 3D reconstructions)."
+                + f" Passed PSF has shape: {psf.shape}, and the reconstruction is {len(self.vol_geom.shape_xyz)}D."
+            )
+        self.psf_vu = self.psf_vwu = psf
+
+    def _apply_psf(self, prj):
+        if self.psf_vu is None:
+            return prj
+        t, host = _arrays.to_dev(prj)
+        t = t.contiguous()
+        n = int(t.shape[-1])
+        kern = torch.as_tensor(self.psf_vu, device=t.device)
+        out = torch.empty_like(t)
+        _native.call("chip_psf_lines", t.data_ptr(), out.data_ptr(), kern.data_ptr(), int(kern.numel()),
+                     t.numel() // n, n, _native.stream())
+        return _arrays.from_dev(out, host)
 
     def __enter__(self):
         self.projector_backend.make_ready()
@@ -129,7 +154,7 @@ class ProjectorUncorrected(operators.ProjectorOperator):
         prj_vu = self.projector_backend.fp(vol, angle_ind)
         if self.prj_intensities is not None:
             prj_vu = prj_vu * np.float32(self.prj_intensities[angle_ind])
-        return prj_vu
+        return self._apply_psf(prj_vu)
 
     def bp_angle(self, prj_vu, angle_ind: int):
         if self.prj_intensities is not None:
@@ -143,7 +168,7 @@ class ProjectorUncorrected(operators.ProjectorOperator):
                 prj_vwu = prj_vwu * self._intensities_dev(prj_vwu.device)[:, None]
             else:
                 prj_vwu *= self.prj_intensities[:, None].astype(np.float32)
-        return prj_vwu
+        return self._apply_psf(prj_vwu)
 
     def bp(self, prj_vwu):
         if self.prj_intensities is not None:
@@ -267,6 +292,7 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         d = len(self.angles_det_rad)
         line = torch.empty((d, n), dtype=torch.float32, device=t.device)
         self._plan().fp_angle(t, self._maps(t.device), line, int(angle_ind))
+        line = self._apply_psf(line)
         if d == 1:
             line = line[0]
         return _arrays.from_dev(line, host)
@@ -298,6 +324,7 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             self._plan().fp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, 1)
         else:
             self._plan().fp(t, self._maps(t.device), out, 1)
+        out = self._apply_psf(out)
         if d == 1:
             out = out[0]
         return _arrays.from_dev(out, host)

@@ -202,6 +202,36 @@ def test_att_maps_parity(pkg, n):
     assert rel_max(ref, got) < TOL
 
 
+def test_att_maps_detector_surface(pkg):
+    """Detectors with a surface: the outgoing factor is averaged over emitted_sub_sampling
+    sub-angles (a 9 mm^2 detector at 10 mm in 4 parts, a 2 mm^2 one at 5 mm in 4, and a point
+    detector), against the oracle's average."""
+    n, a = 33, 10
+    rng = np.random.default_rng(410)
+    ang = _angles(a)
+    mu_in = (rng.random((n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.08).astype(np.float32)
+    from pycorrectedemissionct_amd.physics.attenuation import DetectorXRF
+
+    dets = [DetectorXRF(9.0, 10.0, np.pi / 2), DetectorXRF(2.0, 5.0, -np.pi / 2), DetectorXRF(0.0, 1.0, 0.3)]
+    av = pkg.AttenuationVolume(mu_in, mu_out, ang, dets, emitted_sub_sampling=4)
+    av.compute_maps()
+    got = av.get_maps()
+    subs = [oatt.detector_sub_angles(d.surface_mm2, d.distance_mm, 4) for d in dets]
+    ref = oatt.compute_maps(mu_in, mu_out, ang, [d.angle_rad for d in dets], sub_angles=subs)
+    e = rel_max(ref, got)
+    point = oatt.compute_maps(mu_in, mu_out, ang, [d.angle_rad for d in dets])
+    print(f"surface maps: rel_max {e:.2e} (point-detector maps differ by {rel_max(point, ref):.2e})")
+    assert e < TOL
+    assert np.array_equal(got[:, 2], av_point_maps(pkg, mu_in, mu_out, ang, 0.3))
+
+
+def av_point_maps(pkg, mu_in, mu_out, ang, phi):
+    av = pkg.AttenuationVolume(mu_in, mu_out, ang, [phi])
+    av.compute_maps()
+    return av.get_maps()[:, 0]
+
+
 def test_compute_attenuation_free_function(pkg):
     rng = np.random.default_rng(405)
     mu = (rng.random((24, 24)) * 0.1).astype(np.float32)
@@ -275,3 +305,28 @@ def test_otf_maps_match_precomputed(pkg, n):
         otf.fp_angle(x, 0)
     with pytest.raises(ValueError):
         pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, maps_mode="otf")
+
+
+def test_detector_psf(pkg):
+    """A 1-D detector PSF is convolved along u after the fp (uncorrected and XRF, whole sinogram
+    and per angle) with edge-replicated borders; the bp ignores it, as in the reference."""
+    n, a = 40, 12
+    rng = np.random.default_rng(910)
+    ang = _angles(a)
+    psf = np.array([0.1, 0.2, 0.4, 0.2, 0.1], dtype=np.float32)
+    vol = _phantom(n, rng)
+    pu = pkg.ProjectorUncorrected([n, n], ang, psf=psf)
+    ref = oxrf.psf_lines(oxrf.fp(vol, ang)[0], psf)
+    assert rel_max(ref, pu.fp(vol)) < TOL
+    maps = rng.uniform(0.3, 1.0, (a, 2, n, n)).astype(np.float32)
+    px = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=[np.pi / 2, -np.pi / 2],
+                                     psf=psf)
+    refx = oxrf.psf_lines(oxrf.fp(vol, ang, maps), psf)
+    assert rel_max(refx, px.fp(vol)) < TOL
+    assert rel_max(refx[:, 5], px.fp_angle(vol, 5)) < TOL
+    sino = rng.random((2, a, n)).astype(np.float32)
+    assert rel_max(oxrf.bp_skimage(sino, ang, n), px.bp(sino)) < TOL
+    psf4 = np.array([0.2, 0.3, 0.4, 0.1], dtype=np.float32)  # even length: zero past the low-side padding
+    assert rel_max(oxrf.psf_lines(oxrf.fp(vol, ang)[0], psf4), pkg.ProjectorUncorrected([n, n], ang, psf=psf4).fp(vol)) < TOL
+    with pytest.raises(ValueError):
+        pkg.ProjectorUncorrected([n, n], ang, psf=np.ones((3, 3)))

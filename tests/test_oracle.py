@@ -205,6 +205,35 @@ def test_maps_multiplicative():
     assert np.array_equal(m, (m_in * m_out).astype(F32))
 
 
+def test_detector_sub_angles_closed_form():
+    # surface 4 mm^2 at 1 mm: half-opening atan(1) = pi / 4, so two parts have midpoints -+pi / 8
+    assert np.allclose(oatt.detector_sub_angles(4.0, 1.0, 2), [-np.pi / 8, np.pi / 8], rtol=0, atol=1e-15)
+    assert np.allclose(oatt.detector_sub_angles(4.0, 1.0, 4), np.array([-3, -1, 1, 3]) * np.pi / 16, atol=1e-15)
+    assert np.array_equal(oatt.detector_sub_angles(4.0, 1.0, 1), [0.0])
+    assert np.array_equal(oatt.detector_sub_angles(0.0, 1.0, 5), [0.0])
+
+
+def test_sub_angle_maps_are_mean_of_shifted_detectors():
+    """Averaging over offsets s_k equals the mean of point detectors at phi + s_k (to the
+    rounding of (theta + phi) + s against theta + (phi + s)); asymmetric offsets so a sign or
+    normalisation slip shows; the in-beam factor multiplies the mean."""
+    rng = np.random.default_rng(8)
+    mu_in = (rng.random((16, 16)) * 0.05).astype(F32)
+    mu_out = (rng.random((16, 16)) * 0.1).astype(F32)
+    ang = np.linspace(0, 2 * np.pi, 5, endpoint=False).astype(F32)
+    phi, subs = np.pi / 2, np.array([0.1, 0.25, -0.05])
+    m = oatt.compute_maps(None, mu_out, ang, [phi], sub_angles=[subs])
+    parts = [oatt.compute_maps(None, mu_out, ang, [phi + s]) for s in subs]
+    assert rel_max(np.mean(parts, axis=0), m) < 1e-5
+    assert rel_max(parts[0], m) > 1e-3  # the offsets matter at this mu
+    m_in = oatt.compute_maps(mu_in, None, ang, [phi])
+    m_both = oatt.compute_maps(mu_in, mu_out, ang, [phi], sub_angles=[subs])
+    assert np.array_equal(m_both, (m_in * m).astype(F32))
+    # a single zero offset is the point detector, bit for bit
+    assert np.array_equal(oatt.compute_maps(mu_in, mu_out, ang, [phi], sub_angles=[np.zeros(1)]),
+                          oatt.compute_maps(mu_in, mu_out, ang, [phi]))
+
+
 def test_log_map_linear_in_mu_fp64():
     rng = np.random.default_rng(7)
     mu = rng.random((10, 10)) * 0.05
@@ -367,3 +396,21 @@ def test_oracle_fbp_filters_and_iradon_golden():
     assert np.max(np.abs(got - ref)) < 1e-5 * np.max(np.abs(ref))
     got = ofl.fbp(lambda y: oxrf.bp_skimage(y[None].astype(np.float32), rad, n), sino, len(rad), "hann")
     assert 1e-2 < np.max(np.abs(got - g["fbp_hann_vol"])) / np.max(np.abs(g["fbp_hann_vol"])) < 0.1
+
+
+def test_oracle_psf_matches_scipy_edge_convolution():
+    """The oracle's detector PSF equals the reference's recipe executed with scipy: edge padding
+    by (K-1)//2, scipy.signal.convolve(mode="same"), crop (``operators.py:462-480``)."""
+    import scipy.signal as spsig
+
+    from oracle import xrf as oxrf
+
+    rng = np.random.default_rng(79)
+    x = rng.random((5, 37))
+    for k_len in (1, 3, 4, 7):
+        kern = rng.random(k_len)
+        pw = (k_len - 1) // 2
+        xp = np.pad(x, ((0, 0), (pw, pw)), mode="edge")
+        ref = spsig.convolve(xp, kern[None, :], mode="same")
+        ref = ref[:, pw:xp.shape[1] - pw] if pw else ref
+        assert np.allclose(oxrf.psf_lines(x, kern), ref, rtol=1e-12, atol=1e-12), k_len
