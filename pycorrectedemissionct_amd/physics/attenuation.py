@@ -118,14 +118,15 @@ class AttenuationVolume:
         if n_slices == 1 and len(self.vol_shape_zyx) == 2:
             self.maps_device = maps[0]
         else:
-            # [A, D, Z, N, N] as the reference lays out 3-D maps
-            self.maps_device = maps.permute(1, 2, 0, 3, 4).contiguous()
+            # [A, D, Z, N, N] as the reference lays out 3-D maps: a view of the slice-major
+            # [Z, A, D, N, N] buffer the projector kernels read
+            self.maps_device = maps.permute(1, 2, 0, 3, 4)
 
     @property
     def maps(self):
         if self.maps_device is None:
             raise ValueError("Maps not computed: call `compute_maps` first")
-        return _arrays.from_dev(self.maps_device, self._host)
+        return _arrays.from_dev(self.maps_device.contiguous() if self._host else self.maps_device, self._host)
 
     def get_maps(self, roi=None, rot_ind=None, det_ind=None, binning: int = 1):
         """The maps, optionally restricted to angles / detectors (``attenuation.py:253-297``)."""

@@ -191,8 +191,6 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             self, vol_geom=vol_geom, angles_rot_rad=angles_rot_rad, rot_axis_shift_pix=rot_axis_shift_pix,
             prj_geom=prj_geom, psf=psf, prj_intensities=prj_intensities, backend=backend, create_single_projs=True,
         )
-        if self.vol_geom.is_3D():
-            raise NotImplementedError("the attenuation-corrected HIP projector handles 2D slices")
         if bp_mode not in ("skimage", "adjoint"):
             raise ValueError(f"Unknown bp_mode: {bp_mode}")
         if maps_mode not in ("precomputed", "otf"):
@@ -231,9 +229,8 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
                 raise ValueError("maps_mode='otf' builds the maps from att_in / att_out: do not pass att_maps")
             if att_in is None and att_out is None:
                 raise ValueError("No attenuation volumes were given.")
-            n = int(self.vol_shape[-1])
             for mu in (att_in, att_out):
-                if mu is not None and tuple(np.shape(mu)) != (n, n):
+                if mu is not None and tuple(np.shape(mu)) != tuple(self.vol_shape):
                     raise ValueError(f"Mismatching attenuation volume shape {np.shape(mu)} with vol_shape {self.vol_shape}")
             self.mu_in_dev = None if att_in is None else _arrays.to_dev(att_in)[0].contiguous()
             self.mu_out_dev = None if att_out is None else _arrays.to_dev(att_out)[0].contiguous()
@@ -250,6 +247,8 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
                     f"Mismatching volume shape of input volume ({att_maps.shape[-2:]})"
                     + f" with vol_shape ({self.vol_shape} in 2D -> {self.vol_shape[-2:]})"
                 )
+            if tuple(att_maps.shape[2:]) != tuple(self.vol_shape):
+                raise ValueError(f"Maps of shape {tuple(att_maps.shape)} do not match the volume {tuple(self.vol_shape)}")
             self.att_vol_angles = _arrays.to_dev(att_maps)[0]
         self.executor = None
 
@@ -262,13 +261,32 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             w_ang = w_ang * self.prj_intensities[:, None]
         return self.projector_backend.plan(self.angles_det_rad, self.weights_det, np.ascontiguousarray(w_ang))
 
+    @property
+    def _n_slices(self) -> int:
+        return int(np.prod(self.vol_shape[:-2])) if len(self.vol_shape) > 2 else 1
+
     def _maps(self, device) -> torch.Tensor | None:
+        """The maps in the kernels' layout: [A, D, N, N] for a slice, [Z, A, D, N, N] for a
+        volume (``att_vol_angles`` keeps the reference's [A, D, Z, N, N] view of it)."""
         m = self.att_vol_angles
         if m is None:
             return None
         if m.device != device:
             m = m.to(device)
-        return m.contiguous()
+        if len(self.vol_shape) > 2:
+            z = m.permute(2, 0, 1, 3, 4)
+            if not z.is_contiguous():
+                z = z.contiguous()
+                self.att_vol_angles = z.permute(1, 2, 0, 3, 4)
+            return z
+        if not m.is_contiguous():
+            m = m.contiguous()
+            self.att_vol_angles = m
+        return m
+
+    def _maps_stride(self) -> int:
+        n = int(self.vol_shape[-1])
+        return len(self.angles_rot_rad) * len(self.angles_det_rad) * n * n if len(self.vol_shape) > 2 else 0
 
     def _require_maps(self, what: str) -> None:
         if self.maps_mode == "otf":
@@ -277,7 +295,8 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
     def collapse_detectors(self) -> None:
         """Convert multi-detector configurations into single-detector (``projectors.py:492-500``)."""
         self._require_maps("collapse_detectors")
-        weights = np.reshape(self.weights_det, [1, -1, 1, 1]) / np.sum(self.weights_det)
+        # the detector axis is 1 of [A, D, N, N] and of [A, D, Z, N, N]
+        weights = np.reshape(self.weights_det, [1, -1] + [1] * (self.att_vol_angles.ndim - 2)) / np.sum(self.weights_det)
         w_dev = torch.as_tensor(weights, dtype=torch.float32, device=self.att_vol_angles.device)
         self.att_vol_angles = torch.sum(self.att_vol_angles * w_dev, dim=1, keepdim=True)
         weights = np.squeeze(weights)
@@ -285,13 +304,22 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         self.weights_angles = np.sum(self.weights_angles * weights, axis=1, keepdims=True)
         self.weights_det = np.sum(self.weights_det, keepdims=True)
 
+    # A volume [Z, N, N] is Z slices, each with its own maps; its sinogram is [D, Z, A, N]
+    # (projectors.py:570-596 stacks [D, Z, N] lines over the angle axis -2), squeezed to
+    # [Z, A, N] for one detector. The kernels run slice-major ([Z, D, A, N]).
     def fp_angle(self, vol, angle_ind: int):
         self._require_maps("fp_angle")
         t, host = _arrays.to_dev(vol)
         n = int(self.vol_shape[-1])
         d = len(self.angles_det_rad)
-        line = torch.empty((d, n), dtype=torch.float32, device=t.device)
-        self._plan().fp_angle(t, self._maps(t.device), line, int(angle_ind))
+        z = self._n_slices
+        t = t.reshape(z, n, n)
+        maps = self._maps(t.device)
+        plan = self._plan()
+        lines = torch.empty((z, d, n), dtype=torch.float32, device=t.device)
+        for iz in range(z):
+            plan.fp_angle(t[iz], maps if z == 1 else maps[iz], lines[iz], int(angle_ind))
+        line = lines[0] if len(self.vol_shape) == 2 else lines.permute(1, 0, 2)
         line = self._apply_psf(line)
         if d == 1:
             line = line[0]
@@ -302,48 +330,56 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
         t, host = _arrays.to_dev(sino)
         n = int(self.vol_shape[-1])
         d = len(self.weights_det)
+        z = self._n_slices
         line = t if single_line else t[..., angle_ind, :]
-        line = line.reshape(d, n).contiguous()
-        vol = torch.empty((n, n), dtype=torch.float32, device=t.device)
-        if self.bp_mode == "adjoint":
-            self._plan().bp_angle(line, self._maps(t.device), vol, int(angle_ind), _native.CHIP_BP_ADJOINT)
-        else:
-            maps = self._maps(t.device) if self.is_symmetric else None
-            self._plan().bp_angle(line, maps, vol, int(angle_ind), _native.CHIP_BP_SKIMAGE_SYM)
-        return _arrays.from_dev(vol, host)
+        line = line.reshape(d, z, n).permute(1, 0, 2).contiguous()
+        vol = torch.empty((z, n, n), dtype=torch.float32, device=t.device)
+        plan = self._plan()
+        maps = self._maps(t.device) if (self.bp_mode == "adjoint" or self.is_symmetric) else None
+        mode = _native.CHIP_BP_ADJOINT if self.bp_mode == "adjoint" else _native.CHIP_BP_SKIMAGE_SYM
+        for iz in range(z):
+            plan.bp_angle(line[iz], None if maps is None else (maps if z == 1 else maps[iz]), vol[iz],
+                          int(angle_ind), mode)
+        return _arrays.from_dev(vol.reshape(tuple(self.vol_shape)), host)
 
     def fp(self, vol):
         t, host = _arrays.to_dev(vol)
         n = int(self.vol_shape[-1])
         a = len(self.angles_rot_rad)
         d = len(self.angles_det_rad)
-        if t.numel() != n * n:
+        z = self._n_slices
+        if t.numel() != z * n * n:
             raise ValueError(f"Expected a volume of shape {tuple(self.vol_shape)}, got {tuple(t.shape)}")
-        out = torch.empty((d, a, n), dtype=torch.float32, device=t.device)
+        out = torch.empty((z, d, a, n), dtype=torch.float32, device=t.device)
         if self.maps_mode == "otf":
-            self._plan().fp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, 1)
+            self._plan().fp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, z)
         else:
-            self._plan().fp(t, self._maps(t.device), out, 1)
+            self._plan().fp(t, self._maps(t.device), out, z, self._maps_stride())
+        out = out[0] if len(self.vol_shape) == 2 else out.permute(1, 0, 2, 3)
         out = self._apply_psf(out)
         if d == 1:
             out = out[0]
-        return _arrays.from_dev(out, host)
+        return _arrays.from_dev(out.contiguous(), host)
 
     def bp(self, sino):
         t, host = _arrays.to_dev(sino)
         n = int(self.vol_shape[-1])
         a = len(self.angles_rot_rad)
         d = len(self.weights_det)
-        if t.numel() != d * a * n:
-            raise ValueError(f"Expected a sinogram with {d * a * n} elements, got {tuple(t.shape)}")
-        out = torch.empty((n, n), dtype=torch.float32, device=t.device)
+        z = self._n_slices
+        if t.numel() != d * z * a * n:
+            raise ValueError(f"Expected a sinogram with {d * z * a * n} elements, got {tuple(t.shape)}")
+        if z > 1:
+            t = t.reshape(d, z, a, n).permute(1, 0, 2, 3)
+        t = t.contiguous()
+        out = torch.empty((z, n, n), dtype=torch.float32, device=t.device)
         mode = _native.CHIP_BP_ADJOINT if self.bp_mode == "adjoint" else _native.CHIP_BP_SKIMAGE_SYM
         if self.maps_mode == "otf" and (self.bp_mode == "adjoint" or self.is_symmetric):
-            self._plan().bp_otf(t.contiguous(), self.mu_in_dev, self.mu_out_dev, out, 1, mode)
+            self._plan().bp_otf(t, self.mu_in_dev, self.mu_out_dev, out, z, mode)
         elif self.bp_mode == "adjoint" or self.is_symmetric:
-            self._plan().bp(t, self._maps(t.device), out, 1, mode)
+            self._plan().bp(t, self._maps(t.device), out, z, mode, self._maps_stride())
         else:
             if self.prj_intensities is not None:
-                t = (t.reshape(d, a, n) * self._intensities_dev(t.device)[None, :, None]).contiguous()
-            self._plan(with_intensities=False).bp(t, None, out, 1, _native.CHIP_BP_SKIMAGE)
-        return _arrays.from_dev(out, host)
+                t = (t.reshape(z, d, a, n) * self._intensities_dev(t.device)[None, None, :, None]).contiguous()
+            self._plan(with_intensities=False).bp(t, None, out, z, _native.CHIP_BP_SKIMAGE)
+        return _arrays.from_dev(out.reshape(tuple(self.vol_shape)), host)

@@ -330,3 +330,138 @@ def test_detector_psf(pkg):
     assert rel_max(oxrf.psf_lines(oxrf.fp(vol, ang)[0], psf4), pkg.ProjectorUncorrected([n, n], ang, psf=psf4).fp(vol)) < TOL
     with pytest.raises(ValueError):
         pkg.ProjectorUncorrected([n, n], ang, psf=np.ones((3, 3)))
+
+
+def test_prj_intensities(pkg):
+    """Per-angle beam intensities (projectors.py:275-278, 297-298, 319-320, 339-340) scale every
+    sinogram line in the fp and the line before every bp: uncorrected projector bit-exact against
+    scaling outside; XRF fp / non-symmetric bp / symmetric bp against the oracle with the
+    intensities folded into the angle weights, and the adjoint mode stays the fp's adjoint."""
+    import torch
+
+    n, a = 40, 12
+    rng = np.random.default_rng(930)
+    ang = _angles(a)
+    inten = rng.random(a) + 0.5
+    vol = _phantom(n, rng)
+    plain = pkg.ProjectorUncorrected([n, n], ang)
+    scaled = pkg.ProjectorUncorrected([n, n], ang, prj_intensities=inten)
+    i32 = inten.astype(np.float32)[:, None]
+    assert np.array_equal(scaled.fp(vol), plain.fp(vol) * i32)
+    y1 = rng.random((a, n)).astype(np.float32)
+    assert np.array_equal(scaled.bp(y1), plain.bp(y1 * i32))
+    assert np.array_equal(scaled.fp_angle(vol, 5), plain.fp_angle(vol, 5) * i32[5])
+
+    det = np.array([np.pi / 2, -np.pi / 2])
+    mu_in = (rng.random((n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.08).astype(np.float32)
+    maps = oatt.compute_maps(mu_in, mu_out, ang, det)
+    w_det = np.array([1.0, 0.7])
+    w_ang_i = np.ones((a, 2)) * inten[:, None]
+    y = rng.random((2, a, n)).astype(np.float32)
+    kw = dict(att_maps=maps, angles_detectors_rad=det, weights_detectors=w_det, prj_intensities=inten)
+    xrf = pkg.ProjectorAttenuationXRF([n, n], ang, **kw)
+    assert rel_max(oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang_i), xrf.fp(vol)) < TOL
+    assert rel_max(oxrf.fp(vol, ang, maps, w_det=w_det, w_ang=w_ang_i)[:, 7], xrf.fp_angle(vol, 7)) < TOL
+    ref_ns = oxrf.bp_skimage(y * inten.astype(np.float32)[None, :, None], ang, n)
+    assert rel_max(ref_ns, xrf.bp(y)) < TOL
+    sym = pkg.ProjectorAttenuationXRF([n, n], ang, is_symmetric=True, **kw)
+    ref_s = oxrf.bp_skimage(y, ang, n, maps, w_det=w_det, w_ang=w_ang_i, symmetric=True)
+    assert rel_max(ref_s, sym.bp(y)) < TOL
+    adj = pkg.ProjectorAttenuationXRF([n, n], ang, bp_mode="adjoint", **kw)
+    x1 = torch.as_tensor(rng.random((n, n)), dtype=torch.float32, device="cuda")
+    yt = torch.as_tensor(y, device="cuda")
+    lhs = float(torch.sum(adj.fp(x1).double() * yt.double()))
+    rhs = float(torch.sum(x1.double() * adj.bp(yt).double()))
+    assert abs(lhs - rhs) / abs(lhs) < TOL
+
+
+def test_collapse_detectors(pkg):
+    """collapse_detectors (projectors.py:492-500): the maps become the detector-weight average,
+    one detector of weight sum(w) at the weighted mean angle; the fp then equals the oracle's
+    single-detector fp with those maps and weights. The collapsed maps keep their detector axis
+    ([A, 1, N, N]) so the per-angle products stay slice-shaped."""
+    n, a = 36, 10
+    rng = np.random.default_rng(940)
+    ang = _angles(a)
+    det = np.array([np.pi / 2, -np.pi / 2, 0.4])
+    w_det = np.array([1.0, 3.0, 0.5])
+    w_ang = rng.random((a, 3)) + 0.5
+    mu_in = (rng.random((n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.08).astype(np.float32)
+    maps = oatt.compute_maps(mu_in, mu_out, ang, det)
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=det,
+                                      weights_detectors=w_det, weights_angles=w_ang)
+    prj.collapse_detectors()
+    wn = w_det / w_det.sum()
+    maps_c = np.sum(maps * wn.reshape(1, -1, 1, 1).astype(np.float32), axis=1, keepdims=True)
+    w_ang_c = np.sum(w_ang * wn, axis=1, keepdims=True)
+    assert np.isclose(prj.angles_det_rad[0], np.sum(det * wn))
+    assert prj.weights_det.shape == (1,) and np.isclose(prj.weights_det[0], w_det.sum())
+    vol = _phantom(n, rng)
+    ref = oxrf.fp(vol, ang, maps_c, w_det=[w_det.sum()], w_ang=w_ang_c)[0]
+    got = prj.fp(vol)
+    assert got.shape == (a, n)
+    assert rel_max(ref, got) < TOL
+    y = rng.random((a, n)).astype(np.float32)
+    assert rel_max(oxrf.bp_skimage(y, ang, n), prj.bp(y)) < TOL
+
+
+def test_xrf_volume_equals_slices(pkg):
+    """A [Z, N, N] volume with its own attenuation per slice: maps [A, D, Z, N, N], fp [D, Z, A, N],
+    bp (non-symmetric, symmetric, adjoint), fp_angle / bp_angle and the on-the-fly mode equal the
+    slice-by-slice 2-D projectors (projectors.py:502-622 apply each slice's maps)."""
+    n, a, z = 32, 12, 3
+    rng = np.random.default_rng(950)
+    ang = _angles(a)
+    det = np.array([np.pi / 2, -np.pi / 2])
+    mu_in = (rng.random((z, n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((z, n, n)) * 0.08).astype(np.float32)
+    vol = np.stack([_phantom(n, rng) for _ in range(z)])
+    y = rng.random((2, z, a, n)).astype(np.float32)
+    kw = dict(att_in=mu_in, att_out=mu_out, angles_detectors_rad=det, weights_detectors=[1.0, 0.6])
+    for extra in (dict(), dict(is_symmetric=True), dict(bp_mode="adjoint")):
+        p3 = pkg.ProjectorAttenuationXRF([n, n, z], ang, **kw, **extra)
+        p2 = [pkg.ProjectorAttenuationXRF([n, n], ang, **{**kw, "att_in": mu_in[i], "att_out": mu_out[i]}, **extra)
+              for i in range(z)]
+        assert p3.att_vol_angles.shape == (a, 2, z, n, n)
+        f3 = p3.fp(vol)
+        assert f3.shape == (2, z, a, n)
+        f2 = np.stack([p2[i].fp(vol[i]) for i in range(z)], axis=1)
+        assert rel_max(f2, f3) < 1e-6
+        b3 = p3.bp(y)
+        b2 = np.stack([p2[i].bp(np.ascontiguousarray(y[:, i])) for i in range(z)])
+        assert b3.shape == (z, n, n) and rel_max(b2, b3) < 1e-6
+        la = p3.fp_angle(vol, 4)
+        assert la.shape == (2, z, n) and rel_max(f2[:, :, 4], la) < 1e-6
+        ba = p3.bp_angle(y, 4)
+        assert rel_max(np.stack([p2[i].bp_angle(np.ascontiguousarray(y[:, i]), 4) for i in range(z)]), ba) < 1e-6
+    otf = pkg.ProjectorAttenuationXRF([n, n, z], ang, maps_mode="otf", bp_mode="adjoint", **kw)
+    assert rel_max(f3, otf.fp(vol)) < TOL and rel_max(b3, otf.bp(y)) < TOL
+    # one detector squeezes the detector axis; the given maps in the reference's layout are accepted
+    maps1 = np.ascontiguousarray(p3.att_vol_angles[:, :1].cpu().numpy())
+    one = pkg.ProjectorAttenuationXRF([n, n, z], ang, att_maps=maps1)
+    assert one.fp(vol).shape == (z, a, n)
+    assert rel_max(f2[0], one.fp(vol)) < 1e-6
+
+
+def test_sirt_pdhg_on_volume_equal_slices(pkg):
+    """SIRT and PDHG-TV2D on a [Z, N, N] XRF volume equal the per-slice solves (the scaling and
+    the 2-D TV act slice by slice)."""
+    n, a, z = 32, 16, 2
+    rng = np.random.default_rng(960)
+    ang = _angles(a)
+    mu_in = (rng.random((z, n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((z, n, n)) * 0.08).astype(np.float32)
+    vol = np.stack([_phantom(n, rng) for _ in range(z)])
+    p3 = pkg.ProjectorAttenuationXRF([n, n, z], ang, att_in=mu_in, att_out=mu_out)
+    p2 = [pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in[i], att_out=mu_out[i]) for i in range(z)]
+    b = p3.fp(vol)
+    x3, _ = pkg.SIRT(verbose=False)(p3, b, 20)
+    x2 = np.stack([pkg.SIRT(verbose=False)(p2[i], np.ascontiguousarray(b[i]), 20)[0] for i in range(z)])
+    assert rel_max(x2, x3) < 1e-5
+    reg = pkg.regularizers.Regularizer_TV2D(1e-2)
+    y3, _ = pkg.PDHG(verbose=False, regularizer=reg)(p3, b, 20)
+    y2 = np.stack([pkg.PDHG(verbose=False, regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(
+        p2[i], np.ascontiguousarray(b[i]), 20)[0] for i in range(z)])
+    assert rel_max(y2, y3) < 1e-5
