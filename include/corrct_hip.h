@@ -231,6 +231,28 @@ int chip_filter_lines(const float* in, float* out, const double* h, int64_t n_li
 int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, int64_t n_lines, int n,
                    void* stream);
 
+/*
+ * Gradient-family regularizers over the last nd (1..3) axes, sizes dims[nd]
+ * (slowest first), of a primal laid out [n_chan][n_batch][dims...]; the dual
+ * q is [nd][n_chan][n_batch][dims...] (TransformGradient.adj_shape,
+ * operators.py:767-844).
+ * chip_grad_dual: q += 0.5 * grad(x) (edge padding: the last difference is
+ * 0), then the dual proximal of `mode`: 0 = l12 (isotropic TV,
+ * Regularizer_TV1D/2D/3D, regularizers.py:218-337 with data_terms.py:598-608),
+ * 1 = l1 (anisotropic), 2 = VTV with point-wise derivative and channel norms
+ * der_norm / chan_norm in {1, 2, 0 = inf} (Regularizer_VTV.apply_proximal_dual,
+ * regularizers.py:1362-1430), 3 = TNV (singular values of each C x nd matrix
+ * clipped at 1, Regularizer_TNV with DataFidelity_ln, regularizers.py:1324-1359,
+ * data_terms.py:781-820). Modes 0 and 1 need n_chan == 1; n_chan <= 16.
+ * chip_grad_update_primal: upd += t (or -= t when subtract) with
+ * t = -div(q) times scale when apply_scale (compute_update_primal,
+ * regularizers.py:194-210; PDHG adds, SIRT subtracts, solvers.py:953-958, 1178-1180).
+ */
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
+                   float weight, int der_norm, int chan_norm, void* stream);
+int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
+                            float scale, int apply_scale, int subtract, void* stream);
+
 /* FISTA step (solvers.py:1458-1490): x_new = y + tau * grad (tau per element,
  * or tau_scalar when tau is NULL), soft-thresholded at tau * l1_weight when
  * l1_weight > 0 (Regularizer_l1.apply_proximal_primal, regularizers.py:559-585),

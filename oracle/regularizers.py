@@ -1,0 +1,126 @@
+"""CPU restatement of corrct's gradient-family regularizers (float32 numpy).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+* ``grad_nd`` / ``div_nd`` -- ``TransformGradient`` with edge padding over the
+  last ``nd`` axes (``operators.py:796-844``): forward differences with the
+  last one 0; the divergence's backward differences with the first one 0,
+  summed over the axes in order. ``-div_nd`` is the operator's adjoint.
+* ``prox_l12`` / ``prox_l1`` -- ``DataFidelity_l12`` / ``DataFidelity_l1``
+  dual proximals with a weight (``data_terms.py:549-555, 598-608``).
+* ``prox_vtv`` -- ``Regularizer_VTV.apply_proximal_dual``
+  (``regularizers.py:1394-1430``), channels on axis 1 of the dual.
+* ``prox_tnv`` -- ``DataFidelity_ln.apply_proximal_dual`` with ``ln_axes=(1, 0)``
+  and the l1 spectral norm (``data_terms.py:781-820``): numpy's SVD of each
+  (channels x derivatives) matrix, singular values ``s / max(s, 1)``.
+* ``GradReg`` -- ``Regularizer_Grad`` bookkeeping (``regularizers.py:134-215,
+  260-273``): tau contribution ``2 nd`` (times the weight outside the l1
+  family), dual update ``q += 0.5 grad`` then the prox, primal contribution
+  ``-div q`` (times the weight outside the l1 family).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def grad_nd(x: np.ndarray, nd: int) -> np.ndarray:
+    x = np.asarray(x)
+    g = np.zeros((nd, *x.shape), dtype=x.dtype)
+    for k in range(nd):
+        ax = x.ndim - nd + k
+        sl_lo = [slice(None)] * x.ndim
+        sl_hi = [slice(None)] * x.ndim
+        sl_lo[ax] = slice(0, -1)
+        sl_hi[ax] = slice(1, None)
+        g[(k, *sl_lo)] = x[tuple(sl_hi)] - x[tuple(sl_lo)]
+    return g
+
+
+def div_nd(q: np.ndarray) -> np.ndarray:
+    nd = q.shape[0]
+    ndim = q.ndim - 1
+    out = None
+    for k in range(nd):
+        ax = ndim - nd + k
+        d = np.zeros(q.shape[1:], dtype=q.dtype)
+        sl_lo = [slice(None)] * ndim
+        sl_hi = [slice(None)] * ndim
+        sl_lo[ax] = slice(0, -1)
+        sl_hi[ax] = slice(1, None)
+        d[tuple(sl_hi)] = q[(k, *sl_hi)] - q[(k, *sl_lo)]
+        out = d if out is None else out + d
+    return out
+
+
+def prox_l12(q: np.ndarray, w: float) -> np.ndarray:
+    nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=q.dtype))
+    return (q / np.fmax(nrm, q.dtype.type(w))) * q.dtype.type(w)
+
+
+def prox_l1(q: np.ndarray, w: float) -> np.ndarray:
+    return (q / np.fmax(np.abs(q), q.dtype.type(w))) * q.dtype.type(w)
+
+
+def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None) -> np.ndarray:
+    """``q`` is [nd, C, ...] (or [nd, ...] for one channel when ``q.ndim == nd + 1``)."""
+    nd = q.shape[0] if nd is None else nd
+    scalar = q.ndim == nd + 1
+    t = q[:, None] if scalar else q
+    if der == 1:
+        gn = np.abs(t)
+    elif der == 2:
+        gn = np.linalg.norm(t, axis=0, ord=2, keepdims=True)
+    else:
+        gn = np.linalg.norm(t, axis=0, ord=1, keepdims=True)
+    if chan == 1:
+        dn = gn
+    elif chan == 2:
+        dn = np.linalg.norm(gn, axis=1, ord=2, keepdims=True)
+    else:
+        dn = np.linalg.norm(gn, axis=1, ord=1, keepdims=True)
+    t = (t / np.fmax(dn, t.dtype.type(w))) * t.dtype.type(w)
+    return t[:, 0] if scalar else t
+
+
+def prox_tnv(q: np.ndarray) -> np.ndarray:
+    """``q`` is [nd, C, ...]: each (C x nd) matrix keeps its singular vectors, s -> s / max(s, 1)."""
+    nd, c = q.shape[:2]
+    m = np.moveaxis(q.reshape(nd, c, -1), -1, 0)  # [P, nd, C]
+    m = np.swapaxes(m, 1, 2).astype(np.float64)  # [P, C, nd]
+    u, s, vt = np.linalg.svd(m, full_matrices=False)
+    s = s / np.fmax(s, 1.0)
+    r = np.matmul(u, s[..., None] * vt)  # [P, C, nd]
+    r = np.moveaxis(np.swapaxes(r, 1, 2), 0, -1).reshape(q.shape)
+    return r.astype(q.dtype)
+
+
+class GradReg:
+    """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv"}."""
+
+    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf):
+        self.nd, self.weight, self.kind, self.der, self.chan = nd, float(weight), kind, der, chan
+
+    @property
+    def weighted(self) -> bool:
+        return self.kind == "tnv"  # outside the l1 family
+
+    def tau(self) -> float:
+        t = 2.0 * self.nd
+        return t * self.weight if self.weighted else t
+
+    def dual(self, q: np.ndarray, x: np.ndarray) -> np.ndarray:
+        q = q + F32(0.5) * grad_nd(x, self.nd)
+        if self.kind == "l12":
+            return prox_l12(q, self.weight)
+        if self.kind == "l1":
+            return prox_l1(q, self.weight)
+        if self.kind == "vtv":
+            return prox_vtv(q, self.weight, self.der, self.chan, self.nd)
+        return prox_tnv(q)
+
+    def primal(self, q: np.ndarray) -> np.ndarray:
+        u = -div_nd(q)
+        return u * F32(self.weight) if self.weighted else u

@@ -15,7 +15,8 @@ GPU projector's explicit matrix.
 * ``sart``              -- ``solvers.py:473-631`` (per-angle callables; explicit row order)
 * ``power_method``      -- ``solvers.py:26-57`` (explicit start vector)
 * ``fista``             -- ``solvers.py:1348-1513`` (l2 data term, optional l1 prox)
-* ``pdhg``              -- ``solvers.py:1060-1206`` with ``DataFidelity_l2``
+* ``pdhg``              -- ``solvers.py:1060-1206`` (gradient-family regularizers
+                           from ``oracle/regularizers.py`` via ``regs``) with ``DataFidelity_l2``
                            (``data_terms.py:283-331``) or ``DataFidelity_KL``
                            (``data_terms.py:677-692``)
 """
@@ -78,9 +79,12 @@ def tv_dual(q: np.ndarray, xbar: np.ndarray, weight: float) -> np.ndarray:
     return q * F32(weight)
 
 
-def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=None, b_mask=None, tv_weight=None, relaxation=1.95):
+def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=None, b_mask=None, tv_weight=None,
+         relaxation=1.95, regs=()):
+    """``regs``: ``oracle.regularizers.GradReg`` objects, duals re-initialised every iteration."""
     b = np.asarray(b, dtype=F32)
-    sigma, tau = diagonal_scaling(A, At, b, x_shape, tau_reg=4.0 if tv_weight is not None else 0.0, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
+    tau_reg = (4.0 if tv_weight is not None else 0.0) + sum(r.tau() for r in regs)
+    sigma, tau = diagonal_scaling(A, At, b, x_shape, tau_reg=tau_reg, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
     for _ in range(iterations):
         res = b - A(x)
@@ -90,6 +94,8 @@ def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=
         if tv_weight is not None:
             q = tv_dual(np.zeros((2, *x_shape), dtype=F32), x, tv_weight)
             upd = upd - (-div(q))
+        for r in regs:
+            upd = upd - r.primal(r.dual(np.zeros((r.nd, *x_shape), dtype=F32), x))
         x = x + upd * tau
         if lower is not None or upper is not None:
             x = x.clip(lower, upper)
@@ -206,10 +212,12 @@ def mlem(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=
 
 
 def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None, lower=None, upper=None,
-         x_mask=None, b_mask=None, relaxation=0.95):
+         x_mask=None, b_mask=None, relaxation=0.95, regs=()):
+    """``regs``: ``oracle.regularizers.GradReg`` objects beside (or instead of) the 2-D TV ``tv_weight``."""
     b = np.asarray(b, dtype=F32)
+    tau_reg = (4.0 if tv_weight is not None else 0.0) + sum(r.tau() for r in regs)
     sigma, tau = diagonal_scaling(
-        A, At, b, x_shape, tau_reg=4.0 if tv_weight is not None else 0.0, rho_sigma=relaxation, rho_tau=relaxation,
+        A, At, b, x_shape, tau_reg=tau_reg, rho_sigma=relaxation, rho_tau=relaxation,
         x_mask=x_mask, b_mask=b_mask,
     )
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
@@ -223,6 +231,7 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
         raise ValueError(data_term)
     p = np.zeros_like(b)
     q = np.zeros((2, *x_shape), dtype=F32) if tv_weight is not None else None
+    qs = [np.zeros((r.nd, *x_shape), dtype=F32) for r in regs]
     for _ in range(iterations):
         p = p + A(xr) * sigma
         if data_term == "l2":
@@ -231,10 +240,13 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
             p = (F32(1) + p - np.sqrt((p - F32(1)) ** 2 + sigma_data)) / F32(2)
         if b_mask is not None:
             p = p * b_mask
+        qs = [r.dual(q_r, xr) for r, q_r in zip(regs, qs)]
         upd = At(p)
         if q is not None:
             q = tv_dual(q, xr, tv_weight)
             upd = upd + (-div(q))
+        for r, q_r in zip(regs, qs):
+            upd = upd + r.primal(q_r)
         x_new = x - upd * tau
         if lower is not None or upper is not None:
             x_new = x_new.clip(lower, upper)

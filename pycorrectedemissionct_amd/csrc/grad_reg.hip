@@ -1,0 +1,341 @@
+// Gradient-family regularizers of libcorrct_hip (gfx950): the dual update,
+// dual proximal and primal contribution of Regularizer_Grad and its
+// subclasses TV1D / TV2D / TV3D, VTV and TNV (regularizers.py:134-215,
+// 218-337, 1324-1440) for gradients over the last 1..3 axes.
+//
+// Layout: the primal is [C][B][S] with S the product of the gradient axes'
+// sizes, C the channels a VTV / TNV prox couples (1 for TV) and B the
+// remaining axes, which stay independent; the dual is [nd][C][B][S], the
+// reference's TransformGradient.adj_shape (operators.py:781-793). One thread
+// owns one (b, s) and all C * nd dual entries there, so each prox is local
+// and every dual entry is read and written once (coalesced along s).
+//
+// Float32 order follows the reference's numpy expressions:
+//   dual   q = q + 0.5 * (x[i+1] - x[i])        (0 past the last sample)
+//   l12    q = (q / max(|q|_2 over nd, w)) * w   (data_terms.py:549-555, 607-608)
+//   l1     q = (q / max(|q|, w)) * w
+//   VTV    q = (q / max(dn, w)) * w with dn the point-wise derivative norm
+//          (1, 2, inf) followed by the channel norm (1, 2, inf); the dual
+//          norms of inf are 1, as in regularizers.py:1395-1411
+//   TNV    per (b, s) the C x nd matrix M = U diag(s) V^T becomes
+//          U diag(s / max(s, 1)) V^T (data_terms.py:781-820 with the l1 dual
+//          prox of weight 1); computed as M V diag(1 / max(s, 1)) V^T from a
+//          float64 Jacobi eigen-decomposition of M^T M (nd <= 3)
+//   primal upd = upd +- scale * (-div q), div with the first difference 0
+//          (operators.py:820-844), summed over the axes in order.
+#include "common.hpp"
+
+namespace chip {
+namespace {
+
+constexpr int kGrBlock = 256;
+constexpr int kGrMaxChan = 16;
+
+enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3 };
+
+struct GradGeom {
+  int C;        // coupled channels
+  int nd;       // gradient axes (1..3)
+  int64_t B;    // independent batch
+  int64_t S;    // samples per (c, b)
+  int dim[3];   // sizes of the gradient axes, slowest first
+  int64_t stride[3];
+};
+
+inline dim3 gr_grid(int64_t n) {
+  int64_t blocks = (n + kGrBlock - 1) / kGrBlock;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  return dim3(static_cast<unsigned>(blocks));
+}
+
+__device__ __forceinline__ void sample_index(const GradGeom& g, int64_t s, int idx[3]) {
+  int64_t r = s;
+  for (int k = g.nd - 1; k >= 0; --k) {
+    idx[k] = static_cast<int>(r % g.dim[k]);
+    r /= g.dim[k];
+  }
+}
+
+// Cyclic Jacobi on the symmetric nd x nd matrix a (overwritten by its
+// eigenvalues on the diagonal); v receives the eigenvectors as columns.
+__device__ void jacobi_sym(double a[3][3], double v[3][3], int nd) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) v[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < nd; ++p) {
+      diag += a[p][p] * a[p][p];
+      for (int q = p + 1; q < nd; ++q) off += a[p][q] * a[p][q];
+    }
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int p = 0; p < nd; ++p) {
+      for (int q = p + 1; q < nd; ++q) {
+        const double apq = a[p][q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0);
+        const double s = t * c;
+        // A <- J^T A J with J = [[c, s], [-s, c]] on (p, q); V <- V J
+        for (int k = 0; k < nd; ++k) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < nd; ++k) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < nd; ++k) {
+          const double vkp = v[k][p], vkq = v[k][q];
+          v[k][p] = c * vkp - s * vkq;
+          v[k][q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float pnorm_acc(float acc, float v, int ord) {
+  // running point-wise norm: ord 2 accumulates squares, 1 and 0 (inf's dual, 1) absolutes
+  return ord == 2 ? acc + v * v : acc + fabsf(v);
+}
+
+// ND gradient axes and at most CM channels, both compile-time so the C * ND
+// dual values stay in registers; g.C <= CM channels are live.
+template <int ND, int CM>
+__global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__ q, const float* __restrict__ x,
+                                                             GradGeom g, int mode, float w, int der, int chan) {
+  const int64_t total = g.B * g.S;
+  const int64_t cs = g.B * g.S;  // channel stride of x and of each q component
+  const int64_t qs = static_cast<int64_t>(g.C) * cs;  // component stride of q
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t b = t / g.S;
+    const int64_t s = t - b * g.S;
+    int idx[3];
+    sample_index(g, s, idx);
+    const int64_t off = b * g.S + s;
+    float v[ND][CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      if (c >= g.C) break;
+      const float* xc = x + c * cs + off;
+      const float x0 = xc[0];
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        const float gk = (idx[k] + 1 < g.dim[k]) ? xc[g.stride[k]] - x0 : 0.f;
+        v[k][c] = q[k * qs + c * cs + off] + 0.5f * gk;
+      }
+    }
+    if (mode == kGrL12) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) acc = acc + v[k][c] * v[k][c];
+        const float den = fmaxf(sqrtf(acc), w);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * w;
+      }
+    } else if (mode == kGrL1) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / fmaxf(fabsf(v[k][c]), w)) * w;
+      }
+    } else if (mode == kGrVTV) {
+      if (der == 1) {
+        // gn = |q| per (k, c); the channel norm runs over c for each k
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+          float acc = 0.f;
+#pragma unroll
+          for (int c = 0; c < CM; ++c)
+            if (c < g.C) acc = pnorm_acc(acc, fabsf(v[k][c]), chan);
+          const float dn_k = chan == 2 ? sqrtf(acc) : acc;
+#pragma unroll
+          for (int c = 0; c < CM; ++c) {
+            if (c >= g.C) break;
+            const float dn = chan == 1 ? fabsf(v[k][c]) : dn_k;
+            v[k][c] = (v[k][c] / fmaxf(dn, w)) * w;
+          }
+        }
+      } else {
+        float gn[CM];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+          float acc = 0.f;
+#pragma unroll
+          for (int k = 0; k < ND; ++k) acc = pnorm_acc(acc, c < g.C ? v[k][c] : 0.f, der);
+          gn[c] = der == 2 ? sqrtf(acc) : acc;
+        }
+        float acc = 0.f;
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < g.C) acc = pnorm_acc(acc, gn[c], chan);
+        const float dn_all = chan == 2 ? sqrtf(acc) : acc;
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+          if (c >= g.C) break;
+          const float den = fmaxf(chan == 1 ? gn[c] : dn_all, w);
+#pragma unroll
+          for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * w;
+        }
+      }
+    } else {  // kGrTNV
+      double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, ev[3][3];
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int k = j; k < ND; ++k) {
+          double acc = 0.0;
+#pragma unroll
+          for (int c = 0; c < CM; ++c)
+            if (c < g.C) acc += static_cast<double>(v[j][c]) * v[k][c];
+          a[j][k] = acc;
+          a[k][j] = acc;
+        }
+      jacobi_sym(a, ev, ND);
+      double h[3];
+#pragma unroll
+      for (int i = 0; i < ND; ++i) h[i] = 1.0 / fmax(sqrt(fmax(a[i][i], 0.0)), 1.0);
+      double hm[3][3];
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+          double acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < ND; ++i) acc += ev[j][i] * h[i] * ev[k][i];
+          hm[j][k] = acc;
+        }
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+        double m[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) m[j] = v[j][c];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) acc += m[j] * hm[j][k];
+          v[k][c] = static_cast<float>(acc);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      if (c >= g.C) break;
+#pragma unroll
+      for (int k = 0; k < ND; ++k) q[k * qs + c * cs + off] = v[k][c];
+    }
+  }
+}
+
+template <int ND>
+void launch_grad_dual(float* q, const float* x, const GradGeom& g, int mode, float w, int der, int chan,
+                      hipStream_t st) {
+  const dim3 grid = gr_grid(g.B * g.S);
+  if (g.C == 1)
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 1>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan);
+  else if (g.C <= 4)
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 4>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan);
+  else
+    hipLaunchKernelGGL((grad_dual_kernel<ND, kGrMaxChan>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der,
+                       chan);
+}
+
+__global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict__ upd, const float* __restrict__ q,
+                                                               GradGeom g, float scale, int apply_scale,
+                                                               int subtract) {
+  const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t s = e % g.S;
+    int idx[3];
+    sample_index(g, s, idx);
+    float d = 0.f;
+    for (int k = 0; k < g.nd; ++k) {
+      const float* qk = q + k * total + e;
+      const float dk = (idx[k] > 0) ? qk[0] - qk[-g.stride[k]] : 0.f;
+      d = (k == 0) ? dk : d + dk;
+    }
+    float t = -d;
+    if (apply_scale) t = t * scale;
+    upd[e] = subtract ? upd[e] - t : upd[e] + t;
+  }
+}
+
+int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims) {
+  CHIP_CHECK_ARG(nd >= 1 && nd <= 3, "nd must be 1, 2 or 3");
+  CHIP_CHECK_ARG(dims != nullptr, "NULL dims");
+  CHIP_CHECK_ARG(n_chan >= 1 && n_chan <= kGrMaxChan, "n_chan must be in [1, 16]");
+  CHIP_CHECK_ARG(n_batch >= 1, "n_batch must be >= 1");
+  g.C = n_chan;
+  g.nd = nd;
+  g.B = n_batch;
+  g.S = 1;
+  for (int k = 0; k < 3; ++k) {
+    g.dim[k] = 1;
+    g.stride[k] = 0;
+  }
+  for (int k = 0; k < nd; ++k) {
+    CHIP_CHECK_ARG(dims[k] >= 1, "gradient axis sizes must be >= 1");
+    g.dim[k] = dims[k];
+    g.S *= dims[k];
+  }
+  int64_t st = 1;
+  for (int k = nd - 1; k >= 0; --k) {
+    g.stride[k] = st;
+    st *= g.dim[k];
+  }
+  return CHIP_OK;
+}
+
+}  // namespace
+}  // namespace chip
+
+using namespace chip;
+
+extern "C" {
+
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
+                   float weight, int der_norm, int chan_norm, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrTNV, "unknown mode");
+  CHIP_CHECK_ARG(mode != kGrVTV || ((der_norm == 0 || der_norm == 1 || der_norm == 2) &&
+                                    (chan_norm == 0 || chan_norm == 1 || chan_norm == 2)),
+                 "VTV point-wise norms must be 1, 2 or 0 (inf)");
+  CHIP_CHECK_ARG(mode == kGrVTV || mode == kGrTNV || n_chan == 1, "coupled channels need the VTV or TNV mode");
+  GradGeom g;
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims);
+  if (rc != CHIP_OK) return rc;
+  hipStream_t st = as_stream(stream);
+  if (nd == 1) launch_grad_dual<1>(q, x, g, mode, weight, der_norm, chan_norm, st);
+  else if (nd == 2) launch_grad_dual<2>(q, x, g, mode, weight, der_norm, chan_norm, st);
+  else launch_grad_dual<3>(q, x, g, mode, weight, der_norm, chan_norm, st);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
+                            float scale, int apply_scale, int subtract, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(upd != nullptr && q != nullptr, "NULL argument");
+  GradGeom g;
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims);
+  if (rc != CHIP_OK) return rc;
+  hipLaunchKernelGGL(grad_primal_kernel, gr_grid(static_cast<int64_t>(g.C) * g.B * g.S), dim3(kGrBlock), 0,
+                     as_stream(stream), upd, q, g, scale, apply_scale, subtract);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+}  // extern "C"

@@ -98,6 +98,20 @@ class DataFidelity_l12(DataFidelity_l1):
         self.l2_axis = l2_axis
 
 
+class DataFidelity_ln(DataFidelityBase):
+    """Nuclear-norm data fidelity (``data_terms.py:760-842``): the TNV dual norm. Its dual proximal
+    (singular values of each ``ln_axes`` matrix through ``spectral_norm``'s dual proximal) runs
+    fused in ``chip_grad_dual`` for the ``Regularizer_TNV`` layout."""
+
+    __data_fidelity_name__ = "ln"
+
+    def __init__(self, background=None, ln_axes=(1, -1), spectral_norm: DataFidelityBase | None = None):
+        super().__init__(background=background)
+        self.ln_axes = ln_axes
+        self.spectral_norm = DataFidelity_l1() if spectral_norm is None else spectral_norm
+        self.use_fallback = False
+
+
 class DataFidelity_KL(DataFidelityBase):
     """Kullback-Leibler data fidelity (``data_terms.py:677-757``)."""
 

@@ -9,6 +9,14 @@ its primal proximal (soft thresholding at tau * weight) is fused into
 edge-padded forward gradient, ``sigma = 1/2`` and ``tau = 2 * ndims``. Its dual
 update and prox run fused in ``chip_tv_dual``; its primal contribution
 (``-div q``) is fused into the SIRT / PDHG primal kernels.
+
+``Regularizer_TV1D`` / ``TV3D`` (``regularizers.py:292-337``), any other
+``Regularizer_Grad`` over the trailing axes with the l12 or l1 norm,
+``Regularizer_VTV`` (``regularizers.py:1362-1430``) and ``Regularizer_TNV``
+(``regularizers.py:1324-1359``) run on ``chip_grad_dual`` (dual update and
+prox in one pass) and ``chip_grad_update_primal`` (``-div q``, weighted for
+norms outside the l1 family), with the dual in the reference's
+``[ndims, *primal]`` layout.
 """
 
 from __future__ import annotations
@@ -66,6 +74,41 @@ class Regularizer_Grad(BaseRegularizer):
             tau *= self.weight
         return tau
 
+    def _check_trailing_axes(self, shape) -> None:
+        nd = len(shape)
+        axes = sorted(int(a) % nd for a in np.asarray(self.axes).reshape(-1))
+        if axes != list(range(nd - self.ndims, nd)) or self.ndims > 3:
+            raise NotImplementedError("the HIP gradient kernels take the gradient over the last 1 to 3 axes, in order")
+        if list(np.asarray(self.axes) % nd) != axes:
+            raise NotImplementedError("the gradient axes must be given in increasing order")
+        if self.pad_mode != "edge":
+            raise NotImplementedError("the HIP gradient kernels implement edge padding")
+        if self.upd_mask is not None:
+            raise NotImplementedError("upd_mask is not supported by the HIP gradient kernels")
+        if np.asarray(self.weight).size != 1:
+            raise NotImplementedError("the HIP gradient kernels take a scalar weight")
+
+    def device_config(self, shape) -> dict:
+        """Kernel parameters of ``chip_grad_dual`` / ``chip_grad_update_primal`` for a primal of ``shape``."""
+        shape = tuple(int(v) for v in shape)
+        self._check_trailing_axes(shape)
+        if isinstance(self.norm, dt.DataFidelity_l12):
+            if getattr(self.norm, "l2_axis", 0) != 0:
+                raise NotImplementedError("the l12 norm is supported over the gradient axis (l2_axis=0)")
+            mode = 0
+        elif type(self.norm) is dt.DataFidelity_l1:
+            mode = 1
+        else:
+            raise NotImplementedError(f"norm {self.norm.info()} is not supported by the HIP gradient kernels")
+        return self._config(shape, mode, 1, int(np.prod(shape[: len(shape) - self.ndims])), 0, 0)
+
+    def _config(self, shape, mode, n_chan, n_batch, der, chan) -> dict:
+        l1_family = isinstance(self.norm, dt.DataFidelity_l1)
+        return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=int(self.ndims),
+                    dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
+                    weight=float(self.weight), der=der, chan=chan, scale=float(self.weight),
+                    apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape))
+
     def check_fused(self) -> float:
         """The weight, after checking the fused TV2D kernels implement this configuration."""
         if self.ndims != 2 or list(np.asarray(self.axes) % 100) != list(np.array([-2, -1]) % 100):
@@ -88,6 +131,99 @@ class Regularizer_TV2D(Regularizer_Grad):
 
     def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
         super().__init__(weight=weight, ndims=2, axes=axes, pad_mode=pad_mode, norm=norm, upd_mask=upd_mask)
+
+
+class Regularizer_TV1D(Regularizer_Grad):
+    """Total Variation in 1D (``regularizers.py:292-305``)."""
+
+    __reg_name__ = "TV1D"
+
+    def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
+        super().__init__(weight=weight, ndims=1, axes=axes, pad_mode=pad_mode, norm=norm, upd_mask=upd_mask)
+
+
+class Regularizer_TV3D(Regularizer_Grad):
+    """Total Variation in 3D (``regularizers.py:324-337``)."""
+
+    __reg_name__ = "TV3D"
+
+    def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
+        super().__init__(weight=weight, ndims=3, axes=axes, pad_mode=pad_mode, norm=norm, upd_mask=upd_mask)
+
+
+_PWISE_CODE = {1: 1, 2: 2, np.inf: 0}
+
+
+def _multichannel_layout(shape, ndims: int, what: str) -> tuple[int, int]:
+    """(channels, independent batch) of a primal whose axis 0 holds the channels."""
+    if len(shape) <= ndims:
+        return 1, 1
+    n_chan = int(shape[0])
+    if n_chan > 16:
+        raise NotImplementedError(f"{what} couples at most 16 channels in the HIP kernels")
+    return n_chan, int(np.prod(shape[1: len(shape) - ndims]))
+
+
+class Regularizer_VTV(Regularizer_Grad):
+    """Vectorial Total Variation (``regularizers.py:1362-1430``) for multi-channel volumes
+    ``[C, ..., *gradient axes]``: the dual is divided by ``max(dn, weight)`` and multiplied by
+    ``weight``, with ``dn`` the point-wise derivative norm (1, 2, inf) followed by the norm across
+    channels (1, 2, inf; the dual norms of inf are 1)."""
+
+    __reg_name__ = "VTV"
+
+    def __init__(self, weight, ndims: int = 2, pwise_der_norm=2, pwise_chan_norm=np.inf, x_ref=None,
+                 upd_mask=None):
+        super().__init__(weight=weight, ndims=ndims, upd_mask=upd_mask)
+        self.pwise_der_norm = pwise_der_norm
+        if self.pwise_der_norm not in [1, 2, np.inf]:
+            self._raise_pwise_norm_error()
+        self.pwise_chan_norm = pwise_chan_norm
+        if self.pwise_chan_norm not in [1, 2, np.inf]:
+            self._raise_pwise_norm_error()
+        if x_ref is not None:
+            raise NotImplementedError("a reference volume (x_ref) is not supported by the HIP VTV kernel")
+        self.q_ref = None
+
+    def _raise_pwise_norm_error(self):
+        raise ValueError(
+            "The only supported point-wise norm exponents are: 1, 2, and Inf."
+            + f" Provided the following instead: derivatives={self.pwise_der_norm}, channel={self.pwise_chan_norm}"
+        )
+
+    def device_config(self, shape) -> dict:
+        shape = tuple(int(v) for v in shape)
+        self._check_trailing_axes(shape)
+        n_chan, n_batch = _multichannel_layout(shape, self.ndims, "VTV")
+        return self._config(shape, 2, n_chan, n_batch, _PWISE_CODE[self.pwise_der_norm],
+                            _PWISE_CODE[self.pwise_chan_norm])
+
+
+class Regularizer_TNV(Regularizer_Grad):
+    """Total Nuclear Variation (``regularizers.py:1324-1359``) for multi-channel volumes
+    ``[C, ..., *gradient axes]``: the nuclear norm of each (channels x derivatives) Jacobian,
+    through ``DataFidelity_ln(ln_axes=(1, 0))``; its dual proximal clips the singular values at 1
+    (the l1 spectral norm), and the weight scales tau and the primal update."""
+
+    __reg_name__ = "TNV"
+
+    def __init__(self, weight, ndims: int = 2, axes=None, pad_mode: str = "edge", upd_mask=None,
+                 spectral_norm=None, x_ref=None):
+        super().__init__(weight=weight, ndims=ndims, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask)
+        self.norm = dt.DataFidelity_ln(ln_axes=(1, 0), spectral_norm=spectral_norm)
+        if x_ref is not None:
+            raise NotImplementedError("a reference volume (x_ref) is not supported by the HIP TNV kernel")
+        self.x_ref = None
+
+    def device_config(self, shape) -> dict:
+        shape = tuple(int(v) for v in shape)
+        self._check_trailing_axes(shape)
+        if type(self.norm.spectral_norm) is not dt.DataFidelity_l1:
+            raise NotImplementedError("the HIP TNV kernel implements the l1 spectral norm")
+        if len(shape) <= self.ndims:
+            raise NotImplementedError("TNV needs a channel axis in front of the gradient axes")
+        n_chan, n_batch = _multichannel_layout(shape, self.ndims, "TNV")
+        return self._config(shape, 3, n_chan, n_batch, 0, 0)
 
 
 class Regularizer_l1(BaseRegularizer):

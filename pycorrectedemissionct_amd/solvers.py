@@ -184,14 +184,36 @@ class Solver:
         return b_mask, b_test_mask
 
     def _tv_weight(self) -> float | None:
-        if not self.regularizer:
+        """The weight of a single 2-D TV regularizer that the fused TV2D kernels implement, else None."""
+        if len(self.regularizer) != 1:
             return None
-        if len(self.regularizer) > 1:
-            raise NotImplementedError("one TV2D regularizer is supported by the fused HIP kernels")
         reg = self.regularizer[0]
-        if not isinstance(reg, regularizers.Regularizer_Grad):
-            raise NotImplementedError(f"regularizer {reg.info()} is not supported by the fused HIP kernels")
-        return reg.check_fused()
+        if type(reg) not in (regularizers.Regularizer_Grad, regularizers.Regularizer_TV2D):
+            return None
+        try:
+            return reg.check_fused()
+        except NotImplementedError:
+            return None
+
+    def _grad_regs(self, x: torch.Tensor) -> list:
+        """(kernel parameters, zero dual) of every regularizer for the HIP gradient kernels."""
+        out = []
+        for reg in self.regularizer:
+            if not isinstance(reg, regularizers.Regularizer_Grad):
+                raise NotImplementedError(f"regularizer {reg.info()} is not supported by the HIP kernels")
+            cfg = reg.device_config(tuple(x.shape))
+            out.append((cfg, torch.zeros(cfg["q_shape"], dtype=torch.float32, device=x.device)))
+        return out
+
+    @staticmethod
+    def _grad_dual(cfg: dict, q: torch.Tensor, x: torch.Tensor, st) -> None:
+        _native.call("chip_grad_dual", q.data_ptr(), x.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
+                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"], st)
+
+    @staticmethod
+    def _grad_primal(cfg: dict, upd: torch.Tensor, q: torch.Tensor, subtract: bool, st) -> None:
+        _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
+                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], cfg["apply_scale"], int(subtract), st)
 
     def _track(self, info, ii, Ax, b, b_mask, b_test_mask) -> bool:
         """Residual bookkeeping of one iteration; True when the tolerance is reached."""
@@ -556,6 +578,7 @@ class SIRT(Solver):
         n = int(x.shape[-1])
         batch = x.numel() // (n * n) if x.ndim >= 2 else 1
         q = torch.empty((batch, 2, n, n), dtype=torch.float32, device=x.device) if tv_w is not None else None
+        grad_regs = self._grad_regs(x) if (tv_w is None and self.regularizer) else []
         for ii in range(iterations):
             info.iterations += 1
             Ax = _arrays.to_dev(A(x))[0]
@@ -563,10 +586,16 @@ class SIRT(Solver):
                 break
             _native.call("chip_sirt_residual", res.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
                          _native.ptr(b_mask), sigma.data_ptr(), b_t.numel(), st)
+            # the reference re-initialises the regularizer duals every iteration (solvers.py:950-958)
+            for cfg, q_r in grad_regs:
+                q_r.zero_()
+                self._grad_dual(cfg, q_r, x, st)
             upd = _arrays.to_dev(A.T(res))[0]
             if q is not None:
                 q.zero_()
                 _native.call("chip_tv_dual", q.data_ptr(), x.data_ptr(), n, batch, tv_w, st)
+            for cfg, q_r in grad_regs:
+                self._grad_primal(cfg, upd, q_r, True, st)
             _native.call("chip_sirt_update", x.data_ptr(), upd.data_ptr(), _native.ptr(q), tau.data_ptr(), hl, lo, hu,
                          hi, _native.ptr(x_mask_t), n, batch, st)
         return _arrays.from_dev(x, host), info
@@ -627,6 +656,7 @@ class PDHG(Solver):
             x_relax=x.clone(), p=torch.zeros_like(b_t), q=None if tv_w is None else torch.zeros(
                 (batch, 2, n, n), dtype=torch.float32, device=x.device), tv_w=tv_w, n=n, batch=batch,
             bg=self.data_term.background_dev(b_t), kl=isinstance(self.data_term, data_terms.DataFidelity_KL),
+            grad_regs=self._grad_regs(x) if (tv_w is None and self.regularizer) else [],
         )
 
     def iterate(self, A, s: dict, iterations: int, lower_limit=None, upper_limit=None, info=None):
@@ -645,7 +675,11 @@ class PDHG(Solver):
                          _native.ptr(s["b_mask"]), p.numel(), st)
             if q is not None:
                 _native.call("chip_tv_dual", q.data_ptr(), xr.data_ptr(), s["n"], s["batch"], s["tv_w"], st)
+            for cfg, q_r in s["grad_regs"]:
+                self._grad_dual(cfg, q_r, xr, st)
             upd = _arrays.to_dev(A.T(p))[0]
+            for cfg, q_r in s["grad_regs"]:
+                self._grad_primal(cfg, upd, q_r, False, st)
             _native.call("chip_pdhg_primal", x.data_ptr(), xr.data_ptr(), upd.data_ptr(), _native.ptr(q),
                          s["tau"].data_ptr(), hl, lo, hu, hi, _native.ptr(s["x_mask"]), s["n"], s["batch"], st)
             if track:
