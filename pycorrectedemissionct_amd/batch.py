@@ -177,7 +177,7 @@ class BatchedProjectorXRF:
         return self.plan.bp(sino, maps, out, _BP_MODES[self.bp_mode])
 
 
-def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(1, 2, 3), bp_candidates=(2, 4, 8)) -> dict:
+def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(2, 4, 5, 6), bp_candidates=(2, 4, 8)) -> dict:
     """Pick the persistent-grid sizes of the batched fp and bp by timing them on ``A``'s batch.
 
     Returns ``{"fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``; the

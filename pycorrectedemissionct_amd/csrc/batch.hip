@@ -169,7 +169,7 @@ int create(BatchPlan& b, int n_slices, const int* sizes, int n_angles, const flo
     const int n = sizes[s];
     BatchSlice sl;
     sl.n = n;
-    sl.n_ublk = (n + 63) / 64;
+    sl.n_ublk = (n + kFpBatchBins - 1) / kFpBatchBins;
     sl.size_idx = size_index[n];
     sl.pad = b.size_plans[sl.size_idx]->p.pad;
     sl.vol_off = b.h_vol_start.back();

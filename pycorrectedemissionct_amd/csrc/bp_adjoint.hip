@@ -24,6 +24,7 @@ namespace {
 constexpr int kTileW = 32;
 constexpr int kTileH = 8;
 constexpr int kAngBlk = 32;
+constexpr int kGrp = 8;  // angles whose map values are loaded together
 constexpr int kWin = 40;  // >= 31 |c| + 7 |s| + 2 (|c| + |s|) + 2
 
 // One workgroup's work: the 32 x 8 pixel tile (tx, ty) of one slice.
@@ -36,10 +37,21 @@ struct BpItem {
   int tx, ty;
 };
 
+// Per angle of a block: first window bin, whether every candidate row of the
+// tile lies in [0, N), and the lattice offsets of the nine candidates,
+// cxy[k] = (c ku + s ky, -s ku + c ky) for k = 3 ku + ky.
+struct BpAngle {
+  float cx[9];
+  float cy[9];
+  int wlo;
+  int rows_inside;
+  int pad[2];
+};
+
 template <int D, bool MAPS>
 __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float* __restrict__ w, int n_angles,
                                                  int n_det_total, int det0, int angle0, int n_ang, int maps_a0,
-                                                 int accumulate, float (*win)[D][kWin], int* wlo_s) {
+                                                 int accumulate, float (*win)[D][kWin], BpAngle* ang_s) {
   const int n = itm.n;
   const float4* __restrict__ geom = itm.geom;
   const int tid = threadIdx.x;
@@ -51,7 +63,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
   const bool inside = i < n && j < n;
   const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
   const float fj = static_cast<float>(j), fi = static_cast<float>(i);
-  // tile corners for the per-angle bin window
+  // tile corners for the per-angle bin window and row range
   const float tj0 = static_cast<float>(itm.tx * kTileW), tj1 = tj0 + (kTileW - 1);
   const float ti0 = static_cast<float>(itm.ty * kTileH), ti1 = ti0 + (kTileH - 1);
 
@@ -63,11 +75,25 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const float4 g = geom[a0 + tid];
       const float cs = g.x, sn = g.y;
       const float r = fabsf(cs) + fabsf(sn);
-      // u* = c (x - m02) - s (y - m12) over the tile corners
+      // preimage (u*, y*) = (c (x - m02) - s (y - m12), s (x - m02) + c (y - m12)) over the tile corners
       const float u00 = cs * (tj0 - g.z) - sn * (ti0 - g.w), u01 = cs * (tj1 - g.z) - sn * (ti0 - g.w);
       const float u10 = cs * (tj0 - g.z) - sn * (ti1 - g.w), u11 = cs * (tj1 - g.z) - sn * (ti1 - g.w);
+      const float y00 = sn * (tj0 - g.z) + cs * (ti0 - g.w), y01 = sn * (tj1 - g.z) + cs * (ti0 - g.w);
+      const float y10 = sn * (tj0 - g.z) + cs * (ti1 - g.w), y11 = sn * (tj1 - g.z) + cs * (ti1 - g.w);
       const float umin = fminf(fminf(u00, u01), fminf(u10, u11));
-      wlo_s[tid] = static_cast<int>(floorf(umin - r)) - 1;
+      const float ymin = fminf(fminf(y00, y01), fminf(y10, y11));
+      const float ymax = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
+      BpAngle& ba = ang_s[tid];
+      ba.wlo = static_cast<int>(floorf(umin - r)) - 1;
+      // candidate rows are floor(y* - r) + 1 .. + 3 (one of slack for rounding)
+      ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
+#pragma unroll
+      for (int ku = 0; ku < 3; ++ku)
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+          ba.cx[3 * ku + ky] = cs * static_cast<float>(ku) + sn * static_cast<float>(ky);
+          ba.cy[3 * ku + ky] = cs * static_cast<float>(ky) - sn * static_cast<float>(ku);
+        }
     }
     __syncthreads();
     for (int k = tid; k < na * D * kWin; k += 256) {
@@ -75,57 +101,86 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const int rem = k - al * (D * kWin);
       const int d = rem / kWin;
       const int q = rem - d * kWin;
-      const int u = wlo_s[al] + q;
+      const int u = ang_s[al].wlo + q;
       win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] : 0.f;
     }
     __syncthreads();
     if (inside) {
-      for (int al = 0; al < na; ++al) {
-        const int a = a0 + al;
-        const float4 g = geom[a];
-        const float cs = g.x, sn = g.y, m02 = g.z, m12 = g.w;
-        const float r = fabsf(cs) + fabsf(sn);
-        const float px = fj - m02, py = fi - m12;
-        const float us = cs * px - sn * py;  // preimage of the pixel centre
-        const float ys = sn * px + cs * py;
-        const float fu0 = floorf(us - r) + 1.f;  // first of <= 3 candidate bins / rows
-        const float fy0 = floorf(ys - r) + 1.f;
-        const int iu0 = static_cast<int>(fu0);
-        const int iy0 = static_cast<int>(fy0);
-        // candidate sample coordinates with the fp's float32 rounding sequence
-        float xu[3], yu[3], sy[3], cy[3];
+      // map values of the next group of kGrp angles are loaded while the current group is
+      // computed: with one pixel per thread the per-angle loads would otherwise expose the
+      // full memory latency once per angle
+      float mcur[kGrp][D], mnxt[kGrp][D];
+      auto load_maps = [&](float (*m)[D], int g0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float uf = fu0 + static_cast<float>(k);
-          const float yf = fy0 + static_cast<float>(k);
-          xu[k] = __fmul_rn(cs, uf);
-          yu[k] = __fmul_rn(-sn, uf);
-          sy[k] = (static_cast<unsigned>(iy0 + k) < static_cast<unsigned>(n)) ? __fmul_rn(sn, yf) : 1e30f;
-          cy[k] = __fmul_rn(cs, yf);
-        }
-        float K[3];
+        for (int k = 0; k < kGrp; ++k)
 #pragma unroll
-        for (int ku = 0; ku < 3; ++ku) {
-          float ksum = 0.f;
+          for (int d = 0; d < D; ++d)
+            m[k][d] = (MAPS && g0 + k < na)
+                          ? mb[(static_cast<size_t>(a0 + g0 + k - maps_a0) * n_det_total + det0 + d) * nn + pix]
+                          : 1.f;
+      };
+      load_maps(mcur, 0);
+      for (int g0 = 0; g0 < na; g0 += kGrp) {
+        if (g0 + kGrp < na) load_maps(mnxt, g0 + kGrp);
 #pragma unroll
-          for (int ky = 0; ky < 3; ++ky) {
-            // rows outside [0, N) carry a huge x offset and weigh 0
-            const float xs = __fadd_rn(__fadd_rn(xu[ku], sy[ky]), m02);
-            const float ysc = __fadd_rn(__fadd_rn(yu[ku], cy[ky]), m12);
-            const float wx = fmaxf(1.f - fabsf(xs - fj), 0.f);
-            const float wy = fmaxf(1.f - fabsf(ysc - fi), 0.f);
-            ksum = fmaf(wx, wy, ksum);
+        for (int k = 0; k < kGrp; ++k) {
+          const int al = g0 + k;
+          if (al >= na) break;
+          const int a = a0 + al;
+          const float4 g = geom[a];
+          const float cs = g.x, sn = g.y;
+          const BpAngle& ba = ang_s[al];
+          const float r = fabsf(cs) + fabsf(sn);
+          const float px = fj - g.z, py = fi - g.w;
+          const float us = cs * px - sn * py;  // preimage of the pixel centre
+          const float ys = sn * px + cs * py;
+          const float fu0 = floorf(us - r) + 1.f;  // first of <= 3 candidate bins / rows
+          const float fy0 = floorf(ys - r) + 1.f;
+          // sample offsets from the pixel: (c u + s y + m02 - j, -s u + c y + m12 - i) at (fu0, fy0)
+          const float dx0 = fmaf(cs, fu0, fmaf(sn, fy0, -px));
+          const float dy0 = fmaf(-sn, fu0, fmaf(cs, fy0, -py));
+          float K[3];
+          if (ba.rows_inside) {
+#pragma unroll
+            for (int ku = 0; ku < 3; ++ku) {
+              float ksum = 0.f;
+#pragma unroll
+              for (int ky = 0; ky < 3; ++ky) {
+                const float wx = fmaxf(1.f - fabsf(dx0 + ba.cx[3 * ku + ky]), 0.f);
+                const float wy = fmaxf(1.f - fabsf(dy0 + ba.cy[3 * ku + ky]), 0.f);
+                ksum = fmaf(wx, wy, ksum);
+              }
+              K[ku] = ksum;
+            }
+          } else {
+            // rows outside [0, N) are not samples of the fp and weigh 0
+            const int iy0 = static_cast<int>(fy0);
+#pragma unroll
+            for (int ku = 0; ku < 3; ++ku) {
+              float ksum = 0.f;
+#pragma unroll
+              for (int ky = 0; ky < 3; ++ky) {
+                const float wx = fmaxf(1.f - fabsf(dx0 + ba.cx[3 * ku + ky]), 0.f);
+                float wy = fmaxf(1.f - fabsf(dy0 + ba.cy[3 * ku + ky]), 0.f);
+                wy = (static_cast<unsigned>(iy0 + ky) < static_cast<unsigned>(n)) ? wy : 0.f;
+                ksum = fmaf(wx, wy, ksum);
+              }
+              K[ku] = ksum;
+            }
           }
-          K[ku] = ksum;
-        }
-        const int q0 = iu0 - wlo_s[al];  // bins outside [0, N) read the zero padding
+          const int q0 = static_cast<int>(fu0) - ba.wlo;  // bins outside [0, N) read the zero padding
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
-          float f = w[static_cast<size_t>(a) * n_det_total + det0 + d];
-          if (MAPS) f *= mb[(static_cast<size_t>(a - maps_a0) * n_det_total + det0 + d) * nn + pix];
-          out = fmaf(f, gd, out);
+          for (int d = 0; d < D; ++d) {
+            const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
+            float f = w[static_cast<size_t>(a) * n_det_total + det0 + d];
+            if (MAPS) f *= mcur[k][d];
+            out = fmaf(f, gd, out);
+          }
         }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k)
+#pragma unroll
+          for (int d = 0; d < D; ++d) mcur[k][d] = mnxt[k][d];
       }
     }
   }
@@ -144,7 +199,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
                                                               int n_det_total, int det0, int angle0, int n_ang,
                                                               int maps_a0, int accumulate) {
   __shared__ float win[kAngBlk][D][kWin];
-  __shared__ int wlo_s[kAngBlk];
+  __shared__ BpAngle ang_s[kAngBlk];
   const int b = blockIdx.z;
   const size_t nn = static_cast<size_t>(n) * n;
   BpItem itm;
@@ -155,7 +210,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
   itm.n = n;
   itm.tx = blockIdx.x;
   itm.ty = blockIdx.y;
-  bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, angle0, n_ang, maps_a0, accumulate, win, wlo_s);
+  bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, angle0, n_ang, maps_a0, accumulate, win, ang_s);
 }
 
 // Persistent form over a batch of slices of different sizes: item k is tile
@@ -170,7 +225,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
                                                                const float* __restrict__ w, int n_angles,
                                                                int n_det_total, int det0, int accumulate) {
   __shared__ float win[kAngBlk][D][kWin];
-  __shared__ int wlo_s[kAngBlk];
+  __shared__ BpAngle ang_s[kAngBlk];
   const int64_t total = item_start[n_slices];
   for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
     const int s = find_slice(item_start, n_slices, k);
@@ -185,7 +240,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
     itm.n = sl.n;
     itm.ty = local / ntx;
     itm.tx = local - itm.ty * ntx;
-    bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, 0, n_angles, 0, accumulate, win, wlo_s);
+    bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, 0, n_angles, 0, accumulate, win, ang_s);
     __syncthreads();  // the next item restages the windows
   }
 }

@@ -78,7 +78,7 @@ struct Plan {
   // 2 = LDS-staged with register prefetch, 3 = streamed per-detector tiles,
   // 4 = 3 staging only the footprint of interior chunks
   // (2..4 fall back to 1 when N % 4 != 0)
-  int fp_variant = 3;
+  int fp_variant = 5;
 };
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -86,6 +86,9 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // ---------------------------------------------------------------------------
 // batches of slices of different sizes (one persistent launch per pass)
 // ---------------------------------------------------------------------------
+// detector bins per workgroup of the batched fp (the kernel's tile width)
+constexpr int kFpBatchBins = 32;
+
 struct BatchSlice {
   int n;         // slice side
   int n_ublk;    // 64-bin blocks of the fp
@@ -99,7 +102,7 @@ struct BatchSlice {
 struct BatchPlan {
   int n_slices = 0, n_angles = 0, n_det = 0;
   int n_cu = 256;
-  int fp_wg_per_cu = 2;  // persistent grid of the fp: fp_wg_per_cu x CUs workgroups
+  int fp_wg_per_cu = 5;  // persistent grid of the fp: fp_wg_per_cu x CUs workgroups
   int bp_wg_per_cu = 4;  // same for the bp
   int fp_trim = 0;       // stage only the footprint of interior fp chunks
   std::vector<chip_plan*> size_plans;  // one single-slice plan per distinct size

@@ -31,14 +31,34 @@
 namespace chip {
 namespace {
 
-constexpr int kBins = 64;         // detector bins per workgroup
-constexpr int kRows = 32;         // sample rows per chunk
-constexpr int kRowsPerGroup = 8;  // rows per wave per chunk
-constexpr int kBoxRows = 76;      // >= 63|s| + 31|c| + 3
-constexpr int kSlots = 20;        // float4 slots per box row (<= 76 + 3 alignment columns)
-constexpr int kPitch = 84;        // 16-byte aligned rows: one ds_write_b128 per staged slot
-constexpr int kTile = kBoxRows * kPitch;
-constexpr int kPerThread = (kBoxRows * kSlots + 255) / 256;  // 6
+// Tile shapes: BINS detector bins x ROWS sample rows per chunk, 256 threads in
+// 256 / BINS row groups. The staged box of a chunk spans at most
+// (BINS - 1)|s| + (ROWS - 1)|c| + 3 rows (+1 on the border, where clamped
+// samples tap N and N + 1) and as many columns plus 3 for the float4
+// alignment of its first column; the bounds below hold for every angle
+// (max of a|s| + b|c| is sqrt(a^2 + b^2)).
+template <int BINS, int ROWS, int BOX_ROWS, int SLOTS, int PITCH, int WAVES_PER_SIMD>
+struct FpTile {
+  static constexpr int kWaves = WAVES_PER_SIMD;  // occupancy the register budget is sized for
+  static constexpr int kBins = BINS;
+  static constexpr int kRows = ROWS;
+  static constexpr int kGroups = 256 / BINS;
+  static constexpr int kRowsPerGroup = ROWS / kGroups;
+  static constexpr int kBoxRows = BOX_ROWS;
+  static constexpr int kSlots = SLOTS;  // float4 slots per box row
+  static constexpr int kPitch = PITCH;  // floats per tile row and detector, a multiple of 4
+  static constexpr int kTile = BOX_ROWS * PITCH;  // floats per detector; a pass holds D * kTile
+  static constexpr int kPerThread = (BOX_ROWS * SLOTS + 255) / 256;
+  static_assert(ROWS % kGroups == 0, "rows split evenly over the groups");
+  static_assert(PITCH % 4 == 0 && PITCH >= 4 * SLOTS, "float4-aligned rows holding every slot");
+};
+// 64 x 32: sqrt(63^2 + 31^2) = 70.3 -> 74.3 rows, 77.3 columns
+using Tile64 = FpTile<64, 32, 76, 20, 84, 1>;
+// 32 x 32: sqrt(31^2 + 31^2) = 43.9 -> 47.9 rows, 50.9 columns; a third of the LDS
+using Tile32 = FpTile<32, 32, 48, 13, 52, 1>;
+// the persistent batch kernel's tile; the batch plan counts its bin blocks with kFpBatchBins
+using BatchTile = Tile32;
+static_assert(BatchTile::kBins == kFpBatchBins, "batch plan and kernel agree on the bin block");
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -56,9 +76,10 @@ __device__ __forceinline__ int logical_block(int p, int g) {
   return x * q + min(x, r) + k;
 }
 
+template <class T>
 __device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, float yu_a, float xu_b, float yu_b,
                                          float cs, float sn, float m02, float m12) {
-  const int y_last = min(yc + kRows, n) - 1;
+  const int y_last = min(yc + T::kRows, n) - 1;
   const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
   const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
   const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
@@ -88,6 +109,31 @@ __device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, fl
   return bx;
 }
 
+// A box packed into two ints for LDS: columns / rows offset by 8 in 13 bits
+// each (N <= 4096), the empty and border flags in bits 26 and 27.
+constexpr int kMaxChunks = 128;  // chunk boxes precomputed per workgroup (N <= 128 * rows)
+
+__device__ __forceinline__ int2 pack_box(const Box& b) {
+  int2 p;
+  p.x = (b.clo + 8) | ((b.chi + 8) << 13) | (b.empty << 26) | (b.border << 27);
+  p.y = (b.rlo + 8) | ((b.rhi + 8) << 13);
+  return p;
+}
+
+__device__ __forceinline__ Box unpack_box(int2 p, int yc, int rows, int n) {
+  Box b;
+  b.clo = (p.x & 8191) - 8;
+  b.chi = ((p.x >> 13) & 8191) - 8;
+  b.empty = (p.x >> 26) & 1;
+  b.border = (p.x >> 27) & 1;
+  b.rlo = (p.y & 8191) - 8;
+  b.rhi = ((p.y >> 13) & 8191) - 8;
+  b.ylo = yc;
+  b.yhi = min(yc + rows, n) - 1;
+  b.all_valid = 0;
+  return b;
+}
+
 // One workgroup's work: bins [u0, u0 + 64) of one angle of one slice, for the
 // detectors [det0, det0 + D) of a pass.
 struct FpItem {
@@ -107,13 +153,16 @@ struct FpItem {
 // centre within 1.5 of its pixels -- instead of the whole axis-aligned box,
 // which at 45 degrees is twice the footprint. Border chunks (clamped samples
 // read the zero padding) keep the whole box.
-template <int D, bool MAPS, bool TRIM>
+template <class T, int D, bool MAPS, bool TRIM>
 __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
+  static_assert(D == 1 || D == 2, "one or two detectors per pass");
+  constexpr int kBins = T::kBins, kRows = T::kRows, kGroups = T::kGroups, kRowsPerGroup = T::kRowsPerGroup;
+  constexpr int kSlots = T::kSlots, kPitch = T::kPitch, kPerThread = T::kPerThread;
   const int n = it.n;
   const int u0 = it.u0;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int grp = tid >> 6;
+  const int lane = tid % kBins;
+  const int grp = tid / kBins;
   const int u = u0 + lane;
 
   const float4 gm = it.gm;
@@ -184,20 +233,30 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
     const int cw = bx.chi - bx.clo + 1;
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
-      const int o = s_row[i] * kPitch + s_col[i];
+      // tile layout [row][col][D]: the D detectors' products of one pixel are adjacent
+      const int o = (s_row[i] * kPitch + s_col[i]) * D;
       if (inmask & (1u << i)) {
+        float p[D][4];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
+          p[d][0] = MAPS ? fr[i].x * mr[d][i].x : fr[i].x;
+          p[d][1] = MAPS ? fr[i].y * mr[d][i].y : fr[i].y;
+          p[d][2] = MAPS ? fr[i].z * mr[d][i].z : fr[i].z;
+          p[d][3] = MAPS ? fr[i].w * mr[d][i].w : fr[i].w;
+        }
+#pragma unroll
+        for (int h = 0; h < D; ++h) {
+          // float4 h holds columns 4h / D .. of the slot, detectors interleaved
           float4 v4;
-          v4.x = MAPS ? fr[i].x * mr[d][i].x : fr[i].x;
-          v4.y = MAPS ? fr[i].y * mr[d][i].y : fr[i].y;
-          v4.z = MAPS ? fr[i].z * mr[d][i].z : fr[i].z;
-          v4.w = MAPS ? fr[i].w * mr[d][i].w : fr[i].w;
-          *reinterpret_cast<float4*>(lds + d * kTile + o) = v4;
+          v4.x = p[(4 * h + 0) % D][(4 * h + 0) / D];
+          v4.y = p[(4 * h + 1) % D][(4 * h + 1) / D];
+          v4.z = p[(4 * h + 2) % D][(4 * h + 2) / D];
+          v4.w = p[(4 * h + 3) % D][(4 * h + 3) / D];
+          *reinterpret_cast<float4*>(lds + o + 4 * h) = v4;
         }
       } else if (bx.border && s_row[i] < th && s_col[i] < cw) {  // zero padding outside the slice
 #pragma unroll
-        for (int d = 0; d < D; ++d) *reinterpret_cast<float4*>(lds + d * kTile + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int h = 0; h < D; ++h) *reinterpret_cast<float4*>(lds + o + 4 * h) = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
   };
@@ -206,11 +265,26 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
 #pragma unroll
   for (int d = 0; d < D; ++d) acc[d] = 0.f;
 
+  // the chunk boxes are uniform over the workgroup: computed once, one chunk per thread,
+  // into LDS behind the tile (slices of more than kMaxChunks chunks compute them per chunk)
+  const int n_chunks = (n + kRows - 1) / kRows;
+  const bool lds_boxes = n_chunks <= kMaxChunks;
+  int2* boxes = reinterpret_cast<int2*>(lds + D * T::kTile);
+  if (lds_boxes) {
+    for (int c = tid; c < n_chunks; c += 256)
+      boxes[c] = pack_box(chunk_box<T>(c * kRows, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12));
+    __syncthreads();
+  }
+  auto box_of = [&](int y) -> Box {
+    return lds_boxes ? unpack_box(boxes[y / kRows], y, kRows, n)
+                     : chunk_box<T>(y, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+  };
+
   int yc = 0;
   Box cur;
   for (;; yc += kRows) {
     if (yc >= n) break;
-    cur = chunk_box(yc, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+    cur = box_of(yc);
     if (!cur.empty) break;
   }
   if (yc < n) issue(cur);
@@ -221,7 +295,7 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
     int yn = yc + kRows;  // next chunk meeting the slice; its loads overlap this chunk's gathers
     Box nxt;
     for (; yn < n; yn += kRows) {
-      nxt = chunk_box(yn, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+      nxt = box_of(yn);
       if (!nxt.empty) break;
     }
     if (yn < n) issue(nxt);
@@ -245,23 +319,43 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
         const float dc = xs - c0f;
         const float dr = ys - r0f;
         const int li = static_cast<int>(fmaf(r0f, static_cast<float>(kPitch), c0f)) - base;
+        float t00[D], t01[D], t10[D], t11[D];
+        if (D == 2) {
+          const float2* t = reinterpret_cast<const float2*>(lds) + li;
+          const float2 a0 = t[0], a1 = t[1], b0 = t[kPitch], b1 = t[kPitch + 1];
+          t00[0] = a0.x, t00[D - 1] = a0.y, t01[0] = a1.x, t01[D - 1] = a1.y;
+          t10[0] = b0.x, t10[D - 1] = b0.y, t11[0] = b1.x, t11[D - 1] = b1.y;
+        } else {
+          const float* t = lds + li;
+          t00[0] = t[0], t01[0] = t[1], t10[0] = t[kPitch], t11[0] = t[kPitch + 1];
+        }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const float* t = lds + d * kTile + li;
-          const float t00 = t[0], t01 = t[1], t10 = t[kPitch], t11 = t[kPitch + 1];
-          const float top = fmaf(dc, t01 - t00, t00);
-          const float bot = fmaf(dc, t11 - t10, t10);
+          const float top = fmaf(dc, t01[d] - t00[d], t00[d]);
+          const float bot = fmaf(dc, t11[d] - t10[d], t10[d]);
           acc[d] += fmaf(dr, bot - top, top);
         }
       };
+      // full row groups run straight-line, so the LDS reads of several samples overlap
+      const bool full = yb0 + kRowsPerGroup <= n;
       if (!cur.border) {
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < kRowsPerGroup; ++k)
-          if (yb0 + k < n) sample(yb0 + k, false);
+          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k, false);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kRowsPerGroup; ++k)
+            if (yb0 + k < n) sample(yb0 + k, false);
+        }
       } else {
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < kRowsPerGroup; ++k)
-          if (yb0 + k < n) sample(yb0 + k, true);
+          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k, true);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kRowsPerGroup; ++k)
+            if (yb0 + k < n) sample(yb0 + k, true);
+        }
       }
     }
     yc = yn;
@@ -270,25 +364,26 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
 
   __syncthreads();
 #pragma unroll
-  for (int d = 0; d < D; ++d) lds[(d * 4 + grp) * 64 + lane] = acc[d];
+  for (int d = 0; d < D; ++d) lds[(d * kGroups + grp) * kBins + lane] = acc[d];
   __syncthreads();
   if (grp == 0 && u < n) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const float s = ((lds[(d * 4 + 0) * 64 + lane] + lds[(d * 4 + 1) * 64 + lane]) + lds[(d * 4 + 2) * 64 + lane]) +
-                      lds[(d * 4 + 3) * 64 + lane];
+      float s = lds[(d * kGroups) * kBins + lane];
+#pragma unroll
+      for (int g = 1; g < kGroups; ++g) s = s + lds[(d * kGroups + g) * kBins + lane];
       it.out[d * it.det_stride + u] = s * it.wa[d];
     }
   }
 }
 
-template <int D, bool MAPS, bool TRIM>
-__global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+template <class T, int D, bool MAPS, bool TRIM>
+__global__ __launch_bounds__(256, T::kWaves) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
                                                         int64_t maps_slice_stride, float* __restrict__ sino,
                                                         const float4* __restrict__ geom, const float* __restrict__ w,
                                                         int n, int n_angles, int n_det_total, int det0, int n_ublk,
                                                         int angle0, int maps_a0) {
-  extern __shared__ float lds[];  // [D][kTile]
+  extern __shared__ float lds[];  // [kBoxRows][kPitch][D]
   const int lb = logical_block(blockIdx.x, gridDim.x);
   const int la = lb / n_ublk;
   const int a = angle0 + la;
@@ -296,7 +391,7 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
   const size_t nn = static_cast<size_t>(n) * n;
   FpItem it;
   it.n = n;
-  it.u0 = (lb - la * n_ublk) * kBins;
+  it.u0 = (lb - la * n_ublk) * T::kBins;
   it.gm = geom[a];
   it.v = vol + static_cast<size_t>(b) * nn;
   it.mb = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride +
@@ -305,7 +400,7 @@ __global__ __launch_bounds__(256) void fp_stream_kernel(const float* __restrict_
   it.out = sino + ((static_cast<size_t>(b) * n_det_total + det0) * n_angles + a) * n;
   it.det_stride = static_cast<size_t>(n_angles) * n;
   it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-  fp_stream_block<D, MAPS, TRIM>(it, lds);
+  fp_stream_block<T, D, MAPS, TRIM>(it, lds);
 }
 
 // Persistent form over a batch of slices of different sizes: work item k of
@@ -320,7 +415,7 @@ __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* 
                                                               const float4* __restrict__ geom,
                                                               const float* __restrict__ w, int n_angles,
                                                               int n_det_total, int det0) {
-  extern __shared__ float lds[];  // [D][kTile]
+  extern __shared__ float lds[];  // [kBoxRows][kPitch][D]
   const int64_t total = item_start[n_slices];
   const int g = gridDim.x;
   const int64_t lb = logical_block(blockIdx.x, g);
@@ -332,29 +427,37 @@ __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* 
     const size_t nn = static_cast<size_t>(sl.n) * sl.n;
     FpItem it;
     it.n = sl.n;
-    it.u0 = (local - a * sl.n_ublk) * kBins;
+    it.u0 = (local - a * sl.n_ublk) * BatchTile::kBins;
     it.gm = geom[static_cast<size_t>(sl.size_idx) * n_angles + a];
     it.v = vol + sl.vol_off;
     it.mb = MAPS ? maps + sl.maps_off + (static_cast<size_t>(a) * n_det_total + det0) * nn : nullptr;
     it.out = sino + sl.sino_off + (static_cast<size_t>(det0) * n_angles + a) * sl.n;
     it.det_stride = static_cast<size_t>(n_angles) * sl.n;
     it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-    fp_stream_block<D, MAPS, TRIM>(it, lds);
+    fp_stream_block<BatchTile, D, MAPS, TRIM>(it, lds);
   }
 }
 
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch, int det0,
             int angle0, int n_ang, int maps_a0, hipStream_t st) {
-  const int n_ublk = (p.n + kBins - 1) / kBins;
+  if (p.fp_variant == 5) {
+    const int n_ublk = (p.n + Tile32::kBins - 1) / Tile32::kBins;
+    dim3 grid(n_ublk * n_ang, batch);
+    const size_t shmem = sizeof(float) * D * Tile32::kTile + sizeof(int2) * kMaxChunks;
+    hipLaunchKernelGGL((fp_stream_kernel<Tile32, D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride,
+                       sino, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    return;
+  }
+  const int n_ublk = (p.n + Tile64::kBins - 1) / Tile64::kBins;
   dim3 grid(n_ublk * n_ang, batch);
-  const size_t shmem = sizeof(float) * D * kTile;
+  const size_t shmem = sizeof(float) * D * Tile64::kTile + sizeof(int2) * kMaxChunks;
   if (p.fp_variant == 4) {
-    hipLaunchKernelGGL((fp_stream_kernel<D, MAPS, true>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
+    hipLaunchKernelGGL((fp_stream_kernel<Tile64, D, MAPS, true>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
                        p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
   } else {
-    hipLaunchKernelGGL((fp_stream_kernel<D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
-                       p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    hipLaunchKernelGGL((fp_stream_kernel<Tile64, D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride,
+                       sino, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
   }
 }
 
@@ -363,7 +466,7 @@ void launch_batch(const BatchPlan& bp, const float* vol, const float* maps, floa
   const int64_t total = bp.h_item_start_fp.back();
   const int64_t want = static_cast<int64_t>(bp.fp_wg_per_cu) * bp.n_cu;
   const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
-  const size_t shmem = sizeof(float) * D * kTile;
+  const size_t shmem = sizeof(float) * D * BatchTile::kTile + sizeof(int2) * kMaxChunks;
   if (bp.fp_trim) {
     hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS, true>), dim3(grid), dim3(256), shmem, st, bp.d_slices,
                        bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det,

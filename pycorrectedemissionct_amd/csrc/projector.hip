@@ -859,7 +859,7 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 4, "fp_kernel must be in 0..4");
+    CHIP_CHECK_ARG(value >= 0 && value <= 5, "fp_kernel must be in 0..5");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }
