@@ -331,7 +331,7 @@ def main() -> int:
     log(f"copy 1 GiB: {t_copy * 1e6:.1f} us ({copy_gbs:.0f} GB/s)")
 
     kernels = {
-        "fp": dict(kernel="fp_stream_kernel<2, true>", t=t_fp, bytes=fp_bytes, gbs=fp_gbs,
+        "fp": dict(kernel="fp_stream_kernel<FpTile<32, 32, 48, 13, 52, 1>, 2, true, false>", t=t_fp, bytes=fp_bytes, gbs=fp_gbs,
                    traffic_file="fp_stream_512x720x2_traffic.json"),
         "bp": dict(kernel="bp_adjoint_tile_kernel<2, true>", t=t_bp, bytes=bp_bytes, gbs=bp_gbs,
                    traffic_file="bp_adjoint_512x720x2_traffic.json"),
