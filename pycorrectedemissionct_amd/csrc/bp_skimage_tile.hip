@@ -1,0 +1,169 @@
+// Whole-sinogram skimage back-projection with staged line windows (gfx950).
+//
+// Same arithmetic as bp_skimage_pixel (projector.hip): per pixel and angle
+// t = (j - k) cos - (i - k) sin, np.interp of the line at t with 0 outside
+// [-k, n - 1 - k] (membership decided in float64 near the ends), summed over
+// angles then detectors in that order, times w[a, d] M[a, d][i, j] for the
+// symmetric form (_projector_backends.py:335-359, projectors.py:552-568);
+// iradon's x 2 / pi is folded into the plan's weights by the caller as before.
+//
+// A workgroup owns a 16 x 16 pixel tile and walks the angles in blocks of
+// 32: the bins the tile can touch (<= 15 (|c| + |s|) + 4 per angle and
+// detector) are staged in LDS once per block instead of being gathered
+// through the caches by every pixel, and the symmetric form's map values are
+// loaded 8 angles at a time, the next 8 while the current are summed.
+#include "common.hpp"
+
+namespace chip {
+namespace {
+
+constexpr int kSkTile = 16;
+constexpr int kSkAngBlk = 32;
+constexpr int kSkGrp = 8;
+constexpr int kSkWin = 28;  // >= 15 (|c| + |s|) + 4, with one bin of slack each side
+
+template <int SYM, int D>
+__global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __restrict__ sino,
+                                                              const float* __restrict__ maps,
+                                                              int64_t maps_slice_stride, float* __restrict__ vol,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ w, int n, int n_angles,
+                                                              int n_det, int det0, int angle0, int n_ang,
+                                                              int maps_a0, int accumulate) {
+  __shared__ float win[kSkAngBlk][D][kSkWin];
+  __shared__ int wlo_s[kSkAngBlk];
+  const int tid = threadIdx.x;
+  const int tx = static_cast<int>(blockIdx.x), ty = static_cast<int>(blockIdx.y);
+  const int j = tx * kSkTile + (tid & (kSkTile - 1));
+  const int i = ty * kSkTile + (tid / kSkTile);
+  const int b = blockIdx.z;
+  const int k = n / 2;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float* sb = sino + static_cast<size_t>(b) * n_det * n_angles * n;
+  const float* mb = (SYM && maps) ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  const bool inside = i < n && j < n;
+  const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
+  const float jc = static_cast<float>(j - k);
+  const float ic = static_cast<float>(i - k);
+  const float tlo = static_cast<float>(-k);
+  const float thi = static_cast<float>(n - 1 - k);
+  // tile corners, centred
+  const float jc0 = static_cast<float>(tx * kSkTile - k), jc1 = jc0 + (kSkTile - 1);
+  const float ic0 = static_cast<float>(ty * kSkTile - k), ic1 = ic0 + (kSkTile - 1);
+
+  float acc = 0.f;
+  for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kSkAngBlk) {
+    const int na = min(kSkAngBlk, angle0 + n_ang - a0);
+    __syncthreads();  // the previous block's windows are no longer read
+    if (tid < na) {
+      const float4 g = geom[a0 + tid];
+      const float t00 = fmaf(jc0, g.x, -(ic0 * g.y)), t01 = fmaf(jc1, g.x, -(ic0 * g.y));
+      const float t10 = fmaf(jc0, g.x, -(ic1 * g.y)), t11 = fmaf(jc1, g.x, -(ic1 * g.y));
+      const float tmin = fminf(fminf(t00, t01), fminf(t10, t11));
+      wlo_s[tid] = static_cast<int>(floorf(tmin)) + k - 1;
+    }
+    __syncthreads();
+    for (int q = tid; q < na * D * kSkWin; q += 256) {
+      const int al = q / (D * kSkWin);
+      const int rem = q - al * (D * kSkWin);
+      const int d = rem / kSkWin;
+      const int e = rem - d * kSkWin;
+      const int bin = wlo_s[al] + e;
+      win[al][d][e] =
+          (bin >= 0 && bin < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + bin] : 0.f;
+    }
+    __syncthreads();
+    if (inside) {
+      float mcur[kSkGrp][D], mnxt[kSkGrp][D];
+      auto load_maps = [&](float (*m)[D], int g0) {
+#pragma unroll
+        for (int q = 0; q < kSkGrp; ++q)
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            m[q][d] = (mb != nullptr && g0 + q < na)
+                          ? mb[(static_cast<size_t>(a0 + g0 + q - maps_a0) * n_det + det0 + d) * nn + pix]
+                          : 1.f;
+      };
+      if (SYM) load_maps(mcur, 0);
+      for (int g0 = 0; g0 < na; g0 += kSkGrp) {
+        if (SYM && g0 + kSkGrp < na) load_maps(mnxt, g0 + kSkGrp);
+#pragma unroll
+        for (int q = 0; q < kSkGrp; ++q) {
+          const int al = g0 + q;
+          if (al >= na) break;
+          const int a = a0 + al;
+          const float4 g = geom[a];
+          float t = fmaf(jc, g.x, -(ic * g.y));
+          // np.interp is discontinuous at the ends of the line (0 outside): decide
+          // membership with the reference's double-precision t whenever the
+          // float32 t is within its error bound of an end
+          bool member;
+          if (fabsf(t - tlo) < 1e-3f || fabsf(t - thi) < 1e-3f) {
+            const double td = fma(static_cast<double>(jc), static_cast<double>(g.x),
+                                  -(static_cast<double>(ic) * static_cast<double>(g.y)));
+            member = td >= static_cast<double>(tlo) && td <= static_cast<double>(thi);
+            t = fminf(fmaxf(t, tlo), thi);
+          } else {
+            member = t >= tlo && t <= thi;
+          }
+          if (!member) continue;
+          const float tf = floorf(t);
+          const float fr = t - tf;
+          const int e = static_cast<int>(tf) + k - wlo_s[al];  // bins past n - 1 read the zero padding
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const float s0 = win[al][d][e];
+            const float s1 = win[al][d][e + 1];
+            float val = fmaf(fr, s1 - s0, s0);
+            if (SYM) {
+              float f = w[static_cast<size_t>(a) * n_det + det0 + d];
+              if (mb != nullptr) f *= mcur[q][d];
+              val *= f;
+            }
+            acc += val;
+          }
+        }
+        if (SYM) {
+#pragma unroll
+          for (int q = 0; q < kSkGrp; ++q)
+#pragma unroll
+            for (int d = 0; d < D; ++d) mcur[q][d] = mnxt[q][d];
+        }
+      }
+    }
+  }
+  if (inside) {
+    float* o = vol + static_cast<size_t>(b) * nn + pix;
+    *o = accumulate ? *o + acc : acc;
+  }
+}
+
+template <int SYM, int D>
+void launch_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
+                  int det0, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+  dim3 grid((p.n + kSkTile - 1) / kSkTile, (p.n + kSkTile - 1) / kSkTile, batch);
+  hipLaunchKernelGGL((bp_skimage_tile_kernel<SYM, D>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate);
+}
+
+}  // namespace
+
+// Detectors in passes of two, the passes after the first accumulating; with
+// one or two detectors the sums run in the same order as bp_skimage_kernel.
+void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
+                             int batch, int sym, int angle0, int n_ang, int maps_a0, int accumulate,
+                             hipStream_t st) {
+  for (int det0 = 0; det0 < p.n_det; det0 += 2) {
+    const bool two = p.n_det - det0 >= 2;
+    const int acc = (det0 > 0) ? 1 : accumulate;
+    if (sym) {
+      if (two) launch_tiles<1, 2>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
+      else launch_tiles<1, 1>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
+    } else {
+      if (two) launch_tiles<0, 2>(p, sino, nullptr, 0, vol, batch, det0, angle0, n_ang, 0, acc, st);
+      else launch_tiles<0, 1>(p, sino, nullptr, 0, vol, batch, det0, angle0, n_ang, 0, acc, st);
+    }
+  }
+}
+
+}  // namespace chip

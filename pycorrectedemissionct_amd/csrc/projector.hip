@@ -761,7 +761,11 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float* __restrict
 static void launch_bp(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
                       int mode, int angle0, int n_ang, int line_mode, int accumulate, hipStream_t st) {
   dim3 grid((p.n + kBpTile - 1) / kBpTile, (p.n + kBpTile - 1) / kBpTile, batch);
-  if (mode == CHIP_BP_SKIMAGE) {
+  if (!line_mode && (mode == CHIP_BP_SKIMAGE || mode == CHIP_BP_SKIMAGE_SYM)) {
+    // whole sinograms and angle ranges: line windows staged in LDS per tile
+    launch_bp_skimage_tiles(p, sino, maps, stride, vol, batch, mode == CHIP_BP_SKIMAGE_SYM ? 1 : 0, angle0, n_ang,
+                            0, accumulate, st);
+  } else if (mode == CHIP_BP_SKIMAGE) {
     hipLaunchKernelGGL(bp_skimage_kernel<0>, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w,
                        p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate, 0);
   } else if (mode == CHIP_BP_SKIMAGE_SYM) {
@@ -803,9 +807,7 @@ void launch_bp_range(const Plan& p, const float* sino, const float* maps, float*
     launch_bp_adjoint_tiles(p, sino, maps, 0, vol, 1, angle0, n_ang, maps_a0, accumulate, st);
     return;
   }
-  dim3 grid((p.n + kBpTile - 1) / kBpTile, (p.n + kBpTile - 1) / kBpTile, 1);
-  hipLaunchKernelGGL(bp_skimage_kernel<1>, grid, dim3(256), 0, st, sino, maps, 0, vol, p.d_fp_geom, p.d_w, p.n,
-                     p.n_angles, p.n_det, angle0, n_ang, 0, accumulate, maps_a0);
+  launch_bp_skimage_tiles(p, sino, maps, 0, vol, 1, 1, angle0, n_ang, maps_a0, accumulate, st);
 }
 
 }  // namespace chip
