@@ -9,10 +9,9 @@
 //   E      = exp(-rotate(C, +deg))[e:-e, e:-e]
 //   M[a,d] = E_in(theta_a) * E_out(theta_a + phi_d + pi)
 //
-// Kernel 1 (att_scan_kernel): one thread per (beam, column x of the rotated
-// frame) walks the column top to bottom, sampling mu_p at the rotated
-// position and carrying the trapezoid prefix sum; consecutive threads write
-// consecutive columns, so every row store is coalesced. The running sum is
+// Kernel 1 (att_scan_kernel): per beam, 8-lane groups walk the columns of the
+// rotated frame top to bottom in bands of 8 rows, sampling mu_p at the
+// rotated positions and carrying the trapezoid prefix sum; the running sum is
 // the same float32 sequence numpy's cumsum performs.
 // Kernel 2 (att_compose_kernel): one thread per map pixel samples the
 // beams' prefix-sum images at the back-rotated position, takes exp and
@@ -59,29 +58,58 @@ __device__ __forceinline__ void affine_apply(const Affine& m, float x, float y, 
 
 // beams [0, n_in) use mu_in with rot[rot_in0 + beam], the rest mu_out with
 // rot[rot_out0 + (beam - n_in)]; output C[beam][r][x], side np.
+//
+// A wave covers 8 columns x 8 rows of the rotated frame: lane = 8 * column
+// + row. Its samples form a compact rotated 8 x 8 patch of mu, so each
+// gather touches a few cache lines (a wave along one rotated row touched one
+// line per lane). The prefix sum stays numpy's sequential float32 sum: within
+// a column's 8 lanes the running value is passed from row to row in order
+// (run_r = run_{r-1} + v_r), and lane 7 carries it to the next band of rows.
+constexpr int kScanCols = 8;   // columns per wave
+constexpr int kScanBand = 8;   // rows per band (lanes per column)
+constexpr int kScanWaves = 4;  // waves per workgroup
+
 __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__ mu_in, const float* __restrict__ mu_out,
                                                        const Affine* __restrict__ rot_in,
                                                        const Affine* __restrict__ rot_out, int n_in, int n_beams,
                                                        int n, int pad, int np, float* __restrict__ cum) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col = lane / kScanBand;
+  const int row = lane - col * kScanBand;
+  const int x = (blockIdx.x * kScanWaves + wave) * kScanCols + col;
   const int beam = blockIdx.y;
-  if (x >= np || beam >= n_beams) return;
+  if (beam >= n_beams) return;  // uniform over the workgroup
+  const bool live = x < np;
   const bool inb = beam < n_in;
   const float* mu = inb ? mu_in : mu_out;
   const Affine m = inb ? rot_in[beam] : rot_out[beam - n_in];
-  const float xf = static_cast<float>(x);
+  const float xf = static_cast<float>(min(x, np - 1));
   float* out = cum + static_cast<size_t>(beam) * np * np + x;
   float xs, ys;
+  // R[np - 1]: the roll wraps it onto row 0
   affine_apply(m, xf, static_cast<float>(np - 1), xs, ys);
-  float prev = warp_sample(mu, n, pad, xs, ys);  // R[np-1]: the roll wraps it onto row 0
-  float run = 0.f;
-  for (int r = 0; r < np; ++r) {
-    affine_apply(m, xf, static_cast<float>(r), xs, ys);
+  float last = warp_sample(mu, n, pad, xs, ys);  // R of the row before the band, per column
+  float carry = 0.f;                              // running sum at the row before the band
+  const int src_prev = (lane & ~(kScanBand - 1)) + ((row + kScanBand - 1) & (kScanBand - 1));
+  const int src_last = (lane & ~(kScanBand - 1)) + (kScanBand - 1);
+  for (int r0 = 0; r0 < np; r0 += kScanBand) {
+    const int r = r0 + row;
+    affine_apply(m, xf, static_cast<float>(min(r, np - 1)), xs, ys);
     const float cur = warp_sample(mu, n, pad, xs, ys);
+    // R[r - 1]: the previous lane of the column, or the last row of the previous band
+    const float up = __shfl(cur, src_prev, 64);
+    const float prev = (row == 0) ? last : up;
     const float v = (cur + prev) * 0.5f;
-    run = (r == 0) ? v : run + v;
-    out[static_cast<size_t>(r) * np] = run;
-    prev = cur;
+    float run = (row == 0) ? ((r0 == 0) ? v : carry + v) : v;
+#pragma unroll
+    for (int k = 1; k < kScanBand; ++k) {
+      const float before = __shfl(run, src_prev, 64);
+      if (row == k) run = before + v;
+    }
+    if (live && r < np) out[static_cast<size_t>(r) * np] = run;
+    carry = __shfl(run, src_last, 64);
+    last = __shfl(cur, src_last, 64);
   }
 }
 
@@ -170,7 +198,7 @@ static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int
   const Affine* rout_b = p.d_rot + 2 * A + static_cast<size_t>(A) * D * S;
   const int n_in = mi ? na : 0;
   const int n_beams = n_in + (mo ? na * D * S : 0);
-  dim3 g1((p.np + 255) / 256, n_beams);
+  dim3 g1((p.np + kScanWaves * kScanCols - 1) / (kScanWaves * kScanCols), n_beams);
   hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
                      rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
   CHIP_LAUNCH_CHECK();
