@@ -124,8 +124,11 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
                                                           int has_out, int n_in, int a0, int n_det, int n, int pad,
                                                           int np, float* __restrict__ maps, int out_a0, int n_sub,
                                                           const int* __restrict__ nsub) {
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  // each wave takes an 8 x 8 quadrant of the 16 x 16 tile: its back-rotated
+  // samples form a compact patch of the prefix-sum planes
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+  const int i = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
   const int ad = blockIdx.z;  // chunk-local (la, d)
   if (i >= n || j >= n) return;
   const int la = ad / n_det;
