@@ -26,6 +26,19 @@
 
 #include "common.hpp"
 
+// skimage's separate float32 / float64 products and sums: no contraction of
+// a * b + c into one rounding in this file (explicit fmaf / fma stay fused)
+#pragma clang fp contract(off)
+
+namespace chip {
+namespace {
+__device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
+__device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
+__device__ __forceinline__ double dmul_rn(double a, double b) { return a * b; }
+__device__ __forceinline__ double dadd_rn(double a, double b) { return a + b; }
+}  // namespace
+}  // namespace chip
+
 namespace chip {
 
 __device__ __forceinline__ float warp_sample(const float* __restrict__ img, int side, int off, float xs, float ys) {
@@ -46,14 +59,14 @@ __device__ __forceinline__ float warp_sample(const float* __restrict__ img, int 
   const float tl = px(r0, c0), tr = px(r0, c1), bl = px(r1, c0), br = px(r1, c1);
   const double dc = dcf, dr = drf;
   // explicit roundings: no FMA contraction, as in the x86-64 build of scikit-image
-  const double top = __dadd_rn(__dmul_rn(1.0 - dc, static_cast<double>(tl)), static_cast<double>(__fmul_rn(dcf, tr)));
-  const double bot = __dadd_rn(__dmul_rn(1.0 - dc, static_cast<double>(bl)), static_cast<double>(__fmul_rn(dcf, br)));
-  return static_cast<float>(__dadd_rn(__dmul_rn(1.0 - dr, top), __dmul_rn(dr, bot)));
+  const double top = dadd_rn(dmul_rn(1.0 - dc, static_cast<double>(tl)), static_cast<double>(mul_rn(dcf, tr)));
+  const double bot = dadd_rn(dmul_rn(1.0 - dc, static_cast<double>(bl)), static_cast<double>(mul_rn(dcf, br)));
+  return static_cast<float>(dadd_rn(dmul_rn(1.0 - dr, top), dmul_rn(dr, bot)));
 }
 
 __device__ __forceinline__ void affine_apply(const Affine& m, float x, float y, float& xs, float& ys) {
-  xs = __fadd_rn(__fadd_rn(__fmul_rn(m.m00, x), __fmul_rn(m.m01, y)), m.m02);
-  ys = __fadd_rn(__fadd_rn(__fmul_rn(m.m10, x), __fmul_rn(m.m11, y)), m.m12);
+  xs = add_rn(add_rn(mul_rn(m.m00, x), mul_rn(m.m01, y)), m.m02);
+  ys = add_rn(add_rn(mul_rn(m.m10, x), mul_rn(m.m11, y)), m.m12);
 }
 
 // beams [0, n_in) use mu_in with rot[rot_in0 + beam], the rest mu_out with

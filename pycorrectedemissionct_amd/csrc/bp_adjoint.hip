@@ -11,7 +11,10 @@
 // <= sqrt(2) (same for y) of the pixel's preimage (u*, y*), i.e. in a 3 x 3
 // block of lattice points; each pixel evaluates those nine weights directly
 // (no scatter, no atomics, deterministic). Samples outside [0, N) in u or y do
-// not exist in the fp and get weight 0.
+// not exist in the fp and get weight 0. The offsets (Px - j, Py - i) are one
+// per-pixel offset plus per-angle lattice constants; they differ from the
+// fp's float32 coordinates by rounding only, and the tent weights are
+// continuous, so <A x, y> = <x, A^T y> holds to about 1e-7 relative.
 //
 // A workgroup owns a 32 x 8 pixel tile (rows of 128 contiguous bytes for the
 // map reads) and walks all angles; per block of 32 angles the line segments

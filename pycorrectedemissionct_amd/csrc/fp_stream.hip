@@ -28,8 +28,16 @@
 // bin-blocks of one angle run back to back on one XCD and share its L2.
 #include "common.hpp"
 
+// the sample coordinates are skimage's separate float32 products and sums:
+// no contraction of a * b + c into one rounding (explicit fmaf stays fused)
+#pragma clang fp contract(off)
+
 namespace chip {
 namespace {
+
+// single-rounding float32 product and sum (no contraction under this file's pragma)
+__device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
+__device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
 
 // Tile shapes: BINS detector bins x ROWS sample rows per chunk, 256 threads in
 // 256 / BINS row groups. The staged box of a chunk spans at most
@@ -81,12 +89,12 @@ __device__ __forceinline__ Box chunk_box(int yc, int n, float fn, float xu_a, fl
                                          float cs, float sn, float m02, float m12) {
   const int y_last = min(yc + T::kRows, n) - 1;
   const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
-  const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
-  const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
-  const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
-  const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
-  const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
-  const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+  const float sya = mul_rn(sn, ya), syb = mul_rn(sn, yb);
+  const float cya = mul_rn(cs, ya), cyb = mul_rn(cs, yb);
+  const float x0 = add_rn(add_rn(xu_a, sya), m02), x1 = add_rn(add_rn(xu_b, sya), m02);
+  const float x2 = add_rn(add_rn(xu_a, syb), m02), x3 = add_rn(add_rn(xu_b, syb), m02);
+  const float y0 = add_rn(add_rn(yu_a, cya), m12), y1 = add_rn(add_rn(yu_b, cya), m12);
+  const float y2 = add_rn(add_rn(yu_a, cyb), m12), y3 = add_rn(add_rn(yu_b, cyb), m12);
   const float xmin = fminf(fminf(x0, x1), fminf(x2, x3));
   const float xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
   const float ymin = fminf(fminf(y0, y1), fminf(y2, y3));
@@ -172,11 +180,11 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   const float* mb = it.mb;
   const float fn = static_cast<float>(n);
   const float uf = static_cast<float>(u);
-  const float xu = __fmul_rn(cs, uf);
-  const float yu = __fmul_rn(-sn, uf);
+  const float xu = mul_rn(cs, uf);
+  const float yu = mul_rn(-sn, uf);
   const int u_last = min(u0 + kBins, n) - 1;
-  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
-  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+  const float xu_a = mul_rn(cs, static_cast<float>(u0)), yu_a = mul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = mul_rn(cs, static_cast<float>(u_last)), yu_b = mul_rn(-sn, static_cast<float>(u_last));
   const float uf_lo = static_cast<float>(u0) - 3.5f, uf_hi = static_cast<float>(u_last) + 3.5f;
 
   // staging slots of this thread: box row, first column, offset from the box origin
@@ -306,8 +314,8 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
       // one sample: skimage inverse map, taps from LDS, order-1 blend
       auto sample = [&](int y, bool clamp) {
         const float yf = static_cast<float>(y);
-        float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-        float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+        float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+        float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
         if (clamp) {
           // outside (-1, N) every tap is padding: clamping keeps the taps on
           // the zero rows/columns of the box and the sample at exactly 0

@@ -24,6 +24,19 @@
 //       per-angle geometry is wave-uniform (scalar loads).
 #include "common.hpp"
 
+// skimage's separate float32 / float64 products and sums: no contraction of
+// a * b + c into one rounding in this file (explicit fmaf / fma stay fused)
+#pragma clang fp contract(off)
+
+namespace chip {
+namespace {
+__device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
+__device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
+__device__ __forceinline__ double dmul_rn(double a, double b) { return a * b; }
+__device__ __forceinline__ double dadd_rn(double a, double b) { return a + b; }
+}  // namespace
+}  // namespace chip
+
 namespace chip {
 
 // ---------------------------------------------------------------------------
@@ -66,12 +79,12 @@ __global__ __launch_bounds__(256) void fp_kernel(const float* __restrict__ vol, 
 
   if (u < n) {
     const float uf = static_cast<float>(u);
-    const float xu = __fmul_rn(cs, uf);
-    const float yu = __fmul_rn(-sn, uf);
+    const float xu = mul_rn(cs, uf);
+    const float yu = mul_rn(-sn, uf);
     for (int y = dy; y < n; y += kFpLanesY) {
       const float yf = static_cast<float>(y);
-      const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-      const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+      const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+      const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
       if (!(xs > -1.f && xs < static_cast<float>(n) && ys > -1.f && ys < static_cast<float>(n))) continue;
       const float c0f = floorf(xs);
       const float r0f = floorf(ys);
@@ -195,12 +208,12 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ v
 
   // per-thread constant parts of the sample map
   const float uf = static_cast<float>(u);
-  const float xu = __fmul_rn(cs, uf);
-  const float yu = __fmul_rn(-sn, uf);
+  const float xu = mul_rn(cs, uf);
+  const float yu = mul_rn(-sn, uf);
   // corner columns of the block (clamped to real bins)
   const int u_last = min(u0 + kLdsBins, n) - 1;
-  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
-  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+  const float xu_a = mul_rn(cs, static_cast<float>(u0)), yu_a = mul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = mul_rn(cs, static_cast<float>(u_last)), yu_b = mul_rn(-sn, static_cast<float>(u_last));
 
   float acc[D];
 #pragma unroll
@@ -212,12 +225,12 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ v
     float xmin, xmax, ymin, ymax;
     {
       const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
-      const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
-      const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
-      const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
-      const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
-      const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
-      const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+      const float sya = mul_rn(sn, ya), syb = mul_rn(sn, yb);
+      const float cya = mul_rn(cs, ya), cyb = mul_rn(cs, yb);
+      const float x0 = add_rn(add_rn(xu_a, sya), m02), x1 = add_rn(add_rn(xu_b, sya), m02);
+      const float x2 = add_rn(add_rn(xu_a, syb), m02), x3 = add_rn(add_rn(xu_b, syb), m02);
+      const float y0 = add_rn(add_rn(yu_a, cya), m12), y1 = add_rn(add_rn(yu_b, cya), m12);
+      const float y2 = add_rn(add_rn(yu_a, cyb), m12), y3 = add_rn(add_rn(yu_b, cyb), m12);
       xmin = fminf(fminf(x0, x1), fminf(x2, x3));
       xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
       ymin = fminf(fminf(y0, y1), fminf(y2, y3));
@@ -258,8 +271,8 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ v
         const int y = yb0 + k;
         if (y >= n) break;
         const float yf = static_cast<float>(y);
-        const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-        const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+        const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+        const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
         if (!(xs > -1.f && xs < fn && ys > -1.f && ys < fn)) continue;
         const float c0f = floorf(xs);
         const float r0f = floorf(ys);
@@ -322,12 +335,12 @@ __device__ __forceinline__ FpBox fp_chunk_box(int yc, int n, float fn, float xu_
                                               float cs, float sn, float m02, float m12) {
   const int y_last = min(yc + kLdsRows, n) - 1;
   const float ya = static_cast<float>(yc), yb = static_cast<float>(y_last);
-  const float sya = __fmul_rn(sn, ya), syb = __fmul_rn(sn, yb);
-  const float cya = __fmul_rn(cs, ya), cyb = __fmul_rn(cs, yb);
-  const float x0 = __fadd_rn(__fadd_rn(xu_a, sya), m02), x1 = __fadd_rn(__fadd_rn(xu_b, sya), m02);
-  const float x2 = __fadd_rn(__fadd_rn(xu_a, syb), m02), x3 = __fadd_rn(__fadd_rn(xu_b, syb), m02);
-  const float y0 = __fadd_rn(__fadd_rn(yu_a, cya), m12), y1 = __fadd_rn(__fadd_rn(yu_b, cya), m12);
-  const float y2 = __fadd_rn(__fadd_rn(yu_a, cyb), m12), y3 = __fadd_rn(__fadd_rn(yu_b, cyb), m12);
+  const float sya = mul_rn(sn, ya), syb = mul_rn(sn, yb);
+  const float cya = mul_rn(cs, ya), cyb = mul_rn(cs, yb);
+  const float x0 = add_rn(add_rn(xu_a, sya), m02), x1 = add_rn(add_rn(xu_b, sya), m02);
+  const float x2 = add_rn(add_rn(xu_a, syb), m02), x3 = add_rn(add_rn(xu_b, syb), m02);
+  const float y0 = add_rn(add_rn(yu_a, cya), m12), y1 = add_rn(add_rn(yu_b, cya), m12);
+  const float y2 = add_rn(add_rn(yu_a, cyb), m12), y3 = add_rn(add_rn(yu_b, cyb), m12);
   const float xmin = fminf(fminf(x0, x1), fminf(x2, x3));
   const float xmax = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
   const float ymin = fminf(fminf(y0, y1), fminf(y2, y3));
@@ -369,11 +382,11 @@ __global__ __launch_bounds__(256) void fp_lds2_kernel(const float* __restrict__ 
                             : nullptr;
   const float fn = static_cast<float>(n);
   const float uf = static_cast<float>(u);
-  const float xu = __fmul_rn(cs, uf);
-  const float yu = __fmul_rn(-sn, uf);
+  const float xu = mul_rn(cs, uf);
+  const float yu = mul_rn(-sn, uf);
   const int u_last = min(u0 + kLdsBins, n) - 1;
-  const float xu_a = __fmul_rn(cs, static_cast<float>(u0)), yu_a = __fmul_rn(-sn, static_cast<float>(u0));
-  const float xu_b = __fmul_rn(cs, static_cast<float>(u_last)), yu_b = __fmul_rn(-sn, static_cast<float>(u_last));
+  const float xu_a = mul_rn(cs, static_cast<float>(u0)), yu_a = mul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = mul_rn(cs, static_cast<float>(u_last)), yu_b = mul_rn(-sn, static_cast<float>(u_last));
 
   float4 fr[kP2PerThread];
   float4 mr[D][kP2PerThread];
@@ -452,8 +465,8 @@ __global__ __launch_bounds__(256) void fp_lds2_kernel(const float* __restrict__ 
 #pragma unroll
         for (int k = 0; k < kLdsRowsPerThread; ++k) {
           const float yf = static_cast<float>(yb0 + k);
-          const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-          const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+          const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+          const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
           const float c0f = floorf(xs);
           const float r0f = floorf(ys);
           const float dc = xs - c0f;
@@ -473,8 +486,8 @@ __global__ __launch_bounds__(256) void fp_lds2_kernel(const float* __restrict__ 
           const int y = yb0 + k;
           if (y >= n) break;
           const float yf = static_cast<float>(y);
-          const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-          const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+          const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+          const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
           if (!(xs > -1.f && xs < fn && ys > -1.f && ys < fn)) continue;
           const float c0f = floorf(xs);
           const float r0f = floorf(ys);
@@ -726,13 +739,13 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float* __restrict
     const int y_hi = min(static_cast<int>(floorf(ysm + 1.5f)) + 1, n - 1);
     for (int u = u_lo; u <= u_hi; ++u) {
       const float uf = static_cast<float>(u);
-      const float xu = __fmul_rn(cs, uf);
-      const float yu = __fmul_rn(-sn, uf);
+      const float xu = mul_rn(cs, uf);
+      const float yu = mul_rn(-sn, uf);
       float kern = 0.f;
       for (int y = y_lo; y <= y_hi; ++y) {
         const float yf = static_cast<float>(y);
-        const float xs = __fadd_rn(__fadd_rn(xu, __fmul_rn(sn, yf)), m02);
-        const float ys = __fadd_rn(__fadd_rn(yu, __fmul_rn(cs, yf)), m12);
+        const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+        const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
         const float c0f = floorf(xs);
         const float r0f = floorf(ys);
         const int c0 = static_cast<int>(c0f);
