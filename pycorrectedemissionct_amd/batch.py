@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import _arrays, _native
+from .solvers import Solver
 
 _BP_MODES = {"skimage": _native.CHIP_BP_SKIMAGE, "symmetric": _native.CHIP_BP_SKIMAGE_SYM,
              "adjoint": _native.CHIP_BP_ADJOINT}
@@ -307,17 +308,54 @@ class BatchedSIRT:
 
 
 class BatchedPDHG:
-    """PDHG with an l2 or KL data term and isotropic TV(lambda) (``solvers.py:1060-1206``) on every slice."""
+    """PDHG with an l2 or KL data term and isotropic TV(lambda) (``solvers.py:1060-1206``) on every slice.
+
+    ``regularizer`` (instead of ``tv_lambda``) takes any gradient-family regularizer of
+    ``regularizers.py`` -- TV2D over one slice, or VTV / TNV / TV3D coupling ``channels``
+    consecutive slices of equal size (the emission channels of one slice) as one
+    ``[channels, N, N]`` volume -- run by ``chip_grad_dual`` / ``chip_grad_update_primal`` per
+    group inside the captured iteration.
+    """
 
     def __init__(self, data_term: str = "l2", tv_lambda: float | None = None, relaxation: float = 0.95,
-                 graph_chunk: int = 10, use_graph: bool = True):
+                 graph_chunk: int = 10, use_graph: bool = True, regularizer=None, channels: int = 1):
         if data_term not in ("l2", "kl"):
             raise ValueError(f"Unknown data term: {data_term}")
+        if regularizer is not None and tv_lambda is not None:
+            raise ValueError("Give either tv_lambda or regularizer")
         self.data_term = data_term
         self.tv_lambda = tv_lambda
+        self.regularizer = regularizer
+        self.channels = int(channels)
         self.relaxation = relaxation
         self.graph_chunk = graph_chunk
         self.use_graph = use_graph
+
+    def _groups(self, A: BatchedProjectorXRF, x: torch.Tensor):
+        """(kernel parameters, primal view, dual view) per channel group of the regularizer."""
+        from .regularizers import Regularizer_Grad
+
+        reg = self.regularizer
+        if not isinstance(reg, Regularizer_Grad):
+            raise NotImplementedError("the batched PDHG takes gradient-family regularizers")
+        c = self.channels
+        if len(A.sizes) % c:
+            raise ValueError(f"{len(A.sizes)} slices do not split into groups of {c} channels")
+        groups, q_elems = [], 0
+        for g in range(len(A.sizes) // c):
+            sizes = A.sizes[g * c:(g + 1) * c]
+            if len(set(sizes)) != 1:
+                raise ValueError(f"the channels of group {g} differ in size: {sizes}")
+            shape = (c, sizes[0], sizes[0]) if c > 1 else (sizes[0], sizes[0])
+            cfg = reg.device_config(shape)
+            start = int(A.plan.vol_start[g * c])
+            n_el = int(np.prod(shape))
+            groups.append((cfg, start, n_el, q_elems))
+            q_elems += cfg["nd"] * n_el
+        q = torch.zeros(q_elems, dtype=torch.float32, device=x.device)
+        tau_reg = float(reg.initialize_sigma_tau(np.zeros(groups[0][0]["q_shape"][1:], dtype=np.float32)))
+        out = [(cfg, start, n_el, q[qo:qo + cfg["nd"] * n_el]) for cfg, start, n_el, qo in groups]
+        return out, q, tau_reg
 
     def prepare(self, A: BatchedProjectorXRF, b: torch.Tensor, x0=None, dual_init: bool = False) -> dict:
         """Device state of the solve. With ``x0`` and ``dual_init``, the duals start consistent
@@ -326,6 +364,9 @@ class BatchedPDHG:
         b = _arrays.to_dev(b)[0].reshape(-1).contiguous()
         # TV's tau contribution: 2 * ndims, unweighted for its l12 (an l1-family) norm (regularizers.py:260-273)
         tau_reg = 4.0 if self.tv_lambda is not None else 0.0
+        groups, q_groups = None, None
+        if self.regularizer is not None:
+            groups, q_groups, tau_reg = self._groups(A, torch.empty(0, device=A.device))
         tau = _precondition(A, A.bp(torch.ones_like(b)).clone(), A.plan.vol_start, tau_reg, self.relaxation)
         sigma = _precondition(A, A.fp(torch.ones_like(tau)).clone(), A.plan.sino_start, 0.0, self.relaxation)
         x = torch.zeros_like(tau) if x0 is None else _arrays.to_dev(x0)[0].reshape(-1).clone()
@@ -340,7 +381,7 @@ class BatchedPDHG:
                 for _ in range(8):  # q <- proj(q + grad x / 2) -> lambda grad x / |grad x| where grad x != 0
                     _native.call("chip_batch_tv_dual", A.plan._h, q.data_ptr(), x.data_ptr(), float(self.tv_lambda), st)
         return dict(b=b, tau=tau, sigma=sigma, x=x, xbar=x.clone(), p=p, q=q,
-                    ax=torch.empty_like(b), upd=torch.empty_like(tau))
+                    ax=torch.empty_like(b), upd=torch.empty_like(tau), groups=groups, q_groups=q_groups)
 
     def iterate(self, A: BatchedProjectorXRF, s: dict, iterations: int, lower_limit=None, upper_limit=None):
         hl, lo = _limit(lower_limit)
@@ -355,7 +396,13 @@ class BatchedPDHG:
             if s["q"] is not None:
                 _native.call("chip_batch_tv_dual", A.plan._h, s["q"].data_ptr(), s["xbar"].data_ptr(),
                              float(self.tv_lambda), st)
+            if s["groups"] is not None:
+                for cfg, start, n_el, qg in s["groups"]:
+                    Solver._grad_dual(cfg, qg, s["xbar"][start:start + n_el], st)
             A.bp(s["p"], s["upd"])
+            if s["groups"] is not None:
+                for cfg, start, n_el, qg in s["groups"]:
+                    Solver._grad_primal(cfg, s["upd"][start:start + n_el], qg, False, st)
             _native.call("chip_batch_pdhg_primal", A.plan._h, s["x"].data_ptr(), s["xbar"].data_ptr(),
                          s["upd"].data_ptr(), _native.ptr(s["q"]), s["tau"].data_ptr(), hl, lo, hu, hi, None, st)
 
