@@ -12,6 +12,8 @@
 // detector) are staged in LDS once per block instead of being gathered
 // through the caches by every pixel, and the symmetric form's map values are
 // loaded 8 angles at a time, the next 8 while the current are summed.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace chip {
@@ -22,25 +24,19 @@ constexpr int kSkAngBlk = 32;
 constexpr int kSkGrp = 8;
 constexpr int kSkWin = 28;  // >= 15 (|c| + |s|) + 4, with one bin of slack each side
 
+// One 16 x 16 tile (tx, ty) of one slice: sb = the slice's sinogram [n_det][n_angles][n],
+// mb = its maps or NULL, out = the slice's image.
 template <int SYM, int D>
-__global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __restrict__ sino,
-                                                              const float* __restrict__ maps,
-                                                              int64_t maps_slice_stride, float* __restrict__ vol,
-                                                              const float4* __restrict__ geom,
-                                                              const float* __restrict__ w, int n, int n_angles,
-                                                              int n_det, int det0, int angle0, int n_ang,
-                                                              int maps_a0, int accumulate) {
-  __shared__ float win[kSkAngBlk][D][kSkWin];
-  __shared__ int wlo_s[kSkAngBlk];
+__device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ sb, const float* __restrict__ mb,
+                                                      float* __restrict__ out, const float4* __restrict__ geom,
+                                                      const float* __restrict__ w, int n, int n_angles, int n_det,
+                                                      int det0, int angle0, int n_ang, int maps_a0, int accumulate,
+                                                      int tx, int ty, float (*win)[D][kSkWin], int* wlo_s) {
   const int tid = threadIdx.x;
-  const int tx = static_cast<int>(blockIdx.x), ty = static_cast<int>(blockIdx.y);
   const int j = tx * kSkTile + (tid & (kSkTile - 1));
   const int i = ty * kSkTile + (tid / kSkTile);
-  const int b = blockIdx.z;
   const int k = n / 2;
   const size_t nn = static_cast<size_t>(n) * n;
-  const float* sb = sino + static_cast<size_t>(b) * n_det * n_angles * n;
-  const float* mb = (SYM && maps) ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
   const bool inside = i < n && j < n;
   const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
   const float jc = static_cast<float>(j - k);
@@ -133,8 +129,53 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __res
     }
   }
   if (inside) {
-    float* o = vol + static_cast<size_t>(b) * nn + pix;
+    float* o = out + pix;
     *o = accumulate ? *o + acc : acc;
+  }
+}
+
+template <int SYM, int D>
+__global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __restrict__ sino,
+                                                              const float* __restrict__ maps,
+                                                              int64_t maps_slice_stride, float* __restrict__ vol,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ w, int n, int n_angles,
+                                                              int n_det, int det0, int angle0, int n_ang,
+                                                              int maps_a0, int accumulate) {
+  __shared__ float win[kSkAngBlk][D][kSkWin];
+  __shared__ int wlo_s[kSkAngBlk];
+  const int b = blockIdx.z;
+  const float* sb = sino + static_cast<size_t>(b) * n_det * n_angles * n;
+  const float* mb = (SYM && maps) ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  bp_skimage_tile_block<SYM, D>(sb, mb, vol + static_cast<size_t>(b) * n * n, geom, w, n, n_angles, n_det, det0,
+                                angle0, n_ang, maps_a0, accumulate, static_cast<int>(blockIdx.x),
+                                static_cast<int>(blockIdx.y), win, wlo_s);
+}
+
+// Persistent form over a batch of slices of different sizes: item k is the
+// 16 x 16 tile of slice s found in the tile prefix sums.
+template <int SYM, int D>
+__global__ __launch_bounds__(256) void bp_skimage_tile_batch_kernel(const BatchSlice* __restrict__ slices,
+                                                                    int n_slices,
+                                                                    const int64_t* __restrict__ item_start,
+                                                                    const float* __restrict__ sino,
+                                                                    const float* __restrict__ maps,
+                                                                    float* __restrict__ vol,
+                                                                    const float4* __restrict__ geom,
+                                                                    const float* __restrict__ w, int n_angles,
+                                                                    int n_det, int det0, int accumulate) {
+  __shared__ float win[kSkAngBlk][D][kSkWin];
+  __shared__ int wlo_s[kSkAngBlk];
+  const int64_t total = item_start[n_slices];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_slice(item_start, n_slices, it);
+    const BatchSlice sl = slices[s];
+    const int local = static_cast<int>(it - item_start[s]);
+    const int ntx = (sl.n + kSkTile - 1) / kSkTile;
+    const int ty = local / ntx, tx = local - ty * ntx;
+    bp_skimage_tile_block<SYM, D>(sino + sl.sino_off, (SYM && maps) ? maps + sl.maps_off : nullptr,
+                                  vol + sl.vol_off, geom + static_cast<size_t>(sl.size_idx) * n_angles, w, sl.n,
+                                  n_angles, n_det, det0, 0, n_angles, 0, accumulate, tx, ty, win, wlo_s);
   }
 }
 
@@ -163,6 +204,31 @@ void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps
       if (two) launch_tiles<0, 2>(p, sino, nullptr, 0, vol, batch, det0, angle0, n_ang, 0, acc, st);
       else launch_tiles<0, 1>(p, sino, nullptr, 0, vol, batch, det0, angle0, n_ang, 0, acc, st);
     }
+  }
+}
+
+// Batched form: one persistent launch per pass of two detectors over the 16 x 16
+// tiles of every slice.
+void launch_bp_skimage_tiles_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
+                                   hipStream_t st) {
+  const int64_t total = bp.h_item_start_bps.back();
+  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
+  const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
+  for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
+    const bool two = bp.n_det - det0 >= 2;
+    const int acc = det0 > 0 ? 1 : 0;
+    const float* m = sym ? maps : nullptr;
+#define CHIP_SK_BATCH(S, DD)                                                                                      \
+  hipLaunchKernelGGL((bp_skimage_tile_batch_kernel<S, DD>), dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices, \
+                     bp.d_item_start_bps, sino, m, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det, det0, acc)
+    if (sym) {
+      if (two) CHIP_SK_BATCH(1, 2);
+      else CHIP_SK_BATCH(1, 1);
+    } else {
+      if (two) CHIP_SK_BATCH(0, 2);
+      else CHIP_SK_BATCH(0, 1);
+    }
+#undef CHIP_SK_BATCH
   }
 }
 

@@ -145,6 +145,8 @@ void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_
 
 // exact adjoint of the fp over angles [angle0, angle0 + n_ang), all detectors;
 // maps[0] holds angle maps_a0; accumulate = 1 adds to vol (bp_adjoint.hip)
+void launch_bp_skimage_tiles_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
+                                   hipStream_t st);
 void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
                              int batch, int sym, int angle0, int n_ang, int maps_a0, int accumulate,
                              hipStream_t st);

@@ -660,46 +660,10 @@ __global__ __launch_bounds__(256) void bp_skimage_kernel(const float* __restrict
   *o = accumulate ? *o + acc : acc;
 }
 
-// Persistent form over a batch of slices of different sizes: item k is the
-// 16 x 16 pixel tile of slice s found in the tile prefix sums.
-template <int SYM>
-__global__ __launch_bounds__(256) void bp_skimage_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
-                                                               const int64_t* __restrict__ item_start,
-                                                               const float* __restrict__ sino,
-                                                               const float* __restrict__ maps,
-                                                               float* __restrict__ vol,
-                                                               const float4* __restrict__ geom,
-                                                               const float* __restrict__ w, int n_angles, int n_det) {
-  const int64_t total = item_start[n_slices];
-  for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
-    const int s = find_slice(item_start, n_slices, k);
-    const BatchSlice sl = slices[s];
-    const int local = static_cast<int>(k - item_start[s]);
-    const int ntx = (sl.n + kBpTile - 1) / kBpTile;
-    const int ty = local / ntx, tx = local - ty * ntx;
-    const int j = tx * kBpTile + (threadIdx.x & (kBpTile - 1));
-    const int i = ty * kBpTile + (threadIdx.x / kBpTile);
-    if (i < sl.n && j < sl.n) {
-      const float acc = bp_skimage_pixel<SYM>(sino + sl.sino_off, (SYM && maps) ? maps + sl.maps_off : nullptr,
-                                              geom + static_cast<size_t>(sl.size_idx) * n_angles, w, sl.n, n_angles,
-                                              n_det, 0, n_angles, 0, 0, i, j);
-      vol[sl.vol_off + static_cast<int64_t>(i) * sl.n + j] = acc;
-    }
-  }
-}
-
 void launch_bp_skimage_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
                              hipStream_t st) {
-  const int64_t total = bp.h_item_start_bps.back();
-  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
-  const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
-  if (sym) {
-    hipLaunchKernelGGL(bp_skimage_batch_kernel<1>, dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
-                       bp.d_item_start_bps, sino, maps, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det);
-  } else {
-    hipLaunchKernelGGL(bp_skimage_batch_kernel<0>, dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
-                       bp.d_item_start_bps, sino, nullptr, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det);
-  }
+  // line windows staged per tile, as for single slices (bp_skimage_tile.hip)
+  launch_bp_skimage_tiles_batch(bp, sino, maps, vol, sym, st);
 }
 
 // ---------------------------------------------------------------------------
