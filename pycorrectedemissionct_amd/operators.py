@@ -127,6 +127,21 @@ class ProjectorOperator(BaseTransform):
         return None
 
 
+class TransformIdentity(BaseTransform):
+    """Identity operator (``operators.py:365-384``)."""
+
+    def __init__(self, x_shape):
+        self.dir_shape = np.array(x_shape, ndmin=1, dtype=int)
+        self.adj_shape = np.array(x_shape, ndmin=1, dtype=int)
+        super().__init__()
+
+    def _op_direct(self, x):
+        return x
+
+    def _op_adjoint(self, x):
+        return x
+
+
 class TransformGradient(BaseTransform):
     """Gradient with edge padding and its adjoint ``-div`` (``operators.py:767-844``)."""
 

@@ -23,11 +23,18 @@ from __future__ import annotations
 
 import numpy as np
 
+import torch
+
+from . import _arrays, _native
 from . import data_terms as dt
 from . import operators
 
 
 class BaseRegularizer:
+    """Regularizer object API of ``regularizers.py:36-215``: ``initialize_sigma_tau``,
+    ``initialize_dual``, ``update_dual``, ``apply_proximal_dual`` and ``compute_update_primal`` on
+    NumPy arrays or device tensors (the solvers call the fused kernels directly)."""
+
     __reg_name__ = ""
 
     def __init__(self, weight, norm: dt.DataFidelityBase, upd_mask=None, dtype=np.float32):
@@ -36,6 +43,43 @@ class BaseRegularizer:
         self.op = None
         self.norm = norm
         self.upd_mask = upd_mask
+        self.sigma = 1.0
+
+    def _check_op(self) -> None:
+        if self.op is None:
+            raise ValueError("Regularizer not initialized! Please use method: `initialize_sigma_tau`.")
+
+    def initialize_dual(self):
+        """Zero dual of the operator's adjoint shape (``regularizers.py:120-132``)."""
+        self._check_op()
+        return np.zeros(self.op.adj_shape, dtype=self.dtype)
+
+    def update_dual(self, dual, primal) -> None:
+        """``dual += sigma * op(primal)`` in place (``regularizers.py:134-148``)."""
+        self._check_op()
+        dual += self.sigma * self.op(primal)
+
+    def compute_update_primal(self, dual):
+        """``op^T(dual)``, times the weight outside the l1 family (``regularizers.py:194-210``)."""
+        self._check_op()
+        upd = self.op.T(dual)
+        if self.upd_mask is not None:
+            upd = upd * self.upd_mask
+        if not isinstance(self.norm, dt.DataFidelity_l1):
+            upd = upd * float(self.weight)
+        return upd
+
+    def _prox_on_device(self, dual, cfg: dict) -> None:
+        """The dual proximal of ``cfg`` in place: ``chip_grad_dual`` with a zero primal."""
+        t, host = _arrays.to_dev(dual)
+        t = t.contiguous()
+        zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
+        _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
+                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"], _native.stream())
+        if host:
+            dual[...] = t.reshape(np.shape(dual)).cpu().numpy()
+        elif t.data_ptr() != dual.data_ptr():
+            dual.copy_(t.reshape(dual.shape))
 
     def info(self) -> str:
         return self.__reg_name__ + f"(w:{self.weight.max():.3e})"
@@ -73,6 +117,11 @@ class Regularizer_Grad(BaseRegularizer):
         if not isinstance(self.norm, dt.DataFidelity_l1):
             tau *= self.weight
         return tau
+
+    def apply_proximal_dual(self, dual) -> None:
+        """The norm's dual proximal in place (``regularizers.py:150-162``; VTV / TNV override the
+        norm as in the reference), on the device."""
+        self._prox_on_device(dual, self.device_config(tuple(np.shape(dual))[1:]))
 
     def _check_trailing_axes(self, shape) -> None:
         nd = len(shape)
@@ -235,8 +284,22 @@ class Regularizer_l1(BaseRegularizer):
         super().__init__(weight=weight, norm=dt.DataFidelity_l1() if norm is None else norm, upd_mask=upd_mask)
 
     def initialize_sigma_tau(self, primal) -> float:
-        self.op = None  # identity transform
+        """Identity transform, sigma = 1, tau = 1 (``regularizers.py:544-553``)."""
+        self.op = operators.TransformIdentity(np.shape(primal))
+        self.sigma = 1.0
         return 1.0
+
+    def update_dual(self, dual, primal) -> None:
+        dual += primal
+
+    def apply_proximal_dual(self, dual) -> None:
+        """Clip to [-weight, weight] (``data_terms.py:549-555``), on the device."""
+        if np.ndim(self.weight) != 0:
+            raise NotImplementedError("the device l1 proximal takes a scalar weight")
+        n_el = int(np.prod(np.shape(dual)))
+        cfg = dict(mode=1, n_chan=1, n_batch=1, nd=1, dims=np.array([n_el], dtype=np.int32),
+                   weight=float(self.weight), der=0, chan=0, q_shape=(1, n_el))
+        self._prox_on_device(dual, cfg)
 
     def primal_prox_weight(self) -> float:
         """The weight of the fused soft threshold (prox of tau * weight * ||.||_1)."""
