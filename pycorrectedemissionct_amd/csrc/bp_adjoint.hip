@@ -40,6 +40,10 @@ struct BpItem {
   int tx, ty;
 };
 
+// hat(t) = max(0, 1 - |t|): 1 - |t| <= 1, so clamping it to [0, 1] is the same
+// value, which the hardware applies as an output modifier of the subtraction
+__device__ __forceinline__ float tent(float t) { return __builtin_amdgcn_fmed3f(1.f - fabsf(t), 0.f, 1.f); }
+
 // Per angle of a block: first window bin, whether every candidate row of the
 // tile lies in [0, N), and the lattice offsets of the nine candidates,
 // cxy[k] = (c ku + s ky, -s ku + c ky) for k = 3 ku + ky.
@@ -149,8 +153,8 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
               float ksum = 0.f;
 #pragma unroll
               for (int ky = 0; ky < 3; ++ky) {
-                const float wx = fmaxf(1.f - fabsf(dx0 + ba.cx[3 * ku + ky]), 0.f);
-                const float wy = fmaxf(1.f - fabsf(dy0 + ba.cy[3 * ku + ky]), 0.f);
+                const float wx = tent(dx0 + ba.cx[3 * ku + ky]);
+                const float wy = tent(dy0 + ba.cy[3 * ku + ky]);
                 ksum = fmaf(wx, wy, ksum);
               }
               K[ku] = ksum;
@@ -163,8 +167,8 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
               float ksum = 0.f;
 #pragma unroll
               for (int ky = 0; ky < 3; ++ky) {
-                const float wx = fmaxf(1.f - fabsf(dx0 + ba.cx[3 * ku + ky]), 0.f);
-                float wy = fmaxf(1.f - fabsf(dy0 + ba.cy[3 * ku + ky]), 0.f);
+                const float wx = tent(dx0 + ba.cx[3 * ku + ky]);
+                float wy = tent(dy0 + ba.cy[3 * ku + ky]);
                 wy = (static_cast<unsigned>(iy0 + ky) < static_cast<unsigned>(n)) ? wy : 0.f;
                 ksum = fmaf(wx, wy, ksum);
               }
