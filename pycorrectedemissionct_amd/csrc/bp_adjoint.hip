@@ -45,14 +45,15 @@ struct BpItem {
 __device__ __forceinline__ float tent(float t) { return __builtin_amdgcn_fmed3f(1.f - fabsf(t), 0.f, 1.f); }
 
 // Per angle of a block: first window bin, whether every candidate row of the
-// tile lies in [0, N), and the lattice offsets of the nine candidates,
-// cxy[k] = (c ku + s ky, -s ku + c ky) for k = 3 ku + ky.
+// tile lies in [0, N), 1 - (|c| + |s|), and the lattice offsets of the nine
+// candidates, cxy[k] = (c ku + s ky, -s ku + c ky) for k = 3 ku + ky.
 struct BpAngle {
   float cx[9];
   float cy[9];
   int wlo;
   int rows_inside;
-  int pad[2];
+  float omr;
+  int pad;
 };
 
 template <int D, bool MAPS>
@@ -91,6 +92,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const float ymin = fminf(fminf(y00, y01), fminf(y10, y11));
       const float ymax = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
       BpAngle& ba = ang_s[tid];
+      ba.omr = 1.f - r;
       ba.wlo = static_cast<int>(floorf(umin - r)) - 1;
       // candidate rows are floor(y* - r) + 1 .. + 3 (one of slack for rounding)
       ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
@@ -109,7 +111,10 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const int d = rem / kWin;
       const int q = rem - d * kWin;
       const int u = ang_s[al].wlo + q;
-      win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] : 0.f;
+      // the line times its weight w[a, d], applied once per bin instead of once per pixel
+      win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
+                                              w[static_cast<size_t>(a0 + al) * n_det_total + det0 + d]
+                                        : 0.f;
     }
     __syncthreads();
     if (inside) {
@@ -137,12 +142,11 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
           const float4 g = geom[a];
           const float cs = g.x, sn = g.y;
           const BpAngle& ba = ang_s[al];
-          const float r = fabsf(cs) + fabsf(sn);
           const float px = fj - g.z, py = fi - g.w;
-          const float us = cs * px - sn * py;  // preimage of the pixel centre
-          const float ys = sn * px + cs * py;
-          const float fu0 = floorf(us - r) + 1.f;  // first of <= 3 candidate bins / rows
-          const float fy0 = floorf(ys - r) + 1.f;
+          // first of <= 3 candidate bins / rows: floor(p* - r) + 1 for the pixel centre's
+          // preimage (u*, y*) = (c px - s py, s px + c py) and r = |c| + |s|
+          const float fu0 = floorf(fmaf(cs, px, fmaf(-sn, py, ba.omr)));
+          const float fy0 = floorf(fmaf(sn, px, fmaf(cs, py, ba.omr)));
           // sample offsets from the pixel: (c u + s y + m02 - j, -s u + c y + m12 - i) at (fu0, fy0)
           const float dx0 = fmaf(cs, fu0, fmaf(sn, fy0, -px));
           const float dy0 = fmaf(-sn, fu0, fmaf(cs, fy0, -py));
@@ -179,9 +183,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
 #pragma unroll
           for (int d = 0; d < D; ++d) {
             const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
-            float f = w[static_cast<size_t>(a) * n_det_total + det0 + d];
-            if (MAPS) f *= mcur[k][d];
-            out = fmaf(f, gd, out);
+            out = MAPS ? fmaf(mcur[k][d], gd, out) : out + gd;
           }
         }
 #pragma unroll
