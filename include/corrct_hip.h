@@ -157,9 +157,11 @@ int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const fl
  * 1 (LDS-staged whole-sinogram kernel), 2 (LDS-staged with register
  * prefetch), 3 (streamed LDS tiles of 64 bins x 32 rows, the detectors'
  * products interleaved), 4 (3 staging only the footprint of interior chunks:
- * less HBM traffic, no faster) or 5 (3 with 32-bin tiles: a third of the LDS,
- * twice the workgroups per CU; the default); 2..5 need N % 4 == 0 and 16-byte
- * aligned vol / maps, else 1 runs.
+ * less HBM traffic, no faster), 5 (3 with 32-bin tiles: a third of the LDS,
+ * twice the workgroups per CU; the default) or 6 (workgroups walking
+ * image-space bands of 16 rows or 32 columns that stage each map value once:
+ * 27 % less HBM traffic than 5, slower at 512^2); 2..6 need N % 4 == 0 and
+ * 16-byte aligned vol / maps, else 1 runs.
  */
 int chip_plan_tune(chip_plan* plan, const char* key, int value);
 

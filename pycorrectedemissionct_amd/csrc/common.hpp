@@ -76,8 +76,8 @@ struct Plan {
   int device = 0;
   // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged,
   // 2 = LDS-staged with register prefetch, 3 = streamed per-detector tiles,
-  // 4 = 3 staging only the footprint of interior chunks
-  // (2..4 fall back to 1 when N % 4 != 0)
+  // 4 = 3 staging only the footprint of interior chunks, 5 = 3 with 32-bin tiles,
+  // 6 = image-space bands (2..6 fall back to 1 when N % 4 != 0)
   int fp_variant = 5;
 };
 
@@ -142,6 +142,11 @@ void launch_bp_skimage_batch(const BatchPlan& bp, const float* sino, const float
 // aligned vol / maps (fp_stream.hip)
 void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
                       int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st);
+
+// the same fp in image-space bands of rows or columns, staging each map value once
+// (fp_band.hip); same preconditions as launch_fp_stream
+void launch_fp_band(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
+                    int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st);
 
 // exact adjoint of the fp over angles [angle0, angle0 + n_ang), all detectors;
 // maps[0] holds angle maps_a0; accumulate = 1 adds to vol (bp_adjoint.hip)

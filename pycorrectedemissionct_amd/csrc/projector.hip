@@ -556,7 +556,9 @@ static void launch_fp(const Plan& p, const float* vol, const float* maps, int64_
     // two detectors per pass keep the staged tiles within 64 KiB of LDS
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const bool two = p.n_det - det0 >= 2;
-      if (variant >= 3) {
+      if (variant == 6) {
+        launch_fp_band(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, 0, p.n_angles, 0, st);
+      } else if (variant >= 3) {
         launch_fp_stream(p, vol, maps, stride, sino, batch, det0, two ? 2 : 1, 0, p.n_angles, 0, st);
       } else if (variant == 2) {
         if (two) launch_fp_lds2_group<2>(p, vol, maps, stride, sino, batch, det0, st);
@@ -763,7 +765,8 @@ void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* 
   if (vec_ok && p.fp_variant >= 3) {
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const int nd = p.n_det - det0 >= 2 ? 2 : 1;
-      launch_fp_stream(p, vol, maps, 0, sino, 1, det0, nd, angle0, n_ang, maps_a0, st);
+      if (p.fp_variant == 6) launch_fp_band(p, vol, maps, 0, sino, 1, det0, nd, angle0, n_ang, maps_a0, st);
+      else launch_fp_stream(p, vol, maps, 0, sino, 1, det0, nd, angle0, n_ang, maps_a0, st);
     }
     return;
   }
@@ -838,7 +841,7 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 5, "fp_kernel must be in 0..5");
+    CHIP_CHECK_ARG(value >= 0 && value <= 6, "fp_kernel must be in 0..6");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }
