@@ -21,6 +21,8 @@
 // the tile needs (<= 40 bins per angle and detector) are staged in LDS.
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace chip {
 namespace {
 
@@ -44,16 +46,11 @@ struct BpItem {
 // value, which the hardware applies as an output modifier of the subtraction
 __device__ __forceinline__ float tent(float t) { return __builtin_amdgcn_fmed3f(1.f - fabsf(t), 0.f, 1.f); }
 
-// Per angle of a block: first window bin, whether every candidate row of the
-// tile lies in [0, N), 1 - (|c| + |s|), and the lattice offsets of the nine
-// candidates, cxy[k] = (c ku + s ky, -s ku + c ky) for k = 3 ku + ky.
+// Per angle of a block: first window bin and whether every candidate row of
+// the tile lies in [0, N).
 struct BpAngle {
-  float cx[9];
-  float cy[9];
   int wlo;
   int rows_inside;
-  float omr;
-  int pad;
 };
 
 template <int D, bool MAPS>
@@ -92,17 +89,9 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const float ymin = fminf(fminf(y00, y01), fminf(y10, y11));
       const float ymax = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
       BpAngle& ba = ang_s[tid];
-      ba.omr = 1.f - r;
       ba.wlo = static_cast<int>(floorf(umin - r)) - 1;
       // candidate rows are floor(y* - r) + 1 .. + 3 (one of slack for rounding)
       ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
-#pragma unroll
-      for (int ku = 0; ku < 3; ++ku)
-#pragma unroll
-        for (int ky = 0; ky < 3; ++ky) {
-          ba.cx[3 * ku + ky] = cs * static_cast<float>(ku) + sn * static_cast<float>(ky);
-          ba.cy[3 * ku + ky] = cs * static_cast<float>(ky) - sn * static_cast<float>(ku);
-        }
     }
     __syncthreads();
     for (int k = tid; k < na * D * kWin; k += 256) {
@@ -145,40 +134,40 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
           const float px = fj - g.z, py = fi - g.w;
           // first of <= 3 candidate bins / rows: floor(p* - r) + 1 for the pixel centre's
           // preimage (u*, y*) = (c px - s py, s px + c py) and r = |c| + |s|
-          const float fu0 = floorf(fmaf(cs, px, fmaf(-sn, py, ba.omr)));
-          const float fy0 = floorf(fmaf(sn, px, fmaf(cs, py, ba.omr)));
+          const float omr = 1.f - (fabsf(cs) + fabsf(sn));
+          const float fu0 = floorf(fmaf(cs, px, fmaf(-sn, py, omr)));
+          const float fy0 = floorf(fmaf(sn, px, fmaf(cs, py, omr)));
           // sample offsets from the pixel: (c u + s y + m02 - j, -s u + c y + m12 - i) at (fu0, fy0)
           const float dx0 = fmaf(cs, fu0, fmaf(sn, fy0, -px));
           const float dy0 = fmaf(-sn, fu0, fmaf(cs, fy0, -py));
+          // candidate (ku, ky) sits at (dx0, dy0) + ku (c, -s) + ky (s, c) from the pixel: the
+          // offsets are stepped by adds (rounding of order 1e-7 at |offset| <= 3, below the
+          // float32 rounding of the samples' own coordinates)
           float K[3];
-          if (ba.rows_inside) {
-#pragma unroll
-            for (int ku = 0; ku < 3; ++ku) {
-              float ksum = 0.f;
-#pragma unroll
-              for (int ky = 0; ky < 3; ++ky) {
-                const float wx = tent(dx0 + ba.cx[3 * ku + ky]);
-                const float wy = tent(dy0 + ba.cy[3 * ku + ky]);
-                ksum = fmaf(wx, wy, ksum);
-              }
-              K[ku] = ksum;
-            }
-          } else {
-            // rows outside [0, N) are not samples of the fp and weigh 0
+          auto kernel_weights = [&](auto masked) {
             const int iy0 = static_cast<int>(fy0);
+            float ax = dx0, ay = dy0;
 #pragma unroll
             for (int ku = 0; ku < 3; ++ku) {
-              float ksum = 0.f;
+              float bx = ax, by = ay, ksum = 0.f;
 #pragma unroll
               for (int ky = 0; ky < 3; ++ky) {
-                const float wx = tent(dx0 + ba.cx[3 * ku + ky]);
-                float wy = tent(dy0 + ba.cy[3 * ku + ky]);
-                wy = (static_cast<unsigned>(iy0 + ky) < static_cast<unsigned>(n)) ? wy : 0.f;
+                const float wx = tent(bx);
+                float wy = tent(by);
+                // rows outside [0, N) are not samples of the fp and weigh 0
+                if (decltype(masked)::value)
+                  wy = (static_cast<unsigned>(iy0 + ky) < static_cast<unsigned>(n)) ? wy : 0.f;
                 ksum = fmaf(wx, wy, ksum);
+                bx += sn;
+                by += cs;
               }
               K[ku] = ksum;
+              ax += cs;
+              ay -= sn;
             }
-          }
+          };
+          if (ba.rows_inside) kernel_weights(std::false_type{});
+          else kernel_weights(std::true_type{});
           const int q0 = static_cast<int>(fu0) - ba.wlo;  // bins outside [0, N) read the zero padding
 #pragma unroll
           for (int d = 0; d < D; ++d) {
