@@ -184,10 +184,13 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
   }
   if (inside) {
     float* o = itm.vol + pix;
-    *o = accumulate ? *o + out : out;
+    if (accumulate > 1) unsafeAtomicAdd(o, out);  // angle halves of a zeroed slice (see launch)
+    else *o = accumulate ? *o + out : out;
   }
 }
 
+// split = 1: one slice, blockIdx.z = half of the angle range; both halves add their
+// sums to the zeroed slice (two addends onto 0 give the same bits in either order)
 template <int D, bool MAPS>
 __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __restrict__ sino,
                                                               const float* __restrict__ maps,
@@ -195,10 +198,21 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
                                                               const float4* __restrict__ geom,
                                                               const float* __restrict__ w, int n, int n_angles,
                                                               int n_det_total, int det0, int angle0, int n_ang,
-                                                              int maps_a0, int accumulate) {
+                                                              int maps_a0, int accumulate, int split) {
   __shared__ float win[kAngBlk][D][kWin];
   __shared__ BpAngle ang_s[kAngBlk];
-  const int b = blockIdx.z;
+  int b = blockIdx.z;
+  if (split) {
+    const int first = (n_ang + 1) / 2;
+    b = 0;
+    if (blockIdx.z == 0) {
+      n_ang = first;
+    } else {
+      angle0 += first;
+      n_ang -= first;
+    }
+    accumulate = 2;
+  }
   const size_t nn = static_cast<size_t>(n) * n;
   BpItem itm;
   itm.sino = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
@@ -243,12 +257,25 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
   }
 }
 
+__global__ void zero_kernel(float* __restrict__ v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0.f;
+}
+
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch, int det0,
             int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
-  dim3 grid((p.n + kTileW - 1) / kTileW, (p.n + kTileH - 1) / kTileH, batch);
+  const int tx = (p.n + kTileW - 1) / kTileW, ty = (p.n + kTileH - 1) / kTileH;
+  // one slice fills only n^2 / 64 waves (4 per SIMD at 512^2): with a fresh output and every
+  // detector in this pass, the angle range is split over twice the workgroups
+  const int split = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk) ? 1 : 0;
+  if (split) {
+    const int64_t nn = static_cast<int64_t>(p.n) * p.n;
+    hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, st, vol, nn);
+  }
+  dim3 grid(tx, ty, split ? 2 : batch);
   hipLaunchKernelGGL((bp_adjoint_tile_kernel<D, MAPS>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate);
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split);
 }
 
 template <int D, bool MAPS>

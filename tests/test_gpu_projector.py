@@ -129,9 +129,12 @@ def test_xrf_bp_symmetric_parity(pkg):
     assert rel_max(ref, got) < TOL
 
 
-def test_adjoint_dot_test_and_oracle(pkg):
-    n, a = 48, 40
-    rng = np.random.default_rng(303)
+@pytest.mark.parametrize("n,a", [(48, 40), (70, 97)])
+def test_adjoint_dot_test_and_oracle(pkg, n, a):
+    """Dot test and the scatter oracle. With 64 or more angles a single slice runs its angle
+    range in two halves added into the zeroed output (97 angles: halves of 49 and 48, each with
+    a partial block of 32, on a ragged 70 x 70 slice); that sum must be reproducible bit for bit."""
+    rng = np.random.default_rng(303 + a)
     prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, bp_mode="adjoint")
     x = rng.random((n, n)).astype(np.float32)
     y = rng.random((2, a, n)).astype(np.float32)
@@ -143,6 +146,7 @@ def test_adjoint_dot_test_and_oracle(pkg):
     assert abs(lhs - rhs) / abs(lhs) < TOL
     ref = oxrf.bp_adjoint(y, ang, n, maps, w_det=w_det, w_ang=w_ang)
     assert rel_max(ref, aty) < TOL
+    assert np.array_equal(prj.bp(y), aty.astype(np.float32))
 
 
 def test_adjoint_ragged_three_detectors(pkg):
