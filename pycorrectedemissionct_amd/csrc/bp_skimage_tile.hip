@@ -130,10 +130,13 @@ __device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ 
   }
   if (inside) {
     float* o = out + pix;
-    *o = accumulate ? *o + acc : acc;
+    if (accumulate > 1) unsafeAtomicAdd(o, acc);  // angle halves of a zeroed slice (see launch_tiles)
+    else *o = accumulate ? *o + acc : acc;
   }
 }
 
+// split = 1: one slice, blockIdx.z = half of the angle range; both halves add their
+// sums to the zeroed slice (two addends onto 0 give the same bits in either order)
 template <int SYM, int D>
 __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __restrict__ sino,
                                                               const float* __restrict__ maps,
@@ -141,10 +144,21 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __res
                                                               const float4* __restrict__ geom,
                                                               const float* __restrict__ w, int n, int n_angles,
                                                               int n_det, int det0, int angle0, int n_ang,
-                                                              int maps_a0, int accumulate) {
+                                                              int maps_a0, int accumulate, int split) {
   __shared__ float win[kSkAngBlk][D][kSkWin];
   __shared__ int wlo_s[kSkAngBlk];
-  const int b = blockIdx.z;
+  int b = blockIdx.z;
+  if (split) {
+    const int first = (n_ang + 1) / 2;
+    b = 0;
+    if (blockIdx.z == 0) {
+      n_ang = first;
+    } else {
+      angle0 += first;
+      n_ang -= first;
+    }
+    accumulate = 2;
+  }
   const float* sb = sino + static_cast<size_t>(b) * n_det * n_angles * n;
   const float* mb = (SYM && maps) ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
   bp_skimage_tile_block<SYM, D>(sb, mb, vol + static_cast<size_t>(b) * n * n, geom, w, n, n_angles, n_det, det0,
@@ -179,18 +193,34 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_batch_kernel(const BatchS
   }
 }
 
+__global__ void zero_kernel(float* __restrict__ v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0.f;
+}
+
 template <int SYM, int D>
 void launch_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
                   int det0, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
-  dim3 grid((p.n + kSkTile - 1) / kSkTile, (p.n + kSkTile - 1) / kSkTile, batch);
+  // one slice fills only n^2 / 64 waves (4 per SIMD at 512^2): with a fresh output and every
+  // detector in this pass, the non-symmetric form (VALU-bound) splits the angle range over
+  // twice the workgroups; the symmetric form is bound by its map loads and gains nothing
+  const int split =
+      (!SYM && batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kSkAngBlk) ? 1 : 0;
+  if (split) {
+    const int64_t nn = static_cast<int64_t>(p.n) * p.n;
+    hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, st, vol, nn);
+  }
+  dim3 grid((p.n + kSkTile - 1) / kSkTile, (p.n + kSkTile - 1) / kSkTile, split ? 2 : batch);
   hipLaunchKernelGGL((bp_skimage_tile_kernel<SYM, D>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate);
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split);
 }
 
 }  // namespace
 
 // Detectors in passes of two, the passes after the first accumulating; with
-// one or two detectors the sums run in the same order as bp_skimage_kernel.
+// one or two detectors the sums run in the same order as bp_skimage_kernel,
+// except that a single non-symmetric slice with 64 or more angles adds its two
+// angle halves.
 void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
                              int batch, int sym, int angle0, int n_ang, int maps_a0, int accumulate,
                              hipStream_t st) {

@@ -55,7 +55,7 @@ def test_fp_uncorrected_parity(pkg, n, a):
     assert e32 < TOL
 
 
-@pytest.mark.parametrize("n,a", [(64, 40), (33, 24)])
+@pytest.mark.parametrize("n,a", [(64, 40), (33, 24), (70, 97)])
 def test_bp_skimage_parity(pkg, n, a):
     rng = np.random.default_rng(200 + n)
     ang = _angles(a)
@@ -118,9 +118,11 @@ def test_xrf_bp_nonsymmetric_parity(pkg):
     assert rel_max(ref, got) < TOL
 
 
-def test_xrf_bp_symmetric_parity(pkg):
-    n, a = 40, 30
-    rng = np.random.default_rng(302)
+@pytest.mark.parametrize("n,a", [(40, 30), (48, 80)])
+def test_xrf_bp_symmetric_parity(pkg, n, a):
+    """Symmetric bp (maps and weights) against the oracle; 80 angles: two full and one partial
+    block of 32."""
+    rng = np.random.default_rng(302 + a)
     prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, symmetric=True)
     sino = rng.random((2, a, n)).astype(np.float32)
     got = prj.bp(sino)
