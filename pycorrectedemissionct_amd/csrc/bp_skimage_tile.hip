@@ -23,18 +23,24 @@ constexpr int kSkTile = 16;
 constexpr int kSkAngBlk = 32;
 constexpr int kSkGrp = 8;
 constexpr int kSkWin = 28;  // >= 15 (|c| + |s|) + 4, with one bin of slack each side
+// symmetric single slices: 32 x 8 tiles, so that each wave's map reads are whole 128-byte
+// rows (16-wide rows read half lines, which the caches then fetch twice); the window spans
+// 31 |c| + 7 |s| + 4 <= 36 bins (indices 1..34 checked on every tile and angle of N = 33,
+// 64, 70, 130 and 512 at 0.5 degree steps)
+constexpr int kSymTileW = 32, kSymTileH = 8, kSymWin = 36;
 
-// One 16 x 16 tile (tx, ty) of one slice: sb = the slice's sinogram [n_det][n_angles][n],
-// mb = its maps or NULL, out = the slice's image.
-template <int SYM, int D>
+// One TW x TH tile (tx, ty) of one slice: sb = the slice's sinogram [n_det][n_angles][n],
+// mb = its maps or NULL, out = the slice's image; W window bins per angle and detector.
+template <int SYM, int D, int TW, int TH, int W>
 __device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ sb, const float* __restrict__ mb,
                                                       float* __restrict__ out, const float4* __restrict__ geom,
                                                       const float* __restrict__ w, int n, int n_angles, int n_det,
                                                       int det0, int angle0, int n_ang, int maps_a0, int accumulate,
-                                                      int tx, int ty, float (*win)[D][kSkWin], int* wlo_s) {
+                                                      int tx, int ty, float (*win)[D][W], int* wlo_s) {
+  static_assert(TW * TH == 256, "one pixel per thread");
   const int tid = threadIdx.x;
-  const int j = tx * kSkTile + (tid & (kSkTile - 1));
-  const int i = ty * kSkTile + (tid / kSkTile);
+  const int j = tx * TW + (tid % TW);
+  const int i = ty * TH + (tid / TW);
   const int k = n / 2;
   const size_t nn = static_cast<size_t>(n) * n;
   const bool inside = i < n && j < n;
@@ -44,8 +50,8 @@ __device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ 
   const float tlo = static_cast<float>(-k);
   const float thi = static_cast<float>(n - 1 - k);
   // tile corners, centred
-  const float jc0 = static_cast<float>(tx * kSkTile - k), jc1 = jc0 + (kSkTile - 1);
-  const float ic0 = static_cast<float>(ty * kSkTile - k), ic1 = ic0 + (kSkTile - 1);
+  const float jc0 = static_cast<float>(tx * TW - k), jc1 = jc0 + (TW - 1);
+  const float ic0 = static_cast<float>(ty * TH - k), ic1 = ic0 + (TH - 1);
 
   float acc = 0.f;
   for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kSkAngBlk) {
@@ -59,11 +65,11 @@ __device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ 
       wlo_s[tid] = static_cast<int>(floorf(tmin)) + k - 1;
     }
     __syncthreads();
-    for (int q = tid; q < na * D * kSkWin; q += 256) {
-      const int al = q / (D * kSkWin);
-      const int rem = q - al * (D * kSkWin);
-      const int d = rem / kSkWin;
-      const int e = rem - d * kSkWin;
+    for (int q = tid; q < na * D * W; q += 256) {
+      const int al = q / (D * W);
+      const int rem = q - al * (D * W);
+      const int d = rem / W;
+      const int e = rem - d * W;
       const int bin = wlo_s[al] + e;
       win[al][d][e] =
           (bin >= 0 && bin < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + bin] : 0.f;
@@ -137,7 +143,7 @@ __device__ __forceinline__ void bp_skimage_tile_block(const float* __restrict__ 
 
 // split = 1: one slice, blockIdx.z = half of the angle range; both halves add their
 // sums to the zeroed slice (two addends onto 0 give the same bits in either order)
-template <int SYM, int D>
+template <int SYM, int D, int TW, int TH, int W>
 __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __restrict__ sino,
                                                               const float* __restrict__ maps,
                                                               int64_t maps_slice_stride, float* __restrict__ vol,
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __res
                                                               const float* __restrict__ w, int n, int n_angles,
                                                               int n_det, int det0, int angle0, int n_ang,
                                                               int maps_a0, int accumulate, int split) {
-  __shared__ float win[kSkAngBlk][D][kSkWin];
+  __shared__ float win[kSkAngBlk][D][W];
   __shared__ int wlo_s[kSkAngBlk];
   int b = blockIdx.z;
   if (split) {
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_kernel(const float* __res
   }
   const float* sb = sino + static_cast<size_t>(b) * n_det * n_angles * n;
   const float* mb = (SYM && maps) ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
-  bp_skimage_tile_block<SYM, D>(sb, mb, vol + static_cast<size_t>(b) * n * n, geom, w, n, n_angles, n_det, det0,
+  bp_skimage_tile_block<SYM, D, TW, TH, W>(sb, mb, vol + static_cast<size_t>(b) * n * n, geom, w, n, n_angles, n_det, det0,
                                 angle0, n_ang, maps_a0, accumulate, static_cast<int>(blockIdx.x),
                                 static_cast<int>(blockIdx.y), win, wlo_s);
 }
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(256) void bp_skimage_tile_batch_kernel(const BatchS
     const int local = static_cast<int>(it - item_start[s]);
     const int ntx = (sl.n + kSkTile - 1) / kSkTile;
     const int ty = local / ntx, tx = local - ty * ntx;
-    bp_skimage_tile_block<SYM, D>(sino + sl.sino_off, (SYM && maps) ? maps + sl.maps_off : nullptr,
+    bp_skimage_tile_block<SYM, D, kSkTile, kSkTile, kSkWin>(sino + sl.sino_off, (SYM && maps) ? maps + sl.maps_off : nullptr,
                                   vol + sl.vol_off, geom + static_cast<size_t>(sl.size_idx) * n_angles, w, sl.n,
                                   n_angles, n_det, det0, 0, n_angles, 0, accumulate, tx, ty, win, wlo_s);
   }
@@ -210,9 +216,10 @@ void launch_tiles(const Plan& p, const float* sino, const float* maps, int64_t s
     const int64_t nn = static_cast<int64_t>(p.n) * p.n;
     hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, st, vol, nn);
   }
-  dim3 grid((p.n + kSkTile - 1) / kSkTile, (p.n + kSkTile - 1) / kSkTile, split ? 2 : batch);
-  hipLaunchKernelGGL((bp_skimage_tile_kernel<SYM, D>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split);
+  constexpr int TW = SYM ? kSymTileW : kSkTile, TH = SYM ? kSymTileH : kSkTile, W = SYM ? kSymWin : kSkWin;
+  dim3 grid((p.n + TW - 1) / TW, (p.n + TH - 1) / TH, split ? 2 : batch);
+  hipLaunchKernelGGL((bp_skimage_tile_kernel<SYM, D, TW, TH, W>), grid, dim3(256), 0, st, sino, maps, stride, vol,
+                     p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split);
 }
 
 }  // namespace
