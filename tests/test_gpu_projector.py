@@ -313,6 +313,37 @@ def test_otf_maps_match_precomputed(pkg, n):
         pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, maps_mode="otf")
 
 
+def test_band_fp_paths(pkg):
+    """fp_kernel 6 (image-space bands) on the paths other than the whole two-detector sinogram:
+    no maps (one detector), three detectors (a two- and a one-detector pass), and the
+    on-the-fly angle blocks, which must equal the precomputed-maps run bit for bit (the same
+    kernel on the same angles)."""
+    import torch
+
+    n, a = 100, 45
+    rng = np.random.default_rng(620)
+    ang = _angles(a)
+    vol = rng.random((n, n)).astype(np.float32)
+    pu = pkg.ProjectorUncorrected([n, n], ang)
+    pu.projector_backend.plan().tune("fp_kernel", 6)  # the plan its fp uses (cached per detector set)
+    assert rel_max(oxrf.fp(vol, ang)[0], pu.fp(vol)) < TOL
+    det = [0.5, 1.5, 2.5]
+    maps = rng.uniform(0.2, 1.0, (a, 3, n, n)).astype(np.float32)
+    p3 = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=det)
+    p3._plan().tune("fp_kernel", 6)
+    assert rel_max(oxrf.fp(vol, ang, maps), p3.fp(vol)) < TOL
+    mu_in = (rng.random((n, n)) * 0.02).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.03).astype(np.float32)
+    kw = dict(att_in=mu_in, att_out=mu_out, angles_detectors_rad=[np.pi / 2, -np.pi / 2])
+    ref = pkg.ProjectorAttenuationXRF([n, n], ang, **kw)
+    otf = pkg.ProjectorAttenuationXRF([n, n], ang, maps_mode="otf", **kw)
+    otf._plan().tune("otf_budget_mib", 16)
+    for prj in (ref, otf):
+        prj._plan().tune("fp_kernel", 6)
+    x = torch.as_tensor(vol, device="cuda")
+    assert torch.equal(otf.fp(x), ref.fp(x))
+
+
 def test_detector_psf(pkg):
     """A 1-D detector PSF is convolved along u after the fp (uncorrected and XRF, whole sinogram
     and per angle) with edge-replicated borders; the bp ignores it, as in the reference."""
