@@ -32,8 +32,6 @@ namespace chip {
 namespace {
 __device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
 __device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
-__device__ __forceinline__ double dmul_rn(double a, double b) { return a * b; }
-__device__ __forceinline__ double dadd_rn(double a, double b) { return a + b; }
 }  // namespace
 }  // namespace chip
 
