@@ -213,3 +213,27 @@ def test_fbp_parity(pkg):
     x2, _ = pkg.FBP()(prj, s2, lower_limit=0.0)
     ref2 = np.clip(ofl.fbp(lambda y: oxrf.bp_skimage(y.astype(np.float32), rad, n), s2, a, "ramp"), 0.0, None)
     assert rel_max(ref2, x2) < 1e-5
+
+
+def test_sirt_at_the_reference_cpu_configuration(pkg):
+    """The reference's CPU-runnable case: a 128^2 slice of 10 ellipses, attenuation 0.02 on the
+    support + 0.005 in and 0.05 out, 256 angles over 2 pi, one detector at pi/2, Poisson counts
+    at peak 1e4. The HIP map builder against the oracle's, then 20 SIRT iterations against the
+    oracle's SIRT on the oracle projector."""
+    from oracle import attenuation as oatt
+
+    n, a = 128, 256
+    rng = np.random.default_rng(20260101)
+    f = pkg.testing.phantom_ellipses(n, 10, rng, semi_axes=(5, 40))
+    mu_in, mu_out = pkg.testing.xrf_attenuation(f)
+    ang = np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, angles_detectors_rad=np.pi / 2)
+    maps = oatt.compute_maps(mu_in, mu_out, ang, np.array([np.pi / 2]))
+    em = rel_max(maps, prj.att_vol_angles.cpu().numpy().reshape(maps.shape))
+    print(f"maps 128^2 x 256: rel_max {em:.2e}")
+    assert em < 1e-5
+    b = pkg.testing.poisson_data(oxrf.fp(f, ang, maps)[0], 1e4, rng)
+    x = pkg.SIRT()(prj, b, iterations=20)[0]
+    ref = osol.sirt(lambda v: oxrf.fp(v, ang, maps)[0], lambda y: oxrf.bp_skimage(y[None], ang, n), b, (n, n), 20)
+    print(f"SIRT-20 128^2 x 256: rel_max {rel_max(ref, x):.2e} rel_l2 {rel_l2(ref, x):.2e}")
+    assert rel_max(ref, x) < TOL
