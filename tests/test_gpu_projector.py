@@ -502,3 +502,35 @@ def test_sirt_pdhg_on_volume_equal_slices(pkg):
     y2 = np.stack([pkg.PDHG(verbose=False, regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(
         p2[i], np.ascontiguousarray(b[i]), 20)[0] for i in range(z)])
     assert rel_max(y2, y3) < 1e-5
+
+
+def test_full_size_fp_and_bp_against_c_oracle(pkg):
+    """The headline shape, 512^2 x 720 angles x 2 detectors with random maps and weights: the
+    default fp and the reference-default (non-symmetric skimage) bp, which splits the angle
+    range of a lone slice, against the oracle's C restatement."""
+    from oracle import build_c
+    from tests.test_oracle_c import c_tables
+
+    lib = build_c.load()
+    n, a, d = 512, 720, 2
+    rng = np.random.default_rng(640)
+    ang = _angles(a)
+    maps = rng.uniform(0.2, 1.0, (a, d, n, n)).astype(np.float32)
+    w_det = np.array([1.0, 0.6])
+    w_ang = rng.uniform(0.5, 1.5, (a, d))
+    prj = pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, angles_detectors_rad=[np.pi / 2, -np.pi / 2],
+                                      weights_detectors=w_det, weights_angles=w_ang)
+    vol = rng.random((n, n)).astype(np.float32)
+    geom, trig = c_tables(ang, n)
+    w = np.ascontiguousarray((w_det[None, :] * w_ang).astype(np.float32))
+    want = np.zeros((d, a, n), np.float32)
+    lib.oracle_fp(vol.ctypes.data, maps.ctypes.data, geom.ctypes.data, w.ctypes.data, n, a, d, want.ctypes.data)
+    got = prj.fp(vol)
+    print(f"fp 512 x 720 x 2 vs C oracle: rel_max {rel_max(want, got):.2e}")
+    assert rel_max(want, got) < TOL
+    y = rng.random((d, a, n)).astype(np.float32)
+    want_b = np.zeros((n, n), np.float32)
+    lib.oracle_bp_skimage(y.ctypes.data, trig.ctypes.data, n, a, d, want_b.ctypes.data)
+    got_b = prj.bp(y)
+    print(f"bp (skimage) 512 x 720 x 2 vs C oracle: rel_max {rel_max(want_b, got_b):.2e}")
+    assert rel_max(want_b, got_b) < TOL
