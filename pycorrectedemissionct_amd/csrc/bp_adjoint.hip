@@ -46,10 +46,13 @@ struct BpItem {
 // value, which the hardware applies as an output modifier of the subtraction
 __device__ __forceinline__ float tent(float t) { return __builtin_amdgcn_fmed3f(1.f - fabsf(t), 0.f, 1.f); }
 
-// Per angle of a block: first window bin and whether every candidate row of
-// the tile lies in [0, N).
-struct BpAngle {
-  int wlo;
+// Per angle of a block (one 16-byte LDS read per angle): the constant terms of the
+// first-candidate arguments, the flat window index of bin 0 and whether every
+// candidate row of the tile lies in [0, N).
+struct __align__(16) BpAngle {
+  float au;  // 1 - (|c| + |s|) - c m02 + s m12
+  float ay;  // 1 - (|c| + |s|) - s m02 - c m12
+  int wbase;  // byte offset into the window block of bin 0 (may be negative)
   int rows_inside;
 };
 
@@ -89,7 +92,11 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const float ymin = fminf(fminf(y00, y01), fminf(y10, y11));
       const float ymax = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
       BpAngle& ba = ang_s[tid];
-      ba.wlo = static_cast<int>(floorf(umin - r)) - 1;
+      const int wlo = static_cast<int>(floorf(umin - r)) - 1;  // first window bin
+      ba.wbase = static_cast<int>(sizeof(float)) * (tid * (D * kWin) - wlo);
+      const float omr = 1.f - r;
+      ba.au = omr - cs * g.z + sn * g.w;
+      ba.ay = omr - sn * g.z - cs * g.w;
       // candidate rows are floor(y* - r) + 1 .. + 3 (one of slack for rounding)
       ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
     }
@@ -99,7 +106,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const int rem = k - al * (D * kWin);
       const int d = rem / kWin;
       const int q = rem - d * kWin;
-      const int u = ang_s[al].wlo + q;
+      const int u = al * (D * kWin) - ang_s[al].wbase / static_cast<int>(sizeof(float)) + q;
       // the line times its weight w[a, d], applied once per bin instead of once per pixel
       win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
                                               w[static_cast<size_t>(a0 + al) * n_det_total + det0 + d]
@@ -110,7 +117,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       // map values of the next group of kGrp angles are loaded while the current group is
       // computed: with one pixel per thread the per-angle loads would otherwise expose the
       // full memory latency once per angle
-      float mcur[kGrp][D], mnxt[kGrp][D];
+      float ma[kGrp][D], mb2[kGrp][D];
       auto load_maps = [&](float (*m)[D], int g0) {
 #pragma unroll
         for (int k = 0; k < kGrp; ++k)
@@ -120,26 +127,23 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
                           ? mb[(static_cast<size_t>(a0 + g0 + k - maps_a0) * n_det_total + det0 + d) * nn + pix]
                           : 1.f;
       };
-      load_maps(mcur, 0);
-      for (int g0 = 0; g0 < na; g0 += kGrp) {
-        if (g0 + kGrp < na) load_maps(mnxt, g0 + kGrp);
+      const char* wbytes = reinterpret_cast<const char*>(&win[0][0][0]);
+      auto angle_group = [&](const float (*m)[D], int g0) {
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
           const int al = g0 + k;
           if (al >= na) break;
-          const int a = a0 + al;
-          const float4 g = geom[a];
+          const float4 g = geom[a0 + al];
           const float cs = g.x, sn = g.y;
-          const BpAngle& ba = ang_s[al];
-          const float px = fj - g.z, py = fi - g.w;
+          const BpAngle ba = ang_s[al];
           // first of <= 3 candidate bins / rows: floor(p* - r) + 1 for the pixel centre's
-          // preimage (u*, y*) = (c px - s py, s px + c py) and r = |c| + |s|
-          const float omr = 1.f - (fabsf(cs) + fabsf(sn));
-          const float fu0 = floorf(fmaf(cs, px, fmaf(-sn, py, omr)));
-          const float fy0 = floorf(fmaf(sn, px, fmaf(cs, py, omr)));
+          // preimage (u*, y*) = (c (j - m02) - s (i - m12), s (j - m02) + c (i - m12)) and
+          // r = |c| + |s|; the angle's constant terms are folded into (au, ay)
+          const float fu0 = floorf(fmaf(cs, fj, fmaf(-sn, fi, ba.au)));
+          const float fy0 = floorf(fmaf(sn, fj, fmaf(cs, fi, ba.ay)));
           // sample offsets from the pixel: (c u + s y + m02 - j, -s u + c y + m12 - i) at (fu0, fy0)
-          const float dx0 = fmaf(cs, fu0, fmaf(sn, fy0, -px));
-          const float dy0 = fmaf(-sn, fu0, fmaf(cs, fy0, -py));
+          const float dx0 = fmaf(cs, fu0, fmaf(sn, fy0, g.z - fj));
+          const float dy0 = fmaf(-sn, fu0, fmaf(cs, fy0, g.w - fi));
           // candidate (ku, ky) sits at (dx0, dy0) + ku (c, -s) + ky (s, c) from the pixel: the
           // offsets are stepped by adds (rounding of order 1e-7 at |offset| <= 3, below the
           // float32 rounding of the samples' own coordinates)
@@ -168,17 +172,24 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
           };
           if (ba.rows_inside) kernel_weights(std::false_type{});
           else kernel_weights(std::true_type{});
-          const int q0 = static_cast<int>(fu0) - ba.wlo;  // bins outside [0, N) read the zero padding
+          // window of this angle, bin fu0 (bins outside [0, N) read the zero padding)
+          const float* wq =
+              reinterpret_cast<const float*>(wbytes + (ba.wbase + static_cast<int>(sizeof(float)) * static_cast<int>(fu0)));
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            const float gd = fmaf(K[0], win[al][d][q0], fmaf(K[1], win[al][d][q0 + 1], K[2] * win[al][d][q0 + 2]));
-            out = MAPS ? fmaf(mcur[k][d], gd, out) : out + gd;
+            const float gd = fmaf(K[0], wq[d * kWin], fmaf(K[1], wq[d * kWin + 1], K[2] * wq[d * kWin + 2]));
+            out = MAPS ? fmaf(m[k][d], gd, out) : out + gd;
           }
         }
+      };
+      load_maps(ma, 0);
+      for (int g0 = 0; g0 < na; g0 += kGrp) {
+        if (g0 + kGrp < na) load_maps(mb2, g0 + kGrp);
+        angle_group(ma, g0);
 #pragma unroll
         for (int k = 0; k < kGrp; ++k)
 #pragma unroll
-          for (int d = 0; d < D; ++d) mcur[k][d] = mnxt[k][d];
+          for (int d = 0; d < D; ++d) ma[k][d] = mb2[k][d];
       }
     }
   }
