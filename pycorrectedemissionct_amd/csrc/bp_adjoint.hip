@@ -200,8 +200,10 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
   }
 }
 
-// split = 1: one slice, blockIdx.z = half of the angle range; both halves add their
-// sums to the zeroed slice (two addends onto 0 give the same bits in either order)
+// part != NULL: one slice, blockIdx.z = one of split parts of the angle range, each
+// stored to its own plane of part. split = 1 (part NULL): one slice, blockIdx.z = half of
+// the angle range; both halves add their sums to the zeroed slice (two addends onto 0
+// give the same bits in either order). Otherwise blockIdx.z = slice of a batch.
 template <int D, bool MAPS>
 __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __restrict__ sino,
                                                               const float* __restrict__ maps,
@@ -209,11 +211,21 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
                                                               const float4* __restrict__ geom,
                                                               const float* __restrict__ w, int n, int n_angles,
                                                               int n_det_total, int det0, int angle0, int n_ang,
-                                                              int maps_a0, int accumulate, int split) {
+                                                              int maps_a0, int accumulate, int split,
+                                                              float* __restrict__ part) {
   __shared__ float win[kAngBlk][D][kWin];
   __shared__ BpAngle ang_s[kAngBlk];
   int b = blockIdx.z;
-  if (split) {
+  if (part) {
+    // one slice, blockIdx.z = part s of split parts of the angle range; each part stores its
+    // own sum in part[s] (plain stores: the parts are added in order by a later pass)
+    const int s = blockIdx.z, lo = n_ang * s / split, hi = n_ang * (s + 1) / split;
+    angle0 += lo;
+    n_ang = hi - lo;
+    b = 0;
+    vol = part + static_cast<size_t>(s) * n * n;
+    accumulate = 0;
+  } else if (split) {
     const int first = (n_ang + 1) / 2;
     b = 0;
     if (blockIdx.z == 0) {
@@ -268,6 +280,40 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
   }
 }
 
+// out = the parts' sums added in part order (the same bits on every run)
+__global__ void sum_parts_kernel(float* __restrict__ out, const float* __restrict__ part, int n_parts, int64_t nn) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  float acc = part[i];
+  for (int s = 1; s < n_parts; ++s) acc += part[s * nn + i];
+  out[i] = acc;
+}
+
+// Parts of one slice's angle range (2 .. kBpMaxParts) that make its tiles x parts
+// workgroups the closest to whole waves of what the GPU holds at once (ties: fewer
+// parts); each part keeps at least 16 angles. 0 when the slice needs no split.
+template <int D, bool MAPS>
+int bp_parts(int tiles, int n_ang) {
+  static const int resident = [] {
+    int per_cu = 0, dev = 0, n_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_adjoint_tile_kernel<D, MAPS>, 256, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    return per_cu * n_cu;
+  }();
+  if (resident <= 0 || tiles >= 4 * resident) return 0;
+  int best = 0;
+  double best_eff = 0.0;
+  for (int s = 2; s <= kBpMaxParts && 16 * s <= n_ang; ++s) {
+    const int64_t wg = static_cast<int64_t>(tiles) * s;
+    const int64_t waves = (wg + resident - 1) / resident;
+    const double eff = static_cast<double>(wg) / static_cast<double>(waves * resident);
+    if (eff > best_eff + 1e-9) best_eff = eff, best = s;
+  }
+  return best;
+}
+
 __global__ void zero_kernel(float* __restrict__ v, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) v[i] = 0.f;
@@ -277,16 +323,33 @@ template <int D, bool MAPS>
 void launch(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch, int det0,
             int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
   const int tx = (p.n + kTileW - 1) / kTileW, ty = (p.n + kTileH - 1) / kTileH;
-  // one slice fills only n^2 / 64 waves (4 per SIMD at 512^2): with a fresh output and every
-  // detector in this pass, the angle range is split over twice the workgroups
+  const int64_t nn = static_cast<int64_t>(p.n) * p.n;
+  // one slice fills only n^2 / 64 waves (4 per SIMD at 512^2); two halves of its angle range
+  // (2048 workgroups at 512^2) leave a second wave of 256 on a GPU that holds 1792 at once.
+  // With a fresh output and every detector in this pass, the range is split into the parts
+  // that come closest to whole waves, each summed into the plan's scratch, then added in order
+  const int parts = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk && p.d_bp_part)
+                        ? bp_parts<D, MAPS>(tx * ty, n_ang)
+                        : 0;
+  if (parts) {
+    // each part's sum goes to the plan's scratch; one pass adds them in part order
+    dim3 grid(tx, ty, parts);
+    hipLaunchKernelGGL((bp_adjoint_tile_kernel<D, MAPS>), grid, dim3(256), 0, st, sino, maps, stride, vol,
+                       p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, parts,
+                       p.d_bp_part);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, st, vol,
+                       static_cast<const float*>(p.d_bp_part), parts, nn);
+    return;
+  }
+  // slices too large for the scratch: two halves added atomically into the zeroed output
   const int split = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk) ? 1 : 0;
   if (split) {
-    const int64_t nn = static_cast<int64_t>(p.n) * p.n;
     hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, st, vol, nn);
   }
   dim3 grid(tx, ty, split ? 2 : batch);
   hipLaunchKernelGGL((bp_adjoint_tile_kernel<D, MAPS>), grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom,
-                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split);
+                     p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0, accumulate, split,
+                     static_cast<float*>(nullptr));
 }
 
 template <int D, bool MAPS>

@@ -46,6 +46,9 @@ struct Affine {
 };
 
 // Host-side geometry plan. Device tables are allocated at plan creation.
+constexpr int kBpMaxParts = 8;
+constexpr int kBpPartsMaxN = 1024;
+
 struct Plan {
   int n = 0;          // slice side N (square), also detector pixels
   int n_angles = 0;   // A
@@ -71,6 +74,9 @@ struct Plan {
   // maps of one angle block for the on-the-fly projectors
   float* d_mblock = nullptr;
   size_t mblock_bytes = 0;
+  // per-part sums of a single-slice adjoint bp whose angle range is split over
+  // more workgroups than the tiles alone give ([kBpMaxParts][N][N], N <= 1024)
+  float* d_bp_part = nullptr;
   // bound on d_scratch + d_mblock (chip_plan_tune "otf_budget_mib")
   size_t otf_budget = size_t(1) << 30;
   int device = 0;

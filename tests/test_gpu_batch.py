@@ -4,7 +4,7 @@ The persistent batched kernels compute every work item exactly as the
 single-slice kernels do, so per slice the batched maps, fp and bp are
 bit-identical to the single-slice projector (itself checked against the
 oracle to 1e-5; these cases use fewer than 64 angles, below which a lone
-slice's bp does not split its angle range into two added halves), and the
+slice's bp does not split its angle range into parts summed in order), and the
 batched SIRT / PDHG match per-slice solver runs.
 HIP-graph replays are checked against eager iterations, bit for bit, over
 several replays with eager work in between.

@@ -133,9 +133,9 @@ def test_xrf_bp_symmetric_parity(pkg, n, a):
 
 @pytest.mark.parametrize("n,a", [(48, 40), (70, 97)])
 def test_adjoint_dot_test_and_oracle(pkg, n, a):
-    """Dot test and the scatter oracle. With 64 or more angles a single slice runs its angle
-    range in two halves added into the zeroed output (97 angles: halves of 49 and 48, each with
-    a partial block of 32, on a ragged 70 x 70 slice); that sum must be reproducible bit for bit."""
+    """Dot test and the scatter oracle. With 64 or more angles a single slice splits its angle
+    range into parts whose sums are added in order (97 angles on a ragged 70 x 70 slice: 6 parts
+    of 16 or 17 angles, each a partial block of 32); that sum must be reproducible bit for bit."""
     rng = np.random.default_rng(303 + a)
     prj, ang, maps, w_det, w_ang = _xrf_case(pkg, n, a, rng, bp_mode="adjoint")
     x = rng.random((n, n)).astype(np.float32)
