@@ -117,6 +117,11 @@ int chip_fp(const chip_plan* plan, const float* vol, const float* maps, int64_t 
  * Back-projection vol[b] = BP_mode(sino[b]); see enum chip_bp_mode. Replaces
  * ProjectorAttenuationXRF.bp (projectors.py:598-622) and
  * ProjectorBackendSKimage.bp (_projector_backends.py:335-359).
+ * One slice with 64 or more angles is split over its angle range to fill the
+ * GPU: adjoint mode with N <= 1024 sums 2-8 parts in the plan's scratch and
+ * adds them in order; otherwise two halves are added atomically into the
+ * zeroed vol. Results are reproducible either way; vol must not alias sino,
+ * and calls on one plan must not overlap (they share its scratch).
  */
 int chip_bp(const chip_plan* plan, const float* sino, const float* maps, int64_t maps_slice_stride,
             float* vol, int batch, int mode, void* stream);
