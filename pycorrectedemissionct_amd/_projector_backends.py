@@ -14,6 +14,7 @@
 from __future__ import annotations
 
 import ctypes
+import threading
 from abc import ABC, abstractmethod
 
 import numpy as np
@@ -198,6 +199,9 @@ class ProjectorBackendHIP(ProjectorBackend):
         self.bp_mode = bp_mode
         self.is_ready = True
         self._plans: dict = {}
+        # corrct's projectors call fp / bp per angle from a thread pool
+        # (projectors.py:585-589, 612-616): plans are looked up and created under a lock
+        self._plans_lock = threading.Lock()
 
     def initialize_geometry(self, vol_geom: VolumeGeometry, angles_rot_rad, rot_axis_shift_pix=None, prj_geom=None,
                             create_single_projs: bool = False):
@@ -222,16 +226,18 @@ class ProjectorBackendHIP(ProjectorBackend):
         det = np.asarray(det_angles_rad, dtype=np.float64).reshape(-1)
         key = (det.tobytes(), None if w_det is None else np.asarray(w_det, np.float64).tobytes(),
                None if w_ang is None else np.asarray(w_ang, np.float64).tobytes())
-        p = self._plans.get(key)
-        if p is None:
-            p = HipPlan(self.n, self.angles_w_rad, det, w_det, w_ang)
-            self._plans[key] = p
+        with self._plans_lock:
+            p = self._plans.get(key)
+            if p is None:
+                p = HipPlan(self.n, self.angles_w_rad, det, w_det, w_ang)
+                self._plans[key] = p
         return p
 
     def dispose(self) -> None:
-        for p in self._plans.values():
+        with self._plans_lock:
+            plans, self._plans = self._plans, {}
+        for p in plans.values():
             p.close()
-        self._plans = {}
         super().dispose()
 
     def make_ready(self) -> None:

@@ -311,6 +311,8 @@ extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_i
   int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
   if (rc != CHIP_OK) return rc;
   const size_t nn = static_cast<size_t>(p.n) * p.n;
+  // only the first chunk of each slice writes a fresh output and may split into parts
+  BpPartGuard guard(p, 1, mode, chunk, st);
   for (int b = 0; b < batch; ++b) {
     const float* mi = mu_in ? mu_in + b * nn : nullptr;
     const float* mo = mu_out ? mu_out + b * nn : nullptr;
@@ -320,7 +322,7 @@ extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_i
       const int na = std::min(chunk, p.n_angles - a0);
       rc = build_maps_chunk(p, mi, mo, a0, na, p.d_mblock, a0, st);
       if (rc != CHIP_OK) return rc;
-      launch_bp_range(p, s, p.d_mblock, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st);
+      launch_bp_range(p, s, p.d_mblock, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st, guard.scratch());
       CHIP_LAUNCH_CHECK();
     }
   }

@@ -289,11 +289,15 @@ __global__ void sum_parts_kernel(float* __restrict__ out, const float* __restric
   out[i] = acc;
 }
 
-// Parts of one slice's angle range (2 .. kBpMaxParts) that make its tiles x parts
+// Parts of one slice's angle range (2 .. max_parts) that make its tiles x parts
 // workgroups the closest to whole waves of what the GPU holds at once (ties: fewer
-// parts); each part keeps at least 16 angles. 0 when the slice needs no split.
+// parts); each part keeps at least 16 angles. 0 when the slice needs no split. The
+// count, and with it the order of the sums, follows the kernel's occupancy on this
+// device unless chip_plan_tune "bp_parts" pins it.
 template <int D, bool MAPS>
-int bp_parts(int tiles, int n_ang) {
+int bp_parts(int tiles, int n_ang, int max_parts, int tune) {
+  if (tune == 0 || max_parts < 2) return 0;
+  if (tune > 0) return (tune <= max_parts && 16 * tune <= n_ang) ? tune : 0;
   static const int resident = [] {
     int per_cu = 0, dev = 0, n_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_adjoint_tile_kernel<D, MAPS>, 256, 0) != hipSuccess ||
@@ -305,7 +309,7 @@ int bp_parts(int tiles, int n_ang) {
   if (resident <= 0 || tiles >= 4 * resident) return 0;
   int best = 0;
   double best_eff = 0.0;
-  for (int s = 2; s <= kBpMaxParts && 16 * s <= n_ang; ++s) {
+  for (int s = 2; s <= max_parts && 16 * s <= n_ang; ++s) {
     const int64_t wg = static_cast<int64_t>(tiles) * s;
     const int64_t waves = (wg + resident - 1) / resident;
     const double eff = static_cast<double>(wg) / static_cast<double>(waves * resident);
@@ -321,15 +325,16 @@ __global__ void zero_kernel(float* __restrict__ v, int64_t n) {
 
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch, int det0,
-            int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+            int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st, const BpScratch& scr) {
   const int tx = (p.n + kTileW - 1) / kTileW, ty = (p.n + kTileH - 1) / kTileH;
   const int64_t nn = static_cast<int64_t>(p.n) * p.n;
   // one slice fills only n^2 / 64 waves (4 per SIMD at 512^2); two halves of its angle range
   // (2048 workgroups at 512^2) leave a second wave of 256 on a GPU that holds 1792 at once.
   // With a fresh output and every detector in this pass, the range is split into the parts
   // that come closest to whole waves, each summed into the plan's scratch, then added in order
-  const int parts = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk && p.d_bp_part)
-                        ? bp_parts<D, MAPS>(tx * ty, n_ang)
+  const int max_parts = scr.part ? static_cast<int>(std::min<int64_t>(kBpMaxParts, scr.cap_elems / nn)) : 0;
+  const int parts = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk)
+                        ? bp_parts<D, MAPS>(tx * ty, n_ang, max_parts, scr.tune)
                         : 0;
   if (parts) {
     // each part's sum goes to the plan's scratch; one pass adds them in part order
@@ -380,18 +385,70 @@ void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float
 }
 
 void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
-                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st,
+                             const BpScratch& scr) {
   // detectors in passes of two; later passes add to the first
   for (int det0 = 0; det0 < p.n_det; det0 += 2) {
     const bool two = p.n_det - det0 >= 2;
     const int acc = (det0 > 0 || accumulate) ? 1 : 0;
     if (maps) {
-      if (two) launch<2, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
-      else launch<1, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st);
+      if (two) launch<2, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st, scr);
+      else launch<1, true>(p, sino, maps, stride, vol, batch, det0, angle0, n_ang, maps_a0, acc, st, scr);
     } else {
-      if (two) launch<2, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st);
-      else launch<1, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st);
+      if (two) launch<2, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st, scr);
+      else launch<1, false>(p, sino, nullptr, stride, vol, batch, det0, angle0, n_ang, 0, acc, st, scr);
     }
+  }
+}
+
+bool bp_may_split(const Plan& p, int batch, int mode, int n_ang) {
+  return mode == CHIP_BP_ADJOINT && batch == 1 && p.n_det <= 2 && n_ang >= 2 * kAngBlk && p.n <= kBpPartsMaxN &&
+         p.bp_parts_tune != 0;
+}
+
+int bp_adjoint_parts(const Plan& p, bool maps, int max_parts) {
+  const int tiles = ((p.n + kTileW - 1) / kTileW) * ((p.n + kTileH - 1) / kTileH);
+  const int two = p.n_det >= 2;
+  if (maps) return two ? bp_parts<2, true>(tiles, p.n_angles, max_parts, p.bp_parts_tune)
+                       : bp_parts<1, true>(tiles, p.n_angles, max_parts, p.bp_parts_tune);
+  return two ? bp_parts<2, false>(tiles, p.n_angles, max_parts, p.bp_parts_tune)
+             : bp_parts<1, false>(tiles, p.n_angles, max_parts, p.bp_parts_tune);
+}
+
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+BpPartGuard::BpPartGuard(const Plan& p, int batch, int mode, int n_ang, hipStream_t st) : p_(p), st_(st) {
+  scr_.tune = p.bp_parts_tune;
+  if (!bp_may_split(p, batch, mode, n_ang)) return;
+  lk_ = std::unique_lock<std::mutex>(p.bp_state->mu);
+  const bool capturing = stream_capturing(st);
+  if (!p.d_bp_part && !capturing) {
+    // no room: the call runs as two halves added atomically, which needs no scratch
+    if (hipMalloc(&p.d_bp_part, sizeof(float) * kBpMaxParts * static_cast<size_t>(p.n) * p.n) != hipSuccess) {
+      p.d_bp_part = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  if (!p.d_bp_part) return;
+  BpPartState& s = *p.bp_state;
+  if (!capturing) {
+    if (s.used && s.last != st && s.done) (void)hipStreamWaitEvent(st, s.done, 0);
+    if (!s.done) (void)hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    record_ = s.done != nullptr;
+  }
+  scr_.part = p.d_bp_part;
+  scr_.cap_elems = static_cast<int64_t>(kBpMaxParts) * p.n * p.n;
+}
+
+BpPartGuard::~BpPartGuard() {
+  if (!record_) return;
+  BpPartState& s = *p_.bp_state;
+  if (hipEventRecord(s.done, st_) == hipSuccess) {
+    s.last = st_;
+    s.used = true;
   }
 }
 

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,6 +51,26 @@ struct Affine {
 constexpr int kBpMaxParts = 8;
 constexpr int kBpPartsMaxN = 1024;
 
+// Where the part sums of a split single-slice adjoint bp go: `part` holds
+// cap_elems floats (parts x N^2 must fit); tune = 0 never splits into parts,
+// 2..kBpMaxParts pins the count (the same bits on every device), -1 picks it
+// from the kernel's occupancy on this device.
+struct BpScratch {
+  float* part = nullptr;
+  int64_t cap_elems = 0;
+  int tune = -1;
+};
+
+// Lazily allocated part scratch of a plan and the ordering of its users: the
+// last stream that used it and an event recorded after that use, so a call
+// on another stream waits for it instead of overwriting partial sums in flight.
+struct BpPartState {
+  std::mutex mu;
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+  bool used = false;
+};
+
 struct Plan {
   int n = 0;          // slice side N (square), also detector pixels
   int n_angles = 0;   // A
@@ -75,8 +97,12 @@ struct Plan {
   float* d_mblock = nullptr;
   size_t mblock_bytes = 0;
   // per-part sums of a single-slice adjoint bp whose angle range is split over
-  // more workgroups than the tiles alone give ([kBpMaxParts][N][N], N <= 1024)
-  float* d_bp_part = nullptr;
+  // more workgroups than the tiles alone give ([kBpMaxParts][N][N], N <= 1024),
+  // allocated by the first call that can use it (never under stream capture)
+  mutable float* d_bp_part = nullptr;
+  std::unique_ptr<BpPartState> bp_state{new BpPartState};
+  // chip_plan_tune "bp_parts": -1 = from occupancy, 0 = never, 2..8 = fixed
+  int bp_parts_tune = -1;
   // bound on d_scratch + d_mblock (chip_plan_tune "otf_budget_mib")
   size_t otf_budget = size_t(1) << 30;
   int device = 0;
@@ -162,14 +188,44 @@ void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps
                              int batch, int sym, int angle0, int n_ang, int maps_a0, int accumulate,
                              hipStream_t st);
 void launch_bp_adjoint_tiles(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol,
-                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st);
+                             int batch, int angle0, int n_ang, int maps_a0, int accumulate, hipStream_t st,
+                             const BpScratch& scr = BpScratch{});
+
+// would a bp of `batch` slices in `mode` over n_ang angles split into parts?
+bool bp_may_split(const Plan& p, int batch, int mode, int n_ang);
+// the part count a split single-slice adjoint bp of all angles uses with room for
+// max_parts parts (0 = no split into parts) (bp_adjoint.hip)
+int bp_adjoint_parts(const Plan& p, bool maps, int max_parts);
+// chip_bp's launches for whole sinograms with the part scratch `scr` (projector.hip)
+void launch_bp_sinogram(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
+                        int mode, hipStream_t st, const BpScratch& scr);
+
+// The plan's part scratch for one call (bp_adjoint.hip). While it lives it holds
+// the plan's BpPartState: on entry it allocates the scratch when the call may
+// split into parts and `st` is not being captured, and makes `st` wait for the
+// previous user of the scratch on another stream; on exit it records this use.
+class BpPartGuard {
+ public:
+  BpPartGuard(const Plan& p, int batch, int mode, int n_ang, hipStream_t st);
+  ~BpPartGuard();
+  BpPartGuard(const BpPartGuard&) = delete;
+  BpPartGuard& operator=(const BpPartGuard&) = delete;
+  const BpScratch& scratch() const { return scr_; }
+
+ private:
+  const Plan& p_;
+  hipStream_t st_;
+  std::unique_lock<std::mutex> lk_;
+  BpScratch scr_;
+  bool record_ = false;
+};
 
 // fp / bp (mode CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT) over an angle range
 // with a maps block whose first angle is maps_a0 (projector.hip)
 void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
                      int maps_a0, hipStream_t st);
 void launch_bp_range(const Plan& p, const float* sino, const float* maps, float* vol, int mode, int angle0,
-                     int n_ang, int maps_a0, int accumulate, hipStream_t st);
+                     int n_ang, int maps_a0, int accumulate, hipStream_t st, const BpScratch& scr = BpScratch{});
 
 constexpr int kMaxDetPerPass = 4;
 

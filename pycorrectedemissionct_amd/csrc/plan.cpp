@@ -153,9 +153,6 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
   hipError_t e;
   if ((e = hipMalloc(&p.d_fp_geom, sizeof(float4) * n_angles)) != hipSuccess) return fail(e, "hipMalloc geom");
   if ((e = hipMalloc(&p.d_w, sizeof(float) * p.h_w.size())) != hipSuccess) return fail(e, "hipMalloc w");
-  if (n <= kBpPartsMaxN &&
-      (e = hipMalloc(&p.d_bp_part, sizeof(float) * kBpMaxParts * static_cast<size_t>(n) * n)) != hipSuccess)
-    return fail(e, "hipMalloc bp parts");
   {
     // row terms of the fp inverse map, one float32 product each (as the kernels form them)
     std::vector<float2> ytab(static_cast<size_t>(n_angles) * n);
@@ -199,6 +196,7 @@ int chip_plan_destroy(chip_plan* plan) {
   if (p.d_scratch) (void)hipFree(p.d_scratch);
   if (p.d_mblock) (void)hipFree(p.d_mblock);
   if (p.d_bp_part) (void)hipFree(p.d_bp_part);
+  if (p.bp_state && p.bp_state->done) (void)hipEventDestroy(p.bp_state->done);
   delete plan;
   return CHIP_OK;
 }

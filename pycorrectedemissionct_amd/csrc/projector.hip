@@ -736,7 +736,8 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float* __restrict
 }
 
 static void launch_bp(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
-                      int mode, int angle0, int n_ang, int line_mode, int accumulate, hipStream_t st) {
+                      int mode, int angle0, int n_ang, int line_mode, int accumulate, hipStream_t st,
+                      const BpScratch& scr = BpScratch{}) {
   dim3 grid((p.n + kBpTile - 1) / kBpTile, (p.n + kBpTile - 1) / kBpTile, batch);
   if (!line_mode && (mode == CHIP_BP_SKIMAGE || mode == CHIP_BP_SKIMAGE_SYM)) {
     // whole sinograms and angle ranges: line windows staged in LDS per tile
@@ -749,11 +750,16 @@ static void launch_bp(const Plan& p, const float* sino, const float* maps, int64
     hipLaunchKernelGGL(bp_skimage_kernel<1>, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w,
                        p.n, p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate, 0);
   } else if (!line_mode && angle0 == 0 && n_ang == p.n_angles && !accumulate) {
-    launch_bp_adjoint_tiles(p, sino, maps, stride, vol, batch, 0, p.n_angles, 0, 0, st);
+    launch_bp_adjoint_tiles(p, sino, maps, stride, vol, batch, 0, p.n_angles, 0, 0, st, scr);
   } else {
     hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), 0, st, sino, maps, stride, vol, p.d_fp_geom, p.d_w, p.n,
                        p.n_angles, p.n_det, angle0, n_ang, line_mode, accumulate);
   }
+}
+
+void launch_bp_sinogram(const Plan& p, const float* sino, const float* maps, int64_t stride, float* vol, int batch,
+                        int mode, hipStream_t st, const BpScratch& scr) {
+  launch_bp(p, sino, maps, stride, vol, batch, mode, 0, p.n_angles, 0, 0, st, scr);
 }
 
 void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
@@ -780,9 +786,9 @@ void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* 
 }
 
 void launch_bp_range(const Plan& p, const float* sino, const float* maps, float* vol, int mode, int angle0,
-                     int n_ang, int maps_a0, int accumulate, hipStream_t st) {
+                     int n_ang, int maps_a0, int accumulate, hipStream_t st, const BpScratch& scr) {
   if (mode == CHIP_BP_ADJOINT) {
-    launch_bp_adjoint_tiles(p, sino, maps, 0, vol, 1, angle0, n_ang, maps_a0, accumulate, st);
+    launch_bp_adjoint_tiles(p, sino, maps, 0, vol, 1, angle0, n_ang, maps_a0, accumulate, st, scr);
     return;
   }
   launch_bp_skimage_tiles(p, sino, maps, 0, vol, 1, 1, angle0, n_ang, maps_a0, accumulate, st);
@@ -829,7 +835,9 @@ int chip_bp(const chip_plan* plan, const float* sino, const float* maps, int64_t
   CHIP_CHECK_ARG(mode >= CHIP_BP_SKIMAGE && mode <= CHIP_BP_ADJOINT, "unknown bp mode");
   CHIP_CHECK_ARG(maps_slice_stride >= 0, "maps_slice_stride must be >= 0");
   const Plan& p = plan->p;
-  launch_bp(p, sino, maps, maps_slice_stride, vol, batch, mode, 0, p.n_angles, 0, 0, as_stream(stream));
+  const hipStream_t st = as_stream(stream);
+  BpPartGuard guard(p, batch, mode, p.n_angles, st);
+  launch_bp(p, sino, maps, maps_slice_stride, vol, batch, mode, 0, p.n_angles, 0, 0, st, guard.scratch());
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -841,6 +849,12 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   if (k == "fp_kernel") {
     CHIP_CHECK_ARG(value >= 0 && value <= 6, "fp_kernel must be in 0..6");
     plan->p.fp_variant = value;
+    return CHIP_OK;
+  }
+  if (k == "bp_parts") {
+    CHIP_CHECK_ARG(value == -1 || value == 0 || (value >= 2 && value <= kBpMaxParts),
+                   "bp_parts must be -1 (from occupancy), 0 (never) or 2..8");
+    plan->p.bp_parts_tune = value;
     return CHIP_OK;
   }
   if (k == "otf_budget_mib") {
