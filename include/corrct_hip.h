@@ -118,13 +118,34 @@ int chip_fp(const chip_plan* plan, const float* vol, const float* maps, int64_t 
  * ProjectorAttenuationXRF.bp (projectors.py:598-622) and
  * ProjectorBackendSKimage.bp (_projector_backends.py:335-359).
  * One slice with 64 or more angles is split over its angle range to fill the
- * GPU: adjoint mode with N <= 1024 sums 2-8 parts in the plan's scratch and
- * adds them in order; otherwise two halves are added atomically into the
- * zeroed vol. Results are reproducible either way; vol must not alias sino,
- * and calls on one plan must not overlap (they share its scratch).
+ * GPU: adjoint mode with N <= 1024 and at most two detectors sums 2-8 parts
+ * in plan scratch (8 N^2 floats, allocated by the first such call unless the
+ * stream is being captured) and adds them in order; otherwise two halves are
+ * added atomically into the zeroed vol. The part count follows the kernel's
+ * occupancy on the device, so the bits can differ between GPUs and builds
+ * unless chip_plan_tune "bp_parts" pins it. Calls on one plan from different
+ * streams are ordered on the scratch by an event; a captured graph that uses
+ * the scratch must not replay while another call on the plan runs. vol must
+ * not alias sino.
  */
 int chip_bp(const chip_plan* plan, const float* sino, const float* maps, int64_t maps_slice_stride,
             float* vol, int batch, int mode, void* stream);
+
+/*
+ * chip_bp without device allocation. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, kernels
+ * and results as chip_bp; the part sums of a split single-slice adjoint bp go
+ * to a static 16 MiB device array of the library (one per GPU, held until the
+ * process ends) instead of plan memory that chip_bp allocates on first use.
+ * A call whose parts x n^2 floats exceed 4 Mi (e.g. 5 or more parts at
+ * n = 1024) returns CHIP_EINVAL; chip_plan_tune "bp_parts" can pin fewer.
+ * All calls on one GPU share the array: calls that overlap on one GPU (other
+ * streams, other threads, other plans) race and must be serialised by the
+ * caller. Nothing in the array needs clearing: each part is fully written.
+ */
+int chip_bp_noalloc(const chip_plan* plan, const float* sino, const float* maps, int64_t maps_slice_stride,
+                    float* vol, int batch, int mode, void* stream);
 
 /*
  * Single-angle variants: line[d][u] for angle a (fp), and the back-projection
@@ -166,7 +187,10 @@ int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const fl
  * twice the workgroups per CU; the default) or 6 (workgroups walking
  * image-space bands of 16 rows or 32 columns that stage each map value once:
  * 27 % less HBM traffic than 5, slower at 512^2); 2..6 need N % 4 == 0 and
- * 16-byte aligned vol / maps, else 1 runs.
+ * 16-byte aligned vol / maps, else 1 runs. "bp_parts" = -1 (default: from
+ * the adjoint bp kernel's occupancy), 0 (never split into parts) or 2..8
+ * (pinned: the same bits on every device). "otf_budget_mib" bounds the
+ * on-the-fly scratch.
  */
 int chip_plan_tune(chip_plan* plan, const char* key, int value);
 

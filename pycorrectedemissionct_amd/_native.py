@@ -44,6 +44,7 @@ _SIGNATURES = {
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
+    "chip_bp_noalloc": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_fp_angle": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp_angle": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_absmax": [c_void_p, ctypes.c_int64, c_void_p, c_void_p],
