@@ -75,6 +75,19 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
 int chip_plan_destroy(chip_plan* plan);
 
 /*
+ * Radon tables from the caller's float32 cos / sin: cos_sin[a] = (cos, sin) of
+ * deg2rad(rad2deg(angles_rad[a])) in float32. scikit-image's radon / iradon
+ * take them from numpy's float32 cos / sin (skimage radon_transform.py:104-112,
+ * 295-308, as ProjectorBackendSKimage calls them, _projector_backends.py:299-359),
+ * which can differ by one ulp from the correctly rounded values chip_plan_create
+ * uses; at N = 1024 a one-ulp cosine moves a whole projection line by ~3e-5
+ * pixels. The Python layer passes numpy's values. Synchronises the device
+ * (queued work may read the old tables) and rewrites the plan's fp / bp
+ * geometry; the attenuation maps are unaffected (they use float64 angles).
+ */
+int chip_plan_set_radon_trig(chip_plan* plan, const float* cos_sin);
+
+/*
  * Detector-surface sampling of the emitted beam: detector d's outgoing maps
  * become the float32 mean, in order, of exp(-cumulative mu) over its
  * n_sub_det[d] (1 <= n_sub_det[d] <= n_sub_max <= 64) angles
@@ -320,6 +333,8 @@ int chip_batch_plan_create(chip_batch_plan** out, int n_slices, const int* sizes
                            const float* angles_rad, int n_det, const double* det_angles_rad, const double* w_det,
                            const double* w_ang);
 int chip_batch_plan_destroy(chip_batch_plan* plan);
+/* chip_plan_set_radon_trig for every slice size of a batch plan. */
+int chip_batch_set_radon_trig(chip_batch_plan* plan, const float* cos_sin);
 /* Total elements of the packed volume, sinogram and maps buffers (NULL outputs are skipped). */
 int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* sino_elems, int64_t* maps_elems);
 /* Persistent-grid sizes: "fp_wg_per_cu" (1..8, default 2) and "bp_wg_per_cu"

@@ -39,6 +39,8 @@ def load():
     lib.oracle_fp.restype = None
     lib.oracle_bp_skimage.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.oracle_bp_skimage.restype = None
+    lib.oracle_bp_adjoint.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    lib.oracle_bp_adjoint.restype = None
     lib.oracle_num_threads.restype = ctypes.c_int
     return lib
 

@@ -14,7 +14,10 @@
  *   iradon bp (radon_transform.py:295-314, _projector_backends.py:350-359):
  *   t = (j - k) c - (i - k) s in double from float32 c, s; linear
  *   interpolation of the line, 0 outside [-k, n-1-k]; float32 accumulation.
- * Parallelised with OpenMP over (angle, detector) for fp and over rows for bp.
+ *   exact adjoint: the transpose of the fp as a scatter of each sample's bilinear
+ *   weights (double), no reference counterpart (the north star's exact-adjoint bp).
+ * Parallelised with OpenMP over (angle, detector) for fp and the adjoint, over rows
+ * for the skimage bp.
  */
 #include <math.h>
 #include <stdint.h>
@@ -98,6 +101,63 @@ void oracle_bp_skimage(const float* sino, const float* trig, int n, int A, int D
       }
     }
   }
+}
+
+/* Exact transpose of oracle_fp, written as a scatter: every sample (a, d, y, u)
+ * of the fp adds w[a,d] line[u] times its four bilinear tap weights (in double)
+ * times M[a,d] at the tap into the pixel the fp read. geom / w / maps as in
+ * oracle_fp; sino [D][A][n]; vol [n][n] (the double sums rounded once). */
+void oracle_bp_adjoint(const float* sino, const float* maps, const float* geom, const float* w, int n, int A, int D,
+                       float* vol) {
+  const size_t nn = (size_t)n * n;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  double* acc = (double*)calloc((size_t)nt * nn, sizeof(double));
+#pragma omp parallel
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    double* my = acc + (size_t)t * nn;
+#pragma omp for schedule(dynamic)
+    for (long ad = 0; ad < (long)A * D; ++ad) {
+      const int a = (int)(ad / D), d = (int)(ad % D);
+      const float* m = maps ? maps + ((size_t)a * D + d) * nn : NULL;
+      const float cs = geom[4 * a], sn = geom[4 * a + 1], m02 = geom[4 * a + 2], m12 = geom[4 * a + 3];
+      const float* line = sino + ((size_t)d * A + a) * n;
+      const double wt = w ? (double)w[(size_t)a * D + d] : 1.0;
+      for (int y = 0; y < n; ++y) {
+        const float yf = (float)y;
+        for (int u = 0; u < n; ++u) {
+          const double lv = wt * (double)line[u];
+          if (lv == 0.0) continue;
+          const float uf = (float)u;
+          const float xs = (cs * uf + sn * yf) + m02;
+          const float ys = (-sn * uf + cs * yf) + m12;
+          const double r0f = floor((double)ys), c0f = floor((double)xs);
+          const double dr = (double)ys - r0f, dc = (double)xs - c0f;
+          const long r0 = (long)r0f, c0 = (long)c0f;
+          const double tw[4] = {(1.0 - dr) * (1.0 - dc), (1.0 - dr) * dc, dr * (1.0 - dc), dr * dc};
+          for (int k = 0; k < 4; ++k) {
+            const long r = r0 + (k >> 1), c = c0 + (k & 1);
+            if (r < 0 || r >= n || c < 0 || c >= n || tw[k] == 0.0) continue;
+            const size_t pix = (size_t)r * n + (size_t)c;
+            my[pix] += tw[k] * lv * (m ? (double)m[pix] : 1.0);
+          }
+        }
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < (long)nn; ++k) {
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += acc[(size_t)t * nn + k];
+    vol[k] = (float)s;
+  }
+  free(acc);
 }
 
 int oracle_num_threads(void) {

@@ -67,7 +67,7 @@ def conv3x3_bwd(x, w, dy):
     dw = np.zeros_like(w)
     for tap in range(9):
         dh, dwd = tap // 3, tap % 3
-        dw[tap] = np.einsum("nhwo,nhwi->oi", dy, xp[:, dh:dh + h, dwd:dwd + wd, :])
+        dw[tap] = dy.reshape(-1, dy.shape[-1]).T @ xp[:, dh:dh + h, dwd:dwd + wd, :].reshape(-1, ci)
         dxp[:, dh:dh + h, dwd:dwd + wd, :] += dy @ w[tap]
     return dxp[:, 1:-1, 1:-1], dw, dy.sum(axis=(0, 1, 2))
 

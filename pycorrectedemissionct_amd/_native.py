@@ -11,6 +11,7 @@ import ctypes
 import os
 import threading
 
+import numpy as np
 import torch  # noqa: F401  (must be loaded before the HIP library)
 
 from . import build as _build
@@ -37,6 +38,8 @@ _SIGNATURES = {
     ],
     "chip_plan_destroy": [c_void_p],
     "chip_plan_set_subangles": [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "chip_plan_set_radon_trig": [c_void_p, c_void_p],
+    "chip_batch_set_radon_trig": [c_void_p, c_void_p],
     "chip_grad_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int,
                        ctypes.c_float, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_grad_update_primal": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p,
@@ -168,6 +171,15 @@ def call(name: str, *args) -> None:
 def exported_symbols() -> list[str]:
     L = lib()
     return [L.chip_symbol_name(i).decode() for i in range(L.chip_num_symbols())]
+
+
+def radon_trig(angles_rad_f32) -> np.ndarray:
+    """[A, 2] float32 (cos, sin) as scikit-image's radon / iradon evaluate them: numpy's float32
+    ``cos`` / ``sin`` of ``deg2rad(rad2deg(angle))`` (``_projector_backends.py:148, 299``;
+    skimage ``radon_transform.py:104-112, 295-308``)."""
+    ang = np.asarray(angles_rad_f32, dtype=np.float32).reshape(-1)
+    th = np.deg2rad(np.rad2deg(ang))
+    return np.ascontiguousarray(np.stack([np.cos(th), np.sin(th)], axis=1).astype(np.float32))
 
 
 def ptr(t) -> int | None:

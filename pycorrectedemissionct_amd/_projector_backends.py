@@ -53,6 +53,8 @@ class HipPlan:
             None if self.w_ang is None else self.w_ang.ctypes.data,
         )
         self._h = handle
+        self._trig = _native.radon_trig(self.angles)
+        _native.call("chip_plan_set_radon_trig", handle, self._trig.ctypes.data)
 
     @property
     def handle(self):

@@ -50,6 +50,8 @@ class BatchPlan:
                      None if self.w_det is None else self.w_det.ctypes.data,
                      None if self.w_ang is None else self.w_ang.ctypes.data)
         self._h = h
+        self._trig = _native.radon_trig(self.angles)
+        _native.call("chip_batch_set_radon_trig", h, self._trig.ctypes.data)
         tot = (ctypes.c_int64 * 3)()
         _native.call("chip_batch_info", h, ctypes.addressof(tot), ctypes.addressof(tot) + 8,
                      ctypes.addressof(tot) + 16)

@@ -141,6 +141,20 @@ void release(BatchPlan& b) {
   b = BatchPlan();
 }
 
+// d_geom[size][A] = the size plans' radon tables (allocated on first call)
+int upload_geom(BatchPlan& b) {
+  std::vector<float4> geom;
+  for (chip_plan* p : b.size_plans) {
+    for (int a = 0; a < b.n_angles; ++a) {
+      const float* g = &p->p.h_fp_geom[4 * static_cast<size_t>(a)];
+      geom.push_back(make_float4(g[0], g[1], g[2], g[3]));
+    }
+  }
+  if (!b.d_geom) return upload(b.d_geom, geom);
+  CHIP_HIP(hipMemcpy(b.d_geom, geom.data(), sizeof(float4) * geom.size(), hipMemcpyHostToDevice));
+  return CHIP_OK;
+}
+
 int create(BatchPlan& b, int n_slices, const int* sizes, int n_angles, const float* angles_rad, int n_det,
            const double* det_angles_rad, const double* w_det, const double* w_ang) {
   b.n_slices = n_slices;
@@ -184,20 +198,13 @@ int create(BatchPlan& b, int n_slices, const int* sizes, int n_angles, const flo
     b.h_sino_start.push_back(sl.sino_off + static_cast<int64_t>(n_det) * n_angles * n);
     b.h_maps_start.push_back(sl.maps_off + static_cast<int64_t>(n_angles) * n_det * nn);
   }
-  std::vector<float4> geom;
-  for (chip_plan* p : b.size_plans) {
-    for (int a = 0; a < n_angles; ++a) {
-      const float* g = &p->p.h_fp_geom[4 * static_cast<size_t>(a)];
-      geom.push_back(make_float4(g[0], g[1], g[2], g[3]));
-    }
-  }
   int rc;
   if ((rc = upload(b.d_slices, b.h_slices)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_item_start_fp, b.h_item_start_fp)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_item_start_bpa, b.h_item_start_bpa)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_item_start_bps, b.h_item_start_bps)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_vol_start, b.h_vol_start)) != CHIP_OK) return rc;
-  if ((rc = upload(b.d_geom, geom)) != CHIP_OK) return rc;
+  if ((rc = upload_geom(b)) != CHIP_OK) return rc;
   b.d_w = b.size_plans[0]->p.d_w;
   return CHIP_OK;
 }
@@ -240,6 +247,16 @@ int chip_batch_plan_create(chip_batch_plan** out, int n_slices, const int* sizes
   }
   *out = bp;
   return CHIP_OK;
+}
+
+int chip_batch_set_radon_trig(chip_batch_plan* plan, const float* cos_sin) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && cos_sin != nullptr, "NULL argument");
+  for (chip_plan* p : plan->b.size_plans) {
+    const int rc = chip_plan_set_radon_trig(p, cos_sin);
+    if (rc != CHIP_OK) return rc;
+  }
+  return upload_geom(plan->b);
 }
 
 int chip_batch_plan_destroy(chip_batch_plan* plan) {

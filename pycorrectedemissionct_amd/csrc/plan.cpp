@@ -53,6 +53,36 @@ static Affine rotate_affine(int side, double r) {
   return m;
 }
 
+// radon inverse-map table h_fp_geom[a] = (c, s, -k (c + s - 1), -k (c - s - 1)) in float32
+// from the angle's float32 (cos, sin)
+static void fill_radon_geom(Plan& p, int a, float c, float s) {
+  const float kf = static_cast<float>(-(p.n / 2));
+  p.h_fp_geom[4 * a + 0] = c;
+  p.h_fp_geom[4 * a + 1] = s;
+  p.h_fp_geom[4 * a + 2] = kf * ((c + s) - 1.0f);
+  p.h_fp_geom[4 * a + 3] = kf * ((c - s) - 1.0f);
+}
+
+// device copies of the radon table and of its row terms (fl(s y), fl(c y)), synchronous
+static hipError_t upload_radon_geom(Plan& p) {
+  hipError_t e;
+  const int n = p.n, n_angles = p.n_angles;
+  if (!p.d_fp_geom && (e = hipMalloc(&p.d_fp_geom, sizeof(float4) * n_angles)) != hipSuccess) return e;
+  if ((e = hipMemcpy(p.d_fp_geom, p.h_fp_geom.data(), sizeof(float4) * n_angles, hipMemcpyHostToDevice)) != hipSuccess)
+    return e;
+  // row terms of the fp inverse map, one float32 product each (as the kernels form them)
+  std::vector<float2> ytab(static_cast<size_t>(n_angles) * n);
+  for (int a = 0; a < n_angles; ++a) {
+    const float c = p.h_fp_geom[4 * a + 0], s = p.h_fp_geom[4 * a + 1];
+    for (int y = 0; y < n; ++y) {
+      const float yf = static_cast<float>(y);
+      ytab[static_cast<size_t>(a) * n + y] = make_float2(s * yf, c * yf);
+    }
+  }
+  if (!p.d_ytab && (e = hipMalloc(&p.d_ytab, sizeof(float2) * ytab.size())) != hipSuccess) return e;
+  return hipMemcpy(p.d_ytab, ytab.data(), sizeof(float2) * ytab.size(), hipMemcpyHostToDevice);
+}
+
 }  // namespace chip
 
 using namespace chip;
@@ -70,7 +100,7 @@ static const char* kSymbols[] = {
     "chip_nn_conv3x3",    "chip_nn_conv3x3_wgrad", "chip_nn_bias_grad", "chip_nn_flip_weights",
     "chip_nn_relu_mask",  "chip_nn_avgpool2",  "chip_nn_upsample2", "chip_nn_head_fwd",
     "chip_nn_head_bwd",   "chip_nn_mse",       "chip_nn_adam",      "chip_nn_to_bf16",
-    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",  "chip_psf_lines",     "chip_plan_set_subangles", "chip_grad_dual", "chip_grad_update_primal", "chip_bp_noalloc",
+    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",  "chip_psf_lines",     "chip_plan_set_subangles", "chip_grad_dual", "chip_grad_update_primal", "chip_bp_noalloc", "chip_plan_set_radon_trig", "chip_batch_set_radon_trig",
 };
 
 extern "C" {
@@ -111,7 +141,6 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
   p.np = n + 2 * p.pad;
   (void)hipGetDevice(&p.device);
 
-  const int k = n / 2;
   p.h_fp_geom.resize(static_cast<size_t>(n_angles) * 4);
   p.h_w.resize(static_cast<size_t>(n_angles) * n_det);
   p.h_rot_in_fwd.resize(n_angles);
@@ -121,13 +150,10 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
   for (int a = 0; a < n_angles; ++a) {
     const float deg = rad2deg_f32(angles_rad[a]);
     const float th = deg2rad_f32(deg);
-    const float c = static_cast<float>(std::cos(static_cast<double>(th)));
-    const float s = static_cast<float>(std::sin(static_cast<double>(th)));
-    const float kf = static_cast<float>(-k);
-    p.h_fp_geom[4 * a + 0] = c;
-    p.h_fp_geom[4 * a + 1] = s;
-    p.h_fp_geom[4 * a + 2] = kf * ((c + s) - 1.0f);
-    p.h_fp_geom[4 * a + 3] = kf * ((c - s) - 1.0f);
+    // correctly rounded float32 cos / sin; chip_plan_set_radon_trig can substitute the
+    // values of the caller's float32 math library
+    fill_radon_geom(p, a, static_cast<float>(std::cos(static_cast<double>(th))),
+                    static_cast<float>(std::sin(static_cast<double>(th))));
     for (int d = 0; d < n_det; ++d) {
       const double wd = w_det ? w_det[d] : 1.0;
       const double wa = w_ang ? w_ang[static_cast<size_t>(a) * n_det + d] : 1.0;
@@ -151,22 +177,8 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
     return e == hipErrorOutOfMemory ? CHIP_ENOMEM : CHIP_EHIP;
   };
   hipError_t e;
-  if ((e = hipMalloc(&p.d_fp_geom, sizeof(float4) * n_angles)) != hipSuccess) return fail(e, "hipMalloc geom");
+  if ((e = upload_radon_geom(p)) != hipSuccess) return fail(e, "radon tables");
   if ((e = hipMalloc(&p.d_w, sizeof(float) * p.h_w.size())) != hipSuccess) return fail(e, "hipMalloc w");
-  {
-    // row terms of the fp inverse map, one float32 product each (as the kernels form them)
-    std::vector<float2> ytab(static_cast<size_t>(n_angles) * n);
-    for (int a = 0; a < n_angles; ++a) {
-      const float c = p.h_fp_geom[4 * a + 0], s = p.h_fp_geom[4 * a + 1];
-      for (int y = 0; y < n; ++y) {
-        const float yf = static_cast<float>(y);
-        ytab[static_cast<size_t>(a) * n + y] = make_float2(s * yf, c * yf);
-      }
-    }
-    if ((e = hipMalloc(&p.d_ytab, sizeof(float2) * ytab.size())) != hipSuccess) return fail(e, "hipMalloc ytab");
-    if ((e = hipMemcpy(p.d_ytab, ytab.data(), sizeof(float2) * ytab.size(), hipMemcpyHostToDevice)) != hipSuccess)
-      return fail(e, "hipMemcpy ytab");
-  }
   const size_t n_rot = 2 * static_cast<size_t>(n_angles) + 2 * static_cast<size_t>(n_angles) * n_det;
   if ((e = hipMalloc(&p.d_rot, sizeof(Affine) * n_rot)) != hipSuccess) return fail(e, "hipMalloc rot");
   std::vector<Affine> rot;
@@ -175,8 +187,6 @@ int chip_plan_create(chip_plan** out, int n, int n_angles, const float* angles_r
   rot.insert(rot.end(), p.h_rot_in_back.begin(), p.h_rot_in_back.end());
   rot.insert(rot.end(), p.h_rot_out_fwd.begin(), p.h_rot_out_fwd.end());
   rot.insert(rot.end(), p.h_rot_out_back.begin(), p.h_rot_out_back.end());
-  if ((e = hipMemcpy(p.d_fp_geom, p.h_fp_geom.data(), sizeof(float4) * n_angles, hipMemcpyHostToDevice)) != hipSuccess)
-    return fail(e, "hipMemcpy geom");
   if ((e = hipMemcpy(p.d_w, p.h_w.data(), sizeof(float) * p.h_w.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "hipMemcpy w");
   if ((e = hipMemcpy(p.d_rot, rot.data(), sizeof(Affine) * n_rot, hipMemcpyHostToDevice)) != hipSuccess)
@@ -198,6 +208,21 @@ int chip_plan_destroy(chip_plan* plan) {
   if (p.d_bp_part) (void)hipFree(p.d_bp_part);
   if (p.bp_state && p.bp_state->done) (void)hipEventDestroy(p.bp_state->done);
   delete plan;
+  return CHIP_OK;
+}
+
+int chip_plan_set_radon_trig(chip_plan* plan, const float* cos_sin) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && cos_sin != nullptr, "NULL argument");
+  Plan& p = plan->p;
+  for (int a = 0; a < p.n_angles; ++a) {
+    const float c = cos_sin[2 * a], s = cos_sin[2 * a + 1];
+    CHIP_CHECK_ARG(std::isfinite(c) && std::isfinite(s) && std::fabs(c) <= 1.f && std::fabs(s) <= 1.f,
+                   "cos / sin entries must be finite and in [-1, 1]");
+  }
+  CHIP_HIP(hipDeviceSynchronize());  // queued work may still read the old tables
+  for (int a = 0; a < p.n_angles; ++a) fill_radon_geom(p, a, cos_sin[2 * a], cos_sin[2 * a + 1]);
+  CHIP_HIP(upload_radon_geom(p));
   return CHIP_OK;
 }
 

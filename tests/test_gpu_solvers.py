@@ -219,8 +219,10 @@ def test_sirt_at_the_reference_cpu_configuration(pkg):
     """The reference's CPU-runnable case: a 128^2 slice of 10 ellipses, attenuation 0.02 on the
     support + 0.005 in and 0.05 out, 256 angles over 2 pi, one detector at pi/2, Poisson counts
     at peak 1e4. The HIP map builder against the oracle's, then 20 SIRT iterations against the
-    oracle's SIRT on the oracle projector."""
+    oracle's SIRT on the oracle projector; then the configuration's full 200 iterations against
+    the oracle loop on the C restatement of the projector."""
     from oracle import attenuation as oatt
+    from oracle import cops
 
     n, a = 128, 256
     rng = np.random.default_rng(20260101)
@@ -237,3 +239,9 @@ def test_sirt_at_the_reference_cpu_configuration(pkg):
     ref = osol.sirt(lambda v: oxrf.fp(v, ang, maps)[0], lambda y: oxrf.bp_skimage(y[None], ang, n), b, (n, n), 20)
     print(f"SIRT-20 128^2 x 256: rel_max {rel_max(ref, x):.2e} rel_l2 {rel_l2(ref, x):.2e}")
     assert rel_max(ref, x) < TOL
+    tabs = cops.tables(ang, n)
+    x200 = pkg.SIRT()(prj, b, iterations=200)[0]
+    ref200 = osol.sirt(lambda v: cops.fp(v, ang, maps, tabs=tabs)[0], lambda y: cops.bp_skimage(y, ang, n, tabs=tabs),
+                       b, (n, n), 200)
+    print(f"SIRT-200 128^2 x 256: rel_max {rel_max(ref200, x200):.2e} rel_l2 {rel_l2(ref200, x200):.2e}")
+    assert rel_max(ref200, x200) < TOL
