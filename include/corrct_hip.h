@@ -199,8 +199,12 @@ int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const fl
  * less HBM traffic, no faster), 5 (3 with 32-bin tiles: a third of the LDS,
  * twice the workgroups per CU; the default) or 6 (workgroups walking
  * image-space bands of 16 rows or 32 columns that stage each map value once:
- * 27 % less HBM traffic than 5, slower at 512^2); 2..6 need N % 4 == 0 and
- * 16-byte aligned vol / maps, else 1 runs. "bp_parts" = -1 (default: from
+ * 27 % less HBM traffic than 5, slower at 512^2), 7 (5 with the slice and map
+ * boxes staged by gfx950's direct-to-LDS buffer loads, the products formed at
+ * the gathers; the same bits as 5), 8 (7 staging only the footprint of
+ * interior chunks) or 9 (5 with each wave gathering 8 x 8 sample patches; sums
+ * in another order); 2..9 need N % 4 == 0 and 16-byte aligned vol / maps,
+ * else 1 runs. "bp_parts" = -1 (default: from
  * the adjoint bp kernel's occupancy), 0 (never split into parts) or 2..8
  * (pinned: the same bits on every device). "otf_budget_mib" bounds the
  * on-the-fly scratch.
@@ -337,9 +341,11 @@ int chip_batch_plan_destroy(chip_batch_plan* plan);
 int chip_batch_set_radon_trig(chip_batch_plan* plan, const float* cos_sin);
 /* Total elements of the packed volume, sinogram and maps buffers (NULL outputs are skipped). */
 int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* sino_elems, int64_t* maps_elems);
-/* Persistent-grid sizes: "fp_wg_per_cu" (1..8, default 2) and "bp_wg_per_cu"
- * (1..16, default 4) workgroups per CU; "fp_trim" (0 / 1, default 0) stages
- * only the footprint of interior fp chunks; set by the autotuner. */
+/* Grid sizes: "fp_wg_per_cu" (0..8, default 5) and "bp_wg_per_cu" (0..16,
+ * default 4) workgroups per CU of a persistent grid, 0 = one workgroup per work
+ * item; fp tile variants "fp_trim" (stage only the footprint of interior
+ * chunks), "fp_direct" (direct-to-LDS staging) and "fp_patch" (8 x 8 sample
+ * patches per wave), each 0 / 1, default 0; set by the autotuner. */
 int chip_batch_tune(chip_batch_plan* plan, const char* key, int value);
 /* Maps of every slice (chip_att_maps per slice; mu_* packed like volumes). */
 int chip_batch_att_maps(chip_batch_plan* plan, const float* mu_in, const float* mu_out, float* maps, void* stream);

@@ -180,8 +180,9 @@ class BatchedProjectorXRF:
         return self.plan.bp(sino, maps, out, _BP_MODES[self.bp_mode])
 
 
-def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(2, 4, 5, 6), bp_candidates=(2, 4, 8)) -> dict:
-    """Pick the persistent-grid sizes of the batched fp and bp by timing them on ``A``'s batch.
+def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8)) -> dict:
+    """Pick the grid sizes of the batched fp and bp by timing them on ``A``'s batch: k workgroups
+    per CU of a persistent grid, or 0 for one workgroup per work item.
 
     Returns ``{"fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``; the
     chosen values are applied to ``A.plan``. Results do not depend on the grid size (every

@@ -283,8 +283,18 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_wg_per_cu") {
-    CHIP_CHECK_ARG(value >= 1 && value <= 8, "fp_wg_per_cu must be in 1..8");
+    CHIP_CHECK_ARG(value >= 0 && value <= 8, "fp_wg_per_cu must be in 0..8 (0: one workgroup per work item)");
     plan->b.fp_wg_per_cu = value;
+    return CHIP_OK;
+  }
+  if (k == "fp_patch") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_patch must be 0 or 1");
+    plan->b.fp_patch = value;
+    return CHIP_OK;
+  }
+  if (k == "fp_direct") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_direct must be 0 or 1");
+    plan->b.fp_direct = value;
     return CHIP_OK;
   }
   if (k == "fp_trim") {
@@ -293,7 +303,7 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
     return CHIP_OK;
   }
   if (k == "bp_wg_per_cu") {
-    CHIP_CHECK_ARG(value >= 1 && value <= 16, "bp_wg_per_cu must be in 1..16");
+    CHIP_CHECK_ARG(value >= 0 && value <= 16, "bp_wg_per_cu must be in 0..16 (0: one workgroup per tile)");
     plan->b.bp_wg_per_cu = value;
     return CHIP_OK;
   }

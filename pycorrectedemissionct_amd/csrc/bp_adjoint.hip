@@ -361,7 +361,8 @@ template <int D, bool MAPS>
 void launch_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int det0, int accumulate,
                   hipStream_t st) {
   const int64_t total = bp.h_item_start_bpa.back();
-  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
+  // bp_wg_per_cu = 0: one workgroup per tile (a flat grid), else a persistent grid
+  const int64_t want = bp.bp_wg_per_cu > 0 ? static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu : total;
   const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
   hipLaunchKernelGGL((bp_adjoint_batch_kernel<D, MAPS>), dim3(grid), dim3(256), 0, st, bp.d_slices, bp.n_slices,
                      bp.d_item_start_bpa, sino, maps, vol, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det, det0,

@@ -249,7 +249,8 @@ void launch_bp_skimage_tiles(const Plan& p, const float* sino, const float* maps
 void launch_bp_skimage_tiles_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
                                    hipStream_t st) {
   const int64_t total = bp.h_item_start_bps.back();
-  const int64_t want = static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu;
+  // bp_wg_per_cu = 0: one workgroup per tile (a flat grid), else a persistent grid
+  const int64_t want = bp.bp_wg_per_cu > 0 ? static_cast<int64_t>(bp.bp_wg_per_cu) * bp.n_cu : total;
   const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
   for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
     const bool two = bp.n_det - det0 >= 2;

@@ -137,6 +137,8 @@ struct BatchPlan {
   int fp_wg_per_cu = 5;  // persistent grid of the fp: fp_wg_per_cu x CUs workgroups
   int bp_wg_per_cu = 4;  // same for the bp
   int fp_trim = 0;       // stage only the footprint of interior fp chunks
+  int fp_direct = 0;     // stage the fp chunks with direct-to-LDS loads (fp_kernel 7 / 8 of a plan)
+  int fp_patch = 0;      // gather 8 x 8 sample patches per wave instruction (fp_kernel 9 of a plan)
   std::vector<chip_plan*> size_plans;  // one single-slice plan per distinct size
   std::vector<BatchSlice> h_slices;
   // prefix sums (n_slices + 1 entries): fp items, adjoint-bp tiles, skimage-bp

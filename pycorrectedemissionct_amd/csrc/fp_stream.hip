@@ -161,16 +161,22 @@ struct FpItem {
 // centre within 1.5 of its pixels -- instead of the whole axis-aligned box,
 // which at 45 degrees is twice the footprint. Border chunks (clamped samples
 // read the zero padding) keep the whole box.
-template <class T, int D, bool MAPS, bool TRIM>
+//
+// PATCH (32-bin tiles): wave w gathers bins [8 w, 8 w + 8), lane l sample rows
+// l / 8 + 8 k of each chunk, so one gather instruction of a wave reads an 8 x 8
+// patch of samples (a compact region of the staged tile, fewer LDS bank
+// conflicts than 32 bins of one row at steep angles) instead of 32 bins x 2 rows.
+template <class T, int D, bool MAPS, bool TRIM, bool PATCH = false>
 __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   static_assert(D == 1 || D == 2, "one or two detectors per pass");
+  static_assert(!PATCH || (T::kBins == 32 && T::kRows == 32), "8 x 8 patches tile a 32 x 32 chunk");
   constexpr int kBins = T::kBins, kRows = T::kRows, kGroups = T::kGroups, kRowsPerGroup = T::kRowsPerGroup;
   constexpr int kSlots = T::kSlots, kPitch = T::kPitch, kPerThread = T::kPerThread;
   const int n = it.n;
   const int u0 = it.u0;
   const int tid = threadIdx.x;
-  const int lane = tid % kBins;
-  const int grp = tid / kBins;
+  const int lane = PATCH ? 8 * (tid >> 6) + (tid & 7) : tid % kBins;  // the thread's bin in the block
+  const int grp = PATCH ? (tid & 63) >> 3 : tid / kBins;               // its row group
   const int u = u0 + lane;
 
   const float4 gm = it.gm;
@@ -309,7 +315,9 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
     if (yn < n) issue(nxt);
 
     if (u < n) {
-      const int yb0 = yc + grp * kRowsPerGroup;  // wave-uniform
+      // rows yb0 + k * ystep, k < kRowsPerGroup (PATCH: grp + 8 k; else 4 consecutive rows, wave-uniform)
+      const int yb0 = PATCH ? yc + grp : yc + grp * kRowsPerGroup;
+      constexpr int ystep = PATCH ? 8 : 1;
       const int base = cur.rlo * kPitch + cur.clo;
       // one sample: skimage inverse map, taps from LDS, order-1 blend
       auto sample = [&](int y, bool clamp) {
@@ -345,24 +353,24 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
         }
       };
       // full row groups run straight-line, so the LDS reads of several samples overlap
-      const bool full = yb0 + kRowsPerGroup <= n;
+      const bool full = yb0 + (kRowsPerGroup - 1) * ystep < n;
       if (!cur.border) {
         if (full) {
 #pragma unroll
-          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k, false);
+          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k * ystep, false);
         } else {
 #pragma unroll
           for (int k = 0; k < kRowsPerGroup; ++k)
-            if (yb0 + k < n) sample(yb0 + k, false);
+            if (yb0 + k * ystep < n) sample(yb0 + k * ystep, false);
         }
       } else {
         if (full) {
 #pragma unroll
-          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k, true);
+          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k * ystep, true);
         } else {
 #pragma unroll
           for (int k = 0; k < kRowsPerGroup; ++k)
-            if (yb0 + k < n) sample(yb0 + k, true);
+            if (yb0 + k * ystep < n) sample(yb0 + k * ystep, true);
         }
       }
     }
@@ -385,7 +393,235 @@ __device__ __forceinline__ void fp_stream_block(const FpItem& it, float* lds) {
   }
 }
 
-template <class T, int D, bool MAPS, bool TRIM>
+// ---------------------------------------------------------------------------
+// Direct-to-LDS staging (fp_kernel 7): the chunk boxes of the slice and of the
+// D map planes go from HBM to LDS with gfx950's buffer_load_dwordx4 ... lds,
+// no VGPR round trip and no multiply at staging; the sample gathers read the
+// slice and map taps from their own planes and form the products fl(v * M)
+// there (the same values, so the same bits as fp_kernel 5). Slots outside the
+// slice, outside the chunk's footprint (interior chunks) or past the box get an
+// out-of-range buffer offset: the hardware writes zeros and reads nothing,
+// which is also the zero padding border chunks need. One buffer per
+// workgroup; the other workgroups of the CU overlap a chunk's load latency.
+// ---------------------------------------------------------------------------
+constexpr unsigned kOob = 0x80000000u;  // byte offset past any plane: loads return 0
+constexpr int kDirectPlane = BatchTile::kBoxRows * 4 * BatchTile::kSlots;  // floats per staged plane
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* base, size_t nn) {
+  // raw buffer over one n x n plane (gfx9 word 3: 32-bit data, no swizzle)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, static_cast<int>(nn * sizeof(float)),
+                                           0x00020000);
+}
+
+template <int D, bool TRIM>
+__device__ __forceinline__ void fp_direct_block(const FpItem& it, float* lds) {
+  using T = BatchTile;
+  constexpr int kBins = T::kBins, kRows = T::kRows, kGroups = T::kGroups, kRowsPerGroup = T::kRowsPerGroup;
+  constexpr int kSlots = T::kSlots, kPitch = 4 * T::kSlots;
+  constexpr int kIter = (T::kBoxRows * kSlots + 255) / 256;
+  const int n = it.n;
+  const int u0 = it.u0;
+  const int tid = threadIdx.x;
+  const int lane = tid % kBins;
+  const int grp = tid / kBins;
+  const int u = u0 + lane;
+  const float4 gm = it.gm;
+  const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const float fn = static_cast<float>(n);
+  const float uf = static_cast<float>(u);
+  const float xu = mul_rn(cs, uf);
+  const float yu = mul_rn(-sn, uf);
+  const int u_last = min(u0 + kBins, n) - 1;
+  const float xu_a = mul_rn(cs, static_cast<float>(u0)), yu_a = mul_rn(-sn, static_cast<float>(u0));
+  const float xu_b = mul_rn(cs, static_cast<float>(u_last)), yu_b = mul_rn(-sn, static_cast<float>(u_last));
+  const float uf_lo = static_cast<float>(u0) - 3.5f, uf_hi = static_cast<float>(u_last) + 3.5f;
+
+  // buffer resources of the slice and of the (up to two) map planes (an opaque type: no arrays)
+  const __amdgpu_buffer_rsrc_t rs_v = plane_rsrc(it.v, nn);
+  const __amdgpu_buffer_rsrc_t rs_m0 = plane_rsrc(it.mb, nn);
+  const __amdgpu_buffer_rsrc_t rs_m1 = plane_rsrc(it.mb + (D - 1) * nn, nn);
+
+  // slots of this thread: flat index s = i * 256 + tid of the box in rows of kSlots float4
+  int s_row[kIter], s_col[kIter];
+#pragma unroll
+  for (int i = 0; i < kIter; ++i) {
+    const int sl = i * 256 + tid;
+    s_row[i] = sl / kSlots;
+    s_col[i] = 4 * (sl - s_row[i] * kSlots);
+  }
+  const int wave_slot0 = tid & ~63;  // the wave's first slot of each pass: LDS destinations are wave-uniform
+
+  auto issue = [&](const Box& bx) {
+    const int th = bx.rhi - bx.rlo + 1;
+#pragma unroll
+    for (int i = 0; i < kIter; ++i) {
+      if (s_row[i] < th) {
+        const int r = bx.rlo + s_row[i];
+        const int c = bx.clo + s_col[i];
+        bool in = c <= bx.chi && static_cast<unsigned>(r) < static_cast<unsigned>(n) &&
+                  static_cast<unsigned>(c) < static_cast<unsigned>(n);
+        if (TRIM && !bx.border) {
+          const float px = static_cast<float>(c) + (1.5f - m02);
+          const float py = static_cast<float>(r) - m12;
+          const float us = cs * px - sn * py;
+          const float ysm = sn * px + cs * py;
+          in = in && us >= uf_lo && us <= uf_hi && ysm >= static_cast<float>(bx.ylo) - 3.5f &&
+               ysm <= static_cast<float>(bx.yhi) + 3.5f;
+        }
+        const unsigned off = in ? static_cast<unsigned>(r * n + c) * 4u : kOob;
+        float* dst = lds + 4 * (i * 256 + wave_slot0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs_v, (lds_void_t*)dst, 16, off, 0, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs_m0, (lds_void_t*)(dst + kDirectPlane), 16, off, 0, 0, 0);
+        if (D == 2)
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(rs_m1, (lds_void_t*)(dst + 2 * kDirectPlane), 16, off, 0, 0, 0);
+      }
+    }
+  };
+
+  float acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = 0.f;
+
+  const int n_chunks = (n + kRows - 1) / kRows;
+  const bool lds_boxes = n_chunks <= kMaxChunks;
+  int2* boxes = reinterpret_cast<int2*>(lds + (1 + D) * kDirectPlane);
+  if (lds_boxes) {
+    for (int c = tid; c < n_chunks; c += 256)
+      boxes[c] = pack_box(chunk_box<T>(c * kRows, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12));
+    __syncthreads();
+  }
+  auto box_of = [&](int y) -> Box {
+    return lds_boxes ? unpack_box(boxes[y / kRows], y, kRows, n)
+                     : chunk_box<T>(y, n, fn, xu_a, yu_a, xu_b, yu_b, cs, sn, m02, m12);
+  };
+
+  for (int yc = 0; yc < n; yc += kRows) {
+    const Box cur = box_of(yc);
+    if (cur.empty) continue;
+    __syncthreads();  // the previous chunk's gathers are done with the planes
+    issue(cur);
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's loads have reached LDS
+    __syncthreads();
+    if (u < n) {
+      const int yb0 = yc + grp * kRowsPerGroup;
+      const int base = cur.rlo * kPitch + cur.clo;
+      auto sample = [&](int y, bool clamp) {
+        const float yf = static_cast<float>(y);
+        float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
+        float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
+        if (clamp) {
+          xs = fminf(fmaxf(xs, -1.f), fn);
+          ys = fminf(fmaxf(ys, -1.f), fn);
+        }
+        const float c0f = floorf(xs);
+        const float r0f = floorf(ys);
+        const float dc = xs - c0f;
+        const float dr = ys - r0f;
+        const int li = static_cast<int>(fmaf(r0f, static_cast<float>(kPitch), c0f)) - base;
+        const float* f = lds + li;
+        const float f00 = f[0], f01 = f[1], f10 = f[kPitch], f11 = f[kPitch + 1];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float* m = f + (1 + d) * kDirectPlane;
+          const float t00 = f00 * m[0], t01 = f01 * m[1], t10 = f10 * m[kPitch], t11 = f11 * m[kPitch + 1];
+          const float top = fmaf(dc, t01 - t00, t00);
+          const float bot = fmaf(dc, t11 - t10, t10);
+          acc[d] += fmaf(dr, bot - top, top);
+        }
+      };
+      const bool full = yb0 + kRowsPerGroup <= n;
+      if (!cur.border) {
+        if (full) {
+#pragma unroll
+          for (int k = 0; k < kRowsPerGroup; ++k) sample(yb0 + k, false);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kRowsPerGroup; ++k)
+            if (yb0 + k < n) sample(yb0 + k, false);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < kRowsPerGroup; ++k)
+          if (yb0 + k < n) sample(yb0 + k, true);
+      }
+    }
+  }
+
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D; ++d) lds[(d * kGroups + grp) * kBins + lane] = acc[d];
+  __syncthreads();
+  if (grp == 0 && u < n) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float sum = lds[(d * kGroups) * kBins + lane];
+#pragma unroll
+      for (int g = 1; g < kGroups; ++g) sum = sum + lds[(d * kGroups + g) * kBins + lane];
+      it.out[d * it.det_stride + u] = sum * it.wa[d];
+    }
+  }
+}
+
+template <int D, bool TRIM>
+__global__ __launch_bounds__(256) void fp_direct_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
+                                                         int64_t maps_slice_stride, float* __restrict__ sino,
+                                                         const float4* __restrict__ geom, const float* __restrict__ w,
+                                                         int n, int n_angles, int n_det_total, int det0, int n_ublk,
+                                                         int angle0, int maps_a0) {
+  extern __shared__ float lds[];  // (1 + D) planes [kBoxRows][4 kSlots], then the chunk boxes
+  const int lb = logical_block(blockIdx.x, gridDim.x);
+  const int la = lb / n_ublk;
+  const int a = angle0 + la;
+  const int b = blockIdx.y;
+  const size_t nn = static_cast<size_t>(n) * n;
+  FpItem it;
+  it.n = n;
+  it.u0 = (lb - la * n_ublk) * BatchTile::kBins;
+  it.gm = geom[a];
+  it.v = vol + static_cast<size_t>(b) * nn;
+  it.mb = maps + static_cast<size_t>(b) * maps_slice_stride + (static_cast<size_t>(a - maps_a0) * n_det_total + det0) * nn;
+  it.out = sino + ((static_cast<size_t>(b) * n_det_total + det0) * n_angles + a) * n;
+  it.det_stride = static_cast<size_t>(n_angles) * n;
+  it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
+  fp_direct_block<D, TRIM>(it, lds);
+}
+
+template <int D, bool TRIM>
+__global__ __launch_bounds__(256) void fp_direct_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
+                                                               const int64_t* __restrict__ item_start,
+                                                               const float* __restrict__ vol,
+                                                               const float* __restrict__ maps, float* __restrict__ sino,
+                                                               const float4* __restrict__ geom,
+                                                               const float* __restrict__ w, int n_angles,
+                                                               int n_det_total, int det0) {
+  extern __shared__ float lds[];
+  const int64_t total = item_start[n_slices];
+  const int g = gridDim.x;
+  const int64_t lb = logical_block(blockIdx.x, g);
+  for (int64_t k = lb; k < total; k += g) {
+    const int s = find_slice(item_start, n_slices, k);
+    const BatchSlice sl = slices[s];
+    const int local = static_cast<int>(k - item_start[s]);
+    const int a = local / sl.n_ublk;
+    const size_t nn = static_cast<size_t>(sl.n) * sl.n;
+    FpItem it;
+    it.n = sl.n;
+    it.u0 = (local - a * sl.n_ublk) * BatchTile::kBins;
+    it.gm = geom[static_cast<size_t>(sl.size_idx) * n_angles + a];
+    it.v = vol + sl.vol_off;
+    it.mb = maps + sl.maps_off + (static_cast<size_t>(a) * n_det_total + det0) * nn;
+    it.out = sino + sl.sino_off + (static_cast<size_t>(det0) * n_angles + a) * sl.n;
+    it.det_stride = static_cast<size_t>(n_angles) * sl.n;
+    it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
+    fp_direct_block<D, TRIM>(it, lds);
+    __syncthreads();  // the next item restages the planes
+  }
+}
+
+template <class T, int D, bool MAPS, bool TRIM, bool PATCH = false>
 __global__ __launch_bounds__(256, T::kWaves) void fp_stream_kernel(const float* __restrict__ vol, const float* __restrict__ maps,
                                                         int64_t maps_slice_stride, float* __restrict__ sino,
                                                         const float4* __restrict__ geom, const float* __restrict__ w,
@@ -408,14 +644,14 @@ __global__ __launch_bounds__(256, T::kWaves) void fp_stream_kernel(const float* 
   it.out = sino + ((static_cast<size_t>(b) * n_det_total + det0) * n_angles + a) * n;
   it.det_stride = static_cast<size_t>(n_angles) * n;
   it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-  fp_stream_block<T, D, MAPS, TRIM>(it, lds);
+  fp_stream_block<T, D, MAPS, TRIM, PATCH>(it, lds);
 }
 
 // Persistent form over a batch of slices of different sizes: work item k of
 // the batch is (slice s, angle a, bin block) with s found in the item prefix
 // sums; workgroup p takes items r * G + logical_block(p, G), r = 0, 1, ...
 // (G workgroups), so each XCD walks a contiguous run of bin blocks.
-template <int D, bool MAPS, bool TRIM>
+template <int D, bool MAPS, bool TRIM, bool PATCH = false>
 __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* __restrict__ slices, int n_slices,
                                                               const int64_t* __restrict__ item_start,
                                                               const float* __restrict__ vol,
@@ -442,19 +678,35 @@ __global__ __launch_bounds__(256) void fp_stream_batch_kernel(const BatchSlice* 
     it.out = sino + sl.sino_off + (static_cast<size_t>(det0) * n_angles + a) * sl.n;
     it.det_stride = static_cast<size_t>(n_angles) * sl.n;
     it.wa = w + static_cast<size_t>(a) * n_det_total + det0;
-    fp_stream_block<BatchTile, D, MAPS, TRIM>(it, lds);
+    fp_stream_block<BatchTile, D, MAPS, TRIM, PATCH>(it, lds);
   }
 }
 
 template <int D, bool MAPS>
 void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch, int det0,
             int angle0, int n_ang, int maps_a0, hipStream_t st) {
-  if (p.fp_variant == 5) {
+  if (MAPS && (p.fp_variant == 7 || p.fp_variant == 8) && p.n * p.n <= (1 << 29)) {
+    const int n_ublk = (p.n + BatchTile::kBins - 1) / BatchTile::kBins;
+    dim3 grid(n_ublk * n_ang, batch);
+    const size_t shmem = sizeof(float) * (1 + D) * kDirectPlane + sizeof(int2) * kMaxChunks;
+    if (p.fp_variant == 8)
+      hipLaunchKernelGGL((fp_direct_kernel<D, true>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
+                         p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    else
+      hipLaunchKernelGGL((fp_direct_kernel<D, false>), grid, dim3(256), shmem, st, vol, maps, stride, sino,
+                         p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    return;
+  }
+  if (p.fp_variant == 5 || p.fp_variant == 7 || p.fp_variant == 8 || p.fp_variant == 9) {
     const int n_ublk = (p.n + Tile32::kBins - 1) / Tile32::kBins;
     dim3 grid(n_ublk * n_ang, batch);
     const size_t shmem = sizeof(float) * D * Tile32::kTile + sizeof(int2) * kMaxChunks;
-    hipLaunchKernelGGL((fp_stream_kernel<Tile32, D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride,
-                       sino, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    if (p.fp_variant == 9)
+      hipLaunchKernelGGL((fp_stream_kernel<Tile32, D, MAPS, false, true>), grid, dim3(256), shmem, st, vol, maps,
+                         stride, sino, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
+    else
+      hipLaunchKernelGGL((fp_stream_kernel<Tile32, D, MAPS, false>), grid, dim3(256), shmem, st, vol, maps, stride,
+                         sino, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, n_ublk, angle0, maps_a0);
     return;
   }
   const int n_ublk = (p.n + Tile64::kBins - 1) / Tile64::kBins;
@@ -472,8 +724,28 @@ void launch(const Plan& p, const float* vol, const float* maps, int64_t stride, 
 template <int D, bool MAPS>
 void launch_batch(const BatchPlan& bp, const float* vol, const float* maps, float* sino, int det0, hipStream_t st) {
   const int64_t total = bp.h_item_start_fp.back();
-  const int64_t want = static_cast<int64_t>(bp.fp_wg_per_cu) * bp.n_cu;
+  // fp_wg_per_cu = 0: one workgroup per work item (a flat grid), else a persistent grid
+  const int64_t want = bp.fp_wg_per_cu > 0 ? static_cast<int64_t>(bp.fp_wg_per_cu) * bp.n_cu : total;
   const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, want)));
+  if (bp.fp_patch) {
+    const size_t shmem_p = sizeof(float) * D * BatchTile::kTile + sizeof(int2) * kMaxChunks;
+    hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS, false, true>), dim3(grid), dim3(256), shmem_p, st,
+                       bp.d_slices, bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles,
+                       bp.n_det, det0);
+    return;
+  }
+  if (MAPS && bp.fp_direct) {
+    const size_t shmem_d = sizeof(float) * (1 + D) * kDirectPlane + sizeof(int2) * kMaxChunks;
+    if (bp.fp_trim)
+      hipLaunchKernelGGL((fp_direct_batch_kernel<D, true>), dim3(grid), dim3(256), shmem_d, st, bp.d_slices,
+                         bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det,
+                         det0);
+    else
+      hipLaunchKernelGGL((fp_direct_batch_kernel<D, false>), dim3(grid), dim3(256), shmem_d, st, bp.d_slices,
+                         bp.n_slices, bp.d_item_start_fp, vol, maps, sino, bp.d_geom, bp.d_w, bp.n_angles, bp.n_det,
+                         det0);
+    return;
+  }
   const size_t shmem = sizeof(float) * D * BatchTile::kTile + sizeof(int2) * kMaxChunks;
   if (bp.fp_trim) {
     hipLaunchKernelGGL((fp_stream_batch_kernel<D, MAPS, true>), dim3(grid), dim3(256), shmem, st, bp.d_slices,

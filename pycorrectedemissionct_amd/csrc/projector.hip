@@ -847,7 +847,7 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 6, "fp_kernel must be in 0..6");
+    CHIP_CHECK_ARG(value >= 0 && value <= 9, "fp_kernel must be in 0..9");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }

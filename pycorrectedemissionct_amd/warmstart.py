@@ -113,8 +113,15 @@ class UNet:
         self.head.copy_(torch.as_tensor(np.asarray(d["head"], np.float32)))
         self.refresh()
 
-    def save(self, path: str) -> None:
-        np.savez(path, **self.state_dict())
+    def save(self, path: str, inference_only: bool = False) -> None:
+        """Weights as .npz; ``inference_only`` stores the conv weights rounded to the bfloat16
+        values inference uses (exactly the same network output, about half the compressed size)."""
+        d = self.state_dict()
+        if inference_only:
+            for k, c in self.convs.items():
+                bits = c.w16.view(torch.int16).cpu().numpy().astype(np.uint16).astype(np.uint32) << 16
+                d[f"{k}.w"] = bits.view(np.float32).reshape(d[f"{k}.w"].shape)
+        np.savez_compressed(path, **d)
 
     @classmethod
     def load(cls, path: str) -> "UNet":
@@ -289,22 +296,27 @@ def training_phantom(n: int, rng: np.random.Generator) -> np.ndarray:
 
 def make_pairs(n_images: int, n: int = 256, n_angles: int = 360, sirt_its: int = 20, pdhg_its: int = 2000,
                tv_lambda: float = 1e-2, peak_range=(1e2, 1e4), seed: int = 0, batch: int = 32,
-               data_term: str = "kl") -> dict:
+               data_term: str = "kl", det_angles=(np.pi / 2,), phantom: str = "training") -> dict:
     """(SIRT-``sirt_its``, PDHG-TV-``pdhg_its``) pairs of seeded phantoms, made on the GPU in batches.
 
     Each phantom has its own attenuation (mu_in = 0.02 (f > 0) + 0.005, mu_out = 0.05 (f > 0)),
-    one detector at 90 deg, Poisson counts at a peak drawn from ``peak_range``. Returns device
-    tensors x_sirt, x_ref, phantom [n_images, n, n] and the per-image data needed to re-solve.
+    detectors at ``det_angles``, Poisson counts at a peak drawn from ``peak_range``. ``phantom``:
+    "training" (ellipses + blobs, Voronoi cells, or both) or "ellipses" (30 ellipses + 5 Gaussian
+    blobs, the PDHG benchmark's family). Returns device tensors x_sirt, x_ref, phantom
+    [n_images, n, n] and the per-image data needed to re-solve.
     """
     rng = np.random.default_rng(seed)
     ang = testing.angles_uniform(n_angles)
     xs, xr, fs, items = [], [], [], []
     for i0 in range(0, n_images, batch):
         nb = min(batch, n_images - i0)
-        f = [training_phantom(n, rng) for _ in range(nb)]
+        if phantom == "ellipses":
+            f = [testing.phantom_ellipses(n, 30, rng, n_blobs=5) for _ in range(nb)]
+        else:
+            f = [training_phantom(n, rng) for _ in range(nb)]
         mus = [testing.xrf_attenuation(fi) for fi in f]
         A = _batch.BatchedProjectorXRF([n] * nb, ang, [m[0] for m in mus], [m[1] for m in mus],
-                                       angles_detectors_rad=[np.pi / 2], bp_mode="adjoint")
+                                       angles_detectors_rad=list(det_angles), bp_mode="adjoint")
         clean = A.fp(A.pack_vol(f))
         sinos = A.unpack_sino(clean)
         b = A.pack_sino([testing.poisson_data(s.cpu().numpy(), float(rng.uniform(*peak_range)), rng) for s in sinos])
@@ -443,12 +455,18 @@ def main() -> int:
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--save", default=None)
+    ap.add_argument("--detectors", type=int, default=1, help="1: one detector at 90 deg; 2: at +-90 deg")
+    ap.add_argument("--peak", type=float, nargs=2, default=(1e2, 1e4), help="Poisson peak range")
+    ap.add_argument("--phantom", choices=["training", "ellipses"], default="training")
+    ap.add_argument("--pair-batch", type=int, default=32)
     args = ap.parse_args()
 
     def log(msg):
         print(f"[warmstart] {msg}", file=sys.stderr, flush=True)
 
-    kw = dict(n=args.n, n_angles=args.angles, batch=32)
+    det = (np.pi / 2,) if args.detectors == 1 else (np.pi / 2, -np.pi / 2)
+    kw = dict(n=args.n, n_angles=args.angles, batch=args.pair_batch, det_angles=det, peak_range=tuple(args.peak),
+              phantom=args.phantom)
     t0 = time.perf_counter()
     losses = [float("nan")]
     if args.load:
@@ -456,9 +474,10 @@ def main() -> int:
         t1 = t2 = time.perf_counter()
     else:
         parts = []
-        for i0 in range(0, args.train, 32):
-            parts.append(make_pairs(min(32, args.train - i0), seed=1000 + i0, pdhg_its=args.pdhg_its, **kw))
-            log(f"training pairs {min(i0 + 32, args.train)}/{args.train} ({time.perf_counter() - t0:.0f} s)")
+        step = args.pair_batch
+        for i0 in range(0, args.train, step):
+            parts.append(make_pairs(min(step, args.train - i0), seed=1000 + i0, pdhg_its=args.pdhg_its, **kw))
+            log(f"training pairs {min(i0 + step, args.train)}/{args.train} ({time.perf_counter() - t0:.0f} s)")
         x_in = torch.cat([p["x_sirt"] for p in parts])
         x_tgt = torch.cat([p["x_ref"] for p in parts])
         del parts
@@ -468,7 +487,12 @@ def main() -> int:
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         if args.save:
-            net.save(args.save)
+            net.save(args.save, inference_only=True)
+            log(f"saved {args.save}; loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    if args.test <= 0:
+        print(json.dumps({"loss_first_last": [losses[0], losses[-1]], "s_pairs": t1 - t0, "s_training": t2 - t1,
+                          "params": net.n_params, "train_pairs": args.train}), flush=True)
+        return 0
     test = make_pairs(args.test, seed=99991, pdhg_its=args.ref_its, **kw)
     log(f"evaluating cold vs warm start against {args.ref_its}-iteration references")
     res = evaluate(net, test, max_its=args.eval_its)

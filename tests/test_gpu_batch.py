@@ -93,8 +93,12 @@ def test_autotune_changes_time_not_results(pkg):
     print("autotune:", res)
     assert res["fp_wg_per_cu"] in res["fp_us"] and res["bp_wg_per_cu"] in res["bp_us"]
     assert torch.equal(A.fp(x), f1) and torch.equal(A.bp(y), b1)
+    for key, val in (("fp_patch", 1), ("fp_direct", 1), ("fp_wg_per_cu", 0), ("bp_wg_per_cu", 0)):
+        A.plan.tune(key, val)  # other kernel shapes and flat grids: results equal to 2e-7 (patch: sum order)
+        assert float((A.fp(x) - f1).abs().max() / f1.abs().max()) < 1e-6
+        assert torch.equal(A.bp(y), b1)
     with pytest.raises(pkg._native.NativeError):
-        A.plan.tune("fp_wg_per_cu", 0)
+        A.plan.tune("fp_wg_per_cu", -1)
 
 
 def test_batch_rejects_sizes_not_multiple_of_4(pkg):
