@@ -311,6 +311,17 @@ int chip_fista_update(float* x, float* y, const float* grad, const float* tau, f
                       int has_lower, float lower, int has_upper, float upper, const float* x_mask, float momentum,
                       int64_t n, void* stream);
 
+/* Array-valued clipping (x_new.clip(lower_limit, upper_limit) with array limits,
+ * solvers.py:797-803, 975-982, 1185-1190): x = clip(x, lower, upper) * x_mask,
+ * each limit an array of n elements (lower / upper non-NULL), a scalar (has_* with
+ * a NULL array) or absent; then, when xbar is given, xbar = x + momentum (x - x_old)
+ * (PDHG's relaxation with momentum 1, FISTA's extrapolation). The fused update
+ * kernels run without limits and mask first, so the order of operations is the
+ * reference's. */
+int chip_clip_relax(float* x, const float* lower, const float* upper, int has_lower, float lower_s, int has_upper,
+                    float upper_s, const float* x_mask, float* xbar, const float* x_old, float momentum, int64_t n,
+                    void* stream);
+
 /* MLEM ratio r = b / max(ax + bg, eps_f32). solvers.py:793-797. */
 int chip_mlem_ratio(float* r, const float* b, const float* ax, const float* bg, int64_t n, void* stream);
 

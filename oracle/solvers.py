@@ -212,14 +212,29 @@ def mlem(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=
 
 
 def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None, lower=None, upper=None,
-         x_mask=None, b_mask=None, relaxation=0.95, regs=()):
-    """``regs``: ``oracle.regularizers.GradReg`` objects beside (or instead of) the 2-D TV ``tv_weight``."""
+         x_mask=None, b_mask=None, relaxation=0.95, regs=(), precondition=True, L=None, b_test_mask=None,
+         tolerance=None, track=False):
+    """``regs``: ``oracle.regularizers.GradReg`` objects beside (or instead of) the 2-D TV ``tv_weight``.
+
+    ``precondition=False``: scalar steps sigma = rho / L, tau = rho / (L + tau_reg) from the operator
+    norm ``L`` (``compute_Lipschitz_scaling``, ``solvers.py:120-158``). ``b_test_mask`` splits the
+    data mask as ``_initialize_b_masks`` does (``solvers.py:241-249``). With ``track`` (l2 data term)
+    returns ``(x, info)``: residual0 / residuals of ``||(b - A x) mask||^2`` per iteration and the
+    same for the test mask (``solvers.py:1145-1158, 1192-1204``); ``tolerance`` stops the loop as
+    the reference does."""
     b = np.asarray(b, dtype=F32)
+    if b_test_mask is not None:
+        bm = np.ones_like(b) if b_mask is None else np.asarray(b_mask, dtype=F32)
+        b_mask, b_test_mask = bm * (F32(1) - b_test_mask), b_test_mask * bm
     tau_reg = (4.0 if tv_weight is not None else 0.0) + sum(r.tau() for r in regs)
-    sigma, tau = diagonal_scaling(
-        A, At, b, x_shape, tau_reg=tau_reg, rho_sigma=relaxation, rho_tau=relaxation,
-        x_mask=x_mask, b_mask=b_mask,
-    )
+    if precondition:
+        sigma, tau = diagonal_scaling(
+            A, At, b, x_shape, tau_reg=tau_reg, rho_sigma=relaxation, rho_tau=relaxation,
+            x_mask=x_mask, b_mask=b_mask,
+        )
+    else:
+        sigma = np.full(b.shape, F32(relaxation / L), dtype=F32)
+        tau = np.full(x_shape, F32(relaxation / (L + tau_reg)), dtype=F32)
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
     xr = x.copy()
     if data_term == "l2":
@@ -232,7 +247,21 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
     p = np.zeros_like(b)
     q = np.zeros((2, *x_shape), dtype=F32) if tv_weight is not None else None
     qs = [np.zeros((r.nd, *x_shape), dtype=F32) for r in regs]
+
+    def res_norm(ax, mask):
+        r = (b - ax).astype(F32)
+        if mask is not None:
+            r = r * mask
+        return float(np.linalg.norm(r.ravel().astype(np.float64)) ** 2)
+
+    info = {"residuals": [], "residuals_cv": [], "iterations": 0}
+    if track:
+        ax0 = A(x)
+        info["residual0"] = res_norm(ax0, b_mask)
+        if b_test_mask is not None:
+            info["residual0_cv"] = res_norm(ax0, b_test_mask)
     for _ in range(iterations):
+        info["iterations"] += 1
         p = p + A(xr) * sigma
         if data_term == "l2":
             p = (p - sigma_data) * sigma1
@@ -254,4 +283,11 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
             x_new = x_new * x_mask
         xr = x_new + (x_new - x)
         x = x_new
-    return x
+        if track:
+            ax = A(x)
+            info["residuals"].append(res_norm(ax, b_mask))
+            if b_test_mask is not None:
+                info["residuals_cv"].append(res_norm(ax, b_test_mask))
+            if tolerance is not None and tolerance > info["residuals"][-1]:
+                break
+    return (x, info) if track else x

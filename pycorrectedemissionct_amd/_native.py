@@ -72,6 +72,8 @@ _SIGNATURES = {
         ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p,
     ],
     "chip_mlem_ratio": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_clip_relax": [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                        c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p],
     "chip_mlem_update": [
         c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
         ctypes.c_int64, c_void_p,

@@ -338,11 +338,47 @@ __global__ __launch_bounds__(kEwBlock) void mlem_update_kernel(float* __restrict
   }
 }
 
+// ---- array-valued limits ----------------------------------------------------------
+
+// x = clip(x, lower, upper) * mask, each limit an array (lo_a / hi_a), a scalar
+// (hl / hu without an array) or absent; then, with xbar, xbar = x + momentum (x - x_old)
+__global__ __launch_bounds__(kEwBlock) void clip_relax_kernel(float* __restrict__ x, const float* __restrict__ lo_a,
+                                                              const float* __restrict__ hi_a, int hl, float lo,
+                                                              int hu, float hi, const float* __restrict__ mask,
+                                                              float* __restrict__ xbar,
+                                                              const float* __restrict__ x_old, float momentum,
+                                                              int64_t n) {
+  GRID_STRIDE(i, n) {
+    float v = x[i];
+    if (lo_a) v = fmaxf(v, lo_a[i]);
+    else if (hl) v = fmaxf(v, lo);
+    if (hi_a) v = fminf(v, hi_a[i]);
+    else if (hu) v = fminf(v, hi);
+    if (mask) v = v * mask[i];
+    x[i] = v;
+    if (xbar) xbar[i] = v + momentum * (v - x_old[i]);
+  }
+}
+
 }  // namespace chip
 
 using namespace chip;
 
 extern "C" {
+
+int chip_clip_relax(float* x, const float* lower, const float* upper, int has_lower, float lower_s, int has_upper,
+                    float upper_s, const float* x_mask, float* xbar, const float* x_old, float momentum, int64_t n,
+                    void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(x != nullptr, "x is NULL");
+  CHIP_CHECK_ARG(n >= 0, "n must be >= 0");
+  CHIP_CHECK_ARG(xbar == nullptr || x_old != nullptr, "xbar needs x_old");
+  if (n == 0) return CHIP_OK;
+  hipLaunchKernelGGL(clip_relax_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), x, lower, upper, has_lower,
+                     lower_s, has_upper, upper_s, x_mask, xbar, x_old, momentum, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
 
 int chip_absmax(const float* v, int64_t n, float* out, void* stream) {
   clear_error();

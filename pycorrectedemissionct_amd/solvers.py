@@ -45,12 +45,35 @@ def _dev(x, like: torch.Tensor | None = None):
     return t
 
 
-def _scalar_limit(v):
-    if v is None:
-        return 0, 0.0
-    if np.ndim(v) != 0:
-        raise NotImplementedError("array-valued clipping limits are not supported by the fused kernels")
-    return 1, float(v)
+class _Limits:
+    """Clipping limits of a solve (``x.clip(lower_limit, upper_limit)``, ``solvers.py:797-803,
+    975-982, 1185-1190``). Scalars go into the fused update kernels; with an array-valued limit
+    (broadcast to x) the fused kernel runs without limits and mask and ``chip_clip_relax`` then
+    clips, masks and (PDHG, FISTA) extrapolates, in the reference's order."""
+
+    def __init__(self, lower, upper, like: torch.Tensor):
+        arr_lo = lower is not None and np.ndim(lower) != 0
+        arr_hi = upper is not None and np.ndim(upper) != 0
+        self.fused = not (arr_lo or arr_hi)
+        self.hl, self.lo = (0, 0.0) if (lower is None or arr_lo) else (1, float(lower))
+        self.hu, self.hi = (0, 0.0) if (upper is None or arr_hi) else (1, float(upper))
+        self.lo_t = _dev(lower, like).contiguous() if arr_lo else None
+        self.hi_t = _dev(upper, like).contiguous() if arr_hi else None
+
+    def kernel(self):
+        """(has_lower, lower, has_upper, upper) for a fused update kernel."""
+        return (self.hl, self.lo, self.hu, self.hi) if self.fused else (0, 0.0, 0, 0.0)
+
+    def kernel_mask(self, mask):
+        """The mask a fused update kernel applies (none when chip_clip_relax does)."""
+        return mask if self.fused else None
+
+    def finish(self, x: torch.Tensor, mask, xbar=None, x_old=None, momentum: float = 0.0) -> None:
+        if self.fused:
+            return
+        _native.call("chip_clip_relax", x.data_ptr(), _native.ptr(self.lo_t), _native.ptr(self.hi_t), self.hl,
+                     self.lo, self.hu, self.hi, _native.ptr(mask), _native.ptr(xbar), _native.ptr(x_old),
+                     float(momentum), x.numel(), _native.stream())
 
 
 def compute_diagonal_scaling(A_abs, At_abs, b: torch.Tensor, regs: Sequence, relaxation_sigma: float = 1.0,
@@ -346,8 +369,8 @@ class SART(Solver):
         if self.tolerance is not None:
             info.residual0 = float(torch.linalg.vector_norm(self.compute_residual(A, b_t, x, A_num_rows, b_mask)))
         rows = np.random.permutation(A_num_rows) if rows_sequence is None else np.asarray(rows_sequence)
-        hl, lo = _scalar_limit(lower_limit)
-        hu, hi = _scalar_limit(upper_limit)
+        lim = _Limits(lower_limit, upper_limit, x)
+        hl, lo, hu, hi = lim.kernel()
         n = int(x.shape[-1])
         batch = x.numel() // (n * n)
         res = torch.empty_like(b_rows[0])
@@ -363,7 +386,8 @@ class SART(Solver):
                 upd = _arrays.to_dev(At(res, a))[0].contiguous()
                 # x = clip(x - At_a(res) tau_a) * mask, with res of the opposite sign
                 _native.call("chip_sirt_update", x.data_ptr(), upd.data_ptr(), None, tau[a].data_ptr(), hl, lo, hu,
-                             hi, _native.ptr(x_mask_t), n, batch, st)
+                             hi, _native.ptr(lim.kernel_mask(x_mask_t)), n, batch, st)
+                lim.finish(x, x_mask_t)
             if self.tolerance is not None:
                 info.residuals[ii] = float(torch.linalg.vector_norm(
                     self.compute_residual(A, b_t, x, A_num_rows, b_mask)))
@@ -451,8 +475,9 @@ class FISTA(Solver):
             x = x * x_mask_t
         y = x.clone()
         t = 1.0
-        hl, lo = _scalar_limit(lower_limit)
-        hu, hi = _scalar_limit(upper_limit)
+        lim = _Limits(lower_limit, upper_limit, x)
+        hl, lo, hu, hi = lim.kernel()
+        x_old = None if lim.fused else torch.empty_like(x)
         info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
         track = b_test_mask is not None or self.tolerance is not None
         if track:
@@ -469,8 +494,11 @@ class FISTA(Solver):
             if self.restart_period is not None and (ii + 1) % self.restart_period == 0:
                 t_new = 1.0
             momentum = (t - 1.0) / t_new
+            if x_old is not None:
+                x_old.copy_(x)
             _native.call("chip_fista_update", x.data_ptr(), y.data_ptr(), grad.data_ptr(), tau_ptr, tau_s, l1w, hl, lo,
-                         hu, hi, _native.ptr(x_mask_t), momentum, x.numel(), st)
+                         hu, hi, _native.ptr(lim.kernel_mask(x_mask_t)), momentum, x.numel(), st)
+            lim.finish(x, x_mask_t, y, x_old, momentum)
             t = t_new
             if track:
                 Ax = _arrays.to_dev(A(x))[0]
@@ -510,8 +538,8 @@ class MLEM(Solver):
         x = x.contiguous()
         self.data_term.assign_data(b_t)
         bg = self.data_term.background_dev(b_t)
-        hl, lo = _scalar_limit(lower_limit)
-        hu, hi = _scalar_limit(upper_limit)
+        lim = _Limits(lower_limit, upper_limit, x)
+        hl, lo, hu, hi = lim.kernel()
         info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
         track = b_test_mask is not None or self.tolerance is not None
         if track:
@@ -526,7 +554,8 @@ class MLEM(Solver):
                          b_t.numel(), st)
             upd = _arrays.to_dev(A.T(ratio))[0]
             _native.call("chip_mlem_update", x.data_ptr(), upd.data_ptr(), tau.data_ptr(), hl, lo, hu, hi,
-                         _native.ptr(x_mask), x.numel(), st)
+                         _native.ptr(lim.kernel_mask(x_mask)), x.numel(), st)
+            lim.finish(x, x_mask)
         return _arrays.from_dev(x, host), info
 
 
@@ -567,8 +596,8 @@ class SIRT(Solver):
         x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
         self.data_term.assign_data(b_t, sigma)
         bg = self.data_term.background_dev(b_t)
-        hl, lo = _scalar_limit(lower_limit)
-        hu, hi = _scalar_limit(upper_limit)
+        lim = _Limits(lower_limit, upper_limit, x)
+        hl, lo, hu, hi = lim.kernel()
         info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
         track = b_test_mask is not None or self.tolerance is not None
         if track:
@@ -597,7 +626,8 @@ class SIRT(Solver):
             for cfg, q_r in grad_regs:
                 self._grad_primal(cfg, upd, q_r, True, st)
             _native.call("chip_sirt_update", x.data_ptr(), upd.data_ptr(), _native.ptr(q), tau.data_ptr(), hl, lo, hu,
-                         hi, _native.ptr(x_mask_t), n, batch, st)
+                         hi, _native.ptr(lim.kernel_mask(x_mask_t)), n, batch, st)
+            lim.finish(x, x_mask_t)
         return _arrays.from_dev(x, host), info
 
 
@@ -626,8 +656,12 @@ class PDHG(Solver):
             raise ValueError(f'Unknown data term: "{data_term}", accepted terms are: "l2" | "l1" | "kl".')
         return cp.deepcopy(data_term)
 
-    def prepare(self, A, b, x0=None, x_mask=None, b_mask=None, b_test_mask=None):
-        """Device state of a solve: dict with b, masks, sigma, tau, x, x_relax, p, q, background."""
+    def prepare(self, A, b, x0=None, x_mask=None, b_mask=None, b_test_mask=None, precondition: bool = True):
+        """Device state of a solve: dict with b, masks, sigma, tau, x, x_relax, p, q, background.
+
+        ``precondition=False`` (or an operator without ``absolute()``) takes the scalar step
+        sizes of ``compute_Lipschitz_scaling`` (``solvers.py:120-158, 1127-1131``), held as
+        constant arrays for the fused kernels."""
         b_t, host = _arrays.to_dev(b)
         b_t = b_t.clone()
         b_mask = _dev(b_mask, b_t)
@@ -636,16 +670,26 @@ class PDHG(Solver):
         tv_w = self._tv_weight()
         if not isinstance(self.data_term, (data_terms.DataFidelity_l2, data_terms.DataFidelity_KL)):
             raise NotImplementedError(f"data term {self.data_term.info()} is not supported by the fused HIP kernels")
-        try:
-            At_abs = A.T.absolute()
-            A_abs = A.absolute()
-        except AttributeError:
-            raise NotImplementedError("PDHG without diagonal preconditioning is not supported") from None
+        if precondition:
+            try:
+                At_abs = A.T.absolute()
+                A_abs = A.absolute()
+            except AttributeError:
+                print(A)
+                print("WARNING: Turning off preconditioning because system matrix does not support absolute")
+                precondition = False
         x_mask_t = None if x_mask is None else _arrays.to_dev(x_mask)[0]
-        sigma, tau, x_shape, _ = compute_diagonal_scaling(A_abs, At_abs, b_t, regs=self.regularizer,
-                                                          relaxation_tau=self.relaxation,
-                                                          relaxation_sigma=self.relaxation, x_mask=x_mask_t,
-                                                          b_mask=b_mask)
+        if precondition:
+            sigma, tau, x_shape, _ = compute_diagonal_scaling(A_abs, At_abs, b_t, regs=self.regularizer,
+                                                              relaxation_tau=self.relaxation,
+                                                              relaxation_sigma=self.relaxation, x_mask=x_mask_t,
+                                                              b_mask=b_mask)
+        else:
+            sig, tau_s, x_shape, _ = compute_Lipschitz_scaling(A, b_t, regs=self.regularizer,
+                                                               relaxation_sigma=self.relaxation,
+                                                               relaxation_tau=self.relaxation)
+            sigma = torch.full_like(b_t, float(sig))
+            tau = torch.full(x_shape, float(tau_s), dtype=torch.float32, device=b_t.device)
         x_mask_t = _dev(x_mask_t, tau)
         x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
         n = int(x.shape[-1])
@@ -662,9 +706,10 @@ class PDHG(Solver):
     def iterate(self, A, s: dict, iterations: int, lower_limit=None, upper_limit=None, info=None):
         """Run ``iterations`` PDHG iterations on the device state ``s`` (no host synchronisation)."""
         st = _native.stream()
-        hl, lo = _scalar_limit(lower_limit)
-        hu, hi = _scalar_limit(upper_limit)
         b, p, q, x, xr = s["b"], s["p"], s["q"], s["x"], s["x_relax"]
+        lim = _Limits(lower_limit, upper_limit, x)
+        hl, lo, hu, hi = lim.kernel()
+        x_old = None if lim.fused else torch.empty_like(x)
         dual = "chip_pdhg_dual_kl" if s["kl"] else "chip_pdhg_dual_l2"
         track = info is not None and (s["b_test_mask"] is not None or self.tolerance is not None)
         for ii in range(iterations):
@@ -680,8 +725,12 @@ class PDHG(Solver):
             upd = _arrays.to_dev(A.T(p))[0]
             for cfg, q_r in s["grad_regs"]:
                 self._grad_primal(cfg, upd, q_r, False, st)
+            if x_old is not None:
+                x_old.copy_(x)
             _native.call("chip_pdhg_primal", x.data_ptr(), xr.data_ptr(), upd.data_ptr(), _native.ptr(q),
-                         s["tau"].data_ptr(), hl, lo, hu, hi, _native.ptr(s["x_mask"]), s["n"], s["batch"], st)
+                         s["tau"].data_ptr(), hl, lo, hu, hi, _native.ptr(lim.kernel_mask(s["x_mask"])), s["n"],
+                         s["batch"], st)
+            lim.finish(x, s["x_mask"], xr, x_old, 1.0)
             if track:
                 Axn = _arrays.to_dev(A(x))[0]
                 if self._track(info, ii, Axn, b, s["b_mask"], s["b_test_mask"]):
@@ -690,9 +739,8 @@ class PDHG(Solver):
 
     def __call__(self, A, b, iterations: int, x0=None, lower_limit=None, upper_limit=None, x_mask=None, b_mask=None,
                  b_test_mask=None, precondition: bool = True):
-        if not precondition:
-            raise NotImplementedError("PDHG without diagonal preconditioning is not supported")
-        s = self.prepare(A, b, x0=x0, x_mask=x_mask, b_mask=b_mask, b_test_mask=b_test_mask)
+        s = self.prepare(A, b, x0=x0, x_mask=x_mask, b_mask=b_mask, b_test_mask=b_test_mask,
+                         precondition=precondition)
         info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
         if s["b_test_mask"] is not None or self.tolerance is not None:
             self._init_residuals(info, A, s["x"], s["b"], s["b_mask"], s["b_test_mask"], s["sigma"])

@@ -245,3 +245,79 @@ def test_sirt_at_the_reference_cpu_configuration(pkg):
                        b, (n, n), 200)
     print(f"SIRT-200 128^2 x 256: rel_max {rel_max(ref200, x200):.2e} rel_l2 {rel_l2(ref200, x200):.2e}")
     assert rel_max(ref200, x200) < TOL
+
+
+def test_pdhg_without_preconditioning(pkg, xrf_problem):
+    """``precondition=False``: scalar steps from the power-method norm (``compute_Lipschitz_scaling``,
+    ``solvers.py:120-158, 1127-1131``; the same ``np.random.rand`` start vector) against the oracle."""
+    p = xrf_problem
+    np.random.seed(21)
+    x, info = pkg.PDHG(data_term="l2", regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(
+        p["prj"], p["b"], iterations=30, precondition=False)
+    np.random.seed(21)
+    L = osol.power_method(p["A"], p["At"], np.random.rand(*np.shape(p["b"])).astype(np.float32))
+    ref = osol.pdhg(p["A"], p["At"], p["b"], p["shape"], 30, data_term="l2", tv_weight=1e-2, precondition=False, L=L)
+    e = rel_max(ref, x)
+    print(f"PDHG-l2-TV Lipschitz 30 it: rel_max {e:.2e} (L = {L:.4f})")
+    assert e < TOL and info.iterations == 30
+
+
+def test_residual_tracking_test_mask_and_tolerance(pkg, xrf_problem):
+    """``SolutionInfo`` bookkeeping (``solvers.py:1145-1158, 1192-1204``): residual0 and the per-iteration
+    residuals of the l2 data term over the reconstruction mask and over ``b_test_mask`` (split from the
+    data mask as ``_initialize_b_masks`` does, ``solvers.py:333-343``) against the oracle, and the
+    tolerance stopping at the oracle's iteration."""
+    p = xrf_problem
+    b = p["b"]
+    test = (np.random.default_rng(22).random(b.shape) < 0.1).astype(np.float32)
+    its = 40
+    ref_x, ref = osol.pdhg(p["A"], p["At"], b, p["shape"], its, data_term="l2", tv_weight=1e-2, b_test_mask=test,
+                           track=True)
+    x, info = pkg.PDHG(data_term="l2", regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(
+        p["prj"], b, iterations=its, b_test_mask=test)
+    res, res_cv = np.asarray(ref["residuals"]), np.asarray(ref["residuals_cv"])
+    e_x = rel_max(ref_x, x)
+    e_r = rel_max(res, info.residuals)
+    e_cv = rel_max(res_cv, info.residuals_cv)
+    e_0 = abs(info.residual0 - ref["residual0"]) / ref["residual0"]
+    e_0cv = abs(info.residual0_cv - ref["residual0_cv"]) / ref["residual0_cv"]
+    print(f"tracking: x {e_x:.2e}, residuals {e_r:.2e}, cv {e_cv:.2e}, residual0 {e_0:.2e}, residual0_cv {e_0cv:.2e}")
+    assert e_x < TOL and e_r < TOL and e_cv < TOL and e_0 < TOL and e_0cv < TOL
+    assert np.allclose(info.residuals_rel, info.residuals / info.residual0)
+    # a tolerance between two of the oracle's residuals stops both at the same iteration
+    k = next(i for i in range(15, its) if res[i - 1] > res[i] * (1 + 1e-3))
+    tol = float(0.5 * (res[k - 1] + res[k]))
+    _, ref_t = osol.pdhg(p["A"], p["At"], b, p["shape"], its, data_term="l2", tv_weight=1e-2, b_test_mask=test,
+                         track=True, tolerance=tol)
+    _, info_t = pkg.PDHG(data_term="l2", regularizer=pkg.regularizers.Regularizer_TV2D(1e-2), tolerance=tol)(
+        p["prj"], b, iterations=its, b_test_mask=test)
+    print(f"tolerance {tol:.4g}: GPU stops after {info_t.iterations}, oracle after {ref_t['iterations']} (k = {k})")
+    assert info_t.iterations == ref_t["iterations"] == k + 1
+
+
+def test_array_valued_limits(pkg, xrf_problem):
+    """``x.clip(lower_limit, upper_limit)`` with array limits (``solvers.py:797-803, 975-982, 1185-1190``) in
+    PDHG, SIRT, MLEM and FISTA, with a reconstruction mask, against the oracle."""
+    p = xrf_problem
+    rng = np.random.default_rng(23)
+    lo = (rng.random(p["shape"]) * 0.05 - 0.02).astype(np.float32)
+    hi = (0.5 + 0.3 * rng.random(p["shape"])).astype(np.float32)
+    x_mask = np.ones(p["shape"], np.float32)
+    x_mask[:3] = 0
+    A, At, b, sh = p["A"], p["At"], p["b"], p["shape"]
+    runs = {
+        "PDHG-KL": (pkg.PDHG(data_term="kl", regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(
+            p["prj"], b, 20, lower_limit=lo, upper_limit=hi, x_mask=x_mask)[0],
+            osol.pdhg(A, At, b, sh, 20, data_term="kl", tv_weight=1e-2, lower=lo, upper=hi, x_mask=x_mask)),
+        "SIRT": (pkg.SIRT()(p["prj"], b, 20, lower_limit=lo, upper_limit=0.6, x_mask=x_mask)[0],
+                 osol.sirt(A, At, b, sh, 20, lower=lo, upper=0.6, x_mask=x_mask)),
+        "MLEM": (pkg.MLEM()(p["prj"], b, 20, lower_limit=0.01, upper_limit=hi, x_mask=x_mask)[0],
+                 osol.mlem(A, At, b, sh, 20, lower=0.01, upper=hi, x_mask=x_mask)),
+        "FISTA": (pkg.FISTA()(p["prj"], b, 20, lower_limit=lo, upper_limit=hi, x_mask=x_mask)[0],
+                  osol.fista(A, At, b, sh, 20, lower=lo, upper=hi, x_mask=x_mask)),
+    }
+    for name, (got, ref) in runs.items():
+        e = rel_max(ref, got)
+        print(f"{name} array limits 20 it: rel_max {e:.2e}")
+        assert e < TOL, name
+        assert np.all(got[3:] <= np.maximum(hi[3:], 0.6) + 1e-6) and np.all(got[:3] == 0)
