@@ -383,6 +383,18 @@ def run_otf(args) -> dict:
     mu_in, mu_out = pc.testing.xrf_attenuation(f, a_in=0.01, a_bg=0.0025, a_out=0.025)
     otf = pc.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, bp_mode="adjoint", maps_mode="otf")
     x = torch.as_tensor(f, device="cuda")
+    budgets = {}
+    for mib in args.otf_budgets:
+        otf._plan().tune("otf_budget_mib", mib)
+        otf.fp(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        otf.fp(x)
+        torch.cuda.synchronize()
+        budgets[mib] = 1e3 * (time.perf_counter() - t0)
+        log(f"otf fp with a {mib} MiB scratch budget: {budgets[mib]:.1f} ms")
+    best = min(budgets, key=budgets.get)
+    otf._plan().tune("otf_budget_mib", best)
     y = otf.fp(x)
     otf.bp(y)
     torch.cuda.synchronize()
@@ -421,6 +433,12 @@ def run_otf(args) -> dict:
         "config": {"workload": "1024x1024, 1440 angles, 1 detector at 90 deg, maps rebuilt per angle block "
                                "(1 GiB scratch) instead of 6 GB of precomputed maps",
                    "otf_fp_ms": 1e3 * t_fp, "otf_bp_ms": 1e3 * t_bp, "precomputed_fp_ms": 1e3 * t_fp_pre,
+                   "otf_over_precomputed_fp": t_fp / t_fp_pre, "otf_fp_ms_by_budget_mib": budgets,
+                   "otf_budget_mib": best,
+                   # samples the on-the-fly fp computes: the map builder's rotated-frame scan (two beams of
+                   # the padded side squared per angle), its composition (two beams per map pixel) and the
+                   # projection's own samples (one per pixel and angle)
+                   "otf_fp_gsamples_per_s": a * (2 * otf._plan().n_pad ** 2 + 2 * n * n + n * n) / t_fp / 1e9,
                    "otf_fp_equals_precomputed": same, "otf_bp_rel_diff": dv,
                    "precomputed_maps_gb": 4.0 * a * n * n / 1e9, "unet_ms_per_slice": 1e3 * t_unet,
                    "unet_batch": 8},
@@ -445,6 +463,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--fp-reps", type=int, default=20)
+    ap.add_argument("--otf-budgets", type=int, nargs="+", default=[1024, 2048, 4096],
+                    help="on-the-fly scratch budgets (MiB) to time; the fastest is kept")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (slices would shard across GPUs with "

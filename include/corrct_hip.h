@@ -176,7 +176,7 @@ int chip_bp_angle(const chip_plan* plan, const float* line, const float* maps, f
  * mu_out (batch slices of n*n each, either may be NULL) instead of read from
  * a precomputed [A][D][n][n] set. Angles run in blocks; each block's maps are
  * built by the same kernels as chip_att_maps into plan-owned scratch bounded
- * by the "otf_budget_mib" tune key (default 1024), then projected, so the
+ * by the "otf_budget_mib" tune key (default 4096), then projected, so the
  * results equal chip_fp / chip_bp over chip_att_maps' output. For
  * configurations whose maps do not fit in memory (the fused on-the-fly mode
  * of AttenuationVolume.compute_maps, physics/attenuation.py:122-180, with

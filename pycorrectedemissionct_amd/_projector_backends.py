@@ -29,6 +29,9 @@ class HipPlan:
 
     def __init__(self, n: int, angles_rad_f32, det_angles_rad=(np.pi / 2,), w_det=None, w_ang=None):
         self.n = int(n)
+        # side of compute_attenuation's padded frame: N + 2 ceil((ceil(sqrt(2) N) - N) / 2)
+        # (_projector_backends.py:75-77)
+        self.n_pad = self.n + 2 * int(np.ceil((np.ceil(np.sqrt(2.0 * self.n * self.n)) - self.n) / 2.0))
         self.angles = np.ascontiguousarray(np.asarray(angles_rad_f32, dtype=np.float32).reshape(-1))
         self.det_angles = np.ascontiguousarray(np.asarray(det_angles_rad, dtype=np.float64).reshape(-1))
         self.n_angles = len(self.angles)

@@ -13,15 +13,16 @@
 // rotated frame top to bottom in bands of 8 rows, sampling mu_p at the
 // rotated positions and carrying the trapezoid prefix sum; the running sum is
 // the same float32 sequence numpy's cumsum performs.
-// Kernel 2 (att_compose_kernel): one thread per map pixel samples the
-// beams' prefix-sum images at the back-rotated position, takes exp and
+// Kernel 2 (att_compose_kernel): each thread takes two map pixels, samples the
+// beams' prefix-sum images at the back-rotated positions, takes exp and
 // multiplies in-beam by out-beam.
 //
-// The order-1 blend follows scikit-image 0.18.3's compiled warp: the
-// fractions r - floor(r) are float32, the products with the right-hand
-// neighbours are float32, the rest of the blend is double, rounded to
-// float32 -- so the maps match a float32 scikit-image evaluation to the
-// last bit except where exp differs by an ulp.
+// The order-1 blend: by default in float32 (maps within ~1e-6 of scikit-image's,
+// the bar is 1e-5); with chip_plan_tune "maps_exact" = 1 as scikit-image 0.18.3's
+// compiled warp blends: the fractions r - floor(r) are float32, the products
+// with the right-hand neighbours are float32, the rest of the blend is double,
+// rounded to float32 -- so the maps match a float32 scikit-image evaluation to
+// the last bit except where exp differs by an ulp.
 #include <algorithm>
 
 #include "common.hpp"
@@ -41,6 +42,10 @@ __device__ __forceinline__ double dadd_rn(double a, double b) { return a + b; }
 
 namespace chip {
 
+// EXACT = false: the same taps blended in float32 (t00 + dc (t01 - t00), then the
+// rows): within a few float32 ulps of the exact blend, which keeps the maps within
+// 1e-6 of scikit-image's (the 1e-5 bar) at a fraction of the FP64 cost.
+template <bool EXACT = true>
 __device__ __forceinline__ float warp_sample(const float* __restrict__ img, int side, int off, float xs, float ys) {
   // img is a side x side image embedded at offset `off` inside a zero frame
   // (the padded frame of compute_attenuation): pixel (r, c) of the frame is
@@ -57,11 +62,54 @@ __device__ __forceinline__ float warp_sample(const float* __restrict__ img, int 
     return (r >= 0 && r < side && c >= 0 && c < side) ? img[static_cast<size_t>(r) * side + c] : 0.f;
   };
   const float tl = px(r0, c0), tr = px(r0, c1), bl = px(r1, c0), br = px(r1, c1);
+  if (!EXACT) {
+    const float top = fmaf(dcf, tr - tl, tl);
+    const float bot = fmaf(dcf, br - bl, bl);
+    return fmaf(drf, bot - top, top);
+  }
   const double dc = dcf, dr = drf;
   // explicit roundings: no FMA contraction, as in the x86-64 build of scikit-image
   const double top = dadd_rn(dmul_rn(1.0 - dc, static_cast<double>(tl)), static_cast<double>(mul_rn(dcf, tr)));
   const double bot = dadd_rn(dmul_rn(1.0 - dc, static_cast<double>(bl)), static_cast<double>(mul_rn(dcf, br)));
   return static_cast<float>(dadd_rn(dmul_rn(1.0 - dr, top), dmul_rn(dr, bot)));
+}
+
+// The float32 blend with each row's two taps read as one 8-byte load (pixels c0 and
+// c0 + 1; when xs is an integer the right tap's weight dc is 0, so reading c0 + 1
+// instead of ceil(xs) changes nothing, and likewise for the rows).
+__device__ __forceinline__ float warp_sample_pairs(const float* __restrict__ img, int side, int off, float xs,
+                                                   float ys) {
+  const float c0f = floorf(xs);
+  const float r0f = floorf(ys);
+  const float dc = xs - c0f;
+  const float dr = ys - r0f;
+  const int c0 = static_cast<int>(c0f) - off;
+  const int r0 = static_cast<int>(r0f) - off;
+  auto row = [&](int r, float& a, float& b) {
+    a = b = 0.f;
+    if (static_cast<unsigned>(r) >= static_cast<unsigned>(side)) return;
+    const float* p = img + static_cast<size_t>(r) * side;
+    if (c0 >= 0 && c0 + 1 < side) {
+      float2 v;
+      __builtin_memcpy(&v, p + c0, sizeof(float2));
+      a = v.x;
+      b = v.y;
+    } else {
+      if (static_cast<unsigned>(c0) < static_cast<unsigned>(side)) a = p[c0];
+      if (static_cast<unsigned>(c0 + 1) < static_cast<unsigned>(side)) b = p[c0 + 1];
+    }
+  };
+  float tl, tr, bl, br;
+  row(r0, tl, tr);
+  row(r0 + 1, bl, br);
+  const float top = fmaf(dc, tr - tl, tl);
+  const float bot = fmaf(dc, br - bl, bl);
+  return fmaf(dr, bot - top, top);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ float warp_any(const float* __restrict__ img, int side, int off, float xs, float ys) {
+  return EXACT ? warp_sample<true>(img, side, off, xs, ys) : warp_sample_pairs(img, side, off, xs, ys);
 }
 
 __device__ __forceinline__ void affine_apply(const Affine& m, float x, float y, float& xs, float& ys) {
@@ -82,6 +130,9 @@ constexpr int kScanCols = 8;   // columns per wave
 constexpr int kScanBand = 8;   // rows per band (lanes per column)
 constexpr int kScanWaves = 4;  // waves per workgroup
 
+// Samples whose taps all lie outside the unpadded slice are 0 (the rotated slice
+// fills about half of the padded frame): they skip the gathers and the blend.
+template <bool EXACT>
 __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__ mu_in, const float* __restrict__ mu_out,
                                                        const Affine* __restrict__ rot_in,
                                                        const Affine* __restrict__ rot_out, int n_in, int n_beams,
@@ -100,29 +151,94 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
   const float xf = static_cast<float>(min(x, np - 1));
   float* out = cum + static_cast<size_t>(beam) * np * np + x;
   float xs, ys;
+  // a sample reads the slice only inside (pad - 1, pad + n) in both coordinates
+  const float lo_f = static_cast<float>(pad - 1), hi_f = static_cast<float>(pad + n);
+  auto sample = [&](float sx, float sy) -> float {
+    if (sx > lo_f && sx < hi_f && sy > lo_f && sy < hi_f) return warp_any<EXACT>(mu, n, pad, sx, sy);
+    return 0.f;
+  };
+  auto band_sample = [&](int r0) -> float {  // this lane's sample of the band starting at row r0
+    float bx, by;
+    affine_apply(m, xf, static_cast<float>(min(r0 + row, np - 1)), bx, by);
+    return sample(bx, by);
+  };
   // R[np - 1]: the roll wraps it onto row 0
   affine_apply(m, xf, static_cast<float>(np - 1), xs, ys);
-  float last = warp_sample(mu, n, pad, xs, ys);  // R of the row before the band, per column
+  float last = sample(xs, ys);  // R of the row before the band, per column
   float carry = 0.f;                              // running sum at the row before the band
-  const int src_prev = (lane & ~(kScanBand - 1)) + ((row + kScanBand - 1) & (kScanBand - 1));
-  const int src_last = (lane & ~(kScanBand - 1)) + (kScanBand - 1);
-  for (int r0 = 0; r0 < np; r0 += kScanBand) {
+  // lane-to-lane moves within the 8-lane column groups run on DPP (VALU, no LDS round trip):
+  // row_shr:1 hands each lane its predecessor's value (row r - 1 of the column; unused at row 0),
+  // row_ror:9 hands row 0 the value of row 7 of its own group (a 16-lane DPP row holds two groups)
+  auto from_prev = [](float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+  };
+  auto from_row7 = [](float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x129, 0xF, 0xF, false));
+  };
+  // Rows of this wave's 8 columns that can be read later: the compose kernel reads a plane
+  // only within 1.5 px of where the slice lies in the rotated frame, and R is 0 more than a
+  // pixel outside it. Per column the rows whose samples fall within 3 px of the slice form an
+  // interval; the wave scans the band-aligned union of its columns' intervals. Before it,
+  // every R is 0, so the running sum is C[0] = (0 + R[np - 1]) / 2 (the roll) and R[r - 1]
+  // is 0; rows after it are never read and are not written.
+  int r_first = 0, r_last = np - 1;
+  {
+    const double lo_m = static_cast<double>(pad) - 4.0, hi_m = static_cast<double>(pad + n) + 3.0;
+    double t = 0.0, b = static_cast<double>(np - 1);
+    auto clamp_rows = [&](double a_r, double c0) {  // lo_m < a_r r + c0 < hi_m
+      if (fabs(a_r) < 1e-12) {
+        if (!(c0 > lo_m && c0 < hi_m)) b = -1.0;
+        return;
+      }
+      double r1 = (lo_m - c0) / a_r, r2 = (hi_m - c0) / a_r;
+      if (r1 > r2) { const double tmp = r1; r1 = r2; r2 = tmp; }
+      t = fmax(t, r1);
+      b = fmin(b, r2);
+    };
+    const double xd = static_cast<double>(min(x, np - 1));
+    clamp_rows(m.m01, m.m00 * xd + m.m02);
+    clamp_rows(m.m11, m.m10 * xd + m.m12);
+    int lo_r = b < t ? np : static_cast<int>(floor(t)) - 1;
+    int hi_r = b < t ? -1 : static_cast<int>(ceil(b)) + 1;
+    // union over the wave's columns (lanes 8 c + row; all lanes of a column agree)
+#pragma unroll
+    for (int sh = 8; sh < 64; sh <<= 1) {
+      lo_r = min(lo_r, __shfl_xor(lo_r, sh, 64));
+      hi_r = max(hi_r, __shfl_xor(hi_r, sh, 64));
+    }
+    r_first = max(lo_r, 0) & ~(kScanBand - 1);
+    r_last = min(hi_r, np - 1);
+  }
+  if (r_last < r_first) return;  // no column of this wave meets the slice (uniform over the wave)
+  if (r_first > 0) {
+    carry = (0.f + last) * 0.5f;  // C[0] with R[0] = 0; rows 1 .. r_first - 1 add 0
+    last = 0.f;                   // R[r_first - 1]
+  }
+  // the samples of the next kAhead bands are in flight while this band is summed: only the
+  // running sum is carried from band to band, the gathers do not depend on it
+  constexpr int kAhead = 4;
+  float ahead[kAhead];
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) ahead[k] = band_sample(r_first + k * kScanBand);
+  for (int r0 = r_first; r0 <= r_last; r0 += kScanBand) {
     const int r = r0 + row;
-    affine_apply(m, xf, static_cast<float>(min(r, np - 1)), xs, ys);
-    const float cur = warp_sample(mu, n, pad, xs, ys);
+    const float cur = ahead[0];
+#pragma unroll
+    for (int k = 0; k + 1 < kAhead; ++k) ahead[k] = ahead[k + 1];
+    ahead[kAhead - 1] = band_sample(r0 + kAhead * kScanBand);
     // R[r - 1]: the previous lane of the column, or the last row of the previous band
-    const float up = __shfl(cur, src_prev, 64);
+    const float up = from_prev(cur);
     const float prev = (row == 0) ? last : up;
     const float v = (cur + prev) * 0.5f;
     float run = (row == 0) ? ((r0 == 0) ? v : carry + v) : v;
 #pragma unroll
     for (int k = 1; k < kScanBand; ++k) {
-      const float before = __shfl(run, src_prev, 64);
+      const float before = from_prev(run);
       if (row == k) run = before + v;
     }
-    if (live && r < np) out[static_cast<size_t>(r) * np] = run;
-    carry = __shfl(run, src_last, 64);
-    last = __shfl(cur, src_last, 64);
+    if (live && r <= r_last) out[static_cast<size_t>(r) * np] = run;
+    carry = from_row7(run);  // used by row 0 only
+    last = from_row7(cur);
   }
 }
 
@@ -131,41 +247,69 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
 // (if has_in), out = n_in + (la * D + d) * n_sub + k (if has_out) for the
 // detector's nsub[d] sub-angles k, whose exp(-.) are averaged in float32 in
 // order (physics/attenuation.py:110-119).
+template <bool EXACT>
 __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restrict__ cum,
                                                           const Affine* __restrict__ rot_in_back,
                                                           const Affine* __restrict__ rot_out_back, int has_in,
                                                           int has_out, int n_in, int a0, int n_det, int n, int pad,
                                                           int np, float* __restrict__ maps, int out_a0, int n_sub,
                                                           const int* __restrict__ nsub) {
-  // each wave takes an 8 x 8 quadrant of the 16 x 16 tile: its back-rotated
-  // samples form a compact patch of the prefix-sum planes
+  // a workgroup owns a 16 x 32 tile, each thread two pixels 16 rows apart (their gathers
+  // are in flight together); each wave takes an 8 x 8 quadrant of both 16 x 16 halves,
+  // so its back-rotated samples form compact patches of the prefix-sum planes
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  const int i = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  const int i0 = blockIdx.y * 32 + (wave >> 1) * 8 + (lane >> 3);
   const int ad = blockIdx.z;  // chunk-local (la, d)
-  if (i >= n || j >= n) return;
+  if (j >= n || i0 >= n) return;
   const int la = ad / n_det;
   const int d = ad - la * n_det;
   const int a = a0 + la;
-  const float xf = static_cast<float>(j + pad);
-  const float yf = static_cast<float>(i + pad);
   const size_t npp = static_cast<size_t>(np) * np;
-  float e_in = 1.f, e_out = 1.f;
-  float xs, ys;
+  const float xf = static_cast<float>(j + pad);
+  constexpr int kPix = 2;
+  float e_in[kPix], e_out[kPix];
+#pragma unroll
+  for (int h = 0; h < kPix; ++h) e_in[h] = e_out[h] = 1.f;
   if (has_in) {
-    affine_apply(rot_in_back[a], xf, yf, xs, ys);
-    e_in = expf(-warp_sample(cum + static_cast<size_t>(la) * npp, np, 0, xs, ys));
+    const Affine m = rot_in_back[a];
+    float v[kPix];
+#pragma unroll
+    for (int h = 0; h < kPix; ++h) {
+      float xs, ys;
+      affine_apply(m, xf, static_cast<float>(min(i0 + 16 * h, n - 1) + pad), xs, ys);
+      v[h] = warp_any<EXACT>(cum + static_cast<size_t>(la) * npp, np, 0, xs, ys);
+    }
+#pragma unroll
+    for (int h = 0; h < kPix; ++h) e_in[h] = expf(-v[h]);
   }
   if (has_out) {
     const int cnt = nsub ? nsub[d] : 1;
-    float acc = 0.f;
+    float acc[kPix];
+#pragma unroll
+    for (int h = 0; h < kPix; ++h) acc[h] = 0.f;
     for (int k = 0; k < cnt; ++k) {
-      affine_apply(rot_out_back[(static_cast<size_t>(a) * n_det + d) * n_sub + k], xf, yf, xs, ys);
-      acc += expf(-warp_sample(cum + static_cast<size_t>(n_in + (la * n_det + d) * n_sub + k) * npp, np, 0, xs, ys));
+      const Affine m = rot_out_back[(static_cast<size_t>(a) * n_det + d) * n_sub + k];
+      const float* plane = cum + static_cast<size_t>(n_in + (la * n_det + d) * n_sub + k) * npp;
+      float v[kPix];
+#pragma unroll
+      for (int h = 0; h < kPix; ++h) {
+        float xs, ys;
+        affine_apply(m, xf, static_cast<float>(min(i0 + 16 * h, n - 1) + pad), xs, ys);
+        v[h] = warp_any<EXACT>(plane, np, 0, xs, ys);
+      }
+#pragma unroll
+      for (int h = 0; h < kPix; ++h) acc[h] += expf(-v[h]);
     }
-    e_out = acc / static_cast<float>(cnt);
+#pragma unroll
+    for (int h = 0; h < kPix; ++h) e_out[h] = acc[h] / static_cast<float>(cnt);
   }
-  maps[(static_cast<size_t>(a - out_a0) * n_det + d) * n * n + static_cast<size_t>(i) * n + j] = e_in * e_out;
+  float* dst = maps + (static_cast<size_t>(a - out_a0) * n_det + d) * n * n + j;
+#pragma unroll
+  for (int h = 0; h < kPix; ++h) {
+    const int i = i0 + 16 * h;
+    if (i < n) dst[static_cast<size_t>(i) * n] = e_in[h] * e_out[h];
+  }
 }
 
 // Grows a plan-owned device buffer to `need` bytes. Work already queued on
@@ -215,12 +359,20 @@ static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int
   const int n_in = mi ? na : 0;
   const int n_beams = n_in + (mo ? na * D * S : 0);
   dim3 g1((p.np + kScanWaves * kScanCols - 1) / (kScanWaves * kScanCols), n_beams);
-  hipLaunchKernelGGL(att_scan_kernel, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
-                     rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
-  CHIP_LAUNCH_CHECK();
-  dim3 g2((p.n + 15) / 16, (p.n + 15) / 16, na * D);
-  hipLaunchKernelGGL(att_compose_kernel, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0, mo ? 1 : 0,
-                     n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
+  dim3 g2((p.n + 15) / 16, (p.n + 31) / 32, na * D);
+  if (p.maps_exact) {
+    hipLaunchKernelGGL(att_scan_kernel<true>, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
+                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+    CHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(att_compose_kernel<true>, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
+                       mo ? 1 : 0, n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
+  } else {
+    hipLaunchKernelGGL(att_scan_kernel<false>, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
+                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+    CHIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(att_compose_kernel<false>, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
+                       mo ? 1 : 0, n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
+  }
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

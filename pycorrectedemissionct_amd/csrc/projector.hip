@@ -851,6 +851,11 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
     plan->p.fp_variant = value;
     return CHIP_OK;
   }
+  if (k == "maps_exact") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "maps_exact must be 0 or 1");
+    plan->p.maps_exact = value;
+    return CHIP_OK;
+  }
   if (k == "bp_parts") {
     CHIP_CHECK_ARG(value == -1 || value == 0 || (value >= 2 && value <= kBpMaxParts),
                    "bp_parts must be -1 (from occupancy), 0 (never) or 2..8");
