@@ -9,7 +9,10 @@
 // connections) and the output may be split in two (the gradient of such a
 // concatenation).
 //
-// Matrix core: v_mfma_f32_16x16x16_bf16. Operand layout (lane l, element i):
+// Matrix cores: v_mfma_f32_16x16x32_bf16 for the convolutions whose channel
+// counts are multiples of 32 (conv3x3_mfma_kernel: LDS-staged input patches),
+// v_mfma_f32_16x16x16_bf16 for the rest (the 1-channel input layer) and the
+// weight gradient. 16x16x16 operand layout (lane l, element i):
 // A[m][k] at m = l % 16, k = 4 (l / 16) + i; B[k][n] at n = l % 16,
 // k = 4 (l / 16) + i; D[m][n] at n = l % 16, m = 4 (l / 16) + i.
 //
@@ -23,6 +26,7 @@ namespace chip {
 namespace {
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
@@ -171,6 +175,163 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// forward 3x3 convolution on gfx950's v_mfma_f32_16x16x32_bf16 (channel counts
+// that are multiples of 32: every layer of the U-Net but the first)
+//
+// A workgroup owns a spatial tile of 8 rows x 16 columns of one image and CO_T
+// output channels; wave w computes tile rows 2w and 2w + 1 (two 16-pixel MFMA
+// row blocks) for all CO_T channels. Input channels run in chunks of 32: the
+// chunk's 10 x 18 input patch (the tile plus its one-pixel halo, zero outside
+// the image, from x1 or x2 of a concatenation) is staged in LDS, pixel pitch 40
+// bf16 (80 bytes: the 16 A-operand lanes of a read fall in distinct banks); the
+// 9 taps then read their A operands from the patch and their B operands
+// (weights, L2-resident) straight from global memory.
+// Operand layout (lane l, element i): A[m][k] at m = l % 16, k = 8 (l / 16) + i;
+// B[k][n] at n = l % 16, k = 8 (l / 16) + i; D[m][n] at n = l % 16, m = 4 (l / 16) + i.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+constexpr int kMtRows = 8, kMtCols = 16;                 // spatial tile
+constexpr int kMtPitch = 40;                             // bf16 per staged pixel (32 channels + 8 pad)
+constexpr int kMtPatch = (kMtRows + 2) * (kMtCols + 2);  // staged pixels
+
+template <int CO_T>
+__global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __restrict__ x1, int c1,
+                                                           const uint16_t* __restrict__ x2, int c2,
+                                                           const uint16_t* __restrict__ wt,
+                                                           const float* __restrict__ bias, uint16_t* __restrict__ y1,
+                                                           int o1, uint16_t* __restrict__ y2, Img im, int cout,
+                                                           int relu) {
+  constexpr int kCb = CO_T / 16;  // MFMA column blocks
+  constexpr int kOutPitch = CO_T + 8;  // bf16 per pixel of the staged output tile
+  constexpr int kLdsHalfs = kMtPatch * kMtPitch > kMtRows * kMtCols * kOutPitch ? kMtPatch * kMtPitch
+                                                                                : kMtRows * kMtCols * kOutPitch;
+  __shared__ __align__(16) uint16_t patch[kLdsHalfs];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_w = (im.w + kMtCols - 1) / kMtCols, tiles_h = (im.h + kMtRows - 1) / kMtRows;
+  int t = blockIdx.x;
+  const int tx = t % tiles_w;
+  t /= tiles_w;
+  const int ty = t % tiles_h;
+  const int img = t / tiles_h;
+  const int h0 = ty * kMtRows, w0 = tx * kMtCols;
+  const int cbase = blockIdx.y * CO_T;
+  const int cin = c1 + c2;
+  const int m = lane & 15, q = lane >> 4;
+
+  f4v acc[2][kCb];
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) acc[b][c] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+    // the chunk lies wholly in x1 or in x2 (c1 % 32 == 0)
+    const uint16_t* src = ci0 < c1 ? x1 : x2;
+    const int cs = ci0 < c1 ? c1 : c2;
+    const int co0 = ci0 < c1 ? ci0 : ci0 - c1;
+    __syncthreads();  // the previous chunk's reads are done
+    for (int k = threadIdx.x; k < kMtPatch * 4; k += 256) {
+      const int px = k >> 2, part = k & 3;  // 4 x 16 bytes per pixel
+      const int pr = px / (kMtCols + 2), pc = px - pr * (kMtCols + 2);
+      const int hh = h0 + pr - 1, ww = w0 + pc - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (hh >= 0 && hh < im.h && ww >= 0 && ww < im.w) {
+        const int64_t pix = (static_cast<int64_t>(img) * im.h + hh) * im.w + ww;
+        v = *reinterpret_cast<const uint4*>(src + pix * cs + co0 + 8 * part);
+      }
+      *reinterpret_cast<uint4*>(patch + px * kMtPitch + 8 * part) = v;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dh = tap / 3, dw = tap % 3;  // offsets into the halo-padded patch
+      bf8v bm[kCb];
+#pragma unroll
+      for (int c = 0; c < kCb; ++c) {
+        const uint16_t* wr = wt + (static_cast<size_t>(tap) * cout + cbase + 16 * c + m) * cin + ci0 + 8 * q;
+        bm[c] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wr));
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int pr = 2 * wave + b + dh, pc = m + dw;
+        const bf8v a = __builtin_bit_cast(
+            bf8v, *reinterpret_cast<const uint4*>(patch + (pr * (kMtCols + 2) + pc) * kMtPitch + 8 * q));
+#pragma unroll
+        for (int c = 0; c < kCb; ++c) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bm[c], acc[b][c], 0, 0, 0);
+      }
+    }
+  }
+
+  // epilogue: D[m][n] with n = lane % 16 (channel), m = 4 (lane / 16) + i (tile column); the
+  // tile goes through LDS as [pixel][channel] so that each pixel's channels leave as 16-byte stores
+  __syncthreads();  // every wave is done with the patch
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    const int col = 16 * c + m;
+    const float bb = bias ? bias[cbase + col] : 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = acc[b][c][i] + bb;
+        if (relu) v = fmaxf(v, 0.f);
+        patch[((2 * wave + b) * kMtCols + 4 * q + i) * kOutPitch + col] = f2bf(v);
+      }
+  }
+  __syncthreads();
+  constexpr int kGroups = CO_T / 8;
+  for (int k = threadIdx.x; k < kMtRows * kMtCols * kGroups; k += 256) {
+    const int px = k / kGroups, g = k - px * kGroups;
+    const int hh = h0 + px / kMtCols, ww = w0 + px % kMtCols;
+    if (hh >= im.h || ww >= im.w) continue;
+    const int64_t p = (static_cast<int64_t>(img) * im.h + hh) * im.w + ww;
+    const uint4 v = *reinterpret_cast<const uint4*>(patch + px * kOutPitch + 8 * g);
+    const int co = cbase + 8 * g;  // the 8 channels lie on one side of o1 (o1 % 16 == 0)
+    if (co < o1) *reinterpret_cast<uint4*>(y1 + p * o1 + co) = v;
+    else *reinterpret_cast<uint4*>(y2 + p * (cout - o1) + (co - o1)) = v;
+  }
+}
+
+// the 1-channel input layer (K = 9): a thread per pixel, 8 output channels per pass, the
+// weights and biases in LDS, 16-byte stores
+__global__ __launch_bounds__(256) void conv3x3_c1_kernel(const uint16_t* __restrict__ x,
+                                                         const uint16_t* __restrict__ wt,
+                                                         const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                         Img im, int cout, int relu) {
+  extern __shared__ float wsh[];  // [9][cout] weights, then [cout] biases
+  for (int k = threadIdx.x; k < 9 * cout; k += blockDim.x) wsh[k] = bf2f(wt[k]);  // wt[tap][co][0]
+  for (int k = threadIdx.x; k < cout; k += blockDim.x) wsh[9 * cout + k] = bias ? bias[k] : 0.f;
+  __syncthreads();
+  const int64_t P = im.pixels();
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int ww = static_cast<int>(p % im.w);
+  const int64_t t = p / im.w;
+  const int hh = static_cast<int>(t % im.h);
+  const int64_t img = t / im.h;
+  float xv[9];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int h2 = hh + tap / 3 - 1, w2 = ww + tap % 3 - 1;
+    xv[tap] = (h2 >= 0 && h2 < im.h && w2 >= 0 && w2 < im.w) ? bf2f(x[(img * im.h + h2) * im.w + w2]) : 0.f;
+  }
+  for (int co0 = 0; co0 < cout; co0 += 8) {
+    u16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float v = 0.f;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) v = fmaf(xv[tap], wsh[tap * cout + co0 + e], v);
+      v += wsh[9 * cout + co0 + e];
+      if (relu) v = fmaxf(v, 0.f);
+      o[e] = f2bf(v);
+    }
+    *reinterpret_cast<u16x8*>(y + p * cout + co0) = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // weight gradient: part[chunk][tap][co][ci] = sum over the chunk's pixels p of
 // dy[p][co] * x[p + offset(tap)][ci]   (M = co, N = ci, K = pixels)
 // ---------------------------------------------------------------------------
@@ -307,6 +468,67 @@ __global__ void relu_mask_kernel(uint16_t* __restrict__ d, const uint16_t* __res
 }
 
 // y[n][h][w][c] = (x[2h][2w] + x[2h][2w+1] + x[2h+1][2w] + x[2h+1][2w+1]) / 4; h, w of the input
+// 8 channels per thread (c % 8 == 0): one 16-byte access per pixel and tensor, the pixel
+// index decomposed once per 8 channels; the same float32 sums in the same order as the
+// per-element kernels below.
+__device__ __forceinline__ u16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const u16x8*>(p); }
+__device__ __forceinline__ void st8(uint16_t* p, u16x8 v) { *reinterpret_cast<u16x8*>(p) = v; }
+
+// mode 0: avgpool2 forward (x: n x 2ho x 2wo, y: n x ho x wo); 1: upsample2 backward (the same
+// 2 x 2 sum without the 1/4)
+__global__ void pool_sum_vec_kernel(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int ho, int wo,
+                                    int c, int mode) {
+  const int cg = c / 8;
+  const int64_t total = static_cast<int64_t>(n) * ho * wo * cg;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int g = static_cast<int>(k % cg);
+    const int64_t pix = k / cg;
+    const int j = static_cast<int>(pix % wo);
+    const int64_t t = pix / wo;
+    const int i = static_cast<int>(t % ho);
+    const int64_t b = t / ho;
+    const int64_t r0 = ((b * 2 * ho + 2 * i) * (2 * wo) + 2 * j) * c + 8 * g;
+    const int64_t r1 = r0 + static_cast<int64_t>(2 * wo) * c;
+    const u16x8 a = ld8(x + r0), bb = ld8(x + r0 + c), cc = ld8(x + r1), d = ld8(x + r1 + c);
+    u16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float sum = (bf2f(a[e]) + bf2f(bb[e])) + (bf2f(cc[e]) + bf2f(d[e]));
+      o[e] = f2bf(mode == 0 ? 0.25f * sum : sum);
+    }
+    st8(y + pix * c + 8 * g, o);
+  }
+}
+
+// mode 0: upsample2 forward (x: n x h x w, y: n x 2h x 2w, y[2i + a][2j + b] = x[i][j]);
+// 1: avgpool2 backward (the same copy times 1/4)
+__global__ void spread_vec_kernel(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w, int c,
+                                  int mode) {
+  const int cg = c / 8;
+  const int64_t total = static_cast<int64_t>(n) * h * w * cg;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int g = static_cast<int>(k % cg);
+    const int64_t pix = k / cg;
+    const int j = static_cast<int>(pix % w);
+    const int64_t t = pix / w;
+    const int i = static_cast<int>(t % h);
+    const int64_t b = t / h;
+    u16x8 v = ld8(x + pix * c + 8 * g);
+    if (mode == 1) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = f2bf(0.25f * bf2f(v[e]));
+    }
+    const int64_t r0 = ((b * 2 * h + 2 * i) * (2 * w) + 2 * j) * c + 8 * g;
+    const int64_t r1 = r0 + static_cast<int64_t>(2 * w) * c;
+    st8(y + r0, v);
+    st8(y + r0 + c, v);
+    st8(y + r1, v);
+    st8(y + r1 + c, v);
+  }
+}
+
 __global__ void avgpool2_kernel(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w, int c) {
   const int ho = h / 2, wo = w / 2;
   const int64_t total = static_cast<int64_t>(n) * ho * wo * c;
@@ -502,6 +724,32 @@ int chip_nn_conv3x3(const uint16_t* x1, int c1, const uint16_t* x2, int c2, cons
                      reinterpret_cast<uintptr_t>(wt) % 8 == 0,
                  "x1 / x2 / wt must be 8-byte aligned");
   const int64_t P = static_cast<int64_t>(n) * h * w;
+  const bool al16 = reinterpret_cast<uintptr_t>(y1) % 16 == 0 && reinterpret_cast<uintptr_t>(y2) % 16 == 0;
+  const bool mfma32 = c1 % 32 == 0 && c2 % 32 == 0 && cout % 32 == 0 && (o1 == cout || o1 % 16 == 0) &&
+                      reinterpret_cast<uintptr_t>(x1) % 16 == 0 && reinterpret_cast<uintptr_t>(x2) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(wt) % 16 == 0 && al16;
+  if (c1 == 1 && c2 == 0 && o1 == cout && cout % 8 == 0 && cout <= 1024 && al16) {
+    dim3 grid(static_cast<unsigned>((P + 255) / 256));
+    CHIP_CHECK_ARG(P <= 2147483647LL * 256, "too many pixels");
+    hipLaunchKernelGGL(conv3x3_c1_kernel, grid, dim3(256), sizeof(float) * 10 * cout, as_stream(stream), x1, wt, bias,
+                       y1, Img{n, h, w}, cout, relu);
+    CHIP_LAUNCH_CHECK();
+    return CHIP_OK;
+  }
+  if (mfma32) {
+    const int64_t tiles = static_cast<int64_t>(n) * ((h + kMtRows - 1) / kMtRows) * ((w + kMtCols - 1) / kMtCols);
+    CHIP_CHECK_ARG(tiles <= 2147483647, "too many pixels");
+    const bool co64 = cout % 64 == 0;
+    dim3 grid(static_cast<unsigned>(tiles), cout / (co64 ? 64 : 32));
+    if (co64)
+      hipLaunchKernelGGL(conv3x3_mfma_kernel<64>, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1,
+                         o1, y2, Img{n, h, w}, cout, relu);
+    else
+      hipLaunchKernelGGL(conv3x3_mfma_kernel<32>, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1,
+                         o1, y2, Img{n, h, w}, cout, relu);
+    CHIP_LAUNCH_CHECK();
+    return CHIP_OK;
+  }
   dim3 grid(static_cast<unsigned>((P + kPxBlock - 1) / kPxBlock), (cout + kCoBlock - 1) / kCoBlock);
   CHIP_CHECK_ARG(grid.x <= 2147483647u, "too many pixels");
   hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1, o1, y2,
@@ -567,10 +815,19 @@ int chip_nn_avgpool2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c,
   clear_error();
   CHIP_CHECK_ARG(x && y && n >= 1 && c >= 1 && h >= 2 && w >= 2 && h % 2 == 0 && w % 2 == 0, "bad argument");
   const int64_t count = static_cast<int64_t>(n) * h * w * c;
-  if (backward)
+  const bool vec = c % 8 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0;
+  if (vec) {
+    if (backward)  // x = dy of the pooled shape (h / 2 x w / 2), y = dx of h x w
+      hipLaunchKernelGGL(spread_vec_kernel, grid_for(count / 32), dim3(256), 0, as_stream(stream), x, y, n, h / 2,
+                         w / 2, c, 1);
+    else
+      hipLaunchKernelGGL(pool_sum_vec_kernel, grid_for(count / 32), dim3(256), 0, as_stream(stream), x, y, n, h / 2,
+                         w / 2, c, 0);
+  } else if (backward) {
     hipLaunchKernelGGL(avgpool2_bwd_kernel, grid_for(count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
-  else
+  } else {
     hipLaunchKernelGGL(avgpool2_kernel, grid_for(count / 4), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  }
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -579,10 +836,18 @@ int chip_nn_upsample2(const uint16_t* x, uint16_t* y, int n, int h, int w, int c
   clear_error();
   CHIP_CHECK_ARG(x && y && n >= 1 && c >= 1 && h >= 1 && w >= 1, "bad argument");
   const int64_t count = static_cast<int64_t>(n) * h * w * c;
-  if (backward)
+  const bool vec = c % 8 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0;
+  if (vec) {
+    if (backward)  // x = dy of 2h x 2w, y = dx of h x w
+      hipLaunchKernelGGL(pool_sum_vec_kernel, grid_for(count / 8), dim3(256), 0, as_stream(stream), x, y, n, h, w, c,
+                         1);
+    else
+      hipLaunchKernelGGL(spread_vec_kernel, grid_for(count / 8), dim3(256), 0, as_stream(stream), x, y, n, h, w, c, 0);
+  } else if (backward) {
     hipLaunchKernelGGL(upsample2_bwd_kernel, grid_for(count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
-  else
+  } else {
     hipLaunchKernelGGL(upsample2_kernel, grid_for(4 * count), dim3(256), 0, as_stream(stream), x, y, n, h, w, c);
+  }
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
