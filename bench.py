@@ -243,7 +243,11 @@ def run_headline(args) -> dict:
     log(f"MLEM 300 iterations, one slice: {t_mlem * 1e3:.1f} ms")
 
     kernels = {
-        "fp": dict(kernel="fp_stream_batch_kernel<2, true, false, false>", t=t_fp, bytes=fp_bytes, gbs=fp_gbs,
+        "fp": dict(kernel={"stream": "fp_stream_batch_kernel<2, true, false, false>",
+                           "stream_trim": "fp_stream_batch_kernel<2, true, true, false>",
+                           "direct": "fp_direct_batch_kernel<2, false>",
+                           "direct_trim": "fp_direct_batch_kernel<2, true>"}[tune["fp_shape"]], t=t_fp, bytes=fp_bytes,
+                   gbs=fp_gbs,
                    traffic_file="fp_batch_2x512x720x2_traffic.json"),
         "bp": dict(kernel="bp_adjoint_batch_kernel<2, true>", t=t_bp, bytes=bp_bytes, gbs=bp_gbs,
                    traffic_file="bp_batch_2x512x720x2_traffic.json"),
@@ -276,7 +280,8 @@ def run_headline(args) -> dict:
             "ms_preconditioner": 1e3 * t_pre, "ms_sirt20": 1e3 * t_sirt, "ms_network": 1e3 * t_net,
             "ms_per_iteration_per_slice": ms_per_step / nb, "mlem300_ms_one_slice": 1e3 * t_mlem,
             "fp_us": t_fp * 1e6, "bp_us": t_bp * 1e6, "fp_plus_bp_gbs": (fp_bytes + bp_bytes) / (t_fp + t_bp) / 1e9,
-            "autotune": {"fp_wg_per_cu": tune["fp_wg_per_cu"], "bp_wg_per_cu": tune["bp_wg_per_cu"]},
+            "autotune": {"fp_shape": tune["fp_shape"], "fp_wg_per_cu": tune["fp_wg_per_cu"],
+                         "bp_wg_per_cu": tune["bp_wg_per_cu"]},
             "network": os.path.relpath(args.weights, ROOT),
         },
         "roofline": {

@@ -356,14 +356,29 @@ int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* si
  * default 4) workgroups per CU of a persistent grid, 0 = one workgroup per work
  * item; fp tile variants "fp_trim" (stage only the footprint of interior
  * chunks), "fp_direct" (direct-to-LDS staging) and "fp_patch" (8 x 8 sample
- * patches per wave), each 0 / 1, default 0; set by the autotuner. */
+ * patches per wave), each 0 / 1, default 0; set by the autotuner. "otf_budget_mib" (>= 16,
+ * default 4096) bounds the on-the-fly projectors' scratch. */
 int chip_batch_tune(chip_batch_plan* plan, const char* key, int value);
-/* Maps of every slice (chip_att_maps per slice; mu_* packed like volumes). */
+/* Maps of every slice (chip_att_maps per slice; mu_* packed like volumes), through a prefix-sum
+ * scratch of at most 1 GiB shared by all sizes. */
 int chip_batch_att_maps(chip_batch_plan* plan, const float* mu_in, const float* mu_out, float* maps, void* stream);
 /* fp / bp of every slice (chip_fp / chip_bp semantics per slice); maps may be NULL. */
 int chip_batch_fp(const chip_batch_plan* plan, const float* vol, const float* maps, float* sino, void* stream);
 int chip_batch_bp(const chip_batch_plan* plan, const float* sino, const float* maps, float* vol, int mode,
                   void* stream);
+/* fp / bp of every slice with its maps evaluated on the fly from its own mu_in / mu_out (packed
+ * like volumes, either may be NULL), a block of angles at a time (chip_fp_otf / chip_bp_otf per
+ * slice). All slice sizes share one scratch (prefix sums + maps block) of at most about the batch's
+ * "otf_budget_mib" (default 4096), grown on the first call. For batches whose maps do not fit in
+ * memory (a few hundred slices of up to 768^2 at 360 angles need about 320 GB of maps). The fp
+ * equals chip_batch_fp over chip_batch_att_maps' maps bit for bit with the default fp tiles; the
+ * bp agrees to rounding (a lone slice's adjoint bp may split its angle range into parts). Mode
+ * CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT. Calls on one batch plan must not overlap: they share
+ * its scratch. */
+int chip_batch_fp_otf(chip_batch_plan* plan, const float* vol, const float* mu_in, const float* mu_out,
+                      float* sino, void* stream);
+int chip_batch_bp_otf(chip_batch_plan* plan, const float* sino, const float* mu_in, const float* mu_out,
+                      float* vol, int mode, void* stream);
 /* Spatial solver updates per slice (chip_tv_dual, chip_pdhg_primal,
  * chip_sirt_update semantics); x, tau, masks packed like volumes. */
 int chip_batch_tv_dual(const chip_batch_plan* plan, float* q, const float* x, float lam, void* stream);

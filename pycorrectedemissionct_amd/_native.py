@@ -93,6 +93,8 @@ _SIGNATURES = {
     "chip_batch_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "chip_batch_fp": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "chip_batch_bp": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_batch_fp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "chip_batch_bp_otf": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_batch_tv_dual": [c_void_p, c_void_p, c_void_p, ctypes.c_float, c_void_p],
     "chip_batch_pdhg_primal": [
         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int,

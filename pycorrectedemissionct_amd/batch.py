@@ -104,15 +104,22 @@ class BatchedProjectorXRF:
     None); ``att_maps`` may instead give the packed maps directly. Slices share
     the angles, detectors and weights. ``bp_mode``: "skimage" (corrct's
     default, maps and weights ignored), "symmetric" (``is_symmetric=True``) or
-    "adjoint" (exact transpose of the fp).
+    "adjoint" (exact transpose of the fp). ``maps_mode="otf"`` keeps only the packed
+    attenuation volumes and rebuilds each slice's maps per angle block inside every fp / bp
+    (``chip_batch_fp_otf`` / ``chip_batch_bp_otf``): for batches whose maps do not fit.
     """
 
     def __init__(self, sizes, angles_rot_rad, att_in=None, att_out=None, *, att_maps=None,
                  angles_detectors_rad=(np.pi / 2,), weights_detectors=None, weights_angles=None,
-                 bp_mode: str = "adjoint"):
+                 bp_mode: str = "adjoint", maps_mode: str = "precomputed"):
         if bp_mode not in _BP_MODES:
             raise ValueError(f"Unknown bp_mode: {bp_mode}")
+        if maps_mode not in ("precomputed", "otf"):
+            raise ValueError(f"Unknown maps_mode: {maps_mode}")
+        if maps_mode == "otf" and (att_maps is not None or (att_in is None and att_out is None)):
+            raise ValueError("maps_mode='otf' builds the maps from att_in / att_out: give those, not att_maps")
         self.bp_mode = bp_mode
+        self.maps_mode = maps_mode
         self.plan = BatchPlan(sizes, np.asarray(angles_rot_rad, dtype=np.float32), angles_detectors_rad,
                               weights_detectors, weights_angles)
         self.sizes = [int(s) for s in self.plan.sizes]
@@ -125,8 +132,12 @@ class BatchedProjectorXRF:
         elif att_in is not None or att_out is not None:
             mu_in = None if att_in is None else self.pack_vol(att_in)
             mu_out = None if att_out is None else self.pack_vol(att_out)
-            self.maps = torch.empty(self.plan.maps_elems, dtype=torch.float32, device=self.device)
-            self.plan.att_maps(mu_in, mu_out, self.maps)
+            if maps_mode == "otf":
+                self.mu_in, self.mu_out = mu_in, mu_out
+                self.maps = None
+            else:
+                self.maps = torch.empty(self.plan.maps_elems, dtype=torch.float32, device=self.device)
+                self.plan.att_maps(mu_in, mu_out, self.maps)
         else:
             self.maps = None
 
@@ -155,6 +166,8 @@ class BatchedProjectorXRF:
         return [flat[int(st[s]):int(st[s + 1])].view(self.n_det, self.n_angles, n) for s, n in enumerate(self.sizes)]
 
     def slice_maps(self, s: int) -> torch.Tensor:
+        if self.maps_mode == "otf":
+            raise NotImplementedError("maps_mode='otf' holds no maps")
         st = self.plan.maps_start
         n = self.sizes[s]
         return self.maps[int(st[s]):int(st[s + 1])].view(self.n_angles, self.n_det, n, n)
@@ -171,22 +184,38 @@ class BatchedProjectorXRF:
     def fp(self, vol: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         if out is None:
             out = torch.empty(self.plan.sino_elems, dtype=torch.float32, device=self.device)
+        if self.maps_mode == "otf":
+            _native.call("chip_batch_fp_otf", self.plan._h, _native.ptr(vol), _native.ptr(self.mu_in),
+                         _native.ptr(self.mu_out), _native.ptr(out), _native.stream())
+            return out
         return self.plan.fp(vol, self.maps, out)
 
     def bp(self, sino: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         if out is None:
             out = torch.empty(self.plan.vol_elems, dtype=torch.float32, device=self.device)
+        if self.maps_mode == "otf" and self.bp_mode != "skimage":
+            _native.call("chip_batch_bp_otf", self.plan._h, _native.ptr(sino), _native.ptr(self.mu_in),
+                         _native.ptr(self.mu_out), _native.ptr(out), _BP_MODES[self.bp_mode], _native.stream())
+            return out
         maps = None if self.bp_mode == "skimage" else self.maps
         return self.plan.bp(sino, maps, out, _BP_MODES[self.bp_mode])
 
 
-def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8)) -> dict:
-    """Pick the grid sizes of the batched fp and bp by timing them on ``A``'s batch: k workgroups
-    per CU of a persistent grid, or 0 for one workgroup per work item.
+# fp tile shapes of the batched kernel that give the same bits: LDS staging through registers or
+# gfx950's direct-to-LDS loads, each with the whole chunk box or only its footprint
+FP_SHAPES = {"stream": dict(fp_direct=0, fp_trim=0), "stream_trim": dict(fp_direct=0, fp_trim=1),
+             "direct": dict(fp_direct=1, fp_trim=0), "direct_trim": dict(fp_direct=1, fp_trim=1)}
 
-    Returns ``{"fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``; the
-    chosen values are applied to ``A.plan``. Results do not depend on the grid size (every
-    work item is computed the same way), only the time does.
+
+def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8),
+             fp_shapes=tuple(FP_SHAPES)) -> dict:
+    """Pick the batched fp's tile shape and grid size and the bp's grid size by timing them on
+    ``A``'s batch: k workgroups per CU of a persistent grid, or 0 for one workgroup per work item;
+    the fp shapes of ``FP_SHAPES``.
+
+    Returns ``{"fp_shape": s, "fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``;
+    the chosen values are applied to ``A.plan``. Every candidate computes every work item the same
+    way, so results do not depend on the choice (bit for bit), only the time does.
     """
     x = torch.rand(A.vol_elems, dtype=torch.float32, device=A.device)
     y = torch.empty(A.sino_elems, dtype=torch.float32, device=A.device)
@@ -203,18 +232,25 @@ def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6
         e1.synchronize()
         return e0.elapsed_time(e1) * 1e3 / reps
 
+    A.plan.tune("fp_patch", 0)
     fp_us, bp_us = {}, {}
-    for k in fp_candidates:
-        A.plan.tune("fp_wg_per_cu", k)
-        fp_us[k] = timed(lambda: A.fp(x, y))
+    for shape in fp_shapes:
+        for key, val in FP_SHAPES[shape].items():
+            A.plan.tune(key, val)
+        for k in fp_candidates:
+            A.plan.tune("fp_wg_per_cu", k)
+            fp_us[(shape, k)] = timed(lambda: A.fp(x, y))
     for k in bp_candidates:
         A.plan.tune("bp_wg_per_cu", k)
         bp_us[k] = timed(lambda: A.bp(y, v))
-    best_fp = min(fp_us, key=fp_us.get)
+    best_shape, best_fp = min(fp_us, key=fp_us.get)
     best_bp = min(bp_us, key=bp_us.get)
+    for key, val in FP_SHAPES[best_shape].items():
+        A.plan.tune(key, val)
     A.plan.tune("fp_wg_per_cu", best_fp)
     A.plan.tune("bp_wg_per_cu", best_bp)
-    return {"fp_wg_per_cu": best_fp, "bp_wg_per_cu": best_bp, "fp_us": fp_us, "bp_us": bp_us}
+    return {"fp_shape": best_shape, "fp_wg_per_cu": best_fp, "bp_wg_per_cu": best_bp,
+            "fp_us": {f"{s}/{k}": t for (s, k), t in fp_us.items()}, "bp_us": bp_us}
 
 
 def _precondition(A: BatchedProjectorXRF, vec: torch.Tensor, start, add: float, rho: float) -> torch.Tensor:

@@ -312,9 +312,9 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
   }
 }
 
-// Grows a plan-owned device buffer to `need` bytes. Work already queued on
-// `st` may still read the old buffer, so the stream is drained first.
-static int ensure_buffer(float*& buf, size_t& bytes, size_t need, hipStream_t st, const char* what) {
+// Grows a device buffer to `need` bytes. Work already queued on `st` may still
+// read the old buffer, so the stream is drained first.
+int ensure_buffer(float*& buf, size_t& bytes, size_t need, hipStream_t st, const char* what) {
   if (bytes >= need) return CHIP_OK;
   if (buf) {
     CHIP_HIP(hipStreamSynchronize(st));
@@ -347,9 +347,9 @@ static int chunk_angles(const Plan& p, int beams, bool with_block, size_t budget
 }
 
 // Maps of angles [a0, a0 + na) of one slice into out[(a - out_a0)][d][n][n],
-// through the prefix-sum scratch p.d_scratch (>= na beams' images).
+// through the prefix-sum scratch `scan` (>= na beams' images).
 static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int a0, int na, float* out, int out_a0,
-                            hipStream_t st) {
+                            float* scan, hipStream_t st) {
   const int A = p.n_angles, D = p.n_det;
   const Affine* rin_f = p.d_rot;
   const Affine* rin_b = p.d_rot + A;
@@ -362,18 +362,83 @@ static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int
   dim3 g2((p.n + 15) / 16, (p.n + 31) / 32, na * D);
   if (p.maps_exact) {
     hipLaunchKernelGGL(att_scan_kernel<true>, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
-                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, scan);
     CHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(att_compose_kernel<true>, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
+    hipLaunchKernelGGL(att_compose_kernel<true>, g2, dim3(256), 0, st, scan, rin_b, rout_b, mi ? 1 : 0,
                        mo ? 1 : 0, n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
   } else {
     hipLaunchKernelGGL(att_scan_kernel<false>, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
-                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, p.d_scratch);
+                       rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, scan);
     CHIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(att_compose_kernel<false>, g2, dim3(256), 0, st, p.d_scratch, rin_b, rout_b, mi ? 1 : 0,
+    hipLaunchKernelGGL(att_compose_kernel<false>, g2, dim3(256), 0, st, scan, rin_b, rout_b, mi ? 1 : 0,
                        mo ? 1 : 0, n_in, a0, D, p.n, p.pad, p.np, out, out_a0, S, p.d_nsub);
   }
   CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int att_maps_slices(const Plan& p, const float* mu_in, const float* mu_out, float* maps, int batch, int chunk,
+                    float* scan, hipStream_t st) {
+  const int A = p.n_angles, D = p.n_det, n = p.n;
+  const size_t nn = static_cast<size_t>(n) * n;
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + b * nn : nullptr;
+    const float* mo = mu_out ? mu_out + b * nn : nullptr;
+    float* mb = maps + static_cast<size_t>(b) * A * D * nn;
+    for (int a0 = 0; a0 < A; a0 += chunk) {
+      const int rc = build_maps_chunk(p, mi, mo, a0, std::min(chunk, A - a0), mb, 0, scan, st);
+      if (rc != CHIP_OK) return rc;
+    }
+  }
+  return CHIP_OK;
+}
+
+int maps_block_angles(const Plan& p, bool in, bool out, bool with_block, size_t budget, size_t* scan_bytes,
+                      size_t* block_bytes) {
+  const int beams = (in ? 1 : 0) + (out ? p.n_det * p.n_sub : 0);
+  const int chunk = chunk_angles(p, beams, with_block, budget);
+  if (scan_bytes) *scan_bytes = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
+  if (block_bytes) *block_bytes = with_block ? static_cast<size_t>(chunk) * p.n_det * p.n * p.n * sizeof(float) : 0;
+  return chunk;
+}
+
+int otf_fp(const Plan& p, const float* vol, const float* mu_in, const float* mu_out, float* sino, int batch,
+           int chunk, float* scan, float* block, hipStream_t st) {
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + b * nn : nullptr;
+    const float* mo = mu_out ? mu_out + b * nn : nullptr;
+    const float* v = vol + b * nn;
+    float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
+    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
+      const int na = std::min(chunk, p.n_angles - a0);
+      const int rc = build_maps_chunk(p, mi, mo, a0, na, block, a0, scan, st);
+      if (rc != CHIP_OK) return rc;
+      launch_fp_range(p, v, block, s, a0, na, a0, st);
+      CHIP_LAUNCH_CHECK();
+    }
+  }
+  return CHIP_OK;
+}
+
+int otf_bp(const Plan& p, const float* sino, const float* mu_in, const float* mu_out, float* vol, int batch,
+           int mode, int chunk, float* scan, float* block, hipStream_t st) {
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  // only the first chunk of each slice writes a fresh output and may split into parts
+  BpPartGuard guard(p, 1, mode, chunk, st);
+  for (int b = 0; b < batch; ++b) {
+    const float* mi = mu_in ? mu_in + b * nn : nullptr;
+    const float* mo = mu_out ? mu_out + b * nn : nullptr;
+    const float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
+    float* v = vol + b * nn;
+    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
+      const int na = std::min(chunk, p.n_angles - a0);
+      const int rc = build_maps_chunk(p, mi, mo, a0, na, block, a0, scan, st);
+      if (rc != CHIP_OK) return rc;
+      launch_bp_range(p, s, block, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st, guard.scratch());
+      CHIP_LAUNCH_CHECK();
+    }
+  }
   return CHIP_OK;
 }
 
@@ -390,30 +455,17 @@ extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* m
   CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
   Plan& p = plan->p;
   hipStream_t st = as_stream(stream);
-  const int A = p.n_angles, D = p.n_det, n = p.n;
-  const int beams = (mu_in ? 1 : 0) + (mu_out ? D * p.n_sub : 0);
-  const int chunk = chunk_angles(p, beams, false, size_t(1) << 30);
-  const size_t need = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
-  int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need, st, "chip_att_maps");
+  size_t need = 0;
+  const int chunk = maps_block_angles(p, mu_in != nullptr, mu_out != nullptr, false, kMapsScanBudget, &need, nullptr);
+  const int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need, st, "chip_att_maps");
   if (rc != CHIP_OK) return rc;
-  for (int b = 0; b < batch; ++b) {
-    const float* mi = mu_in ? mu_in + static_cast<size_t>(b) * n * n : nullptr;
-    const float* mo = mu_out ? mu_out + static_cast<size_t>(b) * n * n : nullptr;
-    float* mb = maps + static_cast<size_t>(b) * A * D * n * n;
-    for (int a0 = 0; a0 < A; a0 += chunk) {
-      rc = build_maps_chunk(p, mi, mo, a0, std::min(chunk, A - a0), mb, 0, st);
-      if (rc != CHIP_OK) return rc;
-    }
-  }
-  return CHIP_OK;
+  return att_maps_slices(p, mu_in, mu_out, maps, batch, chunk, p.d_scratch, st);
 }
 
 // Shared set-up of the on-the-fly projectors: chunk size and both scratch buffers.
 static int otf_prepare(Plan& p, const float* mu_in, const float* mu_out, hipStream_t st, int& chunk) {
-  const int beams = (mu_in ? 1 : 0) + (mu_out ? p.n_det * p.n_sub : 0);
-  chunk = chunk_angles(p, beams, true, p.otf_budget);
-  const size_t need_c = static_cast<size_t>(chunk) * beams * p.np * p.np * sizeof(float);
-  const size_t need_m = static_cast<size_t>(chunk) * p.n_det * p.n * p.n * sizeof(float);
+  size_t need_c = 0, need_m = 0;
+  chunk = maps_block_angles(p, mu_in != nullptr, mu_out != nullptr, true, p.otf_budget, &need_c, &need_m);
   int rc = ensure_buffer(p.d_scratch, p.scratch_bytes, need_c, st, "otf prefix sums");
   if (rc != CHIP_OK) return rc;
   return ensure_buffer(p.d_mblock, p.mblock_bytes, need_m, st, "otf maps block");
@@ -429,23 +481,9 @@ extern "C" int chip_fp_otf(chip_plan* plan, const float* vol, const float* mu_in
   Plan& p = plan->p;
   hipStream_t st = as_stream(stream);
   int chunk = 0;
-  int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
+  const int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
   if (rc != CHIP_OK) return rc;
-  const size_t nn = static_cast<size_t>(p.n) * p.n;
-  for (int b = 0; b < batch; ++b) {
-    const float* mi = mu_in ? mu_in + b * nn : nullptr;
-    const float* mo = mu_out ? mu_out + b * nn : nullptr;
-    const float* v = vol + b * nn;
-    float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
-    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
-      const int na = std::min(chunk, p.n_angles - a0);
-      rc = build_maps_chunk(p, mi, mo, a0, na, p.d_mblock, a0, st);
-      if (rc != CHIP_OK) return rc;
-      launch_fp_range(p, v, p.d_mblock, s, a0, na, a0, st);
-      CHIP_LAUNCH_CHECK();
-    }
-  }
-  return CHIP_OK;
+  return otf_fp(p, vol, mu_in, mu_out, sino, batch, chunk, p.d_scratch, p.d_mblock, st);
 }
 
 extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const float* mu_out, float* vol,
@@ -460,23 +498,7 @@ extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_i
   Plan& p = plan->p;
   hipStream_t st = as_stream(stream);
   int chunk = 0;
-  int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
+  const int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
   if (rc != CHIP_OK) return rc;
-  const size_t nn = static_cast<size_t>(p.n) * p.n;
-  // only the first chunk of each slice writes a fresh output and may split into parts
-  BpPartGuard guard(p, 1, mode, chunk, st);
-  for (int b = 0; b < batch; ++b) {
-    const float* mi = mu_in ? mu_in + b * nn : nullptr;
-    const float* mo = mu_out ? mu_out + b * nn : nullptr;
-    const float* s = sino + static_cast<size_t>(b) * p.n_det * p.n_angles * p.n;
-    float* v = vol + b * nn;
-    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
-      const int na = std::min(chunk, p.n_angles - a0);
-      rc = build_maps_chunk(p, mi, mo, a0, na, p.d_mblock, a0, st);
-      if (rc != CHIP_OK) return rc;
-      launch_bp_range(p, s, p.d_mblock, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st, guard.scratch());
-      CHIP_LAUNCH_CHECK();
-    }
-  }
-  return CHIP_OK;
+  return otf_bp(p, sino, mu_in, mu_out, vol, batch, mode, chunk, p.d_scratch, p.d_mblock, st);
 }

@@ -137,6 +137,8 @@ void release(BatchPlan& b) {
   if (b.d_item_start_bps) (void)hipFree(b.d_item_start_bps);
   if (b.d_vol_start) (void)hipFree(b.d_vol_start);
   if (b.d_geom) (void)hipFree(b.d_geom);
+  if (b.d_scan) (void)hipFree(b.d_scan);
+  if (b.d_mblock) (void)hipFree(b.d_mblock);
   for (chip_plan* p : b.size_plans) chip_plan_destroy(p);
   b = BatchPlan();
 }
@@ -302,6 +304,11 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
     plan->b.fp_trim = value;
     return CHIP_OK;
   }
+  if (k == "otf_budget_mib") {
+    CHIP_CHECK_ARG(value >= 16, "otf_budget_mib must be >= 16");
+    plan->b.otf_budget = static_cast<size_t>(value) << 20;
+    return CHIP_OK;
+  }
   if (k == "bp_wg_per_cu") {
     CHIP_CHECK_ARG(value >= 0 && value <= 16, "bp_wg_per_cu must be in 0..16 (0: one workgroup per tile)");
     plan->b.bp_wg_per_cu = value;
@@ -315,11 +322,22 @@ int chip_batch_att_maps(chip_batch_plan* plan, const float* mu_in, const float* 
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && maps != nullptr, "NULL argument");
   CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
-  const BatchPlan& b = plan->b;
-  for (int s = 0; s < b.n_slices; ++s) {
-    const BatchSlice& sl = b.h_slices[s];
-    const int rc = chip_att_maps(b.size_plans[sl.size_idx], mu_in ? mu_in + sl.vol_off : nullptr,
-                                 mu_out ? mu_out + sl.vol_off : nullptr, maps + sl.maps_off, 1, stream);
+  BatchPlan& b = plan->b;
+  hipStream_t st = as_stream(stream);
+  std::vector<int> chunk(b.size_plans.size());
+  size_t need = 0;
+  for (size_t z = 0; z < b.size_plans.size(); ++z) {
+    size_t scan = 0;
+    chunk[z] = maps_block_angles(b.size_plans[z]->p, mu_in != nullptr, mu_out != nullptr, false, kMapsScanBudget,
+                                 &scan, nullptr);
+    need = std::max(need, scan);
+  }
+  int rc = ensure_buffer(b.d_scan, b.scan_bytes, need, st, "chip_batch_att_maps");
+  if (rc != CHIP_OK) return rc;
+  for (const BatchSlice& sl : b.h_slices) {
+    rc = att_maps_slices(b.size_plans[sl.size_idx]->p, mu_in ? mu_in + sl.vol_off : nullptr,
+                         mu_out ? mu_out + sl.vol_off : nullptr, maps + sl.maps_off, 1, chunk[sl.size_idx], b.d_scan,
+                         st);
     if (rc != CHIP_OK) return rc;
   }
   return CHIP_OK;
@@ -343,6 +361,72 @@ int chip_batch_bp(const chip_batch_plan* plan, const float* sino, const float* m
   if (mode == CHIP_BP_ADJOINT) launch_bp_adjoint_batch(plan->b, sino, maps, vol, st);
   else launch_bp_skimage_batch(plan->b, sino, maps, vol, mode == CHIP_BP_SKIMAGE_SYM ? 1 : 0, st);
   CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+}  // extern "C"
+
+namespace chip {
+namespace {
+
+// Block sizes of every slice size within the batch's budget and both shared
+// scratch buffers grown to the largest need.
+int batch_otf_prepare(BatchPlan& b, bool in, bool out, hipStream_t st, std::vector<int>& chunk) {
+  chunk.resize(b.size_plans.size());
+  size_t need_c = 0, need_m = 0;
+  for (size_t z = 0; z < b.size_plans.size(); ++z) {
+    size_t c = 0, m = 0;
+    chunk[z] = maps_block_angles(b.size_plans[z]->p, in, out, true, b.otf_budget, &c, &m);
+    need_c = std::max(need_c, c);
+    need_m = std::max(need_m, m);
+  }
+  const int rc = ensure_buffer(b.d_scan, b.scan_bytes, need_c, st, "otf prefix sums");
+  if (rc != CHIP_OK) return rc;
+  return ensure_buffer(b.d_mblock, b.mblock_bytes, need_m, st, "otf maps block");
+}
+
+}  // namespace
+}  // namespace chip
+
+extern "C" {
+
+int chip_batch_fp_otf(chip_batch_plan* plan, const float* vol, const float* mu_in, const float* mu_out, float* sino,
+                      void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && vol != nullptr && sino != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  BatchPlan& b = plan->b;
+  hipStream_t st = as_stream(stream);
+  std::vector<int> chunk;
+  int rc = batch_otf_prepare(b, mu_in != nullptr, mu_out != nullptr, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  for (const BatchSlice& sl : b.h_slices) {
+    rc = otf_fp(b.size_plans[sl.size_idx]->p, vol + sl.vol_off, mu_in ? mu_in + sl.vol_off : nullptr,
+                mu_out ? mu_out + sl.vol_off : nullptr, sino + sl.sino_off, 1, chunk[sl.size_idx], b.d_scan,
+                b.d_mblock, st);
+    if (rc != CHIP_OK) return rc;
+  }
+  return CHIP_OK;
+}
+
+int chip_batch_bp_otf(chip_batch_plan* plan, const float* sino, const float* mu_in, const float* mu_out, float* vol,
+                      int mode, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && vol != nullptr && sino != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(mode == CHIP_BP_SKIMAGE_SYM || mode == CHIP_BP_ADJOINT,
+                 "mode must be CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT (CHIP_BP_SKIMAGE uses no maps: call chip_batch_bp)");
+  BatchPlan& b = plan->b;
+  hipStream_t st = as_stream(stream);
+  std::vector<int> chunk;
+  int rc = batch_otf_prepare(b, mu_in != nullptr, mu_out != nullptr, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  for (const BatchSlice& sl : b.h_slices) {
+    rc = otf_bp(b.size_plans[sl.size_idx]->p, sino + sl.sino_off, mu_in ? mu_in + sl.vol_off : nullptr,
+                mu_out ? mu_out + sl.vol_off : nullptr, vol + sl.vol_off, 1, mode, chunk[sl.size_idx], b.d_scan,
+                b.d_mblock, st);
+    if (rc != CHIP_OK) return rc;
+  }
   return CHIP_OK;
 }
 

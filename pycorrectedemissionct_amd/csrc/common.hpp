@@ -157,6 +157,13 @@ struct BatchPlan {
   int64_t* d_vol_start = nullptr;
   float4* d_geom = nullptr;   // [size][A]
   const float* d_w = nullptr; // [A][D], owned by size_plans[0]
+  // map-builder scratch shared by all sizes: prefix sums and one maps block
+  // (chip_batch_att_maps, chip_batch_fp_otf / _bp_otf), grown on first use
+  float* d_scan = nullptr;
+  size_t scan_bytes = 0;
+  float* d_mblock = nullptr;
+  size_t mblock_bytes = 0;
+  size_t otf_budget = size_t(4) << 30;  // chip_batch_tune "otf_budget_mib"
   int device = 0;
 };
 
@@ -235,6 +242,30 @@ void launch_bp_range(const Plan& p, const float* sino, const float* maps, float*
                      int n_ang, int maps_a0, int accumulate, hipStream_t st, const BpScratch& scr = BpScratch{});
 
 constexpr int kMaxDetPerPass = 4;
+
+// ---------------------------------------------------------------------------
+// attenuation maps through caller-owned scratch (attenuation.hip)
+// ---------------------------------------------------------------------------
+// prefix-sum scratch bound of the precomputed-maps builder
+constexpr size_t kMapsScanBudget = size_t(1) << 30;
+
+// Grows the device buffer `buf` of `bytes` bytes to at least `need` (draining
+// `st` before it frees the old one).
+int ensure_buffer(float*& buf, size_t& bytes, size_t need, hipStream_t st, const char* what);
+// Angles per map block of a slice of plan p within `budget` bytes of scratch
+// (prefix sums, plus the maps block itself when with_block), and the bytes of
+// each part at that block size.
+int maps_block_angles(const Plan& p, bool in, bool out, bool with_block, size_t budget, size_t* scan_bytes,
+                      size_t* block_bytes);
+// Maps [A][D][N][N] of `batch` slices, `chunk` angles at a time through `scan`.
+int att_maps_slices(const Plan& p, const float* mu_in, const float* mu_out, float* maps, int batch, int chunk,
+                    float* scan, hipStream_t st);
+// fp / bp (mode CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT) of `batch` slices with
+// their maps rebuilt per block of `chunk` angles into `block` through `scan`.
+int otf_fp(const Plan& p, const float* vol, const float* mu_in, const float* mu_out, float* sino, int batch,
+           int chunk, float* scan, float* block, hipStream_t st);
+int otf_bp(const Plan& p, const float* sino, const float* mu_in, const float* mu_out, float* vol, int batch,
+           int mode, int chunk, float* scan, float* block, hipStream_t st);
 
 }  // namespace chip
 

@@ -91,8 +91,12 @@ def test_autotune_changes_time_not_results(pkg):
     f1, b1 = A.fp(x).clone(), A.bp(y).clone()
     res = pkg.batch.autotune(A)
     print("autotune:", res)
-    assert res["fp_wg_per_cu"] in res["fp_us"] and res["bp_wg_per_cu"] in res["bp_us"]
+    assert f"{res['fp_shape']}/{res['fp_wg_per_cu']}" in res["fp_us"] and res["bp_wg_per_cu"] in res["bp_us"]
     assert torch.equal(A.fp(x), f1) and torch.equal(A.bp(y), b1)
+    for shape, keys in pkg.batch.FP_SHAPES.items():  # every tile shape the tuner may pick: the same bits
+        for key, val in keys.items():
+            A.plan.tune(key, val)
+        assert torch.equal(A.fp(x), f1), shape
     for key, val in (("fp_patch", 1), ("fp_direct", 1), ("fp_wg_per_cu", 0), ("bp_wg_per_cu", 0)):
         A.plan.tune(key, val)  # other kernel shapes and flat grids: results equal to 2e-7 (patch: sum order)
         assert float((A.fp(x) - f1).abs().max() / f1.abs().max()) < 1e-6
@@ -224,3 +228,47 @@ def test_batched_pdhg_channel_coupling_equals_volume_solves(pkg, reg_name):
         e = rel_max(st["x"].cpu().numpy(), got.cpu().numpy())
         print(f"batched PDHG-KL-{reg_name} group {g} (n={n}, {c} channels): rel_max vs volume solve {e:.2e}")
         assert e < 1e-4
+
+
+def test_batched_otf_equals_precomputed(pkg):
+    """maps_mode='otf' (each slice's maps rebuilt per angle block inside every fp / bp; a 16 MiB
+    budget gives the 96^2 and 128^2 slices two blocks) equals the precomputed-maps batch: fp bit
+    for bit, bp to rounding (blocks after the first add into the output), and the batched PDHG on
+    it follows the precomputed one."""
+    import torch
+
+    a = 36
+    rng = np.random.default_rng(705)
+    ang = _angles(a)
+    mu_in, mu_out = _mus(rng, SIZES)
+    xs = [rng.random((n, n)).astype(np.float32) for n in SIZES]
+    ys = [rng.random((2, a, n)).astype(np.float32) for n in SIZES]
+    for bp_mode in ("adjoint", "symmetric"):
+        kw = dict(angles_detectors_rad=DET, weights_detectors=W_DET, bp_mode=bp_mode)
+        pre = pkg.batch.BatchedProjectorXRF(SIZES, ang, mu_in, mu_out, **kw)
+        otf = pkg.batch.BatchedProjectorXRF(SIZES, ang, mu_in, mu_out, maps_mode="otf", **kw)
+        assert otf.maps is None
+        otf.plan.tune("otf_budget_mib", 16)
+        x, y = pre.pack_vol(xs), pre.pack_sino(ys)
+        assert torch.equal(otf.fp(x), pre.fp(x)), bp_mode
+        e = rel_max(pre.bp(y).cpu().numpy(), otf.bp(y).cpu().numpy())
+        print(f"batched otf bp ({bp_mode}) vs precomputed: rel_max {e:.2e}")
+        assert e < 1e-5
+    with pytest.raises(pkg._native.NativeError, match="otf_budget_mib"):
+        otf.plan.tune("otf_budget_mib", 8)
+    with pytest.raises(NotImplementedError):
+        otf.slice_maps(0)
+    with pytest.raises(ValueError, match="otf"):
+        pkg.batch.BatchedProjectorXRF(SIZES, ang, maps_mode="otf")
+    # a few graphed PDHG-TV iterations on each
+    kw = dict(angles_detectors_rad=DET, weights_detectors=W_DET, bp_mode="adjoint")
+    sizes = [64, 32, 96]
+    mi, mo = _mus(rng, sizes)
+    pre = pkg.batch.BatchedProjectorXRF(sizes, ang, mi, mo, **kw)
+    otf = pkg.batch.BatchedProjectorXRF(sizes, ang, mi, mo, maps_mode="otf", **kw)
+    b = pre.fp(pre.pack_vol([pkg.testing.phantom_ellipses(n, 4, rng) for n in sizes]))
+    xp = pkg.batch.BatchedPDHG(data_term="l2", tv_lambda=1e-2, use_graph=True)(pre, b, 25)
+    xo = pkg.batch.BatchedPDHG(data_term="l2", tv_lambda=1e-2, use_graph=True)(otf, b, 25)
+    e = rel_max(xp.cpu().numpy(), xo.cpu().numpy())
+    print(f"batched PDHG on otf vs precomputed: rel_max {e:.2e}")
+    assert e < 1e-4
