@@ -307,7 +307,9 @@ def run_batched(args) -> dict:
     each with its own 10-ellipse phantom and attenuation and 4 emission channels (mu_out scales 0.02,
     0.04, 0.08, 0.16, channel intensities U[0, 1]), 360 angles, one detector at 90 deg, Gaussian noise
     at 1 % of the max; batched SIRT (100 iterations) then PDHG-TV(1e-2, l2; 300 iterations), HIP-graph
-    iterations, autotuned persistent grids."""
+    iterations, autotuned persistent grids. ``--maps otf`` rebuilds the maps inside every fp / bp
+    instead (the whole 256-slice batch needs about 320 GB of maps); ``--sirt-iters`` /
+    ``--pdhg-iters`` shorten the timed solves."""
     import torch
 
     import pycorrectedemissionct_amd as pc
@@ -326,24 +328,32 @@ def run_batched(args) -> dict:
             mu_in.append((0.02 * sup + 0.005).astype(np.float32))
             mu_out.append((sc * sup).astype(np.float32))
     log(f"batched: {len(sizes)} slice-channels, {sum(n * n for n in sizes) / 1e6:.1f} Mpixel")
+    otf = args.maps == "otf"
     t0 = time.perf_counter()
-    A = pc.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=[np.pi / 2], bp_mode="adjoint")
+    A = pc.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=[np.pi / 2], bp_mode="adjoint",
+                                     maps_mode=args.maps)
     torch.cuda.synchronize()
     t_maps = time.perf_counter() - t0
-    tune = pc.batch.autotune(A)
-    log(f"maps {t_maps:.1f} s ({A.plan.maps_elems * 4 / 1e9:.1f} GB); autotune {tune}")
+    # the tuned kernels are the persistent batched ones, which the on-the-fly mode does not launch
+    tune = None if otf else pc.batch.autotune(A)
+    log(f"set-up {t_maps:.1f} s ({A.plan.maps_elems * 4 / 1e9:.1f} GB of maps{' not held' if otf else ''}); "
+        f"autotune {tune}")
     clean = A.fp(A.pack_vol(fs))
     gen = torch.Generator(device="cuda").manual_seed(1)
     b = clean + 0.01 * clean.abs().max() * torch.randn(clean.shape, generator=gen, device="cuda")
     sirt = pc.batch.BatchedSIRT()
     pdhg = pc.batch.BatchedPDHG(data_term="l2", tv_lambda=1e-2)
+    n_sirt, n_pdhg = args.sirt_iters, args.pdhg_iters
     s1 = sirt.prepare(A, b)
     sirt.iterate(A, s1, 11)  # eager step + graph capture
     s2 = pdhg.prepare(A, b)
     pdhg.iterate(A, s2, 11)
     torch.cuda.synchronize()
-    t_sirt = pc.batch.time_iterations(lambda k: sirt.iterate(A, s1, k), 100)
-    t_pdhg = pc.batch.time_iterations(lambda k: pdhg.iterate(A, s2, k), 300)
+    log("solver graphs captured")
+    t_sirt = pc.batch.time_iterations(lambda k: sirt.iterate(A, s1, k), n_sirt)
+    log(f"SIRT-{n_sirt}: {t_sirt:.2f} s")
+    t_pdhg = pc.batch.time_iterations(lambda k: pdhg.iterate(A, s2, k), n_pdhg)
+    log(f"PDHG-{n_pdhg}: {t_pdhg:.2f} s")
     x = s2["x"]
     y = torch.empty_like(b)
     v = torch.empty_like(x)
@@ -361,16 +371,22 @@ def run_batched(args) -> dict:
     t_bp = ev[1].elapsed_time(ev[2]) / 5e3
     alg = sum(4.0 * (n * n + 360 * n * n + 360 * n) for n in sizes)
     total = t_sirt + t_pdhg
+    samples = 360.0 * sum(n * n for n in sizes)
     return {
-        "metric": "slice-channels/s, batched SIRT-100 + PDHG-TV-300 (uneven sizes)",
+        "metric": f"slice-channels/s, batched SIRT-{n_sirt} + PDHG-TV-{n_pdhg} (uneven sizes)",
         "value": len(sizes) / total, "unit": "slice-channels/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "32 slices x 4 channels, N in {128..768}, 360 angles, 1 detector; "
-                               "SIRT-100 + PDHG-TV(1e-2, l2)-300, HIP-graph iterations",
+        "config": {"workload": f"{args.slices} slices x 4 channels, N in {{128..768}}, 360 angles, 1 detector; "
+                               f"SIRT-{n_sirt} + PDHG-TV(1e-2, l2)-{n_pdhg}, HIP-graph iterations, "
+                               f"{args.maps} maps",
                    "slice_sizes": [int(v) for v in sizes_sl], "maps_gb": A.plan.maps_elems * 4 / 1e9,
-                   "s_maps": t_maps, "autotune": tune, "ms_sirt_iteration": 1e3 * t_sirt / 100,
-                   "ms_pdhg_iteration": 1e3 * t_pdhg / 300, "fp_ms": 1e3 * t_fp, "bp_ms": 1e3 * t_bp,
-                   "fp_gbs": alg / t_fp / 1e9, "bp_gbs": alg / t_bp / 1e9},
+                   "maps_mode": args.maps, "s_setup": t_maps, "autotune": tune,
+                   "ms_sirt_iteration": 1e3 * t_sirt / n_sirt, "ms_pdhg_iteration": 1e3 * t_pdhg / n_pdhg,
+                   "fp_ms": 1e3 * t_fp, "bp_ms": 1e3 * t_bp,
+                   # with precomputed maps: algorithmic HBM bytes (slice + maps + sinogram) per second;
+                   # on the fly: map samples (pixel-angles) per second, maps built inside the call
+                   "fp_gbs": None if otf else alg / t_fp / 1e9, "bp_gbs": None if otf else alg / t_bp / 1e9,
+                   "fp_gsamples_per_s": samples / t_fp / 1e9, "bp_gsamples_per_s": samples / t_bp / 1e9},
     }
 
 
@@ -435,8 +451,8 @@ def run_otf(args) -> dict:
         "metric": "ms per on-the-fly fp / bp, 1024x1024 slice, 1440 angles",
         "value": 1e3 * (t_fp + t_bp), "unit": "ms per fp+bp pair", "n_gpus": 1, "higher_is_better": False,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "1024x1024, 1440 angles, 1 detector at 90 deg, maps rebuilt per angle block "
-                               "(1 GiB scratch) instead of 6 GB of precomputed maps",
+        "config": {"workload": f"1024x1024, 1440 angles, 1 detector at 90 deg, maps rebuilt per angle block "
+                               f"({best} MiB scratch) instead of 6 GB of precomputed maps",
                    "otf_fp_ms": 1e3 * t_fp, "otf_bp_ms": 1e3 * t_bp, "precomputed_fp_ms": 1e3 * t_fp_pre,
                    "otf_over_precomputed_fp": t_fp / t_fp_pre, "otf_fp_ms_by_budget_mib": budgets,
                    "otf_budget_mib": best,
@@ -450,12 +466,80 @@ def run_otf(args) -> dict:
     }
 
 
+def _unet_flops(levels: int, base: int, h: int, w: int) -> float:
+    """Multiply-add flops (2 per MAC) of the U-Net's 3x3 convolutions on one h x w image."""
+    from pycorrectedemissionct_amd.warmstart import layer_shapes
+
+    total = 0.0
+    for name, cin, cout in layer_shapes(levels, base):
+        lvl = int(name[-2] if name[-1] in "ab" else name[-1])
+        total += 2.0 * 9 * cin * cout * (h >> lvl) * (w >> lvl)
+    return total
+
+
+def run_unet(args) -> dict:
+    """The warm-start network (4 levels, 32 base channels, ~2M parameters; bf16 MFMA with float32
+    accumulation): inference at 256^2 (batch 32) and 1024^2 (batch 2), and Adam training steps at
+    256^2 (batch 8), each timed with HIP events over --steps repetitions; TFLOP/s of the 3x3
+    convolutions (2 flops per MAC; a training step counts forward + data gradient + weight
+    gradient) against the 2.5 PFLOP/s dense bf16 MFMA peak."""
+    import torch
+
+    from pycorrectedemissionct_amd import warmstart
+
+    net = warmstart.UNet(4, 32, seed=0)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    st = torch.cuda.current_stream()
+    reps = max(1, args.steps // 20)
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(k):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / k
+
+    res = {}
+    for n, batch in ((256, 32), (1024, 2)):
+        x = torch.rand((batch, n, n), generator=gen, device="cuda")
+        t = timed(lambda: net.forward(x), reps)
+        fl = batch * _unet_flops(4, 32, n, n)
+        res[f"infer_{n}"] = {"batch": batch, "ms": 1e3 * t, "tflops": fl / t / 1e12, "mfma_frac": fl / t / 2.5e15}
+        log(f"inference {batch} x {n}^2: {1e3 * t:.2f} ms, {fl / t / 1e12:.1f} TFLOP/s")
+    batch, n = 8, 256
+    x = torch.rand((batch, n, n), generator=gen, device="cuda")
+    tgt = torch.rand((batch, n, n), generator=gen, device="cuda")
+    t = timed(lambda: net.train_step(x, tgt), reps)
+    # the first convolution (1 input channel) has no data gradient
+    fl = batch * (3 * _unet_flops(4, 32, n, n) - 2.0 * 9 * 1 * 32 * n * n)
+    res["train_256"] = {"batch": batch, "ms": 1e3 * t, "tflops": fl / t / 1e12, "mfma_frac": fl / t / 2.5e15}
+    log(f"training step {batch} x {n}^2: {1e3 * t:.2f} ms, {fl / t / 1e12:.1f} TFLOP/s")
+    return {
+        "metric": "U-Net TFLOP/s (3x3 convolutions, bf16 MFMA, float32 accumulation)",
+        "value": res["infer_256"]["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"U-Net 4 levels x 32 base channels ({net.n_params} parameters); inference "
+                               f"32 x 256^2 and 2 x 1024^2, Adam step 8 x 256^2",
+                   "reps": reps, **res},
+        "roofline": {"bound": "mfma", "achieved": res["infer_256"]["tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": res["infer_256"]["mfma_frac"], "traffic": None},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["headline", "batched", "otf"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batched", "otf", "unet"], default="headline",
                     help="headline: warm-started vs plain batched PDHG at 512^2 (default); batched / otf: the "
-                         "uneven batch and the on-the-fly maps")
+                         "uneven batch and the on-the-fly maps; unet: the warm-start network's kernels")
     ap.add_argument("--slices", type=int, default=None, help="slices in the batch (headline 2, batched 32)")
+    ap.add_argument("--maps", choices=["precomputed", "otf"], default="precomputed",
+                    help="batched: attenuation maps held in memory or rebuilt inside every fp / bp")
+    ap.add_argument("--sirt-iters", type=int, default=100, help="batched: timed SIRT iterations")
+    ap.add_argument("--pdhg-iters", type=int, default=300, help="batched: timed PDHG iterations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -485,6 +569,8 @@ def main() -> int:
     elif args.workload == "batched":
         args.slices = args.slices or 32
         res = run_batched(args)
+    elif args.workload == "unet":
+        res = run_unet(args)
     else:
         res = run_otf(args)
     print(json.dumps(res), flush=True)

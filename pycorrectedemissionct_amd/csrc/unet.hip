@@ -425,6 +425,196 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const uint16_t* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// weight gradient on the matrix cores (channel counts multiples of 32):
+// dW[tap][co][ci] = sum_p dy[p][co] x[p + tap][ci], a GEMM per tap with
+// M = output channels, N = input channels, K = pixels. A workgroup owns CO_B
+// output and 32 input channels and a run of 8 x 16 spatial tiles. Per tile it
+// stages dy transposed to [co][pixel] and the 10 x 18 input halo transposed to
+// [ci][row][column] in three copies, copy s shifted by s - 1 columns, so that
+// every operand -- 8 consecutive pixels of one channel -- is one aligned
+// 16-byte LDS read for every tap. Wave w computes the 16 output channels
+// w % (CO_B / 16) against CO_B / 32 blocks of 16 input channels, all 9 taps.
+// ---------------------------------------------------------------------------
+constexpr int kWgPx = kMtRows * kMtCols;                 // pixels per tile (128)
+constexpr int kWgDyPitch = kWgPx + 8;                    // bf16 per staged dy channel
+constexpr int kWgXPitch = (kMtRows + 2) * kMtCols + 8;   // bf16 per staged input channel of one copy
+
+template <int CO_B>
+__global__ __launch_bounds__(256) void conv3x3_wgrad_mfma_kernel(const uint16_t* __restrict__ dy,
+                                                                 const uint16_t* __restrict__ x1, int c1,
+                                                                 const uint16_t* __restrict__ x2, int c2, Img im,
+                                                                 int cout, int tiles_per_chunk,
+                                                                 float* __restrict__ part) {
+  constexpr int kCoSub = CO_B / 16;  // 16-channel output blocks of the workgroup
+  constexpr int kNci = CO_B / 32;    // 16-channel input blocks per wave (4 waves cover kCoSub x 2 blocks)
+  constexpr int kG = CO_B / 8;       // 16-byte groups of a dy pixel
+  __shared__ __align__(16) uint16_t dyT[CO_B * kWgDyPitch];
+  __shared__ __align__(16) uint16_t xT[3 * 32 * kWgXPitch];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  const int cin = c1 + c2;
+  const int n_cob = cout / CO_B;
+  const int cob = blockIdx.x % n_cob, cib = blockIdx.x / n_cob;
+  const int co0 = cob * CO_B, ci0 = cib * 32;
+  // the 32 input channels lie wholly in x1 or in x2 (c1 % 32 == 0)
+  const uint16_t* src = ci0 < c1 ? x1 : x2;
+  const int cs = ci0 < c1 ? c1 : c2;
+  const int cc0 = ci0 < c1 ? ci0 : ci0 - c1;
+  const int co_sub = wave % kCoSub;
+  const int ci_sub0 = (wave / kCoSub) * kNci;
+  const int tiles_w = (im.w + kMtCols - 1) / kMtCols, tiles_h = (im.h + kMtRows - 1) / kMtRows;
+  const int64_t n_tiles = static_cast<int64_t>(im.n) * tiles_h * tiles_w;
+  const int64_t t_lo = static_cast<int64_t>(blockIdx.y) * tiles_per_chunk;
+  const int64_t t_hi = min(n_tiles, t_lo + tiles_per_chunk);
+
+  f4v acc[9][kNci];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int j = 0; j < kNci; ++j) acc[tap][j] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t t = t_lo; t < t_hi; ++t) {
+    const int tx = static_cast<int>(t % tiles_w);
+    const int64_t r = t / tiles_w;
+    const int ty = static_cast<int>(r % tiles_h);
+    const int img = static_cast<int>(r / tiles_h);
+    const int h0 = ty * kMtRows, w0 = tx * kMtCols;
+    __syncthreads();  // the previous tile's operand reads are done
+    // dy -> dyT[co][pixel]; consecutive lanes take consecutive pixels (consecutive LDS halves)
+    for (int k = threadIdx.x; k < kWgPx * kG; k += 256) {
+      const int px = k % kWgPx, g = k / kWgPx;
+      const int hh = h0 + px / kMtCols, ww = w0 + px % kMtCols;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (hh < im.h && ww < im.w)
+        v = *reinterpret_cast<const uint4*>(dy + ((static_cast<int64_t>(img) * im.h + hh) * im.w + ww) * cout + co0 +
+                                            8 * g);
+      const u16x8 e = __builtin_bit_cast(u16x8, v);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) dyT[(8 * g + j) * kWgDyPitch + px] = e[j];
+    }
+    // input halo -> xT[s][ci][row][col] = x[h0 + row - 1][w0 + col + s - 1][ci0 + ci], 0 outside the image
+    for (int k = threadIdx.x; k < kMtPatch * 4; k += 256) {
+      const int hp = k % kMtPatch, g = k / kMtPatch;
+      const int hr = hp / (kMtCols + 2), hc = hp - hr * (kMtCols + 2);
+      const int hh = h0 + hr - 1, ww = w0 + hc - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (hh >= 0 && hh < im.h && ww >= 0 && ww < im.w)
+        v = *reinterpret_cast<const uint4*>(src + ((static_cast<int64_t>(img) * im.h + hh) * im.w + ww) * cs + cc0 +
+                                            8 * g);
+      const u16x8 e = __builtin_bit_cast(u16x8, v);
+#pragma unroll
+      for (int sft = 0; sft < 3; ++sft) {
+        const int col = hc - sft;
+        if (col < 0 || col >= kMtCols) continue;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xT[(sft * 32 + 8 * g + j) * kWgXPitch + hr * kMtCols + col] = e[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kc = 0; kc < kWgPx / 32; ++kc) {
+      // K = 32 pixels: tile rows 2 kc, 2 kc + 1; lane group q holds row 2 kc + q / 2, columns 8 (q % 2) + i
+      const bf8v a = __builtin_bit_cast(
+          bf8v, *reinterpret_cast<const uint4*>(dyT + (co_sub * 16 + m) * kWgDyPitch + 32 * kc + 8 * q));
+      const int tr = 2 * kc + (q >> 1), c0 = 8 * (q & 1);
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int dh = tap / 3, dw = tap % 3;  // halo offsets
+#pragma unroll
+        for (int j = 0; j < kNci; ++j) {
+          const bf8v b = __builtin_bit_cast(
+              bf8v, *reinterpret_cast<const uint4*>(xT + (dw * 32 + (ci_sub0 + j) * 16 + m) * kWgXPitch +
+                                                    (tr + dh) * kMtCols + c0));
+          acc[tap][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tap][j], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[m'][n]: n = lane % 16 (input channel), m' = 4 (lane / 16) + i (output channel)
+  const size_t chunk_base = static_cast<size_t>(blockIdx.y) * 9;
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int j = 0; j < kNci; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = co0 + co_sub * 16 + 4 * q + i;
+        const int ci = ci0 + (ci_sub0 + j) * 16 + m;
+        part[((chunk_base + tap) * cout + co) * cin + ci] = acc[tap][j][i];
+      }
+}
+
+// out[i] = sum_c part[c][i]: 32 elements x 8 part groups per workgroup, group g summing parts
+// g, g + 8, ... in order, the 8 group sums then added in order (the bits depend only on n_parts)
+__global__ __launch_bounds__(256) void sum_parts_wide_kernel(const float* __restrict__ part, int n_parts, int64_t len,
+                                                              float* __restrict__ out) {
+  __shared__ float red[8][32];
+  const int e = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 32 + e;
+  float s = 0.f;
+  if (i < len)
+    for (int c = g; c < n_parts; c += 8) s += part[static_cast<size_t>(c) * len + i];
+  red[g][e] = s;
+  __syncthreads();
+  if (g == 0 && i < len) {
+    float r = red[0][e];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r += red[k][e];
+    out[i] = r;
+  }
+}
+
+// part[chunk][ci] = sum over the chunk's pixels of wgt(p) x[p][ci] (wgt = dout[p], or 1 when
+// dout is NULL), and with `extra` part[chunk][c] = sum of wgt(p); with dfeat, dfeat[p][ci] =
+// bf16(dout[p] wb[ci]). c = 8 G channels, G a power of two <= 256: thread (g = tid % G,
+// pl = tid / G) takes channels [8 g, 8 g + 8) of pixels pl, pl + 256 / G, ...; the 256 / G
+// pixel lanes are added in order.
+__global__ __launch_bounds__(256) void channel_sum_vec_kernel(const uint16_t* __restrict__ x, int64_t P, int c,
+                                                               int64_t chunk_px, const float* __restrict__ dout,
+                                                               const float* __restrict__ wb, uint16_t* __restrict__ dfeat,
+                                                               int extra, float* __restrict__ part) {
+  __shared__ float red[256 * 8 + 256];
+  const int G = c / 8, PP = 256 / G;
+  const int g = threadIdx.x % G, pl = threadIdx.x / G;
+  const int64_t p_lo = static_cast<int64_t>(blockIdx.x) * chunk_px;
+  const int64_t p_hi = min(P, p_lo + chunk_px);
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float accw = 0.f;
+  float wbv[8];
+  if (dfeat) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) wbv[j] = wb[8 * g + j];
+  }
+  for (int64_t p = p_lo + pl; p < p_hi; p += PP) {
+    const u16x8 e = __builtin_bit_cast(u16x8, *reinterpret_cast<const uint4*>(x + p * c + 8 * g));
+    const float wgt = dout ? dout[p] : 1.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] += wgt * bf2f(e[j]);
+    accw += wgt;
+    if (dfeat) {
+      u16x8 d;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) d[j] = f2bf(dout[p] * wbv[j]);
+      *reinterpret_cast<uint4*>(dfeat + p * c + 8 * g) = __builtin_bit_cast(uint4, d);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) red[pl * c + 8 * g + j] = acc[j];
+  if (g == 0) red[256 * 8 + pl] = accw;
+  __syncthreads();
+  const int n_out = c + (extra ? 1 : 0);
+  for (int ci = threadIdx.x; ci < n_out; ci += 256) {
+    float s = 0.f;
+    if (ci < c) {
+      for (int k = 0; k < PP; ++k) s += red[k * c + ci];
+    } else {
+      for (int k = 0; k < PP; ++k) s += red[256 * 8 + k];
+    }
+    part[static_cast<size_t>(blockIdx.x) * n_out + ci] = s;
+  }
+}
+
 // out[i] = sum_c part[c][i] (fixed order)
 __global__ void sum_parts_kernel(const float* __restrict__ part, int n_parts, int64_t len, float* __restrict__ out) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
@@ -611,6 +801,21 @@ __global__ void head_fwd_kernel(const uint16_t* __restrict__ feat, int c, const 
   }
 }
 
+// head_fwd_kernel with 16-byte loads of each pixel's features (c % 8 == 0): the same sums in the same order
+__global__ void head_fwd_vec_kernel(const uint16_t* __restrict__ feat, int c, const float* __restrict__ wb,
+                                    const float* __restrict__ xin, float* __restrict__ out, int64_t P) {
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < P;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    float s = wb[c];
+    for (int c8 = 0; c8 < c; c8 += 8) {
+      const u16x8 e = __builtin_bit_cast(u16x8, *reinterpret_cast<const uint4*>(feat + p * c + c8));
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s = fmaf(wb[c8 + j], bf2f(e[j]), s);
+    }
+    out[p] = xin[p] + s;
+  }
+}
+
 // dfeat[p][ci] = dout[p] * wb[ci]; part[chunk][ci] = sum_p dout[p] feat[p][ci], part[chunk][c] = sum_p dout[p]
 __global__ void head_bwd_kernel(const float* __restrict__ dout, const uint16_t* __restrict__ feat, int c,
                                 const float* __restrict__ wb, uint16_t* __restrict__ dfeat, int64_t P,
@@ -691,6 +896,14 @@ __global__ void to_f32_kernel(const uint16_t* __restrict__ x, float* __restrict_
     y[k] = bf2f(x[k]);
 }
 
+bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
+
+// channel counts channel_sum_vec_kernel takes: 8 G with G a power of two <= 256
+bool vec_channels(int c) {
+  const int g = c / 8;
+  return c % 8 == 0 && g >= 1 && g <= 256 && (g & (g - 1)) == 0;
+}
+
 dim3 grid_for(int64_t n, int block = 256) {
   return dim3(static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>((n + block - 1) / block, 1), 65536)));
 }
@@ -767,14 +980,35 @@ int chip_nn_conv3x3_wgrad(const uint16_t* dy, const uint16_t* x1, int c1, const 
   const int64_t P = static_cast<int64_t>(n) * h * w;
   const int64_t row = 9LL * cout * cin;
   CHIP_CHECK_ARG(part_elems >= row, "partial-sum scratch smaller than one weight set");
-  int n_chunks = 1;
-  const int64_t px = chunk_pixels(P, row, part_elems, 256, n_chunks);
-  const int n_tiles = 9 * ((cout + 31) / 32) * ((cin + 31) / 32);
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(n_tiles, n_chunks), dim3(256), 0, st, dy, x1, c1, x2, c2,
-                     Img{n, h, w}, cout, px, part);
+  const bool mfma = cout % 32 == 0 && c1 % 32 == 0 && c2 % 32 == 0 && aligned16(dy) && aligned16(x1) &&
+                    (c2 == 0 || aligned16(x2));
+  int n_chunks = 1;
+  if (mfma) {
+    const int co_b = cout % 64 == 0 ? 64 : 32;
+    const int n_blocks = (cout / co_b) * (cin / 32);
+    const int64_t n_tiles = static_cast<int64_t>(n) * ((h + kMtRows - 1) / kMtRows) * ((w + kMtCols - 1) / kMtCols);
+    // about 1024 workgroups, at most 512 partial sets, within the scratch
+    int64_t chunks = std::max<int64_t>(1, (1024 + n_blocks - 1) / n_blocks);
+    chunks = std::min<int64_t>({chunks, 512, n_tiles, std::max<int64_t>(1, part_elems / row), 65535});
+    const int64_t tpc = (n_tiles + chunks - 1) / chunks;
+    n_chunks = static_cast<int>((n_tiles + tpc - 1) / tpc);
+    const dim3 grid(n_blocks, n_chunks);
+    if (co_b == 64)
+      hipLaunchKernelGGL(conv3x3_wgrad_mfma_kernel<64>, grid, dim3(256), 0, st, dy, x1, c1, x2, c2, Img{n, h, w}, cout,
+                         static_cast<int>(tpc), part);
+    else
+      hipLaunchKernelGGL(conv3x3_wgrad_mfma_kernel<32>, grid, dim3(256), 0, st, dy, x1, c1, x2, c2, Img{n, h, w}, cout,
+                         static_cast<int>(tpc), part);
+  } else {
+    const int64_t px = chunk_pixels(P, row, part_elems, 256, n_chunks);
+    const int n_tiles = 9 * ((cout + 31) / 32) * ((cin + 31) / 32);
+    hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(n_tiles, n_chunks), dim3(256), 0, st, dy, x1, c1, x2, c2,
+                       Img{n, h, w}, cout, px, part);
+  }
   CHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sum_parts_kernel, grid_for(row), dim3(256), 0, st, part, n_chunks, row, dw);
+  hipLaunchKernelGGL(sum_parts_wide_kernel, dim3(static_cast<unsigned>((row + 31) / 32)), dim3(256), 0, st, part,
+                     n_chunks, row, dw);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -785,11 +1019,18 @@ int chip_nn_bias_grad(const uint16_t* dy, int64_t n_pix, int c, float* part, int
   CHIP_CHECK_ARG(dy && part && db && c >= 1 && n_pix >= 1, "bad argument");
   CHIP_CHECK_ARG(part_elems >= c, "partial-sum scratch too small");
   int n_chunks = 1;
-  const int64_t px = chunk_pixels(n_pix, c, part_elems, 1024, n_chunks);
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(channel_sum_kernel, dim3(n_chunks), dim3(256), 0, st, dy, n_pix, c, px, part);
+  if (vec_channels(c) && aligned16(dy)) {
+    const int64_t px = chunk_pixels(n_pix, c, part_elems, 4096, n_chunks);
+    hipLaunchKernelGGL(channel_sum_vec_kernel, dim3(n_chunks), dim3(256), 0, st, dy, n_pix, c, px, nullptr, nullptr,
+                       nullptr, 0, part);
+  } else {
+    const int64_t px = chunk_pixels(n_pix, c, part_elems, 1024, n_chunks);
+    hipLaunchKernelGGL(channel_sum_kernel, dim3(n_chunks), dim3(256), 0, st, dy, n_pix, c, px, part);
+  }
   CHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sum_parts_kernel, grid_for(c), dim3(256), 0, st, part, n_chunks, static_cast<int64_t>(c), db);
+  hipLaunchKernelGGL(sum_parts_wide_kernel, dim3(static_cast<unsigned>((c + 31) / 32)), dim3(256), 0, st, part,
+                     n_chunks, static_cast<int64_t>(c), db);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -856,7 +1097,12 @@ int chip_nn_head_fwd(const uint16_t* feat, int c, const float* wb, const float* 
                      void* stream) {
   clear_error();
   CHIP_CHECK_ARG(feat && wb && xin && out && c >= 1 && n_pix >= 1, "bad argument");
-  hipLaunchKernelGGL(head_fwd_kernel, grid_for(n_pix), dim3(256), 0, as_stream(stream), feat, c, wb, xin, out, n_pix);
+  if (c % 8 == 0 && aligned16(feat))
+    hipLaunchKernelGGL(head_fwd_vec_kernel, grid_for(n_pix), dim3(256), 0, as_stream(stream), feat, c, wb, xin, out,
+                       n_pix);
+  else
+    hipLaunchKernelGGL(head_fwd_kernel, grid_for(n_pix), dim3(256), 0, as_stream(stream), feat, c, wb, xin, out,
+                       n_pix);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -869,10 +1115,14 @@ int chip_nn_head_bwd(const float* dout, const uint16_t* feat, int c, const float
   int n_chunks = 1;
   const int64_t px = chunk_pixels(n_pix, c + 1, part_elems, 4096, n_chunks);
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(head_bwd_kernel, dim3(n_chunks), dim3(256), 0, st, dout, feat, c, wb, dfeat, n_pix, px, part);
+  if (vec_channels(c) && aligned16(feat) && aligned16(dfeat))
+    hipLaunchKernelGGL(channel_sum_vec_kernel, dim3(n_chunks), dim3(256), 0, st, feat, n_pix, c, px, dout, wb, dfeat,
+                       1, part);
+  else
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(n_chunks), dim3(256), 0, st, dout, feat, c, wb, dfeat, n_pix, px, part);
   CHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sum_parts_kernel, grid_for(c + 1), dim3(256), 0, st, part, n_chunks,
-                     static_cast<int64_t>(c + 1), dwb);
+  hipLaunchKernelGGL(sum_parts_wide_kernel, dim3(static_cast<unsigned>((c + 1 + 31) / 32)), dim3(256), 0, st, part,
+                     n_chunks, static_cast<int64_t>(c + 1), dwb);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
