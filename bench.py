@@ -466,6 +466,63 @@ def run_otf(args) -> dict:
     }
 
 
+def run_kernels(args) -> dict:
+    """The headline batch's projector kernels alone, for counter runs: the batched fp and exact-adjoint
+    bp of --slices 512 x 512 slices at 720 angles and 2 detectors (autotuned shapes) and a 1 GiB device
+    copy, each launched --fp-reps times after 3 warm-up launches and timed with HIP events."""
+    import torch
+
+    import pycorrectedemissionct_amd as pc
+    from pycorrectedemissionct_amd import batch as B
+
+    n, n_ang, nb = args.n, args.angles, args.slices
+    det = [np.pi / 2, -np.pi / 2]
+    ang = pc.testing.angles_uniform(n_ang)
+    fs, mis, mos = [], [], []
+    for k in range(nb):
+        wl = pc.testing.workload_pdhg_512(seed=20260102 + k, n=n, n_angles=n_ang)
+        fs.append(wl["f"]), mis.append(wl["mu_in"]), mos.append(wl["mu_out"])
+    A = B.BatchedProjectorXRF([n] * nb, ang, mis, mos, angles_detectors_rad=det, bp_mode="adjoint")
+    tune = B.autotune(A)
+    x = A.pack_vol(fs)
+    y = A.fp(x)
+    v = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    def time_launches(fn, reps) -> float:
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    t_fp = time_launches(lambda: A.fp(x, y), args.fp_reps)
+    t_bp = time_launches(lambda: A.bp(y, v), args.fp_reps)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    t_copy = time_launches(lambda: dst.copy_(src), args.fp_reps)
+    fp_bytes = nb * 4.0 * (n * n + n_ang * 2 * n * n + n_ang * 2 * n)
+    bp_bytes = fp_bytes
+    log(f"fp {1e6 * t_fp:.1f} us, bp {1e6 * t_bp:.1f} us, 1 GiB copy {1e6 * t_copy:.1f} us")
+    fp_gbs = fp_bytes / t_fp / 1e9
+    return {
+        "metric": "batched fp / exact-adjoint bp GB/s (algorithmic bytes)", "value": fp_gbs, "unit": "GB/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nb} x {n}x{n} slices, {n_ang} angles, 2 detectors, precomputed maps; "
+                               f"{args.fp_reps} launches each of the batched fp, the batched adjoint bp and a "
+                               f"1 GiB copy", "autotune": tune,
+                   "fp_us": 1e6 * t_fp, "bp_us": 1e6 * t_bp, "copy_us": 1e6 * t_copy,
+                   "fp_gbs": fp_gbs, "bp_gbs": bp_bytes / t_bp / 1e9, "copy_gbs": 8.0 * src.numel() / t_copy / 1e9,
+                   "algorithmic_bytes_per_launch": fp_bytes},
+        "roofline": {"bound": "hbm", "achieved": fp_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fp_gbs / HBM_PEAK_GBS, "traffic": None},
+    }
+
+
 def _unet_flops(levels: int, base: int, h: int, w: int) -> float:
     """Multiply-add flops (2 per MAC) of the U-Net's 3x3 convolutions on one h x w image."""
     from pycorrectedemissionct_amd.warmstart import layer_shapes
@@ -532,9 +589,10 @@ def run_unet(args) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["headline", "batched", "otf", "unet"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batched", "otf", "unet", "kernels"], default="headline",
                     help="headline: warm-started vs plain batched PDHG at 512^2 (default); batched / otf: the "
-                         "uneven batch and the on-the-fly maps; unet: the warm-start network's kernels")
+                         "uneven batch and the on-the-fly maps; unet: the warm-start network's kernels; kernels: "
+                         "the headline batch's fp / bp alone")
     ap.add_argument("--slices", type=int, default=None, help="slices in the batch (headline 2, batched 32)")
     ap.add_argument("--maps", choices=["precomputed", "otf"], default="precomputed",
                     help="batched: attenuation maps held in memory or rebuilt inside every fp / bp")
@@ -571,6 +629,9 @@ def main() -> int:
         res = run_batched(args)
     elif args.workload == "unet":
         res = run_unet(args)
+    elif args.workload == "kernels":
+        args.slices = args.slices or 2
+        res = run_kernels(args)
     else:
         res = run_otf(args)
     print(json.dumps(res), flush=True)

@@ -178,9 +178,10 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict
 // forward 3x3 convolution on gfx950's v_mfma_f32_16x16x32_bf16 (channel counts
 // that are multiples of 32: every layer of the U-Net but the first)
 //
-// A workgroup owns a spatial tile of 8 rows x 16 columns of one image and CO_T
-// output channels; wave w computes tile rows 2w and 2w + 1 (two 16-pixel MFMA
-// row blocks) for all CO_T channels. Input channels run in chunks of 32: the
+// A workgroup owns a spatial tile of 4 RB rows x 16 columns of one image and
+// CO_T output channels; wave w computes tile rows RB w .. RB w + RB - 1 (RB
+// 16-pixel MFMA row blocks, each weight operand used RB times) for all CO_T
+// channels. Input channels run in chunks of 32: the
 // chunk's 10 x 18 input patch (the tile plus its one-pixel halo, zero outside
 // the image, from x1 or x2 of a concatenation) is staged in LDS, pixel pitch 40
 // bf16 (80 bytes: the 16 A-operand lanes of a read fall in distinct banks); the
@@ -195,7 +196,7 @@ constexpr int kMtRows = 8, kMtCols = 16;                 // spatial tile
 constexpr int kMtPitch = 40;                             // bf16 per staged pixel (32 channels + 8 pad)
 constexpr int kMtPatch = (kMtRows + 2) * (kMtCols + 2);  // staged pixels
 
-template <int CO_T>
+template <int CO_T, int RB>
 __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __restrict__ x1, int c1,
                                                            const uint16_t* __restrict__ x2, int c2,
                                                            const uint16_t* __restrict__ wt,
@@ -203,25 +204,27 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
                                                            int o1, uint16_t* __restrict__ y2, Img im, int cout,
                                                            int relu) {
   constexpr int kCb = CO_T / 16;  // MFMA column blocks
+  constexpr int kRows = 4 * RB;    // tile rows: RB row blocks per wave
+  constexpr int kPatch = (kRows + 2) * (kMtCols + 2);
   constexpr int kOutPitch = CO_T + 8;  // bf16 per pixel of the staged output tile
-  constexpr int kLdsHalfs = kMtPatch * kMtPitch > kMtRows * kMtCols * kOutPitch ? kMtPatch * kMtPitch
-                                                                                : kMtRows * kMtCols * kOutPitch;
+  constexpr int kLdsHalfs = kPatch * kMtPitch > kRows * kMtCols * kOutPitch ? kPatch * kMtPitch
+                                                                            : kRows * kMtCols * kOutPitch;
   __shared__ __align__(16) uint16_t patch[kLdsHalfs];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tiles_w = (im.w + kMtCols - 1) / kMtCols, tiles_h = (im.h + kMtRows - 1) / kMtRows;
+  const int tiles_w = (im.w + kMtCols - 1) / kMtCols, tiles_h = (im.h + kRows - 1) / kRows;
   int t = blockIdx.x;
   const int tx = t % tiles_w;
   t /= tiles_w;
   const int ty = t % tiles_h;
   const int img = t / tiles_h;
-  const int h0 = ty * kMtRows, w0 = tx * kMtCols;
+  const int h0 = ty * kRows, w0 = tx * kMtCols;
   const int cbase = blockIdx.y * CO_T;
   const int cin = c1 + c2;
   const int m = lane & 15, q = lane >> 4;
 
-  f4v acc[2][kCb];
+  f4v acc[RB][kCb];
 #pragma unroll
-  for (int b = 0; b < 2; ++b)
+  for (int b = 0; b < RB; ++b)
 #pragma unroll
     for (int c = 0; c < kCb; ++c) acc[b][c] = f4v{0.f, 0.f, 0.f, 0.f};
 
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
     const int cs = ci0 < c1 ? c1 : c2;
     const int co0 = ci0 < c1 ? ci0 : ci0 - c1;
     __syncthreads();  // the previous chunk's reads are done
-    for (int k = threadIdx.x; k < kMtPatch * 4; k += 256) {
+    for (int k = threadIdx.x; k < kPatch * 4; k += 256) {
       const int px = k >> 2, part = k & 3;  // 4 x 16 bytes per pixel
       const int pr = px / (kMtCols + 2), pc = px - pr * (kMtCols + 2);
       const int hh = h0 + pr - 1, ww = w0 + pc - 1;
@@ -253,8 +256,8 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
         bm[c] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wr));
       }
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int pr = 2 * wave + b + dh, pc = m + dw;
+      for (int b = 0; b < RB; ++b) {
+        const int pr = RB * wave + b + dh, pc = m + dw;
         const bf8v a = __builtin_bit_cast(
             bf8v, *reinterpret_cast<const uint4*>(patch + (pr * (kMtCols + 2) + pc) * kMtPitch + 8 * q));
 #pragma unroll
@@ -271,17 +274,17 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
     const int col = 16 * c + m;
     const float bb = bias ? bias[cbase + col] : 0.f;
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < RB; ++b)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         float v = acc[b][c][i] + bb;
         if (relu) v = fmaxf(v, 0.f);
-        patch[((2 * wave + b) * kMtCols + 4 * q + i) * kOutPitch + col] = f2bf(v);
+        patch[((RB * wave + b) * kMtCols + 4 * q + i) * kOutPitch + col] = f2bf(v);
       }
   }
   __syncthreads();
   constexpr int kGroups = CO_T / 8;
-  for (int k = threadIdx.x; k < kMtRows * kMtCols * kGroups; k += 256) {
+  for (int k = threadIdx.x; k < kRows * kMtCols * kGroups; k += 256) {
     const int px = k / kGroups, g = k - px * kGroups;
     const int hh = h0 + px / kMtCols, ww = w0 + px % kMtCols;
     if (hh >= im.h || ww >= im.w) continue;
@@ -650,6 +653,77 @@ __global__ void flip_weights_kernel(const uint16_t* __restrict__ wt, int cout, i
   }
 }
 
+// relu_mask_kernel / add_bf16_kernel on 8 elements (16 bytes) per step
+__global__ void relu_mask_vec_kernel(uint16_t* __restrict__ d, const uint16_t* __restrict__ y, int64_t count8) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count8;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const u16x8 yv = __builtin_bit_cast(u16x8, reinterpret_cast<const uint4*>(y)[k]);
+    u16x8 dv = __builtin_bit_cast(u16x8, reinterpret_cast<const uint4*>(d)[k]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (!(bf2f(yv[j]) > 0.f)) dv[j] = 0;
+    reinterpret_cast<uint4*>(d)[k] = __builtin_bit_cast(uint4, dv);
+  }
+}
+
+__global__ void add_bf16_vec_kernel(uint16_t* __restrict__ y, const uint16_t* __restrict__ x, int64_t n8) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n8;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const u16x8 xv = __builtin_bit_cast(u16x8, reinterpret_cast<const uint4*>(x)[k]);
+    u16x8 yv = __builtin_bit_cast(u16x8, reinterpret_cast<uint4*>(y)[k]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) yv[j] = f2bf(bf2f(yv[j]) + bf2f(xv[j]));
+    reinterpret_cast<uint4*>(y)[k] = __builtin_bit_cast(uint4, yv);
+  }
+}
+
+// weight gradient of the 1-channel input layer: part[chunk][tap][co] = sum_p dy[p][co] x[p + tap].
+// cout = 8 G (G a power of two <= 256): thread (g = tid % G, pl = tid / G) takes output channels
+// [8 g, 8 g + 8) of pixels pl, pl + 256 / G, ... of the chunk, all 9 taps; the pixel lanes are
+// then added in order, one tap at a time through LDS.
+__global__ __launch_bounds__(256) void conv3x3_wgrad_c1_kernel(const uint16_t* __restrict__ dy,
+                                                               const uint16_t* __restrict__ x, Img im, int cout,
+                                                               int64_t chunk_px, float* __restrict__ part) {
+  __shared__ float red[256 * 8];
+  const int G = cout / 8, PP = 256 / G;
+  const int g = threadIdx.x % G, pl = threadIdx.x / G;
+  const int64_t P = im.pixels();
+  const int64_t p_lo = static_cast<int64_t>(blockIdx.x) * chunk_px;
+  const int64_t p_hi = min(P, p_lo + chunk_px);
+  float acc[9][8];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[t][j] = 0.f;
+  for (int64_t p = p_lo + pl; p < p_hi; p += PP) {
+    const u16x8 e = __builtin_bit_cast(u16x8, *reinterpret_cast<const uint4*>(dy + p * cout + 8 * g));
+    const int pw = static_cast<int>(p % im.w);
+    const int64_t r = p / im.w;
+    const int ph = static_cast<int>(r % im.h);
+    const int64_t base = p - pw - static_cast<int64_t>(ph) * im.w;  // the image's first pixel
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int hh = ph + t / 3 - 1, ww = pw + t % 3 - 1;
+      const float xv = (hh >= 0 && hh < im.h && ww >= 0 && ww < im.w)
+                           ? bf2f(x[base + static_cast<int64_t>(hh) * im.w + ww])
+                           : 0.f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[t][j] += bf2f(e[j]) * xv;
+    }
+  }
+  for (int t = 0; t < 9; ++t) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) red[pl * cout + 8 * g + j] = acc[t][j];
+    __syncthreads();
+    for (int co = threadIdx.x; co < cout; co += 256) {
+      float sum = 0.f;
+      for (int k = 0; k < PP; ++k) sum += red[k * cout + co];
+      part[(static_cast<size_t>(blockIdx.x) * 9 + t) * cout + co] = sum;
+    }
+  }
+}
+
 __global__ void relu_mask_kernel(uint16_t* __restrict__ d, const uint16_t* __restrict__ y, int64_t count) {
   for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
        k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -853,12 +927,20 @@ __global__ void mse_kernel(const float* __restrict__ out, const float* __restric
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void sum_double_kernel(const double* __restrict__ part, int n, double scale, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += part[i];
-    out[0] = s * scale;
+// out = scale * sum part[0 .. n): thread t sums part[t], part[t + 256], ... in order, then a
+// fixed pairwise tree over the 256 thread sums (one workgroup of 256 threads)
+__global__ __launch_bounds__(256) void sum_double_kernel(const double* __restrict__ part, int n, double scale,
+                                                         double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
 }
 
 // Adam on fp32 master weights; optional bf16 copy of the result
@@ -950,16 +1032,27 @@ int chip_nn_conv3x3(const uint16_t* x1, int c1, const uint16_t* x2, int c2, cons
     return CHIP_OK;
   }
   if (mfma32) {
-    const int64_t tiles = static_cast<int64_t>(n) * ((h + kMtRows - 1) / kMtRows) * ((w + kMtCols - 1) / kMtCols);
-    CHIP_CHECK_ARG(tiles <= 2147483647, "too many pixels");
     const bool co64 = cout % 64 == 0;
-    dim3 grid(static_cast<unsigned>(tiles), cout / (co64 ? 64 : 32));
-    if (co64)
-      hipLaunchKernelGGL(conv3x3_mfma_kernel<64>, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1,
-                         o1, y2, Img{n, h, w}, cout, relu);
+    const int n_cob = cout / (co64 ? 64 : 32);
+    // 16-row tiles (4 row blocks per wave) where they still give 1024 workgroups, else 8-row tiles
+    const int64_t tiles16 = static_cast<int64_t>(n) * ((h + 15) / 16) * ((w + kMtCols - 1) / kMtCols);
+    const bool rb4 = tiles16 * n_cob >= 1024;
+    const int64_t tiles = rb4 ? tiles16 : static_cast<int64_t>(n) * ((h + 7) / 8) * ((w + kMtCols - 1) / kMtCols);
+    CHIP_CHECK_ARG(tiles <= 2147483647, "too many pixels");
+    dim3 grid(static_cast<unsigned>(tiles), n_cob);
+    hipStream_t st = as_stream(stream);
+    if (co64 && rb4)
+      hipLaunchKernelGGL((conv3x3_mfma_kernel<64, 4>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
+                         Img{n, h, w}, cout, relu);
+    else if (co64)
+      hipLaunchKernelGGL((conv3x3_mfma_kernel<64, 2>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
+                         Img{n, h, w}, cout, relu);
+    else if (rb4)
+      hipLaunchKernelGGL((conv3x3_mfma_kernel<32, 4>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
+                         Img{n, h, w}, cout, relu);
     else
-      hipLaunchKernelGGL(conv3x3_mfma_kernel<32>, grid, dim3(256), 0, as_stream(stream), x1, c1, x2, c2, wt, bias, y1,
-                         o1, y2, Img{n, h, w}, cout, relu);
+      hipLaunchKernelGGL((conv3x3_mfma_kernel<32, 2>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
+                         Img{n, h, w}, cout, relu);
     CHIP_LAUNCH_CHECK();
     return CHIP_OK;
   }
@@ -1000,6 +1093,10 @@ int chip_nn_conv3x3_wgrad(const uint16_t* dy, const uint16_t* x1, int c1, const 
     else
       hipLaunchKernelGGL(conv3x3_wgrad_mfma_kernel<32>, grid, dim3(256), 0, st, dy, x1, c1, x2, c2, Img{n, h, w}, cout,
                          static_cast<int>(tpc), part);
+  } else if (c1 == 1 && c2 == 0 && vec_channels(cout) && aligned16(dy)) {
+    const int64_t px = chunk_pixels(P, row, part_elems, 1024, n_chunks);
+    hipLaunchKernelGGL(conv3x3_wgrad_c1_kernel, dim3(n_chunks), dim3(256), 0, st, dy, x1, Img{n, h, w}, cout, px,
+                       part);
   } else {
     const int64_t px = chunk_pixels(P, row, part_elems, 256, n_chunks);
     const int n_tiles = 9 * ((cout + 31) / 32) * ((cin + 31) / 32);
@@ -1021,7 +1118,7 @@ int chip_nn_bias_grad(const uint16_t* dy, int64_t n_pix, int c, float* part, int
   int n_chunks = 1;
   hipStream_t st = as_stream(stream);
   if (vec_channels(c) && aligned16(dy)) {
-    const int64_t px = chunk_pixels(n_pix, c, part_elems, 4096, n_chunks);
+    const int64_t px = chunk_pixels(n_pix, c, part_elems, 1024, n_chunks);
     hipLaunchKernelGGL(channel_sum_vec_kernel, dim3(n_chunks), dim3(256), 0, st, dy, n_pix, c, px, nullptr, nullptr,
                        nullptr, 0, part);
   } else {
@@ -1047,7 +1144,10 @@ int chip_nn_flip_weights(const uint16_t* wt, int cout, int cin, uint16_t* wt_t, 
 int chip_nn_relu_mask(uint16_t* d, const uint16_t* y, int64_t count, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(d && y && count >= 0, "bad argument");
-  hipLaunchKernelGGL(relu_mask_kernel, grid_for(count), dim3(256), 0, as_stream(stream), d, y, count);
+  if (count % 8 == 0 && aligned16(d) && aligned16(y))
+    hipLaunchKernelGGL(relu_mask_vec_kernel, grid_for(count / 8), dim3(256), 0, as_stream(stream), d, y, count / 8);
+  else
+    hipLaunchKernelGGL(relu_mask_kernel, grid_for(count), dim3(256), 0, as_stream(stream), d, y, count);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -1113,9 +1213,10 @@ int chip_nn_head_bwd(const float* dout, const uint16_t* feat, int c, const float
   CHIP_CHECK_ARG(dout && feat && wb && dfeat && part && dwb && c >= 1 && n_pix >= 1, "bad argument");
   CHIP_CHECK_ARG(part_elems >= c + 1, "partial-sum scratch too small");
   int n_chunks = 1;
-  const int64_t px = chunk_pixels(n_pix, c + 1, part_elems, 4096, n_chunks);
+  const bool vec = vec_channels(c) && aligned16(feat) && aligned16(dfeat);
+  const int64_t px = chunk_pixels(n_pix, c + 1, part_elems, vec ? 1024 : 4096, n_chunks);
   hipStream_t st = as_stream(stream);
-  if (vec_channels(c) && aligned16(feat) && aligned16(dfeat))
+  if (vec)
     hipLaunchKernelGGL(channel_sum_vec_kernel, dim3(n_chunks), dim3(256), 0, st, feat, n_pix, c, px, dout, wb, dfeat,
                        1, part);
   else
@@ -1135,7 +1236,7 @@ int chip_nn_mse(const float* out, const float* tgt, int64_t n_pix, float* dout, 
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(mse_kernel, dim3(blocks), dim3(256), 0, st, out, tgt, n_pix, dout, part);
   CHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(64), 0, st, part, blocks, 1.0 / static_cast<double>(n_pix),
+  hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(256), 0, st, part, blocks, 1.0 / static_cast<double>(n_pix),
                      loss);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
@@ -1156,7 +1257,10 @@ int chip_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float 
 int chip_nn_add_bf16(uint16_t* y, const uint16_t* x, int64_t n, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(x && y && n >= 0, "bad argument");
-  hipLaunchKernelGGL(add_bf16_kernel, grid_for(n), dim3(256), 0, as_stream(stream), y, x, n);
+  if (n % 8 == 0 && aligned16(x) && aligned16(y))
+    hipLaunchKernelGGL(add_bf16_vec_kernel, grid_for(n / 8), dim3(256), 0, as_stream(stream), y, x, n / 8);
+  else
+    hipLaunchKernelGGL(add_bf16_kernel, grid_for(n), dim3(256), 0, as_stream(stream), y, x, n);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

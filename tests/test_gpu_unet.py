@@ -42,11 +42,15 @@ def _from_bits(t):
 
 
 @pytest.mark.parametrize("c1,c2,cout,o1,h,w", [(1, 0, 24, 24, 9, 13), (8, 8, 32, 32, 16, 16), (16, 8, 40, 24, 6, 10),
+                                               # the input layer's own weight-gradient kernel (cout = 8 * 2^k)
+                                               (1, 0, 32, 32, 37, 29),
                                                # channel counts of 32s: the 16x16x32 MFMA kernel, ragged tiles
                                                (32, 0, 64, 64, 9, 21), (64, 32, 32, 32, 8, 16),
                                                (32, 32, 64, 32, 5, 33), (128, 128, 256, 256, 4, 6),
                                                # several 8 x 16 tiles per weight-gradient workgroup, ragged edges
-                                               (32, 0, 32, 32, 250, 200)])
+                                               (32, 0, 32, 32, 250, 200),
+                                               # 16-row forward tiles (grids of >= 1024 tiles), ragged edges
+                                               (32, 0, 64, 64, 520, 264), (64, 0, 32, 32, 264, 520)])
 def test_conv3x3_and_wgrad(pkg, c1, c2, cout, o1, h, w):
     import torch
 
