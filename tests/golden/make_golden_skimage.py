@@ -13,6 +13,9 @@ line it follows:
                   -> rotate(+deg) -> crop -> exp(-.)              (``_projector_backends.py:63-97``)
 * maps         = ones *= E_in(theta); *= E_out(theta + phi_d + pi) (``physics/attenuation.py:105-161``)
 * XRF fp       = w_d * radon(vol * M[a, d])                       (``projectors.py:519-525``)
+* XRF symmetric bp = sum_a sum_d (w_d * bp_a(sino[d, a])) * M[a, d], bp_a the one-angle
+                  ``iradon(line[:, None], [deg_a], ...) * 2 / pi``   (``projectors.py:556-565, 612-616``,
+                                                                     ``_projector_backends.py:357-359``)
 
 Angles follow corrct's dtypes: rotation angles are float32 radians converted
 with ``np.rad2deg`` (``_projector_backends.py:148, 299``); outgoing-beam
@@ -51,6 +54,10 @@ def corrct_bp(prj, degs, n):
         vol[ii] = skt.iradon(prj[ii, :, np.newaxis], [a], circle=False, output_size=n, filter_name=None)
     vol = vol.sum(axis=0)
     return vol * 2 / np.pi
+
+
+def corrct_bp_angle(line, degs_one, n):
+    return skt.iradon(line[:, np.newaxis], degs_one, circle=False, output_size=n, filter_name=None) * 2 / np.pi
 
 
 def corrct_compute_attenuation(vol, angle_rad, invert=False):
@@ -140,6 +147,21 @@ def main():
     out["xrf_maps"] = maps
     out["xrf_wdet"] = w_det
     out["xrf_sino"] = sino
+
+    # --- XRF symmetric bp (is_symmetric=True): the weights are float64 (weights_det times the
+    # float64 default angle weights), each detector's one-angle bp is scaled, stacked, multiplied
+    # by the angle's maps in place and summed over detectors, then the angles are summed
+    sino_b = rng.random((2, len(rad), n)).astype(np.float32)
+    w_ang = np.ones((len(rad), 2))
+    vols = []
+    for ia in range(len(rad)):
+        weights = w_det * w_ang[ia, :]
+        per = [weights[d] * corrct_bp_angle(sino_b[d, ia], degs[ia:ia + 1], n) for d in range(2)]
+        per = np.stack(per, axis=0)
+        per *= maps[ia]
+        vols.append(np.sum(per, axis=0))
+    out["xrf_bp_sino"] = sino_b
+    out["xrf_bp_sym"] = np.sum(vols, axis=0)
 
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, "with", len(out), "arrays; skimage", skimage.__version__)

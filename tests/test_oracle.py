@@ -70,6 +70,19 @@ def test_golden_xrf_fp_bit_exact(golden):
     assert np.array_equal(sino, golden["xrf_sino"])
 
 
+def test_golden_xrf_symmetric_bp_bit_exact(golden):
+    """The symmetric XRF bp's composition (per angle: weighted one-angle iradon per detector,
+    times that detector's map, summed over detectors; then over angles) against scikit-image
+    composed as corrct does (projectors.py:556-565, 612-616); swapping the detectors' maps, a
+    plausible indexing slip, misses by far."""
+    n = golden["xrf_vol"].shape[-1]
+    args = (golden["xrf_bp_sino"], golden["xrf_rad"], n)
+    got = oxrf.bp_skimage(*args, golden["xrf_maps"], w_det=golden["xrf_wdet"], symmetric=True)
+    assert np.array_equal(got, golden["xrf_bp_sym"])
+    swapped = oxrf.bp_skimage(*args, golden["xrf_maps"][:, ::-1], w_det=golden["xrf_wdet"], symmetric=True)
+    assert np.abs(swapped - golden["xrf_bp_sym"]).max() > 1e-2 * np.abs(golden["xrf_bp_sym"]).max()
+
+
 def test_angle_conversions_match_numpy():
     x = (np.random.default_rng(0).random(10000) * 7).astype(F32)
     assert np.array_equal(sk.rad2deg_f32(x), np.rad2deg(x))
