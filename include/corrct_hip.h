@@ -224,6 +224,13 @@ int chip_absmax(const float* v, int64_t n, float* out, void* stream);
  * max = maxval[0] + add and maxval the device scalar chip_absmax(v) wrote.
  * compute_diagonal_scaling (solvers.py:98-116). */
 int chip_precond_fix(float* v, int64_t n, float add, const float* maxval, float rho, void* stream);
+/* y[i] += a * x[i % x_period]: an array-valued regularizer step-size term (tau * upd_mask,
+ * regularizers.py:260-273) added to tau in compute_diagonal_scaling / compute_Lipschitz_scaling
+ * (solvers.py:99-101, 150-153). */
+int chip_axpy(float* y, float a, const float* x, int64_t x_period, int64_t n, void* stream);
+/* v[i] = rho / v[i]: compute_Lipschitz_scaling's tau = relaxation_tau / tau for an array tau
+ * (solvers.py:155). */
+int chip_rdiv(float* v, float rho, int64_t n, void* stream);
 
 /* p = prox_l2(p + sigma * (ax + bg)) * mask, prox_l2(p) = (p - sigma*b) / (1 + sigma).
  * DataFidelity_l2.update_dual / apply_proximal_dual (data_terms.py:217-230,
@@ -295,13 +302,17 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * clipped at 1, Regularizer_TNV with DataFidelity_ln, regularizers.py:1324-1359,
  * data_terms.py:781-820). Modes 0 and 1 need n_chan == 1; n_chan <= 16.
  * chip_grad_update_primal: upd += t (or -= t when subtract) with
- * t = -div(q) times scale when apply_scale (compute_update_primal,
- * regularizers.py:194-210; PDHG adds, SIRT subtracts, solvers.py:953-958, 1178-1180).
+ * t = -div(q), times mask[e % mask_period] when mask is not NULL (the
+ * regularizer's upd_mask, broadcast over leading axes: mask_period divides
+ * the primal's element count), times scale when apply_scale
+ * (compute_update_primal, regularizers.py:194-210; PDHG adds, SIRT
+ * subtracts, solvers.py:953-958, 1178-1180).
  */
 int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
                    float weight, int der_norm, int chan_norm, void* stream);
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, int apply_scale, int subtract, void* stream);
+                            float scale, int apply_scale, int subtract, const float* mask, int64_t mask_period,
+                            void* stream);
 
 /* FISTA step (solvers.py:1458-1490): x_new = y + tau * grad (tau per element,
  * or tau_scalar when tau is NULL), soft-thresholded at tau * l1_weight when

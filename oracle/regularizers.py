@@ -15,8 +15,9 @@ Test infrastructure only (see ``oracle/__init__.py``).
   (channels x derivatives) matrix, singular values ``s / max(s, 1)``.
 * ``GradReg`` -- ``Regularizer_Grad`` bookkeeping (``regularizers.py:134-215,
   260-273``): tau contribution ``2 nd`` (times the weight outside the l1
-  family), dual update ``q += 0.5 grad`` then the prox, primal contribution
-  ``-div q`` (times the weight outside the l1 family).
+  family, times ``upd_mask`` when given), dual update ``q += 0.5 grad`` then
+  the prox, primal contribution ``-div q`` (times ``upd_mask``, then times the
+  weight outside the l1 family).
 """
 
 from __future__ import annotations
@@ -100,16 +101,18 @@ def prox_tnv(q: np.ndarray) -> np.ndarray:
 class GradReg:
     """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv"}."""
 
-    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf):
+    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None):
         self.nd, self.weight, self.kind, self.der, self.chan = nd, float(weight), kind, der, chan
+        self.upd_mask = None if upd_mask is None else np.asarray(upd_mask, dtype=F32)
 
     @property
     def weighted(self) -> bool:
         return self.kind == "tnv"  # outside the l1 family
 
-    def tau(self) -> float:
+    def tau(self):
         t = 2.0 * self.nd
-        return t * self.weight if self.weighted else t
+        t = t * self.weight if self.weighted else t
+        return t if self.upd_mask is None else t * self.upd_mask
 
     def dual(self, q: np.ndarray, x: np.ndarray) -> np.ndarray:
         q = q + F32(0.5) * grad_nd(x, self.nd)
@@ -123,4 +126,6 @@ class GradReg:
 
     def primal(self, q: np.ndarray) -> np.ndarray:
         u = -div_nd(q)
+        if self.upd_mask is not None:
+            u = u * self.upd_mask
         return u * F32(self.weight) if self.weighted else u

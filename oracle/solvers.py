@@ -41,7 +41,9 @@ def diagonal_scaling(A, At, b, x_shape, tau_reg=0.0, rho_sigma=1.0, rho_tau=1.0,
     if b_mask is not None:
         b_ones = b_ones * b_mask
     tau = np.abs(At(b_ones)).astype(F32)
-    if tau_reg:
+    if np.ndim(tau_reg):  # update masks make the regularizers' contribution an array
+        tau = tau + np.asarray(tau_reg, dtype=F32)
+    elif tau_reg:
         tau = tau + F32(tau_reg)
     tau = _fix(tau)
     tau = (F32(rho_tau) / tau).astype(F32)
@@ -234,7 +236,10 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
         )
     else:
         sigma = np.full(b.shape, F32(relaxation / L), dtype=F32)
-        tau = np.full(x_shape, F32(relaxation / (L + tau_reg)), dtype=F32)
+        if np.ndim(tau_reg):
+            tau = (F32(relaxation) / (F32(L) + np.asarray(tau_reg, dtype=F32))).astype(F32)
+        else:
+            tau = np.full(x_shape, F32(relaxation / (L + tau_reg)), dtype=F32)
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
     xr = x.copy()
     if data_term == "l2":

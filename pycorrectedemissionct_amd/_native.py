@@ -43,7 +43,9 @@ _SIGNATURES = {
     "chip_grad_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int,
                        ctypes.c_float, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_grad_update_primal": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p,
-                                ctypes.c_float, ctypes.c_int, ctypes.c_int, c_void_p],
+                                ctypes.c_float, ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_axpy": [c_void_p, ctypes.c_float, c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p],
+    "chip_rdiv": [c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p],
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],

@@ -377,6 +377,8 @@ class BatchedPDHG:
         reg = self.regularizer
         if not isinstance(reg, Regularizer_Grad):
             raise NotImplementedError("the batched PDHG takes gradient-family regularizers")
+        if reg.upd_mask is not None:
+            raise NotImplementedError("the batched PDHG takes regularizers without an update mask")
         c = self.channels
         if len(A.sizes) % c:
             raise ValueError(f"{len(A.sizes)} slices do not split into groups of {c} channels")

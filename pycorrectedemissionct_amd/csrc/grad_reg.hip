@@ -253,7 +253,8 @@ void launch_grad_dual(float* q, const float* x, const GradGeom& g, int mode, flo
 
 __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict__ upd, const float* __restrict__ q,
                                                                GradGeom g, float scale, int apply_scale,
-                                                               int subtract) {
+                                                               int subtract, const float* __restrict__ mask,
+                                                               int64_t mask_period) {
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict
       d = (k == 0) ? dk : d + dk;
     }
     float t = -d;
+    if (mask) t = t * mask[e % mask_period];
     if (apply_scale) t = t * scale;
     upd[e] = subtract ? upd[e] - t : upd[e] + t;
   }
@@ -326,14 +328,18 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
 }
 
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, int apply_scale, int subtract, void* stream) {
+                            float scale, int apply_scale, int subtract, const float* mask, int64_t mask_period,
+                            void* stream) {
   clear_error();
   CHIP_CHECK_ARG(upd != nullptr && q != nullptr, "NULL argument");
   GradGeom g;
   const int rc = make_geom(g, n_chan, n_batch, nd, dims);
   if (rc != CHIP_OK) return rc;
-  hipLaunchKernelGGL(grad_primal_kernel, gr_grid(static_cast<int64_t>(g.C) * g.B * g.S), dim3(kGrBlock), 0,
-                     as_stream(stream), upd, q, g, scale, apply_scale, subtract);
+  const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
+  CHIP_CHECK_ARG(mask == nullptr || (mask_period >= 1 && total % mask_period == 0),
+                 "mask_period must divide the primal's element count");
+  hipLaunchKernelGGL(grad_primal_kernel, gr_grid(total), dim3(kGrBlock), 0, as_stream(stream), upd, q, g, scale,
+                     apply_scale, subtract, mask, mask_period);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

@@ -100,7 +100,7 @@ static const char* kSymbols[] = {
     "chip_nn_conv3x3",    "chip_nn_conv3x3_wgrad", "chip_nn_bias_grad", "chip_nn_flip_weights",
     "chip_nn_relu_mask",  "chip_nn_avgpool2",  "chip_nn_upsample2", "chip_nn_head_fwd",
     "chip_nn_head_bwd",   "chip_nn_mse",       "chip_nn_adam",      "chip_nn_to_bf16",
-    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",  "chip_psf_lines",     "chip_plan_set_subangles", "chip_grad_dual", "chip_grad_update_primal", "chip_bp_noalloc", "chip_plan_set_radon_trig", "chip_batch_set_radon_trig", "chip_clip_relax", "chip_batch_fp_otf", "chip_batch_bp_otf",
+    "chip_nn_to_f32",     "chip_nn_add_bf16",  "chip_pdhg_dual_init", "chip_fista_update",  "chip_filter_lines",  "chip_psf_lines",     "chip_plan_set_subangles", "chip_grad_dual", "chip_grad_update_primal", "chip_bp_noalloc", "chip_plan_set_radon_trig", "chip_batch_set_radon_trig", "chip_clip_relax", "chip_batch_fp_otf", "chip_batch_bp_otf", "chip_axpy", "chip_rdiv",
 };
 
 extern "C" {

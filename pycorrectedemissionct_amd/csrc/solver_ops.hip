@@ -84,6 +84,17 @@ __global__ __launch_bounds__(kEwBlock) void precond_fix_kernel(float* __restrict
   }
 }
 
+// y[i] += a * x[i % period] (a regularizer's masked step-size term added to tau)
+__global__ __launch_bounds__(kEwBlock) void axpy_kernel(float* __restrict__ y, float a, const float* __restrict__ x,
+                                                        int64_t period, int64_t n) {
+  GRID_STRIDE(i, n) { y[i] = y[i] + a * x[i % period]; }
+}
+
+// v[i] = rho / v[i]
+__global__ __launch_bounds__(kEwBlock) void rdiv_kernel(float* __restrict__ v, float rho, int64_t n) {
+  GRID_STRIDE(i, n) { v[i] = rho / v[i]; }
+}
+
 // ---- PDHG data duals ----------------------------------------------------------
 
 __global__ __launch_bounds__(kEwBlock) void dual_l2_kernel(float* __restrict__ p, const float* __restrict__ ax,
@@ -399,6 +410,23 @@ int chip_precond_fix(float* v, int64_t n, float add, const float* maxval, float 
   CHIP_CHECK_ARG(v != nullptr && maxval != nullptr, "NULL pointer");
   CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
   hipLaunchKernelGGL(precond_fix_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), v, n, add, maxval, rho);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_axpy(float* y, float a, const float* x, int64_t x_period, int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(y != nullptr && x != nullptr, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1 && x_period >= 1 && n % x_period == 0, "x_period must divide n");
+  hipLaunchKernelGGL(axpy_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), y, a, x, x_period, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_rdiv(float* v, float rho, int64_t n, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(v != nullptr && n >= 1, "bad argument");
+  hipLaunchKernelGGL(rdiv_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), v, rho, n);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

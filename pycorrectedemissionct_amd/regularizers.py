@@ -109,14 +109,40 @@ class Regularizer_Grad(BaseRegularizer):
         self.pad_mode = pad_mode.lower()
         self.sigma = 0.5
 
-    def initialize_sigma_tau(self, primal) -> float:
-        """Sets up the gradient operator; returns the tau contribution (``regularizers.py:260-273``)."""
+    def initialize_sigma_tau(self, primal):
+        """Sets up the gradient operator; returns the tau contribution (``regularizers.py:260-273``):
+        a float, or ``tau * upd_mask`` when an update mask is set."""
         self.op = operators.TransformGradient(np.shape(primal), axes=self.axes, pad_mode=self.pad_mode)
         self.sigma = 0.5
         tau = 2 * self.ndims
         if not isinstance(self.norm, dt.DataFidelity_l1):
             tau *= self.weight
+        if self.upd_mask is not None:
+            return tau * self.upd_mask
         return tau
+
+    def tau_terms(self, shape):
+        """(scalar tau contribution, update mask as a flat device tensor or None): the factors of
+        ``initialize_sigma_tau``'s result, for the solvers' device kernels."""
+        tau = 2 * self.ndims
+        if not isinstance(self.norm, dt.DataFidelity_l1):
+            tau *= self.weight
+        return float(tau), (None if self.upd_mask is None else self._device_mask(shape))
+
+    def _device_mask(self, shape) -> torch.Tensor:
+        """The update mask as a contiguous float32 device tensor; its shape must be a trailing part of
+        the primal's (it broadcasts over the leading axes, as NumPy broadcasts it in the reference)."""
+        m = self.upd_mask
+        t = m if isinstance(m, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(m, dtype=np.float32))
+        t = t.to(device=_arrays.device(), dtype=torch.float32).contiguous()
+        mshape = tuple(int(v) for v in t.shape)
+        while mshape and mshape[0] == 1:
+            mshape = mshape[1:]
+        shape = tuple(int(v) for v in shape)
+        if len(mshape) > len(shape) or (mshape and shape[len(shape) - len(mshape):] != mshape):
+            raise ValueError(f"upd_mask of shape {tuple(t.shape)} does not broadcast over a primal of shape {shape} "
+                             "along its leading axes")
+        return t.reshape(-1)
 
     def apply_proximal_dual(self, dual) -> None:
         """The norm's dual proximal in place (``regularizers.py:150-162``; VTV / TNV override the
@@ -132,8 +158,6 @@ class Regularizer_Grad(BaseRegularizer):
             raise NotImplementedError("the gradient axes must be given in increasing order")
         if self.pad_mode != "edge":
             raise NotImplementedError("the HIP gradient kernels implement edge padding")
-        if self.upd_mask is not None:
-            raise NotImplementedError("upd_mask is not supported by the HIP gradient kernels")
         if np.asarray(self.weight).size != 1:
             raise NotImplementedError("the HIP gradient kernels take a scalar weight")
 
@@ -153,10 +177,12 @@ class Regularizer_Grad(BaseRegularizer):
 
     def _config(self, shape, mode, n_chan, n_batch, der, chan) -> dict:
         l1_family = isinstance(self.norm, dt.DataFidelity_l1)
+        mask = None if self.upd_mask is None else self._device_mask(shape)
         return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=int(self.ndims),
                     dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
                     weight=float(self.weight), der=der, chan=chan, scale=float(self.weight),
-                    apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape))
+                    apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape),
+                    mask=mask, mask_period=1 if mask is None else int(mask.numel()))
 
     def check_fused(self) -> float:
         """The weight, after checking the fused TV2D kernels implement this configuration."""

@@ -86,7 +86,11 @@ def compute_diagonal_scaling(A_abs, At_abs, b: torch.Tensor, regs: Sequence, rel
     tau = _arrays.to_dev(At_abs(b_ones))[0].clone()
     tau_reg = 0.0
     for reg in regs:
-        tau_reg += float(reg.initialize_sigma_tau(tau))
+        t_reg, mask = _reg_tau(reg, tau)
+        if mask is None:
+            tau_reg += t_reg
+        else:  # tau += t_reg * upd_mask (regularizers.py:271-272)
+            _native.call("chip_axpy", tau.data_ptr(), t_reg, mask.data_ptr(), mask.numel(), tau.numel(), st)
     mx = torch.empty(1, dtype=torch.float32, device=tau.device)
     _native.call("chip_absmax", tau.data_ptr(), tau.numel(), mx.data_ptr(), st)
     _native.call("chip_precond_fix", tau.data_ptr(), tau.numel(), float(tau_reg), mx.data_ptr(), float(relaxation_tau), st)
@@ -116,14 +120,36 @@ def power_method(A, b, iterations: int = 5):
     return float(L), tuple(x.shape), np.float32
 
 
+def _reg_tau(reg, primal):
+    """(scalar, flat device update mask or None) of a regularizer's tau contribution; sets the
+    regularizer up as ``initialize_sigma_tau`` does."""
+    out = reg.initialize_sigma_tau(primal)
+    if hasattr(reg, "tau_terms"):
+        return reg.tau_terms(tuple(np.shape(primal)))
+    return float(out), None
+
+
 def compute_Lipschitz_scaling(A, b, regs: Sequence, relaxation_sigma: float = 1.0, relaxation_tau: float = 1.0):
-    """``sigma = rho_s / L``, ``tau = rho_t / (L + sum reg_tau)`` (``solvers.py:120-158``)."""
+    """``sigma = rho_s / L``, ``tau = rho_t / (L + sum reg_tau)`` (``solvers.py:120-158``); tau is a
+    float, or a float32 device tensor of the primal's shape when a regularizer has an update mask."""
     L, x_shape, x_dtype = power_method(A, b)
     tau = L
     dummy_x = np.empty(x_shape, dtype=x_dtype)
+    masked = []
     for reg in regs:
-        tau += float(reg.initialize_sigma_tau(dummy_x))
-    return relaxation_sigma / L, relaxation_tau / tau, x_shape, x_dtype
+        t_reg, mask = _reg_tau(reg, dummy_x)
+        if mask is None:
+            tau += t_reg
+        else:
+            masked.append((t_reg, mask))
+    if not masked:
+        return relaxation_sigma / L, relaxation_tau / tau, x_shape, x_dtype
+    st = _native.stream()
+    tau_t = torch.full(x_shape, float(tau), dtype=torch.float32, device=_arrays.device())
+    for t_reg, mask in masked:
+        _native.call("chip_axpy", tau_t.data_ptr(), t_reg, mask.data_ptr(), mask.numel(), tau_t.numel(), st)
+    _native.call("chip_rdiv", tau_t.data_ptr(), float(relaxation_tau), tau_t.numel(), st)
+    return relaxation_sigma / L, tau_t, x_shape, x_dtype
 
 
 class SolutionInfo:
@@ -236,7 +262,8 @@ class Solver:
     @staticmethod
     def _grad_primal(cfg: dict, upd: torch.Tensor, q: torch.Tensor, subtract: bool, st) -> None:
         _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
-                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], cfg["apply_scale"], int(subtract), st)
+                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], cfg["apply_scale"], int(subtract),
+                     _native.ptr(cfg["mask"]), cfg["mask_period"], st)
 
     def _track(self, info, ii, Ax, b, b_mask, b_test_mask) -> bool:
         """Residual bookkeeping of one iteration; True when the tolerance is reached."""
@@ -689,7 +716,8 @@ class PDHG(Solver):
                                                                relaxation_sigma=self.relaxation,
                                                                relaxation_tau=self.relaxation)
             sigma = torch.full_like(b_t, float(sig))
-            tau = torch.full(x_shape, float(tau_s), dtype=torch.float32, device=b_t.device)
+            tau = tau_s if isinstance(tau_s, torch.Tensor) else \
+                torch.full(x_shape, float(tau_s), dtype=torch.float32, device=b_t.device)
         x_mask_t = _dev(x_mask_t, tau)
         x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
         n = int(x.shape[-1])

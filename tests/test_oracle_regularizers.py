@@ -117,3 +117,40 @@ def test_generic_regularizers_reduce_objective():
     for reg in (oreg.GradReg(3, 0.5, "l12"), oreg.GradReg(2, 0.5, "vtv"), oreg.GradReg(2, 0.5, "tnv")):
         x = osol.pdhg(A, At, b, shape, 300, regs=[reg])
         assert np.linalg.norm(x - truth) < e0, reg.kind
+
+
+def test_update_mask_special_cases():
+    """upd_mask (regularizers.py:206-207, 271-272): a mask of ones gives the unmasked run bit for bit;
+    a mask of zeros removes the regularizer (its tau term and primal update vanish) bit for bit, for
+    PDHG (preconditioned and Lipschitz) and SIRT; a half mask lies strictly between. A dropped mask
+    on either term, or the mask applied after the weight of a weighted norm, breaks one of these."""
+    shape = (6, 7)
+    A, At, b = _dense_problem(shape, 60, 4)
+    ones, zeros = np.ones(shape, F32), np.zeros(shape, F32)
+    half = np.zeros(shape, F32)
+    half[:, :4] = 1
+    x = np.ones(shape, F32)
+    for _ in range(60):  # the operator norm, for the Lipschitz steps
+        x = At(A(x))
+        x = x / np.linalg.norm(x)
+    L = float(np.sqrt(np.linalg.norm(At(A(x)))))
+
+    def same(u, v, exact):
+        return np.array_equal(u, v) if exact else np.abs(u - v).max() <= 1e-6 * np.abs(v).max()
+
+    for kind in ("l12", "tnv"):
+        # the Lipschitz tau is rho / (L + terms): one float64 division for scalars, float32 for arrays
+        for run, exact in ((lambda regs: osol.pdhg(A, At, b, shape, 25, data_term="kl", regs=regs), True),
+                           (lambda regs: osol.pdhg(A, At, b, shape, 25, regs=regs, precondition=False, L=L), False),
+                           (lambda regs: osol.sirt(A, At, b, shape, 25, regs=regs), True)):
+            plain_reg = run([oreg.GradReg(2, 0.05, kind)])
+            assert same(run([oreg.GradReg(2, 0.05, kind, upd_mask=ones)]), plain_reg, exact)
+            assert same(run([oreg.GradReg(2, 0.05, kind, upd_mask=zeros)]), run([]), exact)
+            mid = run([oreg.GradReg(2, 0.05, kind, upd_mask=half)])
+            assert not same(mid, plain_reg, False) and not same(mid, run([]), False)
+    # the primal term: -div q masked, then weighted (tnv) -- the order of the reference's two products
+    rng = np.random.default_rng(11)
+    q = rng.standard_normal((2, *shape)).astype(F32)
+    r = oreg.GradReg(2, 0.3, "tnv", upd_mask=half)
+    assert np.array_equal(r.primal(q), (-oreg.div_nd(q) * half) * F32(0.3))
+    assert np.array_equal(r.tau(), F32(4 * 0.3) * half)
