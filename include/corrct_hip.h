@@ -301,6 +301,9 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * regularizers.py:1362-1430), 3 = TNV (singular values of each C x nd matrix
  * clipped at 1, Regularizer_TNV with DataFidelity_ln, regularizers.py:1324-1359,
  * data_terms.py:781-820). Modes 0 and 1 need n_chan == 1; n_chan <= 16.
+ * qref (VTV only, else NULL): the dual of Regularizer_VTV's reference volume,
+ * [nd][n_batch][dims...], a fixed extra channel after the n_chan channels in
+ * every point-wise norm, never updated (regularizers.py:1386-1392, 1410-1411).
  * chip_grad_update_primal: upd += t (or -= t when subtract) with
  * t = -div(q), times mask[e % mask_period] when mask is not NULL (the
  * regularizer's upd_mask, broadcast over leading axes: mask_period divides
@@ -309,7 +312,7 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * subtracts, solvers.py:953-958, 1178-1180).
  */
 int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, int der_norm, int chan_norm, void* stream);
+                   float weight, int der_norm, int chan_norm, const float* qref, void* stream);
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
                             float scale, int apply_scale, int subtract, const float* mask, int64_t mask_period,
                             void* stream);

@@ -65,11 +65,15 @@ def prox_l1(q: np.ndarray, w: float) -> np.ndarray:
     return (q / np.fmax(np.abs(q), q.dtype.type(w))) * q.dtype.type(w)
 
 
-def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None) -> np.ndarray:
-    """``q`` is [nd, C, ...] (or [nd, ...] for one channel when ``q.ndim == nd + 1``)."""
+def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None, q_ref=None) -> np.ndarray:
+    """``q`` is [nd, C, ...] (or [nd, ...] for one channel when ``q.ndim == nd + 1``); ``q_ref``
+    ([nd, ...], the reference volume's dual) joins the norms as one more channel after the C
+    channels and is dropped afterwards (``regularizers.py:1410-1411, 1434-1435``)."""
     nd = q.shape[0] if nd is None else nd
     scalar = q.ndim == nd + 1
     t = q[:, None] if scalar else q
+    if q_ref is not None:
+        t = np.concatenate((t, np.asarray(q_ref, dtype=t.dtype)[:, None]), axis=1)
     if der == 1:
         gn = np.abs(t)
     elif der == 2:
@@ -83,6 +87,8 @@ def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None)
     else:
         dn = np.linalg.norm(gn, axis=1, ord=1, keepdims=True)
     t = (t / np.fmax(dn, t.dtype.type(w))) * t.dtype.type(w)
+    if q_ref is not None:
+        t = t[:, : t.shape[1] - 1]
     return t[:, 0] if scalar else t
 
 
@@ -101,9 +107,11 @@ def prox_tnv(q: np.ndarray) -> np.ndarray:
 class GradReg:
     """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv"}."""
 
-    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None):
+    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None):
         self.nd, self.weight, self.kind, self.der, self.chan = nd, float(weight), kind, der, chan
         self.upd_mask = None if upd_mask is None else np.asarray(upd_mask, dtype=F32)
+        # VTV's reference dual: 0.5 * grad(x_ref) from a zero dual (regularizers.py:1386-1392)
+        self.q_ref = None if x_ref is None else F32(0.5) * grad_nd(np.asarray(x_ref, dtype=F32), nd)
 
     @property
     def weighted(self) -> bool:
@@ -121,7 +129,7 @@ class GradReg:
         if self.kind == "l1":
             return prox_l1(q, self.weight)
         if self.kind == "vtv":
-            return prox_vtv(q, self.weight, self.der, self.chan, self.nd)
+            return prox_vtv(q, self.weight, self.der, self.chan, self.nd, self.q_ref)
         return prox_tnv(q)
 
     def primal(self, q: np.ndarray) -> np.ndarray:

@@ -107,7 +107,8 @@ __device__ __forceinline__ float pnorm_acc(float acc, float v, int ord) {
 // dual values stay in registers; g.C <= CM channels are live.
 template <int ND, int CM>
 __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__ q, const float* __restrict__ x,
-                                                             GradGeom g, int mode, float w, int der, int chan) {
+                                                             GradGeom g, int mode, float w, int der, int chan,
+                                                             const float* __restrict__ qref) {
   const int64_t total = g.B * g.S;
   const int64_t cs = g.B * g.S;  // channel stride of x and of each q component
   const int64_t qs = static_cast<int64_t>(g.C) * cs;  // component stride of q
@@ -149,6 +150,11 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
         for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / fmaxf(fabsf(v[k][c]), w)) * w;
       }
     } else if (mode == kGrVTV) {
+      // qref: the reference volume's fixed dual, one more channel after the C channels in
+      // every norm, never written (regularizers.py:1410-1411, 1434-1435)
+      float r[ND];
+#pragma unroll
+      for (int k = 0; k < ND; ++k) r[k] = qref ? qref[k * total + off] : 0.f;
       if (der == 1) {
         // gn = |q| per (k, c); the channel norm runs over c for each k
 #pragma unroll
@@ -157,6 +163,7 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 #pragma unroll
           for (int c = 0; c < CM; ++c)
             if (c < g.C) acc = pnorm_acc(acc, fabsf(v[k][c]), chan);
+          if (qref) acc = pnorm_acc(acc, fabsf(r[k]), chan);
           const float dn_k = chan == 2 ? sqrtf(acc) : acc;
 #pragma unroll
           for (int c = 0; c < CM; ++c) {
@@ -178,6 +185,12 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 #pragma unroll
         for (int c = 0; c < CM; ++c)
           if (c < g.C) acc = pnorm_acc(acc, gn[c], chan);
+        if (qref) {
+          float ar = 0.f;
+#pragma unroll
+          for (int k = 0; k < ND; ++k) ar = pnorm_acc(ar, r[k], der);
+          acc = pnorm_acc(acc, der == 2 ? sqrtf(ar) : ar, chan);
+        }
         const float dn_all = chan == 2 ? sqrtf(acc) : acc;
 #pragma unroll
         for (int c = 0; c < CM; ++c) {
@@ -240,15 +253,15 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 
 template <int ND>
 void launch_grad_dual(float* q, const float* x, const GradGeom& g, int mode, float w, int der, int chan,
-                      hipStream_t st) {
+                      const float* qref, hipStream_t st) {
   const dim3 grid = gr_grid(g.B * g.S);
   if (g.C == 1)
-    hipLaunchKernelGGL((grad_dual_kernel<ND, 1>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan);
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 1>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan, qref);
   else if (g.C <= 4)
-    hipLaunchKernelGGL((grad_dual_kernel<ND, 4>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan);
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 4>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan, qref);
   else
     hipLaunchKernelGGL((grad_dual_kernel<ND, kGrMaxChan>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der,
-                       chan);
+                       chan, qref);
 }
 
 __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict__ upd, const float* __restrict__ q,
@@ -308,7 +321,7 @@ using namespace chip;
 extern "C" {
 
 int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, int der_norm, int chan_norm, void* stream) {
+                   float weight, int der_norm, int chan_norm, const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
   CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrTNV, "unknown mode");
@@ -316,13 +329,14 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
                                     (chan_norm == 0 || chan_norm == 1 || chan_norm == 2)),
                  "VTV point-wise norms must be 1, 2 or 0 (inf)");
   CHIP_CHECK_ARG(mode == kGrVTV || mode == kGrTNV || n_chan == 1, "coupled channels need the VTV or TNV mode");
+  CHIP_CHECK_ARG(qref == nullptr || mode == kGrVTV, "a reference dual (qref) is taken by the VTV mode only");
   GradGeom g;
   const int rc = make_geom(g, n_chan, n_batch, nd, dims);
   if (rc != CHIP_OK) return rc;
   hipStream_t st = as_stream(stream);
-  if (nd == 1) launch_grad_dual<1>(q, x, g, mode, weight, der_norm, chan_norm, st);
-  else if (nd == 2) launch_grad_dual<2>(q, x, g, mode, weight, der_norm, chan_norm, st);
-  else launch_grad_dual<3>(q, x, g, mode, weight, der_norm, chan_norm, st);
+  if (nd == 1) launch_grad_dual<1>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
+  else if (nd == 2) launch_grad_dual<2>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
+  else launch_grad_dual<3>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

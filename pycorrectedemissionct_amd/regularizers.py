@@ -75,7 +75,8 @@ class BaseRegularizer:
         t = t.contiguous()
         zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
         _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"], _native.stream())
+                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"],
+                     _native.ptr(cfg.get("qref")), _native.stream())
         if host:
             dual[...] = t.reshape(np.shape(dual)).cpu().numpy()
         elif t.data_ptr() != dual.data_ptr():
@@ -182,7 +183,7 @@ class Regularizer_Grad(BaseRegularizer):
                     dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
                     weight=float(self.weight), der=der, chan=chan, scale=float(self.weight),
                     apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape),
-                    mask=mask, mask_period=1 if mask is None else int(mask.numel()))
+                    mask=mask, mask_period=1 if mask is None else int(mask.numel()), qref=None)
 
     def check_fused(self) -> float:
         """The weight, after checking the fused TV2D kernels implement this configuration."""
@@ -229,6 +230,20 @@ class Regularizer_TV3D(Regularizer_Grad):
 _PWISE_CODE = {1: 1, 2: 2, np.inf: 0}
 
 
+def _grad_edge(x: np.ndarray, nd: int) -> np.ndarray:
+    """Forward differences over the last ``nd`` axes with edge padding (the last difference 0),
+    stacked first: ``TransformGradient`` (``operators.py:796-819``) on the host, float32."""
+    x = np.asarray(x, dtype=np.float32)
+    out = np.zeros((nd, *x.shape), dtype=np.float32)
+    for k in range(nd):
+        ax = x.ndim - nd + k
+        hi = [slice(None)] * x.ndim
+        lo = [slice(None)] * x.ndim
+        hi[ax], lo[ax] = slice(1, None), slice(None, -1)
+        out[(k, *lo)] = x[tuple(hi)] - x[tuple(lo)]
+    return out
+
+
 def _multichannel_layout(shape, ndims: int, what: str) -> tuple[int, int]:
     """(channels, independent batch) of a primal whose axis 0 holds the channels."""
     if len(shape) <= ndims:
@@ -256,9 +271,14 @@ class Regularizer_VTV(Regularizer_Grad):
         self.pwise_chan_norm = pwise_chan_norm
         if self.pwise_chan_norm not in [1, 2, np.inf]:
             self._raise_pwise_norm_error()
-        if x_ref is not None:
-            raise NotImplementedError("a reference volume (x_ref) is not supported by the HIP VTV kernel")
-        self.q_ref = None
+        # the reference volume's dual sigma * grad(x_ref), a fixed extra channel of the prox
+        # (regularizers.py:1386-1392): computed once here, on the host, as the reference does
+        if isinstance(x_ref, torch.Tensor):
+            x_ref = x_ref.detach().cpu().numpy()
+        self.x_ref = None if x_ref is None else np.asarray(x_ref, dtype=np.float32)
+        self.q_ref = None if x_ref is None else \
+            np.expand_dims((np.float32(0.5) * _grad_edge(self.x_ref, ndims)).astype(np.float32), axis=1)
+        self._q_ref_dev = None
 
     def _raise_pwise_norm_error(self):
         raise ValueError(
@@ -270,8 +290,16 @@ class Regularizer_VTV(Regularizer_Grad):
         shape = tuple(int(v) for v in shape)
         self._check_trailing_axes(shape)
         n_chan, n_batch = _multichannel_layout(shape, self.ndims, "VTV")
-        return self._config(shape, 2, n_chan, n_batch, _PWISE_CODE[self.pwise_der_norm],
-                            _PWISE_CODE[self.pwise_chan_norm])
+        cfg = self._config(shape, 2, n_chan, n_batch, _PWISE_CODE[self.pwise_der_norm],
+                           _PWISE_CODE[self.pwise_chan_norm])
+        if self.q_ref is not None:
+            want = shape[1:] if len(shape) > self.ndims else shape  # the primal without its channel axis
+            if tuple(self.x_ref.shape) != want:
+                raise ValueError(f"x_ref of shape {self.x_ref.shape} does not match the primal's {want}")
+            if self._q_ref_dev is None:
+                self._q_ref_dev = torch.as_tensor(np.ascontiguousarray(self.q_ref)).to(_arrays.device())
+            cfg["qref"] = self._q_ref_dev
+        return cfg
 
 
 class Regularizer_TNV(Regularizer_Grad):
@@ -286,8 +314,9 @@ class Regularizer_TNV(Regularizer_Grad):
                  spectral_norm=None, x_ref=None):
         super().__init__(weight=weight, ndims=ndims, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask)
         self.norm = dt.DataFidelity_ln(ln_axes=(1, 0), spectral_norm=spectral_norm)
-        if x_ref is not None:
-            raise NotImplementedError("a reference volume (x_ref) is not supported by the HIP TNV kernel")
+        # kept as in the reference, whose TNV computes the reference dual (regularizers.py:1353-1356)
+        # but never reads it: its dual proximal is the norm's (regularizers.py:150-162)
+        self.x_ref = x_ref
         self.x_ref = None
 
     def device_config(self, shape) -> dict:

@@ -257,13 +257,14 @@ class Solver:
     @staticmethod
     def _grad_dual(cfg: dict, q: torch.Tensor, x: torch.Tensor, st) -> None:
         _native.call("chip_grad_dual", q.data_ptr(), x.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"], st)
+                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"],
+                     _native.ptr(cfg.get("qref")), st)
 
     @staticmethod
     def _grad_primal(cfg: dict, upd: torch.Tensor, q: torch.Tensor, subtract: bool, st) -> None:
         _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
                      cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], cfg["apply_scale"], int(subtract),
-                     _native.ptr(cfg["mask"]), cfg["mask_period"], st)
+                     _native.ptr(cfg.get("mask")), cfg.get("mask_period", 1), st)
 
     def _track(self, info, ii, Ax, b, b_mask, b_test_mask) -> bool:
         """Residual bookkeeping of one iteration; True when the tolerance is reached."""

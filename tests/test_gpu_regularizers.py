@@ -196,3 +196,50 @@ def test_update_masks(pkg):
     assert e < TOL
     with pytest.raises(ValueError, match="upd_mask"):
         pkg.PDHG(regularizer=R.Regularizer_TV3D(5e-2, upd_mask=np.ones((5, 5), np.float32)))(prj, b, iterations=2)
+
+
+def test_vtv_reference_volume(pkg):
+    """VTV's ``x_ref`` (``regularizers.py:1386-1392, 1410-1435``): the reference volume's dual joins
+    every point-wise norm as an extra channel. The dual kernel against the oracle for all nine norm
+    pairs, with a batch axis and with one channel; PDHG-KL with it against the oracle; TNV's x_ref,
+    which the reference computes but its proximal never reads, changes nothing."""
+    import torch
+
+    R = pkg.regularizers
+    st = pkg._native.stream()
+    rng = np.random.default_rng(975)
+    inf = np.inf
+    cases = [((3, 12, 14), der, chan) for der in (1, 2, inf) for chan in (1, 2, inf)]
+    cases += [((3, 2, 9, 10), 2, inf), ((12, 14), 2, 2)]
+    for shape, der, chan in cases:
+        x = rng.standard_normal(shape).astype(np.float32)
+        q = (rng.standard_normal((2, *shape)) * 0.4).astype(np.float32)
+        xr = rng.standard_normal(shape[1:] if len(shape) > 2 else shape).astype(np.float32)
+        reg = R.Regularizer_VTV(0.3, pwise_der_norm=der, pwise_chan_norm=chan, x_ref=xr)
+        cfg = reg.device_config(shape)
+        qt = torch.as_tensor(q).cuda()
+        pkg.solvers.Solver._grad_dual(cfg, qt, torch.as_tensor(x).cuda(), st)
+        want = oreg.GradReg(2, 0.3, "vtv", der, chan, x_ref=xr).dual(q, x)
+        e = rel_max(want, qt.cpu().numpy())
+        print(f"vtv x_ref {shape} der={der} chan={chan}: rel_max {e:.2e}")
+        assert e < 1e-5
+    with pytest.raises(ValueError, match="x_ref"):
+        R.Regularizer_VTV(0.3, x_ref=np.zeros((5, 5), np.float32)).device_config((3, 12, 14))
+
+    n, a, c = 24, 16, 3
+    ang = np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+    base = pkg.testing.phantom_ellipses(n, 3, rng, semi_axes=(3, 7))
+    f = np.stack([base * s for s in (1.0, 0.6, 0.3)]).astype(np.float32)
+    A, At = _stack_ops(ang, n)
+    b = pkg.testing.poisson_data(A(f), 1e3, rng)
+    prj = pkg.ProjectorUncorrected([n, n, c], ang)
+    x_ref = (base > 0).astype(np.float32)
+    x = pkg.PDHG(data_term="kl", regularizer=R.Regularizer_VTV(5e-2, x_ref=x_ref))(prj, b, iterations=30)[0]
+    ref = osol.pdhg(A, At, b, f.shape, 30, data_term="kl", regs=[oreg.GradReg(2, 5e-2, "vtv", x_ref=x_ref)])
+    e = rel_max(ref, x)
+    plain = osol.pdhg(A, At, b, f.shape, 30, data_term="kl", regs=[oreg.GradReg(2, 5e-2, "vtv")])
+    print(f"PDHG-KL-VTV with x_ref: rel_max {e:.2e} (x_ref moves the result by {rel_max(plain, ref):.2e})")
+    assert e < TOL and rel_max(plain, ref) > 10 * TOL
+    x1 = pkg.PDHG(data_term="kl", regularizer=R.Regularizer_TNV(5e-2, x_ref=x_ref))(prj, b, iterations=10)[0]
+    x2 = pkg.PDHG(data_term="kl", regularizer=R.Regularizer_TNV(5e-2))(prj, b, iterations=10)[0]
+    assert np.array_equal(x1, x2)

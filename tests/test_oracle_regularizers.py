@@ -154,3 +154,26 @@ def test_update_mask_special_cases():
     r = oreg.GradReg(2, 0.3, "tnv", upd_mask=half)
     assert np.array_equal(r.primal(q), (-oreg.div_nd(q) * half) * F32(0.3))
     assert np.array_equal(r.tau(), F32(4 * 0.3) * half)
+
+
+def test_vtv_reference_volume():
+    """VTV's x_ref (regularizers.py:1386-1392, 1410-1435): its dual 0.5 grad(x_ref) joins every
+    point-wise norm as one more channel. Hand-computed one-pixel case (q = (3, 4), reference dual
+    (0, 12), weight 1): the channel 2-norm gives dn = 13, the channel inf-norm (summed) 17, the
+    per-channel norm 5 (the reference cannot act); a zero reference changes nothing, bit for bit."""
+    q = np.array([3.0, 4.0], F32).reshape(2, 1, 1, 1)  # [nd, C, 1 x 1 image]
+    r = np.array([0.0, 12.0], F32).reshape(2, 1, 1)
+    for chan, dn in ((2, 13.0), (np.inf, 17.0), (1, 5.0)):
+        got = oreg.prox_vtv(q, 1.0, der=2, chan=chan, nd=2, q_ref=r)
+        assert np.allclose(got[:, 0, 0, 0], np.array([3.0, 4.0]) / dn, rtol=1e-6), chan
+    rng = np.random.default_rng(3)
+    q = rng.standard_normal((2, 3, 5, 6)).astype(F32)
+    for der in (1, 2, np.inf):
+        for chan in (1, 2, np.inf):
+            ref0 = oreg.prox_vtv(q, 0.4, der, chan, 2, q_ref=np.zeros((2, 5, 6), F32))
+            assert np.array_equal(ref0, oreg.prox_vtv(q, 0.4, der, chan, 2)), (der, chan)
+    x_ref = rng.random((5, 6)).astype(F32)
+    r = oreg.GradReg(2, 0.4, "vtv", x_ref=x_ref)
+    assert np.array_equal(r.q_ref, F32(0.5) * oreg.grad_nd(x_ref, 2))
+    assert not np.array_equal(r.dual(np.zeros_like(q), rng.random((3, 5, 6)).astype(F32)),
+                              oreg.GradReg(2, 0.4, "vtv").dual(np.zeros_like(q), rng.random((3, 5, 6)).astype(F32)))
