@@ -469,14 +469,16 @@ def run_otf(args) -> dict:
 def run_kernels(args) -> dict:
     """The headline batch's projector kernels alone, for counter runs: the batched fp and exact-adjoint
     bp of --slices 512 x 512 slices at 720 angles and 2 detectors (autotuned shapes) and a 1 GiB device
-    copy, each launched --fp-reps times after 3 warm-up launches and timed with HIP events."""
+    copy, each launched --fp-reps times after 3 warm-up launches and timed with HIP events.
+    ``--n``, ``--angles`` and ``--detectors 1`` give the other configurations' geometry (one detector at
+    90 deg), e.g. the warm-start workload's 256 x 256 x 360."""
     import torch
 
     import pycorrectedemissionct_amd as pc
     from pycorrectedemissionct_amd import batch as B
 
     n, n_ang, nb = args.n, args.angles, args.slices
-    det = [np.pi / 2, -np.pi / 2]
+    det = [np.pi / 2, -np.pi / 2][: args.detectors]
     ang = pc.testing.angles_uniform(n_ang)
     fs, mis, mos = [], [], []
     for k in range(nb):
@@ -505,14 +507,15 @@ def run_kernels(args) -> dict:
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     t_copy = time_launches(lambda: dst.copy_(src), args.fp_reps)
-    fp_bytes = nb * 4.0 * (n * n + n_ang * 2 * n * n + n_ang * 2 * n)
+    nd = len(det)
+    fp_bytes = nb * 4.0 * (n * n + n_ang * nd * n * n + n_ang * nd * n)
     bp_bytes = fp_bytes
     log(f"fp {1e6 * t_fp:.1f} us, bp {1e6 * t_bp:.1f} us, 1 GiB copy {1e6 * t_copy:.1f} us")
     fp_gbs = fp_bytes / t_fp / 1e9
     return {
         "metric": "batched fp / exact-adjoint bp GB/s (algorithmic bytes)", "value": fp_gbs, "unit": "GB/s",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{nb} x {n}x{n} slices, {n_ang} angles, 2 detectors, precomputed maps; "
+        "config": {"workload": f"{nb} x {n}x{n} slices, {n_ang} angles, {nd} detector(s), precomputed maps; "
                                f"{args.fp_reps} launches each of the batched fp, the batched adjoint bp and a "
                                f"1 GiB copy", "autotune": tune,
                    "fp_us": 1e6 * t_fp, "bp_us": 1e6 * t_bp, "copy_us": 1e6 * t_copy,
@@ -602,6 +605,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--detectors", type=int, choices=[1, 2], default=2, help="kernels: detectors (1: at 90 deg)")
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--ref-iters", type=int, default=16000)
     ap.add_argument("--tol", type=float, default=5e-2)
