@@ -296,14 +296,16 @@ def training_phantom(n: int, rng: np.random.Generator) -> np.ndarray:
 
 def make_pairs(n_images: int, n: int = 256, n_angles: int = 360, sirt_its: int = 20, pdhg_its: int = 2000,
                tv_lambda: float = 1e-2, peak_range=(1e2, 1e4), seed: int = 0, batch: int = 32,
-               data_term: str = "kl", det_angles=(np.pi / 2,), phantom: str = "training") -> dict:
+               data_term: str = "kl", det_angles=(np.pi / 2,), phantom: str = "training", keep_chunks: bool = True,
+               log=None) -> dict:
     """(SIRT-``sirt_its``, PDHG-TV-``pdhg_its``) pairs of seeded phantoms, made on the GPU in batches.
 
     Each phantom has its own attenuation (mu_in = 0.02 (f > 0) + 0.005, mu_out = 0.05 (f > 0)),
     detectors at ``det_angles``, Poisson counts at a peak drawn from ``peak_range``. ``phantom``:
     "training" (ellipses + blobs, Voronoi cells, or both) or "ellipses" (30 ellipses + 5 Gaussian
     blobs, the PDHG benchmark's family). Returns device tensors x_sirt, x_ref, phantom
-    [n_images, n, n] and the per-image data needed to re-solve.
+    [n_images, n, n] and, with ``keep_chunks``, each batch's projector and data to re-solve (the
+    projectors hold their maps: training sets leave them out).
     """
     rng = np.random.default_rng(seed)
     ang = testing.angles_uniform(n_angles)
@@ -325,7 +327,11 @@ def make_pairs(n_images: int, n: int = 256, n_angles: int = 360, sirt_its: int =
         xs.append(torch.stack(A.unpack_vol(x_s)))
         xr.append(torch.stack(A.unpack_vol(x_r)))
         fs.append(torch.as_tensor(np.stack(f), device=x_s.device))
-        items.append(dict(A=A, b=b, mus=mus))
+        if keep_chunks:
+            items.append(dict(A=A, b=b, mus=mus))
+        del A
+        if log:
+            log(f"pairs {i0 + nb}/{n_images}")
     return dict(x_sirt=torch.cat(xs), x_ref=torch.cat(xr), phantom=torch.cat(fs), chunks=items,
                 angles=ang, n=n, tv_lambda=tv_lambda, data_term=data_term)
 
@@ -389,12 +395,12 @@ def iterations_to_tolerance(solver, A, b, x_ref_flat: torch.Tensor, x0, tols, ma
 
 
 def evaluate(net: UNet, test: dict, tols=(1e-1, 5e-2, 2e-2, 1e-2, 5e-3, 1e-3), max_its: int = 2000,
-             check_every: int = 10) -> dict:
+             check_every: int = 10, variants=("cold", "warm", "warm_dual"), log=None) -> dict:
     """Cold vs warm-started batched PDHG on the held-out chunk(s) of ``test``: iterations to each
     tolerance against the reference ``test["x_ref"]``, for the warm start alone ("warm") and with
     duals initialised from it ("warm_dual"), and the slices/s ratios counting the warm start's
     SIRT-20 + network time."""
-    runs = {"cold": [], "warm": [], "warm_dual": []}
+    runs = {k: [] for k in variants}
     t_it = t_sirt = t_net = 0.0
     n_sl = 0
     for ch in test["chunks"]:
@@ -410,10 +416,11 @@ def evaluate(net: UNet, test: dict, tols=(1e-1, 5e-2, 2e-2, 1e-2, 5e-3, 1e-3), m
         x0 = A.pack_vol(list(warm_start(net, xs)))
         st = solver.prepare(A, b)
         t_it += _batch.time_iterations(lambda its: solver.iterate(A, st, its), 200) / 200
-        runs["cold"].append(iterations_to_tolerance(solver, A, b, x_ref, None, tols, max_its, check_every))
-        runs["warm"].append(iterations_to_tolerance(solver, A, b, x_ref, x0, tols, max_its, check_every))
-        runs["warm_dual"].append(iterations_to_tolerance(solver, A, b, x_ref, x0, tols, max_its, check_every,
-                                                         dual_init=True))
+        for key in variants:
+            runs[key].append(iterations_to_tolerance(solver, A, b, x_ref, None if key == "cold" else x0, tols,
+                                                     max_its, check_every, dual_init=key == "warm_dual"))
+            if log:
+                log(f"evaluated {key} on slices {n_sl}..{n_sl + k - 1}")
         n_sl += k
     its = {key: np.concatenate(v, axis=1) for key, v in runs.items()}
     n_ch = len(test["chunks"])
@@ -424,7 +431,7 @@ def evaluate(net: UNet, test: dict, tols=(1e-1, 5e-2, 2e-2, 1e-2, 5e-3, 1e-3), m
         c_med = float(np.median(its["cold"][i]))
         row["cold_median_its"] = c_med
         row["cold_reached"] = int(np.sum(its["cold"][i] <= max_its))
-        for key in ("warm", "warm_dual"):
+        for key in (v for v in variants if v != "cold"):
             w_med = float(np.median(its[key][i]))
             row[f"{key}_median_its"] = w_med
             row[f"{key}_reached"] = int(np.sum(its[key][i] <= max_its))
@@ -459,6 +466,7 @@ def main() -> int:
     ap.add_argument("--peak", type=float, nargs=2, default=(1e2, 1e4), help="Poisson peak range")
     ap.add_argument("--phantom", choices=["training", "ellipses"], default="training")
     ap.add_argument("--pair-batch", type=int, default=32)
+    ap.add_argument("--no-dual-init", action="store_true", help="evaluate without the dual-initialised variant")
     args = ap.parse_args()
 
     def log(msg):
@@ -476,7 +484,8 @@ def main() -> int:
         parts = []
         step = args.pair_batch
         for i0 in range(0, args.train, step):
-            parts.append(make_pairs(min(step, args.train - i0), seed=1000 + i0, pdhg_its=args.pdhg_its, **kw))
+            parts.append(make_pairs(min(step, args.train - i0), seed=1000 + i0, pdhg_its=args.pdhg_its,
+                                    keep_chunks=False, **kw))
             log(f"training pairs {min(i0 + step, args.train)}/{args.train} ({time.perf_counter() - t0:.0f} s)")
         x_in = torch.cat([p["x_sirt"] for p in parts])
         x_tgt = torch.cat([p["x_ref"] for p in parts])
@@ -493,9 +502,10 @@ def main() -> int:
         print(json.dumps({"loss_first_last": [losses[0], losses[-1]], "s_pairs": t1 - t0, "s_training": t2 - t1,
                           "params": net.n_params, "train_pairs": args.train}), flush=True)
         return 0
-    test = make_pairs(args.test, seed=99991, pdhg_its=args.ref_its, **kw)
+    test = make_pairs(args.test, seed=99991, pdhg_its=args.ref_its, log=log, **kw)
     log(f"evaluating cold vs warm start against {args.ref_its}-iteration references")
-    res = evaluate(net, test, max_its=args.eval_its)
+    variants = ("cold", "warm") if args.no_dual_init else ("cold", "warm", "warm_dual")
+    res = evaluate(net, test, max_its=args.eval_its, variants=variants, log=log)
     res.update({"n": args.n, "angles": args.angles, "train_pairs": args.train, "test_slices": args.test,
                 "train_target_its": args.pdhg_its, "test_reference_its": args.ref_its, "epochs": args.epochs,
                 "params": net.n_params, "loss_first_last": [losses[0], losses[-1]], "s_pairs": t1 - t0,
