@@ -181,12 +181,11 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t* __restrict
 // A workgroup owns a spatial tile of 4 RB rows x 16 columns of one image and
 // CO_T output channels; wave w computes tile rows RB w .. RB w + RB - 1 (RB
 // 16-pixel MFMA row blocks, each weight operand used RB times) for all CO_T
-// channels. Input channels run in chunks of 32: the
-// chunk's 10 x 18 input patch (the tile plus its one-pixel halo, zero outside
-// the image, from x1 or x2 of a concatenation) is staged in LDS, pixel pitch 40
-// bf16 (80 bytes: the 16 A-operand lanes of a read fall in distinct banks); the
-// 9 taps then read their A operands from the patch and their B operands
-// (weights, L2-resident) straight from global memory.
+// channels. Input channels run in chunks of 32: the chunk's (4 RB + 2) x 18
+// input patch (the tile plus its one-pixel halo, zero outside the image, from
+// x1 or x2 of a concatenation) is staged in LDS, pixel pitch 40 bf16 (80 bytes:
+// the 16 A-operand lanes of a read fall in distinct banks), and so are the
+// chunk's weights; the 9 taps then read both operands from LDS.
 // Operand layout (lane l, element i): A[m][k] at m = l % 16, k = 8 (l / 16) + i;
 // B[k][n] at n = l % 16, k = 8 (l / 16) + i; D[m][n] at n = l % 16, m = 4 (l / 16) + i.
 // ---------------------------------------------------------------------------
@@ -196,6 +195,10 @@ constexpr int kMtRows = 8, kMtCols = 16;                 // spatial tile
 constexpr int kMtPitch = 40;                             // bf16 per staged pixel (32 channels + 8 pad)
 constexpr int kMtPatch = (kMtRows + 2) * (kMtCols + 2);  // staged pixels
 
+// The chunk's 9 x CO_T x 32 weights are staged in LDS too ([tap][co][ci], pitch 40 bf16: the
+// 16 B-operand lanes of a read fall in distinct banks) after the patch, so the four waves read
+// their B operands from LDS instead of each from L2 (85 against 117 us per 16-row, 64-channel
+// launch at 256^2); the epilogue reuses the whole area.
 template <int CO_T, int RB>
 __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __restrict__ x1, int c1,
                                                            const uint16_t* __restrict__ x2, int c2,
@@ -207,9 +210,11 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
   constexpr int kRows = 4 * RB;    // tile rows: RB row blocks per wave
   constexpr int kPatch = (kRows + 2) * (kMtCols + 2);
   constexpr int kOutPitch = CO_T + 8;  // bf16 per pixel of the staged output tile
-  constexpr int kLdsHalfs = kPatch * kMtPitch > kRows * kMtCols * kOutPitch ? kPatch * kMtPitch
-                                                                            : kRows * kMtCols * kOutPitch;
+  constexpr int kWPitch = 40;          // bf16 per staged (tap, co) weight row of 32 ci
+  constexpr int kInHalfs = kPatch * kMtPitch + 9 * CO_T * kWPitch;
+  constexpr int kLdsHalfs = kInHalfs > kRows * kMtCols * kOutPitch ? kInHalfs : kRows * kMtCols * kOutPitch;
   __shared__ __align__(16) uint16_t patch[kLdsHalfs];
+  uint16_t* const wl = patch + kPatch * kMtPitch;  // staged weights
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tiles_w = (im.w + kMtCols - 1) / kMtCols, tiles_h = (im.h + kRows - 1) / kRows;
   int t = blockIdx.x;
@@ -245,6 +250,12 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
       }
       *reinterpret_cast<uint4*>(patch + px * kMtPitch + 8 * part) = v;
     }
+    for (int k = threadIdx.x; k < 9 * CO_T * 4; k += 256) {
+      const int row = k >> 2, part = k & 3;  // row = tap * CO_T + co, 4 x 16 bytes of 32 ci
+      const int tap = row / CO_T, co = row - tap * CO_T;
+      const uint16_t* wr = wt + (static_cast<size_t>(tap) * cout + cbase + co) * cin + ci0 + 8 * part;
+      *reinterpret_cast<uint4*>(wl + row * kWPitch + 8 * part) = *reinterpret_cast<const uint4*>(wr);
+    }
     __syncthreads();
 #pragma unroll 1
     for (int tap = 0; tap < 9; ++tap) {
@@ -252,8 +263,8 @@ __global__ __launch_bounds__(256) void conv3x3_mfma_kernel(const uint16_t* __res
       bf8v bm[kCb];
 #pragma unroll
       for (int c = 0; c < kCb; ++c) {
-        const uint16_t* wr = wt + (static_cast<size_t>(tap) * cout + cbase + 16 * c + m) * cin + ci0 + 8 * q;
-        bm[c] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wr));
+        bm[c] = __builtin_bit_cast(
+            bf8v, *reinterpret_cast<const uint4*>(wl + (tap * CO_T + 16 * c + m) * kWPitch + 8 * q));
       }
 #pragma unroll
       for (int b = 0; b < RB; ++b) {
@@ -1041,18 +1052,13 @@ int chip_nn_conv3x3(const uint16_t* x1, int c1, const uint16_t* x2, int c2, cons
     CHIP_CHECK_ARG(tiles <= 2147483647, "too many pixels");
     dim3 grid(static_cast<unsigned>(tiles), n_cob);
     hipStream_t st = as_stream(stream);
-    if (co64 && rb4)
-      hipLaunchKernelGGL((conv3x3_mfma_kernel<64, 4>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
-                         Img{n, h, w}, cout, relu);
-    else if (co64)
-      hipLaunchKernelGGL((conv3x3_mfma_kernel<64, 2>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
-                         Img{n, h, w}, cout, relu);
-    else if (rb4)
-      hipLaunchKernelGGL((conv3x3_mfma_kernel<32, 4>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
-                         Img{n, h, w}, cout, relu);
-    else
-      hipLaunchKernelGGL((conv3x3_mfma_kernel<32, 2>), grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2,
-                         Img{n, h, w}, cout, relu);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, x1, c1, x2, c2, wt, bias, y1, o1, y2, Img{n, h, w}, cout, relu);
+    };
+    if (co64 && rb4) go(conv3x3_mfma_kernel<64, 4>);
+    else if (co64) go(conv3x3_mfma_kernel<64, 2>);
+    else if (rb4) go(conv3x3_mfma_kernel<32, 4>);
+    else go(conv3x3_mfma_kernel<32, 2>);
     CHIP_LAUNCH_CHECK();
     return CHIP_OK;
   }
