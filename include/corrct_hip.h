@@ -49,6 +49,20 @@ enum chip_bp_mode {
   CHIP_BP_ADJOINT = 2
 };
 
+/* Data-fidelity terms of chip_pdhg_dual_norm / chip_data_residual
+ * (src/corrct/data_terms.py). `e` is the term's local_error, `w` its weights,
+ * both read as param[i % param_period]. */
+enum chip_data_term {
+  CHIP_DATA_L2 = 0,    /* DataFidelity_l2     data_terms.py:283-331 */
+  CHIP_DATA_KL = 1,    /* DataFidelity_KL     data_terms.py:677-757 */
+  CHIP_DATA_WL2 = 2,   /* DataFidelity_wl2    data_terms.py:334-395 (param = w) */
+  CHIP_DATA_L2B = 3,   /* DataFidelity_l2b    data_terms.py:398-464 (param = e) */
+  CHIP_DATA_HUBER = 4, /* DataFidelity_Huber  data_terms.py:467-532 (param = e) */
+  CHIP_DATA_L1 = 5,    /* DataFidelity_l1     data_terms.py:535-597 */
+  CHIP_DATA_L12 = 6,   /* DataFidelity_l12    data_terms.py:598-625 */
+  CHIP_DATA_L1B = 7    /* DataFidelity_l1b    data_terms.py:628-674 (param = e) */
+};
+
 /* Library version (major*10000 + minor*100 + patch). */
 int chip_version(void);
 
@@ -269,6 +283,38 @@ int chip_pdhg_primal(float* x, float* xbar, const float* atp, const float* q, co
 int chip_sirt_residual(float* res, const float* b, const float* ax, const float* bg, const float* b_mask,
                        const float* sigma, int64_t n, void* stream);
 
+/* PDHG data dual of the norm-type terms: v = p + sigma (ax + bg) - sigma b, then
+ * apply_proximal_dual of `term` (update_dual data_terms.py:217-230), times b_mask
+ * (solvers.py:1172-1173):
+ *   L2     v / (1 + sigma)
+ *   WL2    v (1 - z) / (1 + sigma / (w + z)),  z = (w == 0)       (data_terms.py:353-354, 301-304)
+ *   L2B    soft(v, sigma sqrt(e)) / (1 + sigma e)                (data_terms.py:410-415, 425-429)
+ *   HUBER  u = v / (1 + sigma e);  u / max(1, |u|)               (data_terms.py:480-501)
+ *   L1     v / max(|v|, 1)                                       (data_terms.py:549-555)
+ *   L12    v / max(|v|_2 over the group, 1)                      (data_terms.py:607-608)
+ *   L1B    s = soft(v, e);  s / max(|s|, 1)                      (data_terms.py:644-645, 549-555)
+ * soft(v, t) = sign(v) max(|v| - t, 0). Groups (L12, and HUBER with an l2_axis)
+ * hold group_len elements group_stride apart: element (o, g, i) sits at
+ * (o * group_len + g) * group_stride + i, n a multiple of group_len * group_stride.
+ * param: w or e as the term needs it (NULL otherwise), param_period >= 1 dividing n.
+ * KL has its own entry (chip_pdhg_dual_kl). */
+int chip_pdhg_dual_norm(float* p, const float* ax, const float* sigma, const float* b, const float* bg,
+                        const float* b_mask, int64_t n, int term, const float* param, int64_t param_period,
+                        int64_t group_len, int64_t group_stride, void* stream);
+
+/* res = compute_residual(ax) * scale for the terms SIRT and FISTA accept
+ * (l2-based and KL; Solver._initialize_data_fidelity_function, solvers.py:299-310),
+ * with a = ax + bg and the term's assigned sigma `asig` (NULL = 1):
+ *   L2   (b - a) mask                                  (data_terms.py:136-161)
+ *   WL2  (b - a) w mask                                (data_terms.py:356-365)
+ *   L2B  soft((b - a) mask, asig sqrt(e))              (data_terms.py:419-423)
+ *   KL   -prox_KL(max(a, eps) asig) mask               (data_terms.py:721-740)
+ * `scale` (NULL = 1) is the solver's sigma of upd = A^T(res * sigma)
+ * (solvers.py:975, 1471). */
+int chip_data_residual(float* res, const float* b, const float* ax, const float* bg, const float* b_mask,
+                       const float* asig, const float* scale, int64_t n, int term, const float* param,
+                       int64_t param_period, void* stream);
+
 /* SIRT update x = clip(x + (upd - reg_upd) * tau) * mask, reg_upd = -div(q)
  * of a TV dual q (NULL = none). solvers.py:975-982. */
 int chip_sirt_update(float* x, const float* upd, const float* q, const float* tau, int has_lower,
@@ -300,7 +346,12 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * der_norm / chan_norm in {1, 2, 0 = inf} (Regularizer_VTV.apply_proximal_dual,
  * regularizers.py:1362-1430), 3 = TNV (singular values of each C x nd matrix
  * clipped at 1, Regularizer_TNV with DataFidelity_ln, regularizers.py:1324-1359,
- * data_terms.py:781-820). Modes 0 and 1 need n_chan == 1; n_chan <= 16.
+ * data_terms.py:781-820), 4 = l2 (q *= weight, the caller passing
+ * 1 / (1 + 0.5): Regularizer_smooth1D/2D/3D, regularizers.py:386-433 with
+ * data_terms.py:294-304), 5 = Huber over the gradient components (q *= weight,
+ * the caller passing 1 / (1 + 0.5 huber_size); q /= max(1, |q|_2):
+ * Regularizer_HubTV2D/3D, regularizers.py:340-384 with data_terms.py:482-501).
+ * Modes 0, 1, 4 and 5 need n_chan == 1; n_chan <= 16.
  * qref (VTV only, else NULL): the dual of Regularizer_VTV's reference volume,
  * [nd][n_batch][dims...], a fixed extra channel after the n_chan channels in
  * every point-wise norm, never updated (regularizers.py:1386-1392, 1410-1411).

@@ -20,6 +20,8 @@ What it restates (reference citations are relative to the corrct tree):
   the l2 / KL data terms and the TV2D regularizer (``solvers.py:60-117,
   688-1206``; ``data_terms.py:217-331, 535-625, 677-757``;
   ``regularizers.py:120-321``; ``operators.py:767-844``).
+* ``data_terms``   -- the data dual step and ``compute_residual`` of the l2, wl2,
+  l2b, Huber, l1, l12, l1b and KL data terms (``data_terms.py:25-757``).
 
 Every function has an FP32 mode (the arithmetic the reference performs,
 operation by operation) and, where it matters, an FP64 mode (same formulas
@@ -38,4 +40,7 @@ How each function is pinned (beyond retyping its formula):
   dense explicit matrices at N <= 16 whose transpose must equal the adjoint.
 * solvers: a dense-matrix restatement (independent code path) and the
   reference's own convergence tests (``tests/test_solvers.py:100-164``).
+* data_terms: each dual step equals the proximal map of sigma times the term's
+  convex conjugate found by brute-force 1-D minimisation; special cases that
+  reduce one term to another; the group projection's ball property.
 """

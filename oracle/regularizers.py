@@ -105,17 +105,26 @@ def prox_tnv(q: np.ndarray) -> np.ndarray:
 
 
 class GradReg:
-    """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv"}."""
+    """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv", "l2", "hub"}.
 
-    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None):
+    "l2" is ``Regularizer_smooth1D/2D/3D`` (``regularizers.py:386-431``): the l2 norm assigned
+    sigma = 1/2 without data scales the dual by ``1 / (1 + 1/2)`` (``data_terms.py:294-304``).
+    "hub" is ``Regularizer_HubTV2D/3D`` (``regularizers.py:340-383``): Huber over the gradient
+    axis, the dual scaled by ``1 / (1 + huber_size / 2)`` (a Python float, rounded to float32 when
+    numpy multiplies, ``data_terms.py:482-495``) then divided by ``max(1, |q|_2)``
+    (``data_terms.py:500-501``). Both weight the primal update (outside the l1 family)."""
+
+    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
+                 huber_size: float = 0.0):
         self.nd, self.weight, self.kind, self.der, self.chan = nd, float(weight), kind, der, chan
+        self.huber_size = float(huber_size)
         self.upd_mask = None if upd_mask is None else np.asarray(upd_mask, dtype=F32)
         # VTV's reference dual: 0.5 * grad(x_ref) from a zero dual (regularizers.py:1386-1392)
         self.q_ref = None if x_ref is None else F32(0.5) * grad_nd(np.asarray(x_ref, dtype=F32), nd)
 
     @property
     def weighted(self) -> bool:
-        return self.kind == "tnv"  # outside the l1 family
+        return self.kind in ("tnv", "l2", "hub")  # outside the l1 family
 
     def tau(self):
         t = 2.0 * self.nd
@@ -130,6 +139,12 @@ class GradReg:
             return prox_l1(q, self.weight)
         if self.kind == "vtv":
             return prox_vtv(q, self.weight, self.der, self.chan, self.nd, self.q_ref)
+        if self.kind == "l2":
+            return (q * F32(1 / (1 + 0.5))).astype(F32)
+        if self.kind == "hub":
+            q = (q * F32(1.0 / (1.0 + 0.5 * self.huber_size))).astype(F32)
+            nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=F32))
+            return (q / np.fmax(F32(1), nrm)).astype(F32)
         return prox_tnv(q)
 
     def primal(self, q: np.ndarray) -> np.ndarray:

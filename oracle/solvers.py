@@ -18,7 +18,10 @@ GPU projector's explicit matrix.
 * ``pdhg``              -- ``solvers.py:1060-1206`` (gradient-family regularizers
                            from ``oracle/regularizers.py`` via ``regs``) with ``DataFidelity_l2``
                            (``data_terms.py:283-331``) or ``DataFidelity_KL``
-                           (``data_terms.py:677-692``)
+                           (``data_terms.py:677-692``), or any ``oracle.data_terms.DataTerm``
+
+``sirt`` and ``fista`` take an optional ``oracle.data_terms.DataTerm`` whose ``compute_residual``
+replaces the plain l2 residual (``solvers.py:951-975, 1470-1471``).
 """
 
 from __future__ import annotations
@@ -82,16 +85,21 @@ def tv_dual(q: np.ndarray, xbar: np.ndarray, weight: float) -> np.ndarray:
 
 
 def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=None, b_mask=None, tv_weight=None,
-         relaxation=1.95, regs=()):
+         relaxation=1.95, regs=(), data_term=None):
     """``regs``: ``oracle.regularizers.GradReg`` objects, duals re-initialised every iteration."""
     b = np.asarray(b, dtype=F32)
     tau_reg = (4.0 if tv_weight is not None else 0.0) + sum(r.tau() for r in regs)
     sigma, tau = diagonal_scaling(A, At, b, x_shape, tau_reg=tau_reg, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
+    if data_term is not None:
+        data_term.assign(b, sigma)
     for _ in range(iterations):
-        res = b - A(x)
-        if b_mask is not None:
-            res = res * b_mask
+        if data_term is not None:
+            res = data_term.residual(A(x), b_mask)
+        else:
+            res = b - A(x)
+            if b_mask is not None:
+                res = res * b_mask
         upd = At(res * sigma)
         if tv_weight is not None:
             q = tv_dual(np.zeros((2, *x_shape), dtype=F32), x, tv_weight)
@@ -156,7 +164,7 @@ def power_method(A, At, x_start, iterations: int = 5) -> float:
 
 
 def fista(A, At, b, x_shape, iterations, precondition=True, L=None, relaxation=1.0, l1=0.0, lower=None, upper=None,
-          x_mask=None, b_mask=None, restart_period=None):
+          x_mask=None, b_mask=None, restart_period=None, data_term=None):
     """FISTA: y-gradient step with diagonal (SIRT-style) or 1/L scaling, soft threshold at
     tau * l1, clip, mask, Beck-Teboulle momentum (``solvers.py:1391-1490``)."""
     b = np.asarray(b, dtype=F32)
@@ -169,10 +177,15 @@ def fista(A, At, b, x_shape, iterations, precondition=True, L=None, relaxation=1
         x = x * x_mask
     y = x.copy()
     t = 1.0
+    if data_term is not None:
+        data_term.assign(b)
     for ii in range(iterations):
-        res = b - A(y)
-        if b_mask is not None:
-            res = res * b_mask
+        if data_term is not None:
+            res = data_term.residual(A(y), b_mask)
+        else:
+            res = b - A(y)
+            if b_mask is not None:
+                res = res * b_mask
         xn = y + tau * At(res * sigma)
         if l1 > 0:
             thr = tau * F32(l1)
@@ -242,7 +255,10 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
             tau = np.full(x_shape, F32(relaxation / (L + tau_reg)), dtype=F32)
     x = np.zeros(x_shape, dtype=F32) if x0 is None else np.array(x0, dtype=F32)
     xr = x.copy()
-    if data_term == "l2":
+    term = None if isinstance(data_term, str) else data_term
+    if term is not None:
+        term.assign(b, sigma)
+    elif data_term == "l2":
         sigma_data = sigma * b
         sigma1 = (F32(1) / (F32(1) + sigma)).astype(F32)
     elif data_term == "kl":
@@ -267,11 +283,14 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
             info["residual0_cv"] = res_norm(ax0, b_test_mask)
     for _ in range(iterations):
         info["iterations"] += 1
-        p = p + A(xr) * sigma
-        if data_term == "l2":
-            p = (p - sigma_data) * sigma1
+        if term is not None:
+            p = term.dual(p, A(xr))
         else:
-            p = (F32(1) + p - np.sqrt((p - F32(1)) ** 2 + sigma_data)) / F32(2)
+            p = p + A(xr) * sigma
+            if data_term == "l2":
+                p = (p - sigma_data) * sigma1
+            else:
+                p = (F32(1) + p - np.sqrt((p - F32(1)) ** 2 + sigma_data)) / F32(2)
         if b_mask is not None:
             p = p * b_mask
         qs = [r.dual(q_r, xr) for r, q_r in zip(regs, qs)]

@@ -63,6 +63,10 @@ _SIGNATURES = {
         c_void_p, ctypes.c_int, ctypes.c_int, c_void_p,
     ],
     "chip_sirt_residual": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
+    "chip_pdhg_dual_norm": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int,
+                            c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, c_void_p],
+    "chip_data_residual": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64,
+                           ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p],
     "chip_sirt_update": [
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, c_void_p,
         ctypes.c_int, ctypes.c_int, c_void_p,

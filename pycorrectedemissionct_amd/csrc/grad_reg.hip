@@ -1,7 +1,8 @@
 // Gradient-family regularizers of libcorrct_hip (gfx950): the dual update,
 // dual proximal and primal contribution of Regularizer_Grad and its
-// subclasses TV1D / TV2D / TV3D, VTV and TNV (regularizers.py:134-215,
-// 218-337, 1324-1440) for gradients over the last 1..3 axes.
+// subclasses TV1D / TV2D / TV3D, HubTV2D / HubTV3D, smooth1D / 2D / 3D, VTV and
+// TNV (regularizers.py:134-215, 218-433, 1324-1440) for gradients over the last
+// 1..3 axes.
 //
 // Layout: the primal is [C][B][S] with S the product of the gradient axes'
 // sizes, C the channels a VTV / TNV prox couples (1 for TV) and B the
@@ -14,6 +15,9 @@
 //   dual   q = q + 0.5 * (x[i+1] - x[i])        (0 past the last sample)
 //   l12    q = (q / max(|q|_2 over nd, w)) * w   (data_terms.py:549-555, 607-608)
 //   l1     q = (q / max(|q|, w)) * w
+//   l2     q = q * w,  w = 1 / (1 + 0.5)             (data_terms.py:294-304, no data)
+//   Huber  q = q * w,  w = 1 / (1 + 0.5 huber_size); q = q / max(1, |q|_2 over nd)
+//          (data_terms.py:482-501 with l2_axis 0)
 //   VTV    q = (q / max(dn, w)) * w with dn the point-wise derivative norm
 //          (1, 2, inf) followed by the channel norm (1, 2, inf); the dual
 //          norms of inf are 1, as in regularizers.py:1395-1411
@@ -31,7 +35,7 @@ namespace {
 constexpr int kGrBlock = 256;
 constexpr int kGrMaxChan = 16;
 
-enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3 };
+enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3, kGrL2 = 4, kGrHuber = 5 };
 
 struct GradGeom {
   int C;        // coupled channels
@@ -141,6 +145,30 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
         const float den = fmaxf(sqrtf(acc), w);
 #pragma unroll
         for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * w;
+      }
+    } else if (mode == kGrL2) {
+      // DataFidelity_l2 without data: q = q * (1 / (1 + sigma)), w holding that factor
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) v[k][c] = v[k][c] * w;
+      }
+    } else if (mode == kGrHuber) {
+      // DataFidelity_Huber over l2_axis 0: q = q * w (w = 1 / (1 + sigma huber_size)),
+      // then q / max(1, |q|_2 over the nd components)
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+          v[k][c] = v[k][c] * w;
+          acc = acc + v[k][c] * v[k][c];
+        }
+        const float den = fmaxf(1.f, sqrtf(acc));
+#pragma unroll
+        for (int k = 0; k < ND; ++k) v[k][c] = v[k][c] / den;
       }
     } else if (mode == kGrL1) {
 #pragma unroll
@@ -324,7 +352,7 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
                    float weight, int der_norm, int chan_norm, const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
-  CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrTNV, "unknown mode");
+  CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrHuber, "unknown mode");
   CHIP_CHECK_ARG(mode != kGrVTV || ((der_norm == 0 || der_norm == 1 || der_norm == 2) &&
                                     (chan_norm == 0 || chan_norm == 1 || chan_norm == 2)),
                  "VTV point-wise norms must be 1, 2 or 0 (inf)");

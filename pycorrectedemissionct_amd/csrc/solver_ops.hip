@@ -130,6 +130,120 @@ __global__ __launch_bounds__(kEwBlock) void dual_kl_kernel(float* __restrict__ p
   }
 }
 
+// ---- norm-type data duals (enum chip_data_term) -------------------------------
+
+// sign(v) * max(|v| - t, 0)
+__device__ __forceinline__ float soft_threshold(float v, float t) {
+  const float m = fmaxf(fabsf(v) - t, 0.f);
+  return v > 0.f ? m : (v < 0.f ? -m : 0.f);
+}
+
+// v = p + sigma (ax + bg) - sigma b, then the elementwise part of the term's
+// dual proximal; for the group norms only the part before the group scaling.
+__device__ __forceinline__ float dual_norm_pre(int term, float v, float s, float bv, float e) {
+  v = v - s * bv;
+  switch (term) {
+    case CHIP_DATA_L2: return v * (1.f / (1.f + s));
+    case CHIP_DATA_WL2: {
+      const float z = e == 0.f ? 1.f : 0.f;  // e holds the weight
+      return v * ((1.f / (1.f + s / (e + z))) * (1.f - z));
+    }
+    case CHIP_DATA_L2B: return soft_threshold(v, s * sqrtf(e)) * (1.f / (1.f + s * e));
+    case CHIP_DATA_HUBER: return v * (1.f / (1.f + s * e));
+    case CHIP_DATA_L1B: return soft_threshold(v, e);
+    default: return v;  // L1, L12
+  }
+}
+
+__device__ __forceinline__ bool dual_norm_projects(int term) {
+  return term == CHIP_DATA_HUBER || term == CHIP_DATA_L1 || term == CHIP_DATA_L12 || term == CHIP_DATA_L1B;
+}
+
+__global__ __launch_bounds__(kEwBlock) void dual_norm_kernel(float* __restrict__ p, const float* __restrict__ ax,
+                                                             const float* __restrict__ sigma,
+                                                             const float* __restrict__ b,
+                                                             const float* __restrict__ bg,
+                                                             const float* __restrict__ mask, int64_t n, int term,
+                                                             const float* __restrict__ prm, int64_t period) {
+  const bool proj = dual_norm_projects(term);
+  GRID_STRIDE(i, n) {
+    const float s = sigma[i];
+    float a = ax[i];
+    if (bg) a = a + bg[i];
+    const float e = prm ? prm[i % period] : 0.f;
+    float v = dual_norm_pre(term, p[i] + a * s, s, b[i], e);
+    if (proj) v = v / fmaxf(fabsf(v), 1.f);
+    if (mask) v = v * mask[i];
+    p[i] = v;
+  }
+}
+
+// one thread per group (o, i): members (o * G + g) * S + i, g < G; the
+// pre-scaled values go to p in a first pass while the squared norm builds up
+// in member order, then a second pass divides by max(|group|_2, 1)
+__global__ __launch_bounds__(kEwBlock) void dual_group_kernel(float* __restrict__ p, const float* __restrict__ ax,
+                                                              const float* __restrict__ sigma,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ bg,
+                                                              const float* __restrict__ mask, int64_t n_groups,
+                                                              int term, const float* __restrict__ prm,
+                                                              int64_t period, int64_t G, int64_t S) {
+  GRID_STRIDE(t, n_groups) {
+    const int64_t o = t / S;
+    const int64_t base = o * G * S + (t - o * S);
+    float acc = 0.f;
+    for (int64_t g = 0; g < G; ++g) {
+      const int64_t k = base + g * S;
+      const float s = sigma[k];
+      float a = ax[k];
+      if (bg) a = a + bg[k];
+      const float e = prm ? prm[k % period] : 0.f;
+      const float v = dual_norm_pre(term, p[k] + a * s, s, b[k], e);
+      acc = acc + v * v;
+      p[k] = v;
+    }
+    const float den = fmaxf(sqrtf(acc), 1.f);
+    for (int64_t g = 0; g < G; ++g) {
+      const int64_t k = base + g * S;
+      float v = p[k] / den;
+      if (mask) v = v * mask[k];
+      p[k] = v;
+    }
+  }
+}
+
+// compute_residual of the l2-based terms and KL, times the solver's sigma
+__global__ __launch_bounds__(kEwBlock) void data_residual_kernel(float* __restrict__ res, const float* __restrict__ b,
+                                                                 const float* __restrict__ ax,
+                                                                 const float* __restrict__ bg,
+                                                                 const float* __restrict__ mask,
+                                                                 const float* __restrict__ asig,
+                                                                 const float* __restrict__ scale, int64_t n,
+                                                                 int term, const float* __restrict__ prm,
+                                                                 int64_t period) {
+  GRID_STRIDE(i, n) {
+    float a = ax[i];
+    if (bg) a = a + bg[i];
+    const float sa = asig ? asig[i] : 1.f;
+    float r;
+    if (term == CHIP_DATA_KL) {
+      const float v = fmaxf(a, FLT_EPSILON) * sa;
+      const float sd = (4.f * sa) * fmaxf(b[i], 0.f);
+      const float dm = v - 1.f;
+      r = ((1.f + v) - sqrtf(dm * dm + sd)) / 2.f;
+      if (mask) r = r * mask[i];
+      r = -r;
+    } else {
+      r = b[i] - a;
+      if (term == CHIP_DATA_WL2) r = r * prm[i % period];
+      if (mask) r = r * mask[i];
+      if (term == CHIP_DATA_L2B) r = soft_threshold(r, sa * sqrtf(prm[i % period]));
+    }
+    if (scale) r = r * scale[i];
+    res[i] = r;
+  }
+}
+
 // dual variable consistent with a primal estimate: the data term's gradient at
 // A x (the fixed point of the dual updates above): l2 p = (A x + bg) - b,
 // KL p = 1 - b / (A x + bg) (0 where A x + bg <= 0)
@@ -480,6 +594,52 @@ int chip_pdhg_primal(float* x, float* xbar, const float* atp, const float* q, co
   const int64_t total = static_cast<int64_t>(n) * n * batch;
   hipLaunchKernelGGL(pdhg_primal_kernel, ew_grid(total), dim3(kEwBlock), 0, as_stream(stream), x, xbar, atp, q, tau,
                      has_lower, lower, has_upper, upper, x_mask, n, batch);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_pdhg_dual_norm(float* p, const float* ax, const float* sigma, const float* b, const float* bg,
+                        const float* b_mask, int64_t n, int term, const float* param, int64_t param_period,
+                        int64_t group_len, int64_t group_stride, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(p && ax && sigma && b, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  CHIP_CHECK_ARG(term == CHIP_DATA_L2 || (term >= CHIP_DATA_WL2 && term <= CHIP_DATA_L1B),
+                 "term must be L2, WL2, L2B, HUBER, L1, L12 or L1B");
+  const bool needs_param = term == CHIP_DATA_WL2 || term == CHIP_DATA_L2B || term == CHIP_DATA_HUBER ||
+                           term == CHIP_DATA_L1B;
+  CHIP_CHECK_ARG(!needs_param || param != nullptr, "this data term needs its parameter array");
+  CHIP_CHECK_ARG(!param || (param_period >= 1 && n % param_period == 0), "param_period must divide n");
+  CHIP_CHECK_ARG(group_len >= 1 && group_stride >= 1 && n % (group_len * group_stride) == 0,
+                 "n must be a multiple of group_len * group_stride");
+  CHIP_CHECK_ARG(group_len == 1 || term == CHIP_DATA_L12 || term == CHIP_DATA_HUBER,
+                 "groups apply to L12 and HUBER only");
+  const int64_t period = param ? param_period : 1;
+  if (group_len == 1) {
+    hipLaunchKernelGGL(dual_norm_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), p, ax, sigma, b, bg,
+                       b_mask, n, term, param, period);
+  } else {
+    const int64_t groups = n / group_len;
+    hipLaunchKernelGGL(dual_group_kernel, ew_grid(groups), dim3(kEwBlock), 0, as_stream(stream), p, ax, sigma, b,
+                       bg, b_mask, groups, term, param, period, group_len, group_stride);
+  }
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_data_residual(float* res, const float* b, const float* ax, const float* bg, const float* b_mask,
+                       const float* asig, const float* scale, int64_t n, int term, const float* param,
+                       int64_t param_period, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(res && b && ax, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  CHIP_CHECK_ARG(term == CHIP_DATA_L2 || term == CHIP_DATA_KL || term == CHIP_DATA_WL2 || term == CHIP_DATA_L2B,
+                 "term must be L2, KL, WL2 or L2B");
+  const bool needs_param = term == CHIP_DATA_WL2 || term == CHIP_DATA_L2B;
+  CHIP_CHECK_ARG(!needs_param || param != nullptr, "this data term needs its parameter array");
+  CHIP_CHECK_ARG(!param || (param_period >= 1 && n % param_period == 0), "param_period must divide n");
+  hipLaunchKernelGGL(data_residual_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), res, b, ax, bg, b_mask,
+                     asig, scale, n, term, param, param ? param_period : int64_t{1});
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

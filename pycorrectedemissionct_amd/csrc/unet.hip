@@ -629,16 +629,6 @@ __global__ __launch_bounds__(256) void channel_sum_vec_kernel(const uint16_t* __
   }
 }
 
-// out[i] = sum_c part[c][i] (fixed order)
-__global__ void sum_parts_kernel(const float* __restrict__ part, int n_parts, int64_t len, float* __restrict__ out) {
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    float s = 0.f;
-    for (int c = 0; c < n_parts; ++c) s += part[static_cast<size_t>(c) * len + i];
-    out[i] = s;
-  }
-}
-
 // part[chunk][co] = sum over the chunk's pixels of dy[p][co]
 __global__ void channel_sum_kernel(const uint16_t* __restrict__ dy, int64_t P, int c, int64_t chunk_px,
                                    float* __restrict__ part) {

@@ -10,8 +10,10 @@ edge-padded forward gradient, ``sigma = 1/2`` and ``tau = 2 * ndims``. Its dual
 update and prox run fused in ``chip_tv_dual``; its primal contribution
 (``-div q``) is fused into the SIRT / PDHG primal kernels.
 
-``Regularizer_TV1D`` / ``TV3D`` (``regularizers.py:292-337``), any other
-``Regularizer_Grad`` over the trailing axes with the l12 or l1 norm,
+``Regularizer_TV1D`` / ``TV3D`` (``regularizers.py:292-337``), ``HubTV2D`` /
+``HubTV3D`` (Huber norm, ``regularizers.py:340-383``), ``smooth1D/2D/3D`` (l2
+norm, ``regularizers.py:386-431``), any other ``Regularizer_Grad`` over the
+trailing axes with the l12, l1, l2 or Huber (``l2_axis=0``) norm,
 ``Regularizer_VTV`` (``regularizers.py:1362-1430``) and ``Regularizer_TNV``
 (``regularizers.py:1324-1359``) run on ``chip_grad_dual`` (dual update and
 prox in one pass) and ``chip_grad_update_primal`` (``-div q``, weighted for
@@ -166,15 +168,30 @@ class Regularizer_Grad(BaseRegularizer):
         """Kernel parameters of ``chip_grad_dual`` / ``chip_grad_update_primal`` for a primal of ``shape``."""
         shape = tuple(int(v) for v in shape)
         self._check_trailing_axes(shape)
+        n_batch = int(np.prod(shape[: len(shape) - self.ndims]))
         if isinstance(self.norm, dt.DataFidelity_l12):
             if getattr(self.norm, "l2_axis", 0) != 0:
                 raise NotImplementedError("the l12 norm is supported over the gradient axis (l2_axis=0)")
             mode = 0
         elif type(self.norm) is dt.DataFidelity_l1:
             mode = 1
+        elif type(self.norm) is dt.DataFidelity_l2:
+            # assign_data(None, sigma=0.5) leaves the dual step q * 1 / (1 + 0.5) (data_terms.py:294-304)
+            cfg = self._config(shape, 4, 1, n_batch, 0, 0)
+            cfg["weight"] = float(np.float32(1 / (1 + self.sigma)))
+            return cfg
+        elif type(self.norm) is dt.DataFidelity_Huber:
+            if self.norm.l2_axis != 0:
+                raise NotImplementedError("the Huber norm is supported over the gradient axis (l2_axis=0)")
+            if np.asarray(self.norm.local_error).size != 1:
+                raise NotImplementedError("the HIP gradient kernels take a scalar Huber size")
+            cfg = self._config(shape, 5, 1, n_batch, 0, 0)
+            # one_sigma_error = 1 / (1 + sigma huber_size) (data_terms.py:482-483)
+            cfg["weight"] = float(np.float32(1.0 / (1.0 + self.sigma * float(self.norm.local_error))))
+            return cfg
         else:
             raise NotImplementedError(f"norm {self.norm.info()} is not supported by the HIP gradient kernels")
-        return self._config(shape, mode, 1, int(np.prod(shape[: len(shape) - self.ndims])), 0, 0)
+        return self._config(shape, mode, 1, n_batch, 0, 0)
 
     def _config(self, shape, mode, n_chan, n_batch, der, chan) -> dict:
         l1_family = isinstance(self.norm, dt.DataFidelity_l1)
@@ -225,6 +242,57 @@ class Regularizer_TV3D(Regularizer_Grad):
 
     def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
         super().__init__(weight=weight, ndims=3, axes=axes, pad_mode=pad_mode, norm=norm, upd_mask=upd_mask)
+
+
+class Regularizer_HubTV2D(Regularizer_Grad):
+    """Huber total variation in 2D (``regularizers.py:340-360``): the gradient's dual scaled by
+    ``1 / (1 + 0.5 huber_size)`` then projected onto the unit l2 ball over the gradient axis."""
+
+    __reg_name__ = "HubTV2D"
+
+    def __init__(self, weight, huber_size: float, axes=None, pad_mode: str = "edge", upd_mask=None):
+        super().__init__(weight=weight, ndims=2, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask,
+                         norm=dt.DataFidelity_Huber(huber_size, l2_axis=0))
+
+
+class Regularizer_HubTV3D(Regularizer_Grad):
+    """Huber total variation in 3D (``regularizers.py:363-383``)."""
+
+    __reg_name__ = "HubTV3D"
+
+    def __init__(self, weight, huber_size: float, axes=None, pad_mode: str = "edge", upd_mask=None):
+        super().__init__(weight=weight, ndims=3, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask,
+                         norm=dt.DataFidelity_Huber(huber_size, l2_axis=0))
+
+
+class Regularizer_smooth1D(Regularizer_Grad):
+    """Smoothness (l2-norm of the gradient) in 1D (``regularizers.py:386-399``)."""
+
+    __reg_name__ = "smooth1D"
+
+    def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
+        super().__init__(weight=weight, ndims=1, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask,
+                         norm=dt.DataFidelity_l2() if norm is None else norm)
+
+
+class Regularizer_smooth2D(Regularizer_Grad):
+    """Smoothness in 2D (``regularizers.py:402-415``)."""
+
+    __reg_name__ = "smooth2D"
+
+    def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
+        super().__init__(weight=weight, ndims=2, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask,
+                         norm=dt.DataFidelity_l2() if norm is None else norm)
+
+
+class Regularizer_smooth3D(Regularizer_Grad):
+    """Smoothness in 3D (``regularizers.py:418-431``)."""
+
+    __reg_name__ = "smooth3D"
+
+    def __init__(self, weight, axes=None, pad_mode: str = "edge", upd_mask=None, norm=None):
+        super().__init__(weight=weight, ndims=3, axes=axes, pad_mode=pad_mode, upd_mask=upd_mask,
+                         norm=dt.DataFidelity_l2() if norm is None else norm)
 
 
 _PWISE_CODE = {1: 1, 2: 2, np.inf: 0}

@@ -45,6 +45,19 @@ def _dev(x, like: torch.Tensor | None = None):
     return t
 
 
+def _residual(res, b, ax, bg, b_mask, asig, scale, term, st) -> None:
+    """``res = data_term.compute_residual(ax, b_mask) * scale`` on the device, for the l2-based terms and
+    KL (``asig``: the sigma the term was assigned, None = 1)."""
+    code, prm, period, _, _ = term
+    if code == data_terms.TERM_L2:
+        _native.call("chip_sirt_residual", res.data_ptr(), b.data_ptr(), ax.data_ptr(), _native.ptr(bg),
+                     _native.ptr(b_mask), _native.ptr(scale), b.numel(), st)
+    else:
+        _native.call("chip_data_residual", res.data_ptr(), b.data_ptr(), ax.data_ptr(), _native.ptr(bg),
+                     _native.ptr(b_mask), _native.ptr(asig), _native.ptr(scale), b.numel(), code, _native.ptr(prm),
+                     period, st)
+
+
 class _Limits:
     """Clipping limits of a solve (``x.clip(lower_limit, upper_limit)``, ``solvers.py:797-803,
     975-982, 1185-1190``). Scalars go into the fused update kernels; with an array-valued limit
@@ -495,6 +508,7 @@ class FISTA(Solver):
             l1w = self.regularizer[0].primal_prox_weight()
         self.data_term.assign_data(b_t)
         bg = self.data_term.background_dev(b_t)
+        term = self.data_term.device_term(b_t)
         dev = b_t.device
         x = torch.zeros(x_shape, dtype=torch.float32, device=dev) if x0 is None else _dev(x0, b_t).clone()
         x = x.reshape(x_shape).contiguous()
@@ -514,9 +528,8 @@ class FISTA(Solver):
         for ii in range(iterations):
             info.iterations += 1
             Ay = _arrays.to_dev(A(y))[0]
-            # residual = (b - (A y + bg)) * mask * sigma
-            _native.call("chip_sirt_residual", res.data_ptr(), b_t.data_ptr(), Ay.data_ptr(), _native.ptr(bg),
-                         _native.ptr(b_mask), sigma_t.data_ptr(), b_t.numel(), st)
+            # residual = compute_residual(A y) * sigma, the data term assigned with sigma = 1
+            _residual(res, b_t, Ay, bg, b_mask, None, sigma_t, term, st)
             grad = _arrays.to_dev(A.T(res))[0].contiguous()
             t_new = float(1.0 + np.sqrt(1.0 + 4.0 * t * t)) / 2.0
             if self.restart_period is not None and (ii + 1) % self.restart_period == 0:
@@ -624,6 +637,7 @@ class SIRT(Solver):
         x = torch.zeros_like(tau) if x0 is None else _dev(x0, tau).clone().contiguous()
         self.data_term.assign_data(b_t, sigma)
         bg = self.data_term.background_dev(b_t)
+        term = self.data_term.device_term(b_t)
         lim = _Limits(lower_limit, upper_limit, x)
         hl, lo, hu, hi = lim.kernel()
         info = SolutionInfo(self.info(), max_iterations=iterations, tolerance=self.tolerance)
@@ -641,8 +655,7 @@ class SIRT(Solver):
             Ax = _arrays.to_dev(A(x))[0]
             if track and self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
                 break
-            _native.call("chip_sirt_residual", res.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
-                         _native.ptr(b_mask), sigma.data_ptr(), b_t.numel(), st)
+            _residual(res, b_t, Ax, bg, b_mask, sigma, sigma, term, st)
             # the reference re-initialises the regularizer duals every iteration (solvers.py:950-958)
             for cfg, q_r in grad_regs:
                 q_r.zero_()
@@ -680,7 +693,7 @@ class PDHG(Solver):
             if data_term.lower() == "kl":
                 return data_terms.DataFidelity_KL()
             if data_term.lower() == "l1":
-                raise NotImplementedError("the l1 data term is not supported by the fused HIP kernels")
+                return data_terms.DataFidelity_l1()
             raise ValueError(f'Unknown data term: "{data_term}", accepted terms are: "l2" | "l1" | "kl".')
         return cp.deepcopy(data_term)
 
@@ -696,8 +709,7 @@ class PDHG(Solver):
         b_test_mask = _dev(b_test_mask, b_t)
         b_mask, b_test_mask = self._initialize_b_masks(b_t, b_mask, b_test_mask)
         tv_w = self._tv_weight()
-        if not isinstance(self.data_term, (data_terms.DataFidelity_l2, data_terms.DataFidelity_KL)):
-            raise NotImplementedError(f"data term {self.data_term.info()} is not supported by the fused HIP kernels")
+        term = self.data_term.device_term(b_t)
         if precondition:
             try:
                 At_abs = A.T.absolute()
@@ -728,7 +740,7 @@ class PDHG(Solver):
             b=b_t, host=host, b_mask=b_mask, b_test_mask=b_test_mask, sigma=sigma, tau=tau, x_mask=x_mask_t, x=x,
             x_relax=x.clone(), p=torch.zeros_like(b_t), q=None if tv_w is None else torch.zeros(
                 (batch, 2, n, n), dtype=torch.float32, device=x.device), tv_w=tv_w, n=n, batch=batch,
-            bg=self.data_term.background_dev(b_t), kl=isinstance(self.data_term, data_terms.DataFidelity_KL),
+            bg=self.data_term.background_dev(b_t), term=term,
             grad_regs=self._grad_regs(x) if (tv_w is None and self.regularizer) else [],
         )
 
@@ -739,14 +751,20 @@ class PDHG(Solver):
         lim = _Limits(lower_limit, upper_limit, x)
         hl, lo, hu, hi = lim.kernel()
         x_old = None if lim.fused else torch.empty_like(x)
-        dual = "chip_pdhg_dual_kl" if s["kl"] else "chip_pdhg_dual_l2"
+        code, prm, period, g_len, g_stride = s["term"]
+        dual = {data_terms.TERM_L2: "chip_pdhg_dual_l2", data_terms.TERM_KL: "chip_pdhg_dual_kl"}.get(code)
         track = info is not None and (s["b_test_mask"] is not None or self.tolerance is not None)
         for ii in range(iterations):
             if info is not None:
                 info.iterations += 1
             Ax = _arrays.to_dev(A(xr))[0]
-            _native.call(dual, p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(), _native.ptr(s["bg"]),
-                         _native.ptr(s["b_mask"]), p.numel(), st)
+            if dual is not None:
+                _native.call(dual, p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(),
+                             _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), st)
+            else:
+                _native.call("chip_pdhg_dual_norm", p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(),
+                             _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), code, _native.ptr(prm),
+                             period, g_len, g_stride, st)
             if q is not None:
                 _native.call("chip_tv_dual", q.data_ptr(), xr.data_ptr(), s["n"], s["batch"], s["tv_w"], st)
             for cfg, q_r in s["grad_regs"]:

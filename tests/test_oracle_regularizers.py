@@ -177,3 +177,55 @@ def test_vtv_reference_volume():
     assert np.array_equal(r.q_ref, F32(0.5) * oreg.grad_nd(x_ref, 2))
     assert not np.array_equal(r.dual(np.zeros_like(q), rng.random((3, 5, 6)).astype(F32)),
                               oreg.GradReg(2, 0.4, "vtv").dual(np.zeros_like(q), rng.random((3, 5, 6)).astype(F32)))
+
+
+def _grad_div_matrices(shape, nd):
+    """Explicit G (edge-padded gradient) and D (the primal update's ``-div``) of ``grad_nd`` / ``div_nd``."""
+    n = int(np.prod(shape))
+    G = np.stack([oreg.grad_nd(np.eye(n)[i].reshape(shape), nd).reshape(-1) for i in range(n)], axis=1)
+    D = np.stack([-oreg.div_nd(np.eye(nd * n)[i].reshape((nd, *shape))).reshape(-1) for i in range(nd * n)], axis=1)
+    return G, D
+
+
+def test_smooth_and_huber_tv_fixed_points():
+    """PDHG's fixed point (p = M x - b, q the regularizer's dual at G x) solves
+    M^T (M x - b) + w D psi'(G x) = 0, with psi'(g) = g for ``smooth2D`` (the normal equations
+    (M^T M + w D G) x = M^T b) and psi'(g) = g / h where |g|_2 <= h, g / |g|_2 beyond, per pixel, for
+    ``HubTV2D``. D is the reference's ``-div`` (operators.py:818-844), which is G^T except at the
+    first sample of each axis (the edge-mode divergence drops -q[0] there)."""
+    from scipy.optimize import root
+
+    shape = (5, 6)
+    rng = np.random.default_rng(12)
+    n = int(np.prod(shape))
+    mat = rng.random((20, n)).astype(F32)
+    truth = np.zeros(shape, F32)
+    truth[1:4, 2:5] = 1
+    b = (mat @ truth.reshape(-1) + rng.normal(0, 0.05, 20)).astype(F32)
+    A = lambda v: (mat @ v.reshape(-1)).astype(F32)  # noqa: E731
+    At = lambda y: (mat.T @ y).astype(F32).reshape(shape)  # noqa: E731
+    G, D = _grad_div_matrices(shape, 2)
+    assert np.abs(D - G.T).max() == 1.0  # the quirk: not the transpose
+    M = mat.astype(np.float64)
+    w = 0.5
+    want = np.linalg.solve(M.T @ M + w * D @ G, M.T @ b.astype(np.float64)).reshape(shape)
+    got = osol.pdhg(A, At, b, shape, 3000, regs=[oreg.GradReg(2, w, "l2")])
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"smooth2D fixed point vs normal equations: {err:.2e}")
+    assert err < 1e-5
+
+    w, h = 0.2, 0.05
+
+    def fixed_point(x):
+        g = (G @ x).reshape(2, -1)
+        r = np.sqrt(np.sum(g * g, axis=0))
+        dpsi = np.where(r <= h, 1 / h, 1 / np.maximum(r, 1e-300)) * g
+        return M.T @ (M @ x - b) + w * D @ dpsi.reshape(-1)
+
+    sol = root(fixed_point, want.reshape(-1), method="hybr", options=dict(xtol=1e-14))
+    assert sol.success and np.max(np.abs(fixed_point(sol.x))) < 1e-10
+    want = sol.x.reshape(shape)
+    got = osol.pdhg(A, At, b, shape, 6000, regs=[oreg.GradReg(2, w, "hub", huber_size=h)])
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"HubTV2D fixed point vs root of the optimality system: {err:.2e}")
+    assert err < 1e-4
