@@ -352,29 +352,52 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * the caller passing 1 / (1 + 0.5 huber_size); q /= max(1, |q|_2):
  * Regularizer_HubTV2D/3D, regularizers.py:340-384 with data_terms.py:482-501).
  * Modes 0, 1, 4 and 5 need n_chan == 1; n_chan <= 16.
+ * weights (modes 0, 1, 2; else NULL): the regularizer's weight as an array
+ * broadcast over the primal, weights[e % weights_period] for primal element e
+ * (weights_period divides the primal's element count); it replaces `weight` in
+ * the prox of every dual component at that sample (the reference's array
+ * weight broadcast against the dual, data_terms.py:549-555,
+ * regularizers.py:1413-1430).
  * qref (VTV only, else NULL): the dual of Regularizer_VTV's reference volume,
  * [nd][n_batch][dims...], a fixed extra channel after the n_chan channels in
  * every point-wise norm, never updated (regularizers.py:1386-1392, 1410-1411).
  * chip_grad_update_primal: upd += t (or -= t when subtract) with
  * t = -div(q), times mask[e % mask_period] when mask is not NULL (the
  * regularizer's upd_mask, broadcast over leading axes: mask_period divides
- * the primal's element count), times scale when apply_scale
+ * the primal's element count), times scale (or scales[e % scales_period]
+ * when scales is not NULL: an array weight) when apply_scale
  * (compute_update_primal, regularizers.py:194-210; PDHG adds, SIRT
  * subtracts, solvers.py:953-958, 1178-1180).
  */
 int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, int der_norm, int chan_norm, const float* qref, void* stream);
+                   float weight, const float* weights, int64_t weights_period, int der_norm, int chan_norm,
+                   const float* qref, void* stream);
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, int apply_scale, int subtract, const float* mask, int64_t mask_period,
-                            void* stream);
+                            float scale, const float* scales, int64_t scales_period, int apply_scale, int subtract,
+                            const float* mask, int64_t mask_period, void* stream);
+
+/* Regularizer_l1 in PDHG / SIRT (identity transform, regularizers.py:531-557 and
+ * the base class's dual / primal steps, regularizers.py:134-210):
+ * chip_l1_dual: q = q + x, then q = (q / max(|q|, w)) * w with w = weight, or
+ * weights[i % weights_period] when weights is not NULL (data_terms.py:549-555).
+ * chip_l1_update_primal: upd += q (or -= q when subtract), times
+ * mask[i % mask_period] when mask is not NULL (upd_mask). */
+int chip_l1_dual(float* q, const float* x, int64_t n, float weight, const float* weights, int64_t weights_period,
+                 void* stream);
+int chip_l1_update_primal(float* upd, const float* q, int64_t n, const float* mask, int64_t mask_period, int subtract,
+                          void* stream);
 
 /* FISTA step (solvers.py:1458-1490): x_new = y + tau * grad (tau per element,
- * or tau_scalar when tau is NULL), soft-thresholded at tau * l1_weight when
- * l1_weight > 0 (Regularizer_l1.apply_proximal_primal, regularizers.py:559-585),
+ * or tau_scalar when tau is NULL); then the l1 prox with weight w = l1_weight, or
+ * l1_weights[i % l1_weights_period] when l1_weights is not NULL, when l1_weight > 0
+ * or l1_weights is given: x_new = soft(x_new, tau * w), or with an update mask m =
+ * l1_mask[i % l1_mask_period], x_new = x_new (1 - m) + soft(x_new m, tau * w)
+ * (Regularizer_l1.apply_proximal_primal, regularizers.py:559-585); then
  * clipped, masked; then y = x_new + momentum (x_new - x) and x = x_new. */
 int chip_fista_update(float* x, float* y, const float* grad, const float* tau, float tau_scalar, float l1_weight,
-                      int has_lower, float lower, int has_upper, float upper, const float* x_mask, float momentum,
-                      int64_t n, void* stream);
+                      const float* l1_weights, int64_t l1_weights_period, const float* l1_mask,
+                      int64_t l1_mask_period, int has_lower, float lower, int has_upper, float upper,
+                      const float* x_mask, float momentum, int64_t n, void* stream);
 
 /* Array-valued clipping (x_new.clip(lower_limit, upper_limit) with array limits,
  * solvers.py:797-803, 975-982, 1185-1190): x = clip(x, lower, upper) * x_mask,

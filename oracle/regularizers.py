@@ -13,6 +13,7 @@ Test infrastructure only (see ``oracle/__init__.py``).
 * ``prox_tnv`` -- ``DataFidelity_ln.apply_proximal_dual`` with ``ln_axes=(1, 0)``
   and the l1 spectral norm (``data_terms.py:781-820``): numpy's SVD of each
   (channels x derivatives) matrix, singular values ``s / max(s, 1)``.
+* ``L1Reg`` -- ``Regularizer_l1`` with its identity transform in PDHG / SIRT.
 * ``GradReg`` -- ``Regularizer_Grad`` bookkeeping (``regularizers.py:134-215,
   260-273``): tau contribution ``2 nd`` (times the weight outside the l1
   family, times ``upd_mask`` when given), dual update ``q += 0.5 grad`` then
@@ -56,13 +57,20 @@ def div_nd(q: np.ndarray) -> np.ndarray:
     return out
 
 
-def prox_l12(q: np.ndarray, w: float) -> np.ndarray:
+def _w(w, dtype):
+    """A weight as the dual's dtype: a scalar, or an array broadcast against the primal's trailing axes."""
+    return np.asarray(w, dtype=dtype)
+
+
+def prox_l12(q: np.ndarray, w) -> np.ndarray:
     nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=q.dtype))
-    return (q / np.fmax(nrm, q.dtype.type(w))) * q.dtype.type(w)
+    w = _w(w, q.dtype)
+    return (q / np.fmax(nrm, w)) * w
 
 
-def prox_l1(q: np.ndarray, w: float) -> np.ndarray:
-    return (q / np.fmax(np.abs(q), q.dtype.type(w))) * q.dtype.type(w)
+def prox_l1(q: np.ndarray, w) -> np.ndarray:
+    w = _w(w, q.dtype)
+    return (q / np.fmax(np.abs(q), w)) * w
 
 
 def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None, q_ref=None) -> np.ndarray:
@@ -86,9 +94,12 @@ def prox_vtv(q: np.ndarray, w: float, der=2, chan=np.inf, nd: int | None = None,
         dn = np.linalg.norm(gn, axis=1, ord=2, keepdims=True)
     else:
         dn = np.linalg.norm(gn, axis=1, ord=1, keepdims=True)
-    t = (t / np.fmax(dn, t.dtype.type(w))) * t.dtype.type(w)
-    if q_ref is not None:
+    if q_ref is not None:  # the weight applies to the C channels; the reference channel is dropped
         t = t[:, : t.shape[1] - 1]
+        if dn.shape[1] > 1:
+            dn = dn[:, : dn.shape[1] - 1]
+    w = _w(w, t.dtype)
+    t = (t / np.fmax(dn, w)) * w
     return t[:, 0] if scalar else t
 
 
@@ -114,9 +125,11 @@ class GradReg:
     numpy multiplies, ``data_terms.py:482-495``) then divided by ``max(1, |q|_2)``
     (``data_terms.py:500-501``). Both weight the primal update (outside the l1 family)."""
 
-    def __init__(self, nd: int, weight: float, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
+    def __init__(self, nd: int, weight, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
                  huber_size: float = 0.0):
-        self.nd, self.weight, self.kind, self.der, self.chan = nd, float(weight), kind, der, chan
+        # an array weight broadcasts against the primal (and the dual's trailing axes)
+        weight = float(weight) if np.ndim(weight) == 0 else np.asarray(weight, dtype=F32)
+        self.nd, self.weight, self.kind, self.der, self.chan = nd, weight, kind, der, chan
         self.huber_size = float(huber_size)
         self.upd_mask = None if upd_mask is None else np.asarray(upd_mask, dtype=F32)
         # VTV's reference dual: 0.5 * grad(x_ref) from a zero dual (regularizers.py:1386-1392)
@@ -147,8 +160,34 @@ class GradReg:
             return (q / np.fmax(F32(1), nrm)).astype(F32)
         return prox_tnv(q)
 
+    def zeros(self, x_shape) -> np.ndarray:
+        return np.zeros((self.nd, *x_shape), dtype=F32)
+
     def primal(self, q: np.ndarray) -> np.ndarray:
         u = -div_nd(q)
         if self.upd_mask is not None:
             u = u * self.upd_mask
-        return u * F32(self.weight) if self.weighted else u
+        return u * np.asarray(self.weight, dtype=F32) if self.weighted else u
+
+
+class L1Reg:
+    """``Regularizer_l1`` in PDHG / SIRT (``regularizers.py:531-557`` with the base class's steps,
+    ``regularizers.py:134-210``): identity transform, ``q += x`` then the l1 prox with the weight
+    (scalar or array), primal contribution ``q`` (times ``upd_mask``), tau contribution 1 (times
+    ``upd_mask``)."""
+
+    def __init__(self, weight, upd_mask=None):
+        self.weight = float(weight) if np.ndim(weight) == 0 else np.asarray(weight, dtype=F32)
+        self.upd_mask = None if upd_mask is None else np.asarray(upd_mask, dtype=F32)
+
+    def zeros(self, x_shape) -> np.ndarray:
+        return np.zeros(x_shape, dtype=F32)
+
+    def tau(self):
+        return 1.0 if self.upd_mask is None else F32(1.0) * self.upd_mask
+
+    def dual(self, q: np.ndarray, x: np.ndarray) -> np.ndarray:
+        return prox_l1((q + x).astype(F32), self.weight)
+
+    def primal(self, q: np.ndarray) -> np.ndarray:
+        return q * self.upd_mask if self.upd_mask is not None else q

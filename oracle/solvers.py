@@ -105,7 +105,7 @@ def sirt(A, At, b, x_shape, iterations, x0=None, lower=None, upper=None, x_mask=
             q = tv_dual(np.zeros((2, *x_shape), dtype=F32), x, tv_weight)
             upd = upd - (-div(q))
         for r in regs:
-            upd = upd - r.primal(r.dual(np.zeros((r.nd, *x_shape), dtype=F32), x))
+            upd = upd - r.primal(r.dual(r.zeros(x_shape), x))
         x = x + upd * tau
         if lower is not None or upper is not None:
             x = x.clip(lower, upper)
@@ -164,9 +164,11 @@ def power_method(A, At, x_start, iterations: int = 5) -> float:
 
 
 def fista(A, At, b, x_shape, iterations, precondition=True, L=None, relaxation=1.0, l1=0.0, lower=None, upper=None,
-          x_mask=None, b_mask=None, restart_period=None, data_term=None):
+          x_mask=None, b_mask=None, restart_period=None, data_term=None, l1_mask=None):
     """FISTA: y-gradient step with diagonal (SIRT-style) or 1/L scaling, soft threshold at
-    tau * l1, clip, mask, Beck-Teboulle momentum (``solvers.py:1391-1490``)."""
+    tau * l1 (``l1`` a scalar or an array), clip, mask, Beck-Teboulle momentum
+    (``solvers.py:1391-1490``). ``l1_mask``: ``Regularizer_l1``'s update mask, thresholding
+    ``x m`` and keeping ``x (1 - m)`` (``regularizers.py:578-583``)."""
     b = np.asarray(b, dtype=F32)
     if precondition:
         sigma, tau = diagonal_scaling(A, At, b, x_shape, rho_tau=relaxation, x_mask=x_mask, b_mask=b_mask)
@@ -187,9 +189,14 @@ def fista(A, At, b, x_shape, iterations, precondition=True, L=None, relaxation=1
             if b_mask is not None:
                 res = res * b_mask
         xn = y + tau * At(res * sigma)
-        if l1 > 0:
-            thr = tau * F32(l1)
-            xn = np.sign(xn) * np.fmax(np.abs(xn) - thr, 0)
+        if np.ndim(l1) or l1 > 0:
+            thr = (tau * np.asarray(l1, dtype=F32)).astype(F32)
+            if l1_mask is not None:
+                xm = (xn * l1_mask).astype(F32)
+                xm = np.sign(xm) * np.fmax(np.abs(xm) - thr, 0)
+                xn = (xn * (F32(1) - l1_mask)).astype(F32) + xm
+            else:
+                xn = np.sign(xn) * np.fmax(np.abs(xn) - thr, 0)
         if lower is not None or upper is not None:
             xn = xn.clip(lower, upper)
         if x_mask is not None:
@@ -267,7 +274,7 @@ def pdhg(A, At, b, x_shape, iterations, data_term="l2", tv_weight=None, x0=None,
         raise ValueError(data_term)
     p = np.zeros_like(b)
     q = np.zeros((2, *x_shape), dtype=F32) if tv_weight is not None else None
-    qs = [np.zeros((r.nd, *x_shape), dtype=F32) for r in regs]
+    qs = [r.zeros(x_shape) for r in regs]
 
     def res_norm(ax, mask):
         r = (b - ax).astype(F32)

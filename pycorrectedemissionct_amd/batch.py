@@ -379,6 +379,8 @@ class BatchedPDHG:
             raise NotImplementedError("the batched PDHG takes gradient-family regularizers")
         if reg.upd_mask is not None:
             raise NotImplementedError("the batched PDHG takes regularizers without an update mask")
+        if np.ndim(reg.weight) != 0:
+            raise NotImplementedError("the batched PDHG takes regularizers with a scalar weight")
         c = self.channels
         if len(A.sizes) % c:
             raise ValueError(f"{len(A.sizes)} slices do not split into groups of {c} channels")

@@ -111,8 +111,9 @@ __device__ __forceinline__ float pnorm_acc(float acc, float v, int ord) {
 // dual values stay in registers; g.C <= CM channels are live.
 template <int ND, int CM>
 __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__ q, const float* __restrict__ x,
-                                                             GradGeom g, int mode, float w, int der, int chan,
-                                                             const float* __restrict__ qref) {
+                                                             GradGeom g, int mode, float w,
+                                                             const float* __restrict__ wt, int64_t wt_period,
+                                                             int der, int chan, const float* __restrict__ qref) {
   const int64_t total = g.B * g.S;
   const int64_t cs = g.B * g.S;  // channel stride of x and of each q component
   const int64_t qs = static_cast<int64_t>(g.C) * cs;  // component stride of q
@@ -135,16 +136,20 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
         v[k][c] = q[k * qs + c * cs + off] + 0.5f * gk;
       }
     }
+    // the weight of channel c at this sample: an array broadcast over the primal (period
+    // dividing its size), else the scalar
+    auto wch = [&](int c) -> float { return wt ? wt[(c * cs + off) % wt_period] : w; };
     if (mode == kGrL12) {
 #pragma unroll
       for (int c = 0; c < CM; ++c) {
         if (c >= g.C) break;
+        const float wc = wch(c);
         float acc = 0.f;
 #pragma unroll
         for (int k = 0; k < ND; ++k) acc = acc + v[k][c] * v[k][c];
-        const float den = fmaxf(sqrtf(acc), w);
+        const float den = fmaxf(sqrtf(acc), wc);
 #pragma unroll
-        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * w;
+        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * wc;
       }
     } else if (mode == kGrL2) {
       // DataFidelity_l2 without data: q = q * (1 / (1 + sigma)), w holding that factor
@@ -174,8 +179,9 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 #pragma unroll
       for (int c = 0; c < CM; ++c) {
         if (c >= g.C) break;
+        const float wc = wch(c);
 #pragma unroll
-        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / fmaxf(fabsf(v[k][c]), w)) * w;
+        for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / fmaxf(fabsf(v[k][c]), wc)) * wc;
       }
     } else if (mode == kGrVTV) {
       // qref: the reference volume's fixed dual, one more channel after the C channels in
@@ -197,7 +203,8 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
           for (int c = 0; c < CM; ++c) {
             if (c >= g.C) break;
             const float dn = chan == 1 ? fabsf(v[k][c]) : dn_k;
-            v[k][c] = (v[k][c] / fmaxf(dn, w)) * w;
+            const float wc = wch(c);
+            v[k][c] = (v[k][c] / fmaxf(dn, wc)) * wc;
           }
         }
       } else {
@@ -223,9 +230,10 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 #pragma unroll
         for (int c = 0; c < CM; ++c) {
           if (c >= g.C) break;
-          const float den = fmaxf(chan == 1 ? gn[c] : dn_all, w);
+          const float wc = wch(c);
+          const float den = fmaxf(chan == 1 ? gn[c] : dn_all, wc);
 #pragma unroll
-          for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * w;
+          for (int k = 0; k < ND; ++k) v[k][c] = (v[k][c] / den) * wc;
         }
       }
     } else {  // kGrTNV
@@ -280,20 +288,24 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
 }
 
 template <int ND>
-void launch_grad_dual(float* q, const float* x, const GradGeom& g, int mode, float w, int der, int chan,
-                      const float* qref, hipStream_t st) {
+void launch_grad_dual(float* q, const float* x, const GradGeom& g, int mode, float w, const float* wt,
+                      int64_t wt_period, int der, int chan, const float* qref, hipStream_t st) {
   const dim3 grid = gr_grid(g.B * g.S);
   if (g.C == 1)
-    hipLaunchKernelGGL((grad_dual_kernel<ND, 1>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan, qref);
-  else if (g.C <= 4)
-    hipLaunchKernelGGL((grad_dual_kernel<ND, 4>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der, chan, qref);
-  else
-    hipLaunchKernelGGL((grad_dual_kernel<ND, kGrMaxChan>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, der,
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 1>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, wt, wt_period, der,
                        chan, qref);
+  else if (g.C <= 4)
+    hipLaunchKernelGGL((grad_dual_kernel<ND, 4>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, wt, wt_period, der,
+                       chan, qref);
+  else
+    hipLaunchKernelGGL((grad_dual_kernel<ND, kGrMaxChan>), grid, dim3(kGrBlock), 0, st, q, x, g, mode, w, wt,
+                       wt_period, der, chan, qref);
 }
 
 __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict__ upd, const float* __restrict__ q,
-                                                               GradGeom g, float scale, int apply_scale,
+                                                               GradGeom g, float scale,
+                                                               const float* __restrict__ scales,
+                                                               int64_t scales_period, int apply_scale,
                                                                int subtract, const float* __restrict__ mask,
                                                                int64_t mask_period) {
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict
     }
     float t = -d;
     if (mask) t = t * mask[e % mask_period];
-    if (apply_scale) t = t * scale;
+    if (apply_scale) t = t * (scales ? scales[e % scales_period] : scale);
     upd[e] = subtract ? upd[e] - t : upd[e] + t;
   }
 }
@@ -349,7 +361,8 @@ using namespace chip;
 extern "C" {
 
 int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, int der_norm, int chan_norm, const float* qref, void* stream) {
+                   float weight, const float* weights, int64_t weights_period, int der_norm, int chan_norm,
+                   const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
   CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrHuber, "unknown mode");
@@ -358,20 +371,26 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
                  "VTV point-wise norms must be 1, 2 or 0 (inf)");
   CHIP_CHECK_ARG(mode == kGrVTV || mode == kGrTNV || n_chan == 1, "coupled channels need the VTV or TNV mode");
   CHIP_CHECK_ARG(qref == nullptr || mode == kGrVTV, "a reference dual (qref) is taken by the VTV mode only");
+  CHIP_CHECK_ARG(weights == nullptr || mode == kGrL12 || mode == kGrL1 || mode == kGrVTV,
+                 "a weight array is taken by the l12, l1 and VTV modes");
   GradGeom g;
   const int rc = make_geom(g, n_chan, n_batch, nd, dims);
   if (rc != CHIP_OK) return rc;
+  const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
+  CHIP_CHECK_ARG(weights == nullptr || (weights_period >= 1 && total % weights_period == 0),
+                 "weights_period must divide the primal's element count");
+  const int64_t wp = weights ? weights_period : 1;
   hipStream_t st = as_stream(stream);
-  if (nd == 1) launch_grad_dual<1>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
-  else if (nd == 2) launch_grad_dual<2>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
-  else launch_grad_dual<3>(q, x, g, mode, weight, der_norm, chan_norm, qref, st);
+  if (nd == 1) launch_grad_dual<1>(q, x, g, mode, weight, weights, wp, der_norm, chan_norm, qref, st);
+  else if (nd == 2) launch_grad_dual<2>(q, x, g, mode, weight, weights, wp, der_norm, chan_norm, qref, st);
+  else launch_grad_dual<3>(q, x, g, mode, weight, weights, wp, der_norm, chan_norm, qref, st);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
 
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, int apply_scale, int subtract, const float* mask, int64_t mask_period,
-                            void* stream) {
+                            float scale, const float* scales, int64_t scales_period, int apply_scale, int subtract,
+                            const float* mask, int64_t mask_period, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(upd != nullptr && q != nullptr, "NULL argument");
   GradGeom g;
@@ -380,8 +399,10 @@ int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_ba
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   CHIP_CHECK_ARG(mask == nullptr || (mask_period >= 1 && total % mask_period == 0),
                  "mask_period must divide the primal's element count");
+  CHIP_CHECK_ARG(scales == nullptr || (scales_period >= 1 && total % scales_period == 0),
+                 "scales_period must divide the primal's element count");
   hipLaunchKernelGGL(grad_primal_kernel, gr_grid(total), dim3(kGrBlock), 0, as_stream(stream), upd, q, g, scale,
-                     apply_scale, subtract, mask, mask_period);
+                     scales, scales ? scales_period : int64_t{1}, apply_scale, subtract, mask, mask_period);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

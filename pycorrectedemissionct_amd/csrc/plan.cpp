@@ -92,7 +92,7 @@ static const char* kSymbols[] = {
     "chip_plan_create",   "chip_plan_destroy", "chip_att_maps",     "chip_fp",
     "chip_bp",            "chip_fp_angle",     "chip_bp_angle",     "chip_absmax",
     "chip_precond_fix",   "chip_pdhg_dual_l2", "chip_pdhg_dual_kl", "chip_pdhg_dual_norm",
-    "chip_data_residual", "chip_tv_dual",
+    "chip_data_residual", "chip_l1_dual",      "chip_l1_update_primal", "chip_tv_dual",
     "chip_pdhg_primal",   "chip_sirt_residual", "chip_sirt_update", "chip_mlem_ratio",
     "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",       "chip_plan_tune",
     "chip_fp_otf",        "chip_bp_otf",       "chip_batch_plan_create", "chip_batch_plan_destroy",

@@ -420,21 +420,51 @@ __global__ __launch_bounds__(256) void psf_lines_kernel(const float* __restrict_
 __global__ __launch_bounds__(kEwBlock) void fista_update_kernel(float* __restrict__ x, float* __restrict__ y,
                                                                 const float* __restrict__ grad,
                                                                 const float* __restrict__ tau, float tau_s, float l1w,
+                                                                const float* __restrict__ l1wa, int64_t l1w_period,
+                                                                const float* __restrict__ l1m, int64_t l1m_period,
                                                                 int hl, float lo, int hu, float hi,
                                                                 const float* __restrict__ mask, float momentum,
                                                                 int64_t n) {
+  const bool prox = l1wa != nullptr || l1w > 0.f;
   GRID_STRIDE(i, n) {
     const float t = tau ? tau[i] : tau_s;
     float xn = y[i] + t * grad[i];
-    if (l1w > 0.f) {
-      const float thr = t * l1w;
-      const float ax = fabsf(xn) - thr;
-      xn = ax > 0.f ? copysignf(ax, xn) : 0.f;
+    if (prox) {
+      const float thr = t * (l1wa ? l1wa[i % l1w_period] : l1w);
+      if (l1m) {  // thresholded inside the update mask, blended outside
+        const float m = l1m[i % l1m_period];
+        const float xm = soft_threshold(xn * m, thr);
+        xn = xn * (1.f - m) + xm;
+      } else {
+        xn = soft_threshold(xn, thr);
+      }
     }
     xn = clipf(xn, hl, lo, hu, hi);
     if (mask) xn = xn * mask[i];
     y[i] = xn + momentum * (xn - x[i]);
     x[i] = xn;
+  }
+}
+
+// ---- Regularizer_l1 (identity transform) ------------------------------------------
+
+__global__ __launch_bounds__(kEwBlock) void l1_dual_kernel(float* __restrict__ q, const float* __restrict__ x,
+                                                           int64_t n, float w, const float* __restrict__ wa,
+                                                           int64_t w_period) {
+  GRID_STRIDE(i, n) {
+    const float wi = wa ? wa[i % w_period] : w;
+    const float v = q[i] + x[i];
+    q[i] = (v / fmaxf(fabsf(v), wi)) * wi;
+  }
+}
+
+__global__ __launch_bounds__(kEwBlock) void l1_primal_kernel(float* __restrict__ upd, const float* __restrict__ q,
+                                                             int64_t n, const float* __restrict__ mask,
+                                                             int64_t m_period, int subtract) {
+  GRID_STRIDE(i, n) {
+    float t = q[i];
+    if (mask) t = t * mask[i % m_period];
+    upd[i] = subtract ? upd[i] - t : upd[i] + t;
   }
 }
 
@@ -693,14 +723,43 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
 }
 
 int chip_fista_update(float* x, float* y, const float* grad, const float* tau, float tau_scalar, float l1_weight,
-                      int has_lower, float lower, int has_upper, float upper, const float* x_mask, float momentum,
-                      int64_t n, void* stream) {
+                      const float* l1_weights, int64_t l1_weights_period, const float* l1_mask,
+                      int64_t l1_mask_period, int has_lower, float lower, int has_upper, float upper,
+                      const float* x_mask, float momentum, int64_t n, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(x && y && grad, "NULL pointer");
   CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
   CHIP_CHECK_ARG(l1_weight >= 0.f, "l1_weight must be >= 0");
+  CHIP_CHECK_ARG(!l1_weights || (l1_weights_period >= 1 && n % l1_weights_period == 0),
+                 "l1_weights_period must divide n");
+  CHIP_CHECK_ARG(!l1_mask || (l1_mask_period >= 1 && n % l1_mask_period == 0), "l1_mask_period must divide n");
   hipLaunchKernelGGL(fista_update_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), x, y, grad, tau,
-                     tau_scalar, l1_weight, has_lower, lower, has_upper, upper, x_mask, momentum, n);
+                     tau_scalar, l1_weight, l1_weights, l1_weights ? l1_weights_period : int64_t{1}, l1_mask,
+                     l1_mask ? l1_mask_period : int64_t{1}, has_lower, lower, has_upper, upper, x_mask, momentum, n);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_l1_dual(float* q, const float* x, int64_t n, float weight, const float* weights, int64_t weights_period,
+                 void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(q && x, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  CHIP_CHECK_ARG(!weights || (weights_period >= 1 && n % weights_period == 0), "weights_period must divide n");
+  hipLaunchKernelGGL(l1_dual_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), q, x, n, weight, weights,
+                     weights ? weights_period : int64_t{1});
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
+
+int chip_l1_update_primal(float* upd, const float* q, int64_t n, const float* mask, int64_t mask_period, int subtract,
+                          void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(upd && q, "NULL pointer");
+  CHIP_CHECK_ARG(n >= 1, "n must be >= 1");
+  CHIP_CHECK_ARG(!mask || (mask_period >= 1 && n % mask_period == 0), "mask_period must divide n");
+  hipLaunchKernelGGL(l1_primal_kernel, ew_grid(n), dim3(kEwBlock), 0, as_stream(stream), upd, q, n, mask,
+                     mask ? mask_period : int64_t{1}, subtract);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

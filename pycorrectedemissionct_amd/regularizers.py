@@ -32,6 +32,19 @@ from . import data_terms as dt
 from . import operators
 
 
+def _product(arrs, shape) -> torch.Tensor | None:
+    """The elementwise product of flat arrays that each broadcast over ``shape`` (None if there are none)."""
+    if not arrs:
+        return None
+    if len(arrs) == 1:
+        return arrs[0]
+    total = int(np.prod(shape))
+    out = arrs[0].repeat(total // arrs[0].numel())
+    for a in arrs[1:]:
+        out = out * a.repeat(total // a.numel())
+    return out.contiguous()
+
+
 class BaseRegularizer:
     """Regularizer object API of ``regularizers.py:36-215``: ``initialize_sigma_tau``,
     ``initialize_dual``, ``update_dual``, ``apply_proximal_dual`` and ``compute_update_primal`` on
@@ -68,17 +81,47 @@ class BaseRegularizer:
         if self.upd_mask is not None:
             upd = upd * self.upd_mask
         if not isinstance(self.norm, dt.DataFidelity_l1):
-            upd = upd * float(self.weight)
+            w = float(self.weight) if self.weight.ndim == 0 else self.weight.astype(np.float32)
+            if isinstance(upd, torch.Tensor) and not np.isscalar(w):
+                w = torch.as_tensor(w, device=upd.device)
+            upd = upd * w
         return upd
 
+    @staticmethod
+    def _bcast_dev(value, shape, what: str) -> torch.Tensor:
+        """``value`` as a flat contiguous float32 device tensor that broadcasts over a primal of ``shape``:
+        its shape (leading 1s dropped) must be a trailing part of ``shape``, so element e of the primal
+        reads entry ``e % numel``, as NumPy broadcasts it in the reference."""
+        t = value if isinstance(value, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(value, dtype=np.float32))
+        t = t.to(device=_arrays.device(), dtype=torch.float32).contiguous()
+        vshape = tuple(int(v) for v in t.shape)
+        while vshape and vshape[0] == 1:
+            vshape = vshape[1:]
+        shape = tuple(int(v) for v in shape)
+        if len(vshape) > len(shape) or (vshape and shape[len(shape) - len(vshape):] != vshape):
+            raise ValueError(f"{what} of shape {tuple(t.shape)} does not broadcast over a primal of shape {shape} "
+                             "along its leading axes")
+        return t.reshape(-1)
+
+    def _weight_dev(self, shape) -> torch.Tensor | None:
+        """The weight as a device array over the primal, or None for a scalar weight."""
+        return None if self.weight.ndim == 0 else self._bcast_dev(self.weight, shape, "weight")
+
     def _prox_on_device(self, dual, cfg: dict) -> None:
-        """The dual proximal of ``cfg`` in place: ``chip_grad_dual`` with a zero primal."""
+        """The dual proximal of ``cfg`` in place: ``chip_grad_dual`` (``chip_l1_dual`` for the identity
+        transform) with a zero primal."""
         t, host = _arrays.to_dev(dual)
         t = t.contiguous()
-        zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
-        _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"],
-                     _native.ptr(cfg.get("qref")), _native.stream())
+        if cfg.get("identity"):
+            zero = torch.zeros_like(t)
+            _native.call("chip_l1_dual", t.data_ptr(), zero.data_ptr(), t.numel(), cfg["weight"],
+                         _native.ptr(cfg.get("weights")), cfg.get("weights_period", 1), _native.stream())
+        else:
+            zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
+            _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
+                         cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], _native.ptr(cfg.get("weights")),
+                         cfg.get("weights_period", 1), cfg["der"], cfg["chan"], _native.ptr(cfg.get("qref")),
+                         _native.stream())
         if host:
             dual[...] = t.reshape(np.shape(dual)).cpu().numpy()
         elif t.data_ptr() != dual.data_ptr():
@@ -125,27 +168,22 @@ class Regularizer_Grad(BaseRegularizer):
         return tau
 
     def tau_terms(self, shape):
-        """(scalar tau contribution, update mask as a flat device tensor or None): the factors of
-        ``initialize_sigma_tau``'s result, for the solvers' device kernels."""
+        """(scalar, flat device array or None) whose product is ``initialize_sigma_tau``'s result,
+        for the solvers' device kernels: ``2 ndims`` (times a scalar weight outside the l1 family),
+        and the update mask, an array weight outside the l1 family, or their product."""
         tau = 2 * self.ndims
+        arrs = [] if self.upd_mask is None else [self._device_mask(shape)]
         if not isinstance(self.norm, dt.DataFidelity_l1):
-            tau *= self.weight
-        return float(tau), (None if self.upd_mask is None else self._device_mask(shape))
+            w = self._weight_dev(shape)
+            if w is None:
+                tau *= float(self.weight)
+            else:
+                arrs.insert(0, w)
+        return float(tau), _product(arrs, shape)
 
     def _device_mask(self, shape) -> torch.Tensor:
-        """The update mask as a contiguous float32 device tensor; its shape must be a trailing part of
-        the primal's (it broadcasts over the leading axes, as NumPy broadcasts it in the reference)."""
-        m = self.upd_mask
-        t = m if isinstance(m, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(m, dtype=np.float32))
-        t = t.to(device=_arrays.device(), dtype=torch.float32).contiguous()
-        mshape = tuple(int(v) for v in t.shape)
-        while mshape and mshape[0] == 1:
-            mshape = mshape[1:]
-        shape = tuple(int(v) for v in shape)
-        if len(mshape) > len(shape) or (mshape and shape[len(shape) - len(mshape):] != mshape):
-            raise ValueError(f"upd_mask of shape {tuple(t.shape)} does not broadcast over a primal of shape {shape} "
-                             "along its leading axes")
-        return t.reshape(-1)
+        """The update mask as a flat device tensor broadcast over the primal's leading axes."""
+        return self._bcast_dev(self.upd_mask, shape, "upd_mask")
 
     def apply_proximal_dual(self, dual) -> None:
         """The norm's dual proximal in place (``regularizers.py:150-162``; VTV / TNV override the
@@ -161,8 +199,6 @@ class Regularizer_Grad(BaseRegularizer):
             raise NotImplementedError("the gradient axes must be given in increasing order")
         if self.pad_mode != "edge":
             raise NotImplementedError("the HIP gradient kernels implement edge padding")
-        if np.asarray(self.weight).size != 1:
-            raise NotImplementedError("the HIP gradient kernels take a scalar weight")
 
     def device_config(self, shape) -> dict:
         """Kernel parameters of ``chip_grad_dual`` / ``chip_grad_update_primal`` for a primal of ``shape``."""
@@ -196,9 +232,17 @@ class Regularizer_Grad(BaseRegularizer):
     def _config(self, shape, mode, n_chan, n_batch, der, chan) -> dict:
         l1_family = isinstance(self.norm, dt.DataFidelity_l1)
         mask = None if self.upd_mask is None else self._device_mask(shape)
+        w_arr = self._weight_dev(shape)
+        w = 0.0 if w_arr is not None else float(self.weight)
+        # an array weight enters the prox of the l12 / l1 / VTV modes and the primal scale outside the
+        # l1 family; TNV, l2 and Huber do not read it in the prox
+        dual_w = w_arr if mode in (0, 1, 2) else None
+        scales = None if l1_family else w_arr
         return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=int(self.ndims),
                     dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
-                    weight=float(self.weight), der=der, chan=chan, scale=float(self.weight),
+                    weight=w, weights=dual_w, weights_period=1 if dual_w is None else int(dual_w.numel()),
+                    der=der, chan=chan, scale=w, scales=scales,
+                    scales_period=1 if scales is None else int(scales.numel()),
                     apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape),
                     mask=mask, mask_period=1 if mask is None else int(mask.numel()), qref=None)
 
@@ -399,35 +443,54 @@ class Regularizer_TNV(Regularizer_Grad):
 
 
 class Regularizer_l1(BaseRegularizer):
-    """l1-norm regularizer: promotes sparse reconstructions (``regularizers.py:531-585``)."""
+    """l1-norm regularizer: promotes sparse reconstructions (``regularizers.py:531-585``). In PDHG and
+    SIRT its identity transform runs on ``chip_l1_dual`` / ``chip_l1_update_primal``; in FISTA its
+    primal proximal is fused into ``chip_fista_update``. The weight may be an array and the update
+    mask any array, each broadcast over the primal's leading axes."""
 
     __reg_name__ = "l1"
 
     def __init__(self, weight, upd_mask=None, norm=None):
         super().__init__(weight=weight, norm=dt.DataFidelity_l1() if norm is None else norm, upd_mask=upd_mask)
 
-    def initialize_sigma_tau(self, primal) -> float:
-        """Identity transform, sigma = 1, tau = 1 (``regularizers.py:544-553``)."""
+    def initialize_sigma_tau(self, primal):
+        """Identity transform, sigma = 1, tau = 1 times the update mask (``regularizers.py:544-553``)."""
         self.op = operators.TransformIdentity(np.shape(primal))
         self.sigma = 1.0
+        if self.upd_mask is not None:
+            return 1.0 * self.upd_mask
         return 1.0
+
+    def tau_terms(self, shape):
+        """(1, update mask as a flat device array or None), as ``Regularizer_Grad.tau_terms``."""
+        return 1.0, (None if self.upd_mask is None else self._bcast_dev(self.upd_mask, shape, "upd_mask"))
 
     def update_dual(self, dual, primal) -> None:
         dual += primal
 
+    def _check_norm(self) -> None:
+        if type(self.norm) is not dt.DataFidelity_l1:
+            raise NotImplementedError(f"norm {self.norm.info()} of Regularizer_l1 is not supported by the HIP kernels")
+
+    def device_config(self, shape) -> dict:
+        """Kernel parameters of ``chip_l1_dual`` / ``chip_l1_update_primal`` for a primal of ``shape``."""
+        self._check_norm()
+        shape = tuple(int(v) for v in shape)
+        w_arr = self._weight_dev(shape)
+        mask = None if self.upd_mask is None else self._bcast_dev(self.upd_mask, shape, "upd_mask")
+        return dict(identity=True, n=int(np.prod(shape)), q_shape=shape,
+                    weight=0.0 if w_arr is not None else float(self.weight), weights=w_arr,
+                    weights_period=1 if w_arr is None else int(w_arr.numel()),
+                    mask=mask, mask_period=1 if mask is None else int(mask.numel()))
+
     def apply_proximal_dual(self, dual) -> None:
         """Clip to [-weight, weight] (``data_terms.py:549-555``), on the device."""
-        if np.ndim(self.weight) != 0:
-            raise NotImplementedError("the device l1 proximal takes a scalar weight")
-        n_el = int(np.prod(np.shape(dual)))
-        cfg = dict(mode=1, n_chan=1, n_batch=1, nd=1, dims=np.array([n_el], dtype=np.int32),
-                   weight=float(self.weight), der=0, chan=0, q_shape=(1, n_el))
-        self._prox_on_device(dual, cfg)
+        self._prox_on_device(dual, self.device_config(tuple(np.shape(dual))))
 
-    def primal_prox_weight(self) -> float:
-        """The weight of the fused soft threshold (prox of tau * weight * ||.||_1)."""
-        if self.upd_mask is not None:
-            raise NotImplementedError("an update mask is not supported by the fused l1 proximal")
-        if np.ndim(self.weight) != 0:
-            raise NotImplementedError("the fused l1 proximal takes a scalar weight")
-        return float(self.weight)
+    def fista_prox(self, shape) -> dict:
+        """Parameters of the proximal fused into ``chip_fista_update``: the prox of tau * weight * |.|_1,
+        inside the update mask when there is one (``regularizers.py:559-585``)."""
+        self._check_norm()
+        cfg = self.device_config(shape)
+        return dict(l1w=cfg["weight"], l1wa=cfg["weights"], l1w_period=cfg["weights_period"], l1m=cfg["mask"],
+                    l1m_period=cfg["mask_period"])

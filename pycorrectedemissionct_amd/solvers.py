@@ -261,7 +261,7 @@ class Solver:
         """(kernel parameters, zero dual) of every regularizer for the HIP gradient kernels."""
         out = []
         for reg in self.regularizer:
-            if not isinstance(reg, regularizers.Regularizer_Grad):
+            if not isinstance(reg, (regularizers.Regularizer_Grad, regularizers.Regularizer_l1)):
                 raise NotImplementedError(f"regularizer {reg.info()} is not supported by the HIP kernels")
             cfg = reg.device_config(tuple(x.shape))
             out.append((cfg, torch.zeros(cfg["q_shape"], dtype=torch.float32, device=x.device)))
@@ -269,15 +269,24 @@ class Solver:
 
     @staticmethod
     def _grad_dual(cfg: dict, q: torch.Tensor, x: torch.Tensor, st) -> None:
+        if cfg.get("identity"):
+            _native.call("chip_l1_dual", q.data_ptr(), x.data_ptr(), cfg["n"], cfg["weight"],
+                         _native.ptr(cfg["weights"]), cfg["weights_period"], st)
+            return
         _native.call("chip_grad_dual", q.data_ptr(), x.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], cfg["der"], cfg["chan"],
-                     _native.ptr(cfg.get("qref")), st)
+                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], _native.ptr(cfg.get("weights")),
+                     cfg.get("weights_period", 1), cfg["der"], cfg["chan"], _native.ptr(cfg.get("qref")), st)
 
     @staticmethod
     def _grad_primal(cfg: dict, upd: torch.Tensor, q: torch.Tensor, subtract: bool, st) -> None:
+        if cfg.get("identity"):
+            _native.call("chip_l1_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n"], _native.ptr(cfg["mask"]),
+                         cfg["mask_period"], int(subtract), st)
+            return
         _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
-                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], cfg["apply_scale"], int(subtract),
-                     _native.ptr(cfg.get("mask")), cfg.get("mask_period", 1), st)
+                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], _native.ptr(cfg.get("scales")),
+                     cfg.get("scales_period", 1), cfg["apply_scale"], int(subtract), _native.ptr(cfg.get("mask")),
+                     cfg.get("mask_period", 1), st)
 
     def _track(self, info, ii, Ax, b, b_mask, b_test_mask) -> bool:
         """Residual bookkeeping of one iteration; True when the tolerance is reached."""
@@ -502,10 +511,10 @@ class FISTA(Solver):
             sig, tau_v, x_shape, _ = compute_Lipschitz_scaling(A, b_t, regs=[], relaxation_tau=self.relaxation)
             tau_ptr, tau_s = None, float(tau_v)
             sigma_t = torch.full_like(b_t, float(sig))
-        l1w = 0.0
+        prox = dict(l1w=0.0, l1wa=None, l1w_period=1, l1m=None, l1m_period=1)
         if self.regularizer:
             self.regularizer[0].initialize_sigma_tau(np.zeros(x_shape, np.float32))
-            l1w = self.regularizer[0].primal_prox_weight()
+            prox = self.regularizer[0].fista_prox(x_shape)
         self.data_term.assign_data(b_t)
         bg = self.data_term.background_dev(b_t)
         term = self.data_term.device_term(b_t)
@@ -537,8 +546,10 @@ class FISTA(Solver):
             momentum = (t - 1.0) / t_new
             if x_old is not None:
                 x_old.copy_(x)
-            _native.call("chip_fista_update", x.data_ptr(), y.data_ptr(), grad.data_ptr(), tau_ptr, tau_s, l1w, hl, lo,
-                         hu, hi, _native.ptr(lim.kernel_mask(x_mask_t)), momentum, x.numel(), st)
+            _native.call("chip_fista_update", x.data_ptr(), y.data_ptr(), grad.data_ptr(), tau_ptr, tau_s,
+                         prox["l1w"], _native.ptr(prox["l1wa"]), prox["l1w_period"], _native.ptr(prox["l1m"]),
+                         prox["l1m_period"], hl, lo, hu, hi, _native.ptr(lim.kernel_mask(x_mask_t)), momentum,
+                         x.numel(), st)
             lim.finish(x, x_mask_t, y, x_old, momentum)
             t = t_new
             if track:

@@ -229,3 +229,46 @@ def test_smooth_and_huber_tv_fixed_points():
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
     print(f"HubTV2D fixed point vs root of the optimality system: {err:.2e}")
     assert err < 1e-4
+
+
+def test_array_weights_reduce_to_scalar_weights():
+    """A constant weight array gives the scalar weight's iterates bit for bit, for every kind."""
+    shape = (3, 6, 7)
+    A, At, b = _dense_problem(shape, 70, 13)
+    w = np.full(shape[1:], 0.05, F32)
+    for kind, nd, kw in (("l12", 2, {}), ("l1", 3, {}), ("vtv", 2, {}), ("tnv", 2, {}), ("l2", 2, {}),
+                         ("hub", 3, dict(huber_size=0.1))):
+        ref = osol.pdhg(A, At, b, shape, 15, data_term="kl", regs=[oreg.GradReg(nd, 0.05, kind, **kw)])
+        got = osol.pdhg(A, At, b, shape, 15, data_term="kl", regs=[oreg.GradReg(nd, w, kind, **kw)])
+        assert np.array_equal(ref, got), kind
+    ref = osol.sirt(A, At, b, shape, 15, regs=[oreg.L1Reg(0.05)])
+    got = osol.sirt(A, At, b, shape, 15, regs=[oreg.L1Reg(w)])
+    assert np.array_equal(ref, got)
+
+
+def test_l1_regularizer_fixed_point_and_masked_fista_prox():
+    """PDHG with the identity-transform l1 regularizer minimises 1/2 |M x - b|^2 + sum w |x|; FISTA
+    with 1/L steps (gradient step tau sigma = 1/L^2, threshold tau w' = w' / L) minimises
+    1/2 |M x - b|^2 + L sum w' |x|, so with w' = w / L the two algorithms meet at the same
+    minimiser, with a scalar and with a spatially varying weight. The masked FISTA prox reduces to
+    the unmasked one for a mask of ones and to no prox for a mask of zeros."""
+    shape = (5, 6)
+    rng = np.random.default_rng(14)
+    n = int(np.prod(shape))
+    mat = rng.random((40, n)).astype(F32)
+    truth = np.where(rng.random(shape) < 0.3, rng.random(shape), 0).astype(F32)
+    b = (mat @ truth.reshape(-1) + rng.normal(0, 0.02, 40)).astype(F32)
+    A = lambda v: (mat @ v.reshape(-1)).astype(F32)  # noqa: E731
+    At = lambda y: (mat.T @ y).astype(F32).reshape(shape)  # noqa: E731
+    L = float(np.linalg.norm(mat.astype(np.float64), 2))
+    for w in (0.3, np.linspace(0.1, 0.6, n, dtype=F32).reshape(shape)):
+        pd = osol.pdhg(A, At, b, shape, 8000, regs=[oreg.L1Reg(w)])
+        fi = osol.fista(A, At, b, shape, 8000, precondition=False, L=L, l1=(np.asarray(w) / F32(L)).astype(F32))
+        err = np.max(np.abs(pd - fi)) / np.max(np.abs(fi))
+        print(f"l1 fixed point, weight {'array' if np.ndim(w) else w}: PDHG vs FISTA {err:.2e}")
+        assert err < 1e-4 and np.count_nonzero(fi == 0) > 0
+    l1 = np.linspace(0.1, 0.6, n, dtype=F32).reshape(shape)
+    plain = osol.fista(A, At, b, shape, 20, l1=l1)
+    assert np.array_equal(osol.fista(A, At, b, shape, 20, l1=l1, l1_mask=np.ones(shape, F32)), plain)
+    assert np.array_equal(osol.fista(A, At, b, shape, 20, l1=l1, l1_mask=np.zeros(shape, F32)),
+                          osol.fista(A, At, b, shape, 20))
