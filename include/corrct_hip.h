@@ -339,8 +339,11 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * (slowest first), of a primal laid out [n_chan][n_batch][dims...]; the dual
  * q is [nd][n_chan][n_batch][dims...] (TransformGradient.adj_shape,
  * operators.py:767-844).
- * chip_grad_dual: q += 0.5 * grad(x) (edge padding: the last difference is
- * 0), then the dual proximal of `mode`: 0 = l12 (isotropic TV,
+ * pad_mode: TransformGradient's np.pad mode (operators.py:796-835): 0 = edge
+ * (the gradient's last difference is 0 and so is the divergence's first),
+ * 1 = constant (last difference -x, first divergence term q), 2 = wrap,
+ * 3 = reflect, 4 = symmetric (as edge).
+ * chip_grad_dual: q += 0.5 * grad(x), then the dual proximal of `mode`: 0 = l12 (isotropic TV,
  * Regularizer_TV1D/2D/3D, regularizers.py:218-337 with data_terms.py:598-608),
  * 1 = l1 (anisotropic), 2 = VTV with point-wise derivative and channel norms
  * der_norm / chan_norm in {1, 2, 0 = inf} (Regularizer_VTV.apply_proximal_dual,
@@ -369,12 +372,12 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * (compute_update_primal, regularizers.py:194-210; PDHG adds, SIRT
  * subtracts, solvers.py:953-958, 1178-1180).
  */
-int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, const float* weights, int64_t weights_period, int der_norm, int chan_norm,
-                   const float* qref, void* stream);
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int pad_mode,
+                   int mode, float weight, const float* weights, int64_t weights_period, int der_norm,
+                   int chan_norm, const float* qref, void* stream);
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, const float* scales, int64_t scales_period, int apply_scale, int subtract,
-                            const float* mask, int64_t mask_period, void* stream);
+                            int pad_mode, float scale, const float* scales, int64_t scales_period, int apply_scale,
+                            int subtract, const float* mask, int64_t mask_period, void* stream);
 
 /* Regularizer_l1 in PDHG / SIRT (identity transform, regularizers.py:531-557 and
  * the base class's dual / primal steps, regularizers.py:134-210):

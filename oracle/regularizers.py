@@ -2,10 +2,12 @@
 
 Test infrastructure only (see ``oracle/__init__.py``).
 
-* ``grad_nd`` / ``div_nd`` -- ``TransformGradient`` with edge padding over the
-  last ``nd`` axes (``operators.py:796-844``): forward differences with the
-  last one 0; the divergence's backward differences with the first one 0,
-  summed over the axes in order. ``-div_nd`` is the operator's adjoint.
+* ``grad_nd`` / ``div_nd`` -- ``TransformGradient`` over the last ``nd`` axes
+  (``operators.py:796-844``) with its ``np.pad`` mode (edge by default: the
+  last forward difference is 0, and so is the divergence's first backward
+  difference), summed over the axes in order. ``-div_nd`` is the operator's
+  adjoint for the constant mode only; for edge it drops ``-q[0]`` at the first
+  sample.
 * ``prox_l12`` / ``prox_l1`` -- ``DataFidelity_l12`` / ``DataFidelity_l1``
   dual proximals with a weight (``data_terms.py:549-555, 598-608``).
 * ``prox_vtv`` -- ``Regularizer_VTV.apply_proximal_dual``
@@ -28,31 +30,28 @@ import numpy as np
 F32 = np.float32
 
 
-def grad_nd(x: np.ndarray, nd: int) -> np.ndarray:
+def grad_nd(x: np.ndarray, nd: int, pad_mode: str = "edge") -> np.ndarray:
+    """``np.diff(np.pad(x, (0, 1) along the axis, mode=pad_mode))`` per axis (``operators.py:796-816``)."""
     x = np.asarray(x)
     g = np.zeros((nd, *x.shape), dtype=x.dtype)
     for k in range(nd):
         ax = x.ndim - nd + k
-        sl_lo = [slice(None)] * x.ndim
-        sl_hi = [slice(None)] * x.ndim
-        sl_lo[ax] = slice(0, -1)
-        sl_hi[ax] = slice(1, None)
-        g[(k, *sl_lo)] = x[tuple(sl_hi)] - x[tuple(sl_lo)]
+        padding = [(0, 0)] * x.ndim
+        padding[ax] = (0, 1)
+        g[k] = np.diff(np.pad(x, padding, mode=pad_mode), n=1, axis=ax)
     return g
 
 
-def div_nd(q: np.ndarray) -> np.ndarray:
+def div_nd(q: np.ndarray, pad_mode: str = "edge") -> np.ndarray:
+    """``sum_k np.diff(np.pad(q[k], (1, 0) along the axis, mode=pad_mode))`` (``operators.py:818-835``)."""
     nd = q.shape[0]
     ndim = q.ndim - 1
     out = None
     for k in range(nd):
         ax = ndim - nd + k
-        d = np.zeros(q.shape[1:], dtype=q.dtype)
-        sl_lo = [slice(None)] * ndim
-        sl_hi = [slice(None)] * ndim
-        sl_lo[ax] = slice(0, -1)
-        sl_hi[ax] = slice(1, None)
-        d[tuple(sl_hi)] = q[(k, *sl_hi)] - q[(k, *sl_lo)]
+        padding = [(0, 0)] * ndim
+        padding[ax] = (1, 0)
+        d = np.diff(np.pad(q[k], padding, mode=pad_mode), n=1, axis=ax)
         out = d if out is None else out + d
     return out
 
@@ -126,7 +125,8 @@ class GradReg:
     (``data_terms.py:500-501``). Both weight the primal update (outside the l1 family)."""
 
     def __init__(self, nd: int, weight, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
-                 huber_size: float = 0.0):
+                 huber_size: float = 0.0, pad_mode: str = "edge"):
+        self.pad_mode = pad_mode
         # an array weight broadcasts against the primal (and the dual's trailing axes)
         weight = float(weight) if np.ndim(weight) == 0 else np.asarray(weight, dtype=F32)
         self.nd, self.weight, self.kind, self.der, self.chan = nd, weight, kind, der, chan
@@ -145,7 +145,7 @@ class GradReg:
         return t if self.upd_mask is None else t * self.upd_mask
 
     def dual(self, q: np.ndarray, x: np.ndarray) -> np.ndarray:
-        q = q + F32(0.5) * grad_nd(x, self.nd)
+        q = q + F32(0.5) * grad_nd(x, self.nd, self.pad_mode)
         if self.kind == "l12":
             return prox_l12(q, self.weight)
         if self.kind == "l1":
@@ -164,7 +164,7 @@ class GradReg:
         return np.zeros((self.nd, *x_shape), dtype=F32)
 
     def primal(self, q: np.ndarray) -> np.ndarray:
-        u = -div_nd(q)
+        u = -div_nd(q, self.pad_mode)
         if self.upd_mask is not None:
             u = u * self.upd_mask
         return u * np.asarray(self.weight, dtype=F32) if self.weighted else u

@@ -27,6 +27,9 @@
 //          float64 Jacobi eigen-decomposition of M^T M (nd <= 3)
 //   primal upd = upd +- scale * (-div q), div with the first difference 0
 //          (operators.py:820-844), summed over the axes in order.
+// Padding (np.pad modes of TransformGradient): only the gradient's last
+// difference and the divergence's first one depend on it (grad_last,
+// div_first); edge is the default.
 #include "common.hpp"
 
 namespace chip {
@@ -37,6 +40,9 @@ constexpr int kGrMaxChan = 16;
 
 enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3, kGrL2 = 4, kGrHuber = 5 };
 
+// np.pad modes of TransformGradient (operators.py:796-835)
+enum GradPad { kPadEdge = 0, kPadConstant = 1, kPadWrap = 2, kPadReflect = 3, kPadSymmetric = 4 };
+
 struct GradGeom {
   int C;        // coupled channels
   int nd;       // gradient axes (1..3)
@@ -44,7 +50,30 @@ struct GradGeom {
   int64_t S;    // samples per (c, b)
   int dim[3];   // sizes of the gradient axes, slowest first
   int64_t stride[3];
+  int pad;      // GradPad
 };
+
+// forward difference at the last sample of an axis: the padded value minus x0
+// (edge and symmetric repeat x0, constant pads 0, wrap the first sample,
+// reflect the one before the last); p points at x0, the axis has `dim` samples
+__device__ __forceinline__ float grad_last(const float* p, float x0, int dim, int64_t stride, int pad) {
+  switch (pad) {
+    case kPadConstant: return -x0;
+    case kPadWrap: return p[-static_cast<int64_t>(dim - 1) * stride] - x0;
+    case kPadReflect: return dim > 1 ? p[-stride] - x0 : 0.f;
+    default: return 0.f;
+  }
+}
+
+// backward difference of the divergence at the first sample: q0 minus the padded value
+__device__ __forceinline__ float div_first(const float* p, float q0, int dim, int64_t stride, int pad) {
+  switch (pad) {
+    case kPadConstant: return q0;
+    case kPadWrap: return q0 - p[static_cast<int64_t>(dim - 1) * stride];
+    case kPadReflect: return dim > 1 ? q0 - p[stride] : 0.f;
+    default: return 0.f;
+  }
+}
 
 inline dim3 gr_grid(int64_t n) {
   int64_t blocks = (n + kGrBlock - 1) / kGrBlock;
@@ -132,7 +161,8 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
       const float x0 = xc[0];
 #pragma unroll
       for (int k = 0; k < ND; ++k) {
-        const float gk = (idx[k] + 1 < g.dim[k]) ? xc[g.stride[k]] - x0 : 0.f;
+        const float gk = (idx[k] + 1 < g.dim[k]) ? xc[g.stride[k]] - x0
+                                                 : grad_last(xc, x0, g.dim[k], g.stride[k], g.pad);
         v[k][c] = q[k * qs + c * cs + off] + 0.5f * gk;
       }
     }
@@ -317,7 +347,7 @@ __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict
     float d = 0.f;
     for (int k = 0; k < g.nd; ++k) {
       const float* qk = q + k * total + e;
-      const float dk = (idx[k] > 0) ? qk[0] - qk[-g.stride[k]] : 0.f;
+      const float dk = (idx[k] > 0) ? qk[0] - qk[-g.stride[k]] : div_first(qk, qk[0], g.dim[k], g.stride[k], g.pad);
       d = (k == 0) ? dk : d + dk;
     }
     float t = -d;
@@ -327,8 +357,11 @@ __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict
   }
 }
 
-int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims) {
+int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims, int pad) {
   CHIP_CHECK_ARG(nd >= 1 && nd <= 3, "nd must be 1, 2 or 3");
+  CHIP_CHECK_ARG(pad >= kPadEdge && pad <= kPadSymmetric, "pad_mode must be 0..4 (edge, constant, wrap, reflect, "
+                 "symmetric)");
+  g.pad = pad;
   CHIP_CHECK_ARG(dims != nullptr, "NULL dims");
   CHIP_CHECK_ARG(n_chan >= 1 && n_chan <= kGrMaxChan, "n_chan must be in [1, 16]");
   CHIP_CHECK_ARG(n_batch >= 1, "n_batch must be >= 1");
@@ -360,9 +393,9 @@ using namespace chip;
 
 extern "C" {
 
-int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int mode,
-                   float weight, const float* weights, int64_t weights_period, int der_norm, int chan_norm,
-                   const float* qref, void* stream) {
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int pad_mode,
+                   int mode, float weight, const float* weights, int64_t weights_period, int der_norm,
+                   int chan_norm, const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
   CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrHuber, "unknown mode");
@@ -374,7 +407,7 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
   CHIP_CHECK_ARG(weights == nullptr || mode == kGrL12 || mode == kGrL1 || mode == kGrVTV,
                  "a weight array is taken by the l12, l1 and VTV modes");
   GradGeom g;
-  const int rc = make_geom(g, n_chan, n_batch, nd, dims);
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims, pad_mode);
   if (rc != CHIP_OK) return rc;
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   CHIP_CHECK_ARG(weights == nullptr || (weights_period >= 1 && total % weights_period == 0),
@@ -389,12 +422,12 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
 }
 
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            float scale, const float* scales, int64_t scales_period, int apply_scale, int subtract,
-                            const float* mask, int64_t mask_period, void* stream) {
+                            int pad_mode, float scale, const float* scales, int64_t scales_period, int apply_scale,
+                            int subtract, const float* mask, int64_t mask_period, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(upd != nullptr && q != nullptr, "NULL argument");
   GradGeom g;
-  const int rc = make_geom(g, n_chan, n_batch, nd, dims);
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims, pad_mode);
   if (rc != CHIP_OK) return rc;
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   CHIP_CHECK_ARG(mask == nullptr || (mask_period >= 1 && total % mask_period == 0),

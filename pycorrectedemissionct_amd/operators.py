@@ -143,7 +143,8 @@ class TransformIdentity(BaseTransform):
 
 
 class TransformGradient(BaseTransform):
-    """Gradient with edge padding and its adjoint ``-div`` (``operators.py:767-844``)."""
+    """Gradient with np.pad padding (edge, constant, wrap, reflect, symmetric) and its adjoint ``-div``
+    (``operators.py:767-844``)."""
 
     def __init__(self, x_shape, axes=None, pad_mode: str = "edge"):
         x_shape = np.array(x_shape, ndmin=1, dtype=int)
@@ -152,7 +153,7 @@ class TransformGradient(BaseTransform):
         self.axes = np.array(axes, ndmin=1, dtype=int)
         self.ndims = len(x_shape)
         self.pad_mode = pad_mode.lower()
-        if self.pad_mode not in ("edge", "constant"):
+        if self.pad_mode not in ("edge", "constant", "wrap", "reflect", "symmetric"):
             raise ValueError(f"Unsupported padding mode: {pad_mode}")
         self.dir_shape = x_shape
         self.adj_shape = np.array((len(self.axes), *self.dir_shape), ndmin=1, dtype=int)
@@ -168,8 +169,14 @@ class TransformGradient(BaseTransform):
             hi = t.narrow(ax, 1, n - 1)
             lo = t.narrow(ax, 0, n - 1)
             d.narrow(ax, 0, n - 1).copy_(hi - lo)
+            # the last difference: the np.pad value past the end minus the last sample
+            last = t.narrow(ax, n - 1, 1)
             if self.pad_mode == "constant":
-                d.narrow(ax, n - 1, 1).copy_(-t.narrow(ax, n - 1, 1))
+                d.narrow(ax, n - 1, 1).copy_(-last)
+            elif self.pad_mode == "wrap":
+                d.narrow(ax, n - 1, 1).copy_(t.narrow(ax, 0, 1) - last)
+            elif self.pad_mode == "reflect" and n > 1:
+                d.narrow(ax, n - 1, 1).copy_(t.narrow(ax, n - 2, 1) - last)
             out.append(d)
         return _arrays.from_dev(torch.stack(out, dim=0), host)
 
@@ -182,8 +189,14 @@ class TransformGradient(BaseTransform):
             n = q.shape[ax]
             d = torch.zeros_like(q)
             d.narrow(ax, 1, n - 1).copy_(q.narrow(ax, 1, n - 1) - q.narrow(ax, 0, n - 1))
+            # the first difference: the first sample minus the np.pad value before it
+            first = q.narrow(ax, 0, 1)
             if self.pad_mode == "constant":
-                d.narrow(ax, 0, 1).copy_(q.narrow(ax, 0, 1))
+                d.narrow(ax, 0, 1).copy_(first)
+            elif self.pad_mode == "wrap":
+                d.narrow(ax, 0, 1).copy_(first - q.narrow(ax, n - 1, 1))
+            elif self.pad_mode == "reflect" and n > 1:
+                d.narrow(ax, 0, 1).copy_(first - q.narrow(ax, 1, 1))
             acc = acc + d
         return _arrays.from_dev(acc, host)
 

@@ -32,6 +32,10 @@ from . import data_terms as dt
 from . import operators
 
 
+# TransformGradient's np.pad modes (operators.py:796-835) as the kernels' pad_mode codes
+_PAD_CODE = {"edge": 0, "constant": 1, "wrap": 2, "reflect": 3, "symmetric": 4}
+
+
 def _product(arrs, shape) -> torch.Tensor | None:
     """The elementwise product of flat arrays that each broadcast over ``shape`` (None if there are none)."""
     if not arrs:
@@ -119,7 +123,8 @@ class BaseRegularizer:
         else:
             zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
             _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                         cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], _native.ptr(cfg.get("weights")),
+                         cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["mode"], cfg["weight"],
+                         _native.ptr(cfg.get("weights")),
                          cfg.get("weights_period", 1), cfg["der"], cfg["chan"], _native.ptr(cfg.get("qref")),
                          _native.stream())
         if host:
@@ -197,8 +202,8 @@ class Regularizer_Grad(BaseRegularizer):
             raise NotImplementedError("the HIP gradient kernels take the gradient over the last 1 to 3 axes, in order")
         if list(np.asarray(self.axes) % nd) != axes:
             raise NotImplementedError("the gradient axes must be given in increasing order")
-        if self.pad_mode != "edge":
-            raise NotImplementedError("the HIP gradient kernels implement edge padding")
+        if self.pad_mode not in _PAD_CODE:
+            raise ValueError(f"Unsupported padding mode: {self.pad_mode}")
 
     def device_config(self, shape) -> dict:
         """Kernel parameters of ``chip_grad_dual`` / ``chip_grad_update_primal`` for a primal of ``shape``."""
@@ -240,7 +245,7 @@ class Regularizer_Grad(BaseRegularizer):
         scales = None if l1_family else w_arr
         return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=int(self.ndims),
                     dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
-                    weight=w, weights=dual_w, weights_period=1 if dual_w is None else int(dual_w.numel()),
+                    pad=_PAD_CODE[self.pad_mode], weight=w, weights=dual_w, weights_period=1 if dual_w is None else int(dual_w.numel()),
                     der=der, chan=chan, scale=w, scales=scales,
                     scales_period=1 if scales is None else int(scales.numel()),
                     apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape),

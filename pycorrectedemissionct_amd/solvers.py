@@ -274,8 +274,9 @@ class Solver:
                          _native.ptr(cfg["weights"]), cfg["weights_period"], st)
             return
         _native.call("chip_grad_dual", q.data_ptr(), x.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg["mode"], cfg["weight"], _native.ptr(cfg.get("weights")),
-                     cfg.get("weights_period", 1), cfg["der"], cfg["chan"], _native.ptr(cfg.get("qref")), st)
+                     cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["mode"], cfg["weight"],
+                     _native.ptr(cfg.get("weights")), cfg.get("weights_period", 1), cfg["der"], cfg["chan"],
+                     _native.ptr(cfg.get("qref")), st)
 
     @staticmethod
     def _grad_primal(cfg: dict, upd: torch.Tensor, q: torch.Tensor, subtract: bool, st) -> None:
@@ -284,7 +285,7 @@ class Solver:
                          cfg["mask_period"], int(subtract), st)
             return
         _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
-                     cfg["nd"], cfg["dims"].ctypes.data, cfg["scale"], _native.ptr(cfg.get("scales")),
+                     cfg["nd"], cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["scale"], _native.ptr(cfg.get("scales")),
                      cfg.get("scales_period", 1), cfg["apply_scale"], int(subtract), _native.ptr(cfg.get("mask")),
                      cfg.get("mask_period", 1), st)
 

@@ -213,6 +213,41 @@ def test_array_weights_and_l1_regularizer(pkg):
         pkg.PDHG(regularizer=R.Regularizer_TV2D(np.ones((5, 5), np.float32)))(prj, b, iterations=1)
 
 
+def test_gradient_padding_modes(pkg):
+    """TransformGradient's np.pad modes (``operators.py:796-835``) in the HIP gradient kernels and the
+    host operator: PDHG-l2 and SIRT with TV3D / TV2D / smooth2D under constant, wrap, reflect and
+    symmetric padding against the oracle, and the operator's gradient / divergence against it."""
+    n, a, c = 24, 16, 3
+    rng = np.random.default_rng(988)
+    ang = np.linspace(0, 2 * np.pi, a, endpoint=False).astype(np.float32)
+    base = pkg.testing.phantom_ellipses(n, 3, rng, semi_axes=(3, 7))
+    f = np.stack([base * s for s in (1.0, 0.6, 0.3)]).astype(np.float32)
+    A, At = _stack_ops(ang, n)
+    b = pkg.testing.poisson_data(A(f), 1e3, rng)
+    prj = pkg.ProjectorUncorrected([n, n, c], ang)
+    R = pkg.regularizers
+    for mode in ("constant", "wrap", "reflect", "symmetric"):
+        cases = [
+            ("TV3D", lambda: R.Regularizer_TV3D(5e-2, pad_mode=mode), oreg.GradReg(3, 5e-2, "l12", pad_mode=mode)),
+            ("TV2D", lambda: R.Regularizer_TV2D(5e-2, pad_mode=mode), oreg.GradReg(2, 5e-2, "l12", pad_mode=mode)),
+            ("smooth2D", lambda: R.Regularizer_smooth2D(5e-2, pad_mode=mode),
+             oreg.GradReg(2, 5e-2, "l2", pad_mode=mode)),
+        ]
+        for name, make, orc in cases:
+            x = pkg.PDHG(regularizer=make())(prj, b, iterations=30)[0]
+            ref = osol.pdhg(A, At, b, f.shape, 30, regs=[orc])
+            x2 = pkg.SIRT(regularizer=make())(prj, b, iterations=30)[0]
+            ref2 = osol.sirt(A, At, b, f.shape, 30, regs=[orc])
+            print(f"{mode} {name}: PDHG rel_max {rel_max(ref, x):.2e}, SIRT {rel_max(ref2, x2):.2e}")
+            assert rel_max(ref, x) < TOL and rel_max(ref2, x2) < TOL, (mode, name)
+        op = pkg.operators.TransformGradient(f.shape, pad_mode=mode)
+        q = rng.normal(size=(3, *f.shape)).astype(np.float32)
+        assert np.allclose(op(f), oreg.grad_nd(f, 3, mode), atol=1e-6), mode
+        assert np.allclose(op.T(q), -oreg.div_nd(q, mode), atol=1e-6), mode
+    with pytest.raises(ValueError):
+        pkg.PDHG(regularizer=R.Regularizer_TV3D(5e-2, pad_mode="mean"))(prj, b, iterations=1)
+
+
 def test_pdhg_vtv_xrf_channels(pkg):
     """Emission channels as slices of one XRF projector, each with its own outgoing attenuation
     (mu_out scaled per channel), coupled by VTV: PDHG-KL against the oracle."""

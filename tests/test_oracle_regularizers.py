@@ -272,3 +272,28 @@ def test_l1_regularizer_fixed_point_and_masked_fista_prox():
     assert np.array_equal(osol.fista(A, At, b, shape, 20, l1=l1, l1_mask=np.ones(shape, F32)), plain)
     assert np.array_equal(osol.fista(A, At, b, shape, 20, l1=l1, l1_mask=np.zeros(shape, F32)),
                           osol.fista(A, At, b, shape, 20))
+
+
+def test_gradient_padding_modes():
+    """The np.pad modes of TransformGradient: -div is the exact adjoint of the gradient for the
+    constant (zero) and wrap (periodic) modes, for any dual; reflect and symmetric by hand on a 1-D
+    ramp (symmetric equals edge)."""
+    rng = np.random.default_rng(15)
+    for shape, nd in (((5, 7), 2), ((3, 4, 6), 3), ((2, 9), 1)):
+        x = rng.normal(size=shape)
+        q = rng.normal(size=(nd, *shape))
+        for mode in ("constant", "wrap"):
+            lhs = np.sum(oreg.grad_nd(x, nd, mode) * q)
+            rhs = np.sum(x * -oreg.div_nd(q, mode))
+            assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs)), (mode, shape)
+        for mode in ("edge", "reflect", "symmetric"):  # not adjoint: the boundary terms differ
+            lhs = np.sum(oreg.grad_nd(x, nd, mode) * q)
+            assert abs(lhs - np.sum(x * -oreg.div_nd(q, mode))) > 1e-6
+    x = np.array([1.0, 2.0, 4.0])
+    q = np.array([[1.0, 2.0, 4.0]])
+    assert np.array_equal(oreg.grad_nd(x, 1, "reflect")[0], [1.0, 2.0, -2.0])
+    assert np.array_equal(oreg.div_nd(q, "reflect"), [-1.0, 1.0, 2.0])
+    assert np.array_equal(oreg.grad_nd(x, 1, "symmetric"), oreg.grad_nd(x, 1, "edge"))
+    assert np.array_equal(oreg.div_nd(q, "symmetric"), oreg.div_nd(q, "edge"))
+    assert np.array_equal(oreg.grad_nd(x, 1, "edge")[0], [1.0, 2.0, 0.0])
+    assert np.array_equal(oreg.div_nd(q, "edge"), [0.0, 1.0, 2.0])
