@@ -129,6 +129,17 @@ int chip_att_maps(chip_plan* plan, const float* mu_in, const float* mu_out, floa
                   void* stream);
 
 /*
+ * Binned maps: each of n_img images of nd (2 or 3) axes dims[nd] (slowest
+ * first) is cropped to a multiple of `binning` per axis, dropping excess - excess/2
+ * samples at the start and excess/2 at the end, and averaged over blocks of
+ * binning^nd samples into out[n_img][dims[k] / binning ...]. Replaces
+ * AttenuationVolume.get_maps(binning=...) (physics/attenuation.py:294-295) over
+ * bin_imgs(auto_crop=True) (processing/pre.py:180-220: skimage block_reduce
+ * with np.mean). Sums in row-major block order in float32.
+ */
+int chip_bin_images(const float* in, float* out, int64_t n_img, int nd, const int* dims, int binning, void* stream);
+
+/*
  * Forward projection sino[b][d][a][u] = w[a,d] * radon_a(vol[b] * M[a,d])[u]
  * for `batch` slices. maps may be NULL (M = 1: ProjectorUncorrected.fp,
  * projectors.py:302-321); maps_slice_stride is the element stride between

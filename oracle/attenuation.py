@@ -7,6 +7,10 @@ Test infrastructure only (see ``oracle/__init__.py``).
   outgoing factor of each detector averaged (float32, in order) over its
   sub-angle offsets (``attenuation.py:110-119``; one offset 0 for a point
   detector)
+* ``bin_maps``             -- ``get_maps(binning=b)`` (``physics/attenuation.py:294-295``) over
+  ``bin_imgs(auto_crop=True)`` (``processing/pre.py:180-220``): crop each volume
+  axis to a multiple of b (``excess - excess // 2`` from the start), then block
+  means (scikit-image ``block_reduce`` with ``np.mean``).
 * ``detector_sub_angles``  -- the offsets: midpoints of S equal parts of
   ``[-atan2(sqrt(surface) / 2, distance), +...]`` (``physics/xrf.py:238-246``,
   ``attenuation.py:112-113``)
@@ -107,3 +111,20 @@ def detector_sub_angles(surface_mm2: float, distance_mm: float, n_sub: int) -> n
     half = math.atan2(math.sqrt(surface_mm2) / 2, distance_mm)
     edges = np.linspace(-half, half, n_sub + 1)
     return np.convolve(edges, np.ones(2), "valid") / 2
+
+
+def bin_maps(maps: np.ndarray, binning: int, nd: int) -> np.ndarray:
+    """Block means over the last ``nd`` axes after the auto-crop (float64 sums, rounded to float32)."""
+    maps = np.asarray(maps)
+    sl = [slice(None)] * maps.ndim
+    for ax in range(maps.ndim - nd, maps.ndim):
+        excess = maps.shape[ax] % binning
+        sl[ax] = slice(excess - excess // 2, maps.shape[ax] - excess // 2)
+    m = maps[tuple(sl)].astype(np.float64)
+    lead = m.shape[: m.ndim - nd]
+    shp = list(lead)
+    for d in m.shape[m.ndim - nd:]:
+        shp += [d // binning, binning]
+    m = m.reshape(shp)
+    axes = tuple(len(lead) + 2 * k + 1 for k in range(nd))
+    return m.mean(axis=axes).astype(np.float32)

@@ -51,6 +51,7 @@ _SIGNATURES = {
     "chip_axpy": [c_void_p, ctypes.c_float, c_void_p, ctypes.c_int64, ctypes.c_int64, c_void_p],
     "chip_rdiv": [c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p],
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    "chip_bin_images": [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_bp_noalloc": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],

@@ -442,9 +442,64 @@ int otf_bp(const Plan& p, const float* sino, const float* mu_in, const float* mu
   return CHIP_OK;
 }
 
+// out[i][z][y][x] = mean of the b^nd inputs of the block at (z, y, x) after
+// cropping the volume axes at (o0, o1, o2): summed in row-major order in
+// float32, divided by b^nd (skimage block_reduce with np.mean)
+__global__ __launch_bounds__(256) void bin_blocks_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                         int64_t n_img, int d0, int d1, int d2, int o0, int o1,
+                                                         int o2, int b0, int b, int e0, int e1, int e2) {
+  const int64_t per = static_cast<int64_t>(e0) * e1 * e2;
+  const int64_t total = n_img * per;
+  const float cnt = static_cast<float>(b0 * b * b);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t img = t / per;
+    int64_t r = t - img * per;
+    const int z = static_cast<int>(r / (static_cast<int64_t>(e1) * e2));
+    r -= static_cast<int64_t>(z) * e1 * e2;
+    const int y = static_cast<int>(r / e2);
+    const int x = static_cast<int>(r - static_cast<int64_t>(y) * e2);
+    const float* src = in + img * static_cast<int64_t>(d0) * d1 * d2;
+    float acc = 0.f;
+    for (int k = 0; k < b0; ++k)
+      for (int i = 0; i < b; ++i) {
+        const float* row = src + (static_cast<int64_t>(o0 + z * b0 + k) * d1 + (o1 + y * b + i)) * d2 + o2 + x * b;
+        for (int j = 0; j < b; ++j) acc += row[j];
+      }
+    out[t] = acc / cnt;
+  }
+}
+
 }  // namespace chip
 
 using namespace chip;
+
+extern "C" int chip_bin_images(const float* in, float* out, int64_t n_img, int nd, const int* dims, int binning,
+                               void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(in != nullptr && out != nullptr && dims != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(n_img >= 1 && (nd == 2 || nd == 3) && binning >= 1, "need n_img >= 1, nd in {2, 3}, binning >= 1");
+  int d[3] = {1, 1, 1};
+  for (int k = 0; k < nd; ++k) {
+    CHIP_CHECK_ARG(dims[k] >= binning, "every binned axis must hold at least one block");
+    d[3 - nd + k] = dims[k];
+  }
+  const int b0 = nd == 3 ? binning : 1;
+  int o[3], e[3];
+  for (int k = 0; k < 3; ++k) {
+    const int bk = k == 0 ? b0 : binning;
+    const int excess = d[k] % bk;
+    o[k] = excess - excess / 2;  // auto_crop: the larger half of the excess from the start
+    e[k] = d[k] / bk;
+  }
+  const int64_t total = n_img * e[0] * e[1] * e[2];
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(bin_blocks_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, as_stream(stream), in, out,
+                     n_img, d[0], d[1], d[2], o[0], o[1], o[2], b0, binning, e[0], e[1], e[2]);
+  CHIP_LAUNCH_CHECK();
+  return CHIP_OK;
+}
 
 extern "C" int chip_att_maps(chip_plan* plan, const float* mu_in, const float* mu_out, float* maps, int batch,
                              void* stream) {

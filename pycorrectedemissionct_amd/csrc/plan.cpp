@@ -89,7 +89,7 @@ using namespace chip;
 
 static const char* kSymbols[] = {
     "chip_version",       "chip_last_error",   "chip_num_symbols",  "chip_symbol_name",
-    "chip_plan_create",   "chip_plan_destroy", "chip_att_maps",     "chip_fp",
+    "chip_plan_create",   "chip_plan_destroy", "chip_att_maps",     "chip_bin_images",   "chip_fp",
     "chip_bp",            "chip_fp_angle",     "chip_bp_angle",     "chip_absmax",
     "chip_precond_fix",   "chip_pdhg_dual_l2", "chip_pdhg_dual_kl", "chip_pdhg_dual_norm",
     "chip_data_residual", "chip_l1_dual",      "chip_l1_update_primal", "chip_tv_dual",

@@ -21,7 +21,7 @@ from collections.abc import Sequence
 import numpy as np
 import torch
 
-from .. import _arrays
+from .. import _arrays, _native
 from .._projector_backends import HipPlan
 
 
@@ -144,8 +144,24 @@ class AttenuationVolume:
         if roi is not None:
             raise NotImplementedError("Extracting a region of interest is not supported, yet.")
         if binning > 1:
-            raise NotImplementedError("binning of the maps is not supported")
+            maps = self._bin_maps(maps.contiguous(), binning)
         return _arrays.from_dev(maps.contiguous(), self._host)
+
+    def _bin_maps(self, maps, binning):
+        """``bin_imgs(maps, binning, axes=<volume axes>, auto_crop=True)`` (``processing/pre.py:180-220``):
+        block means over the volume axes after cropping each to a multiple of ``binning``
+        (``chip_bin_images``)."""
+        if not float(binning).is_integer():
+            raise NotImplementedError("the device binning takes an integer factor (fractional binning rescales)")
+        binning = int(binning)
+        nd = len(self.vol_shape_zyx)
+        dims = [int(v) for v in maps.shape[-nd:]]
+        lead = tuple(int(v) for v in maps.shape[:-nd])
+        out = torch.empty((*lead, *[d // binning for d in dims]), dtype=torch.float32, device=maps.device)
+        dims_c = np.ascontiguousarray(dims, dtype=np.int32)
+        _native.call("chip_bin_images", maps.data_ptr(), out.data_ptr(), int(np.prod(lead, dtype=np.int64)), nd,
+                     dims_c.ctypes.data, binning, _native.stream())
+        return out
 
     def get_projector_args(self, roi=None, rot_ind=None, det_ind=None, binning: int = 1) -> dict:
         """``att_maps`` and ``angles_detectors_rad`` for the projector (``attenuation.py:299-331``)."""

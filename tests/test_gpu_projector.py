@@ -208,6 +208,40 @@ def test_att_maps_parity(pkg, n):
     assert rel_max(ref, got) < TOL
 
 
+def test_att_maps_binning(pkg):
+    """get_maps(binning=b) / get_projector_args(binning=b) (physics/attenuation.py:294-295 over
+    bin_imgs with auto_crop): 2-D maps of an odd size binned by 2 and 3, and maps of a 3-D volume
+    binned over all three volume axes, against the oracle's block means."""
+    rng = np.random.default_rng(405)
+    ang = _angles(8)
+    det = np.array([np.pi / 2, -np.pi / 2])
+    n = 37
+    mu_in = (rng.random((n, n)) * 0.05).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.08).astype(np.float32)
+    av = pkg.AttenuationVolume(mu_in, mu_out, ang, det)
+    av.compute_maps()
+    full = av.get_maps()
+    for b in (2, 3):
+        got = av.get_maps(binning=b)
+        ref = oatt.bin_maps(full, b, 2)
+        print(f"binning {b}: {full.shape} -> {got.shape}, rel_max {rel_max(ref, got):.2e}")
+        assert got.shape == ref.shape and rel_max(ref, got) < 1e-6
+    args = av.get_projector_args(binning=2, rot_ind=slice(0, 4))
+    assert args["att_maps"].shape == (4, 2, 18, 18)
+    z = 5
+    mu3_in = (rng.random((z, 20, 20)) * 0.05).astype(np.float32)
+    mu3_out = (rng.random((z, 20, 20)) * 0.08).astype(np.float32)
+    av3 = pkg.AttenuationVolume(mu3_in, mu3_out, ang, det)
+    av3.compute_maps()
+    full3 = av3.get_maps()
+    got3 = av3.get_maps(binning=2)
+    ref3 = oatt.bin_maps(full3, 2, 3)
+    print(f"3-D binning 2: {full3.shape} -> {got3.shape}, rel_max {rel_max(ref3, got3):.2e}")
+    assert got3.shape == ref3.shape == (8, 2, 2, 10, 10) and rel_max(ref3, got3) < 1e-6
+    with pytest.raises(NotImplementedError):
+        av.get_maps(binning=1.5)
+
+
 def test_att_maps_detector_surface(pkg):
     """Detectors with a surface: the outgoing factor is averaged over emitted_sub_sampling
     sub-angles (a 9 mm^2 detector at 10 mm in 4 parts, a 2 mm^2 one at 5 mm in 4, and a point

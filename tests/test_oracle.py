@@ -427,3 +427,21 @@ def test_oracle_psf_matches_scipy_edge_convolution():
         ref = spsig.convolve(xp, kern[None, :], mode="same")
         ref = ref[:, pw:xp.shape[1] - pw] if pw else ref
         assert np.allclose(oxrf.psf_lines(x, kern), ref, rtol=1e-12, atol=1e-12), k_len
+
+
+def test_bin_maps_known_answers():
+    """get_maps(binning) = bin_imgs(auto_crop=True): a 5 x 5 image drops its first row and column
+    (excess 1: 1 - 0 from the start, 0 from the end) and averages 2 x 2 blocks; a 3-D volume bins
+    its slice axis too; constants stay constant."""
+    from oracle import attenuation as oatt
+
+    img = np.arange(25, dtype=np.float32).reshape(1, 5, 5)
+    want = np.array([[[(6 + 7 + 11 + 12) / 4, (8 + 9 + 13 + 14) / 4],
+                      [(16 + 17 + 21 + 22) / 4, (18 + 19 + 23 + 24) / 4]]], np.float32)
+    assert np.array_equal(oatt.bin_maps(img, 2, 2), want)
+    vol = np.arange(2 * 4 * 6, dtype=np.float32).reshape(1, 2, 4, 6)
+    got = oatt.bin_maps(vol, 2, 3)
+    assert got.shape == (1, 1, 2, 3)
+    assert got[0, 0, 0, 0] == np.mean([0, 1, 6, 7, 24, 25, 30, 31])
+    c = np.full((3, 2, 7, 9), 0.25, np.float32)
+    assert np.array_equal(oatt.bin_maps(c, 3, 2), np.full((3, 2, 2, 3), 0.25, np.float32))
