@@ -350,6 +350,12 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * (slowest first), of a primal laid out [n_chan][n_batch][dims...]; the dual
  * q is [nd][n_chan][n_batch][dims...] (TransformGradient.adj_shape,
  * operators.py:767-844).
+ * strides: NULL when the gradient axes are the trailing nd axes in order (as
+ * above); else the gradient axes are any nd distinct axes of a channel's block
+ * of n_batch samples (C-contiguous), given by their sizes dims[k] and element
+ * strides strides[k] in the order of Regularizer_Grad.axes (the dual's
+ * component k differentiates along axis k; regularizers.py:247-258,
+ * operators.py:781-816), and the block is not split into batches.
  * pad_mode: TransformGradient's np.pad mode (operators.py:796-835): 0 = edge
  * (the gradient's last difference is 0 and so is the divergence's first),
  * 1 = constant (last difference -x, first divergence term q), 2 = wrap,
@@ -383,12 +389,13 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * (compute_update_primal, regularizers.py:194-210; PDHG adds, SIRT
  * subtracts, solvers.py:953-958, 1178-1180).
  */
-int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int pad_mode,
-                   int mode, float weight, const float* weights, int64_t weights_period, int der_norm,
-                   int chan_norm, const float* qref, void* stream);
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims,
+                   const int64_t* strides, int pad_mode, int mode, float weight, const float* weights,
+                   int64_t weights_period, int der_norm, int chan_norm, const float* qref, void* stream);
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            int pad_mode, float scale, const float* scales, int64_t scales_period, int apply_scale,
-                            int subtract, const float* mask, int64_t mask_period, void* stream);
+                            const int64_t* strides, int pad_mode, float scale, const float* scales,
+                            int64_t scales_period, int apply_scale, int subtract, const float* mask,
+                            int64_t mask_period, void* stream);
 
 /* Regularizer_l1 in PDHG / SIRT (identity transform, regularizers.py:531-557 and
  * the base class's dual / primal steps, regularizers.py:134-210):

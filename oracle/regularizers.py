@@ -30,25 +30,28 @@ import numpy as np
 F32 = np.float32
 
 
-def grad_nd(x: np.ndarray, nd: int, pad_mode: str = "edge") -> np.ndarray:
-    """``np.diff(np.pad(x, (0, 1) along the axis, mode=pad_mode))`` per axis (``operators.py:796-816``)."""
+def _axes(ndim: int, nd: int, axes) -> list:
+    return [ndim - nd + k for k in range(nd)] if axes is None else [int(a) % ndim for a in axes]
+
+
+def grad_nd(x: np.ndarray, nd: int, pad_mode: str = "edge", axes=None) -> np.ndarray:
+    """``np.diff(np.pad(x, (0, 1) along the axis, mode=pad_mode))`` per axis (``operators.py:796-816``);
+    ``axes`` in their given order (default: the last ``nd``)."""
     x = np.asarray(x)
     g = np.zeros((nd, *x.shape), dtype=x.dtype)
-    for k in range(nd):
-        ax = x.ndim - nd + k
+    for k, ax in enumerate(_axes(x.ndim, nd, axes)):
         padding = [(0, 0)] * x.ndim
         padding[ax] = (0, 1)
         g[k] = np.diff(np.pad(x, padding, mode=pad_mode), n=1, axis=ax)
     return g
 
 
-def div_nd(q: np.ndarray, pad_mode: str = "edge") -> np.ndarray:
+def div_nd(q: np.ndarray, pad_mode: str = "edge", axes=None) -> np.ndarray:
     """``sum_k np.diff(np.pad(q[k], (1, 0) along the axis, mode=pad_mode))`` (``operators.py:818-835``)."""
     nd = q.shape[0]
     ndim = q.ndim - 1
     out = None
-    for k in range(nd):
-        ax = ndim - nd + k
+    for k, ax in enumerate(_axes(ndim, nd, axes)):
         padding = [(0, 0)] * ndim
         padding[ax] = (1, 0)
         d = np.diff(np.pad(q[k], padding, mode=pad_mode), n=1, axis=ax)
@@ -125,8 +128,9 @@ class GradReg:
     (``data_terms.py:500-501``). Both weight the primal update (outside the l1 family)."""
 
     def __init__(self, nd: int, weight, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
-                 huber_size: float = 0.0, pad_mode: str = "edge"):
+                 huber_size: float = 0.0, pad_mode: str = "edge", axes=None):
         self.pad_mode = pad_mode
+        self.axes = axes
         # an array weight broadcasts against the primal (and the dual's trailing axes)
         weight = float(weight) if np.ndim(weight) == 0 else np.asarray(weight, dtype=F32)
         self.nd, self.weight, self.kind, self.der, self.chan = nd, weight, kind, der, chan
@@ -145,7 +149,7 @@ class GradReg:
         return t if self.upd_mask is None else t * self.upd_mask
 
     def dual(self, q: np.ndarray, x: np.ndarray) -> np.ndarray:
-        q = q + F32(0.5) * grad_nd(x, self.nd, self.pad_mode)
+        q = q + F32(0.5) * grad_nd(x, self.nd, self.pad_mode, self.axes)
         if self.kind == "l12":
             return prox_l12(q, self.weight)
         if self.kind == "l1":
@@ -164,7 +168,7 @@ class GradReg:
         return np.zeros((self.nd, *x_shape), dtype=F32)
 
     def primal(self, q: np.ndarray) -> np.ndarray:
-        u = -div_nd(q, self.pad_mode)
+        u = -div_nd(q, self.pad_mode, self.axes)
         if self.upd_mask is not None:
             u = u * self.upd_mask
         return u * np.asarray(self.weight, dtype=F32) if self.weighted else u

@@ -40,10 +40,10 @@ _SIGNATURES = {
     "chip_plan_set_subangles": [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "chip_plan_set_radon_trig": [c_void_p, c_void_p],
     "chip_batch_set_radon_trig": [c_void_p, c_void_p],
-    "chip_grad_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int,
-                       ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p,
-                       c_void_p],
-    "chip_grad_update_primal": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p,
+    "chip_grad_dual": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p,
+                       ctypes.c_int, ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                       c_void_p, c_void_p],
+    "chip_grad_update_primal": [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p,
                                 ctypes.c_int, ctypes.c_float, c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                 c_void_p, ctypes.c_int64, c_void_p],
     "chip_l1_dual": [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, ctypes.c_int64, c_void_p],

@@ -82,12 +82,10 @@ inline dim3 gr_grid(int64_t n) {
   return dim3(static_cast<unsigned>(blocks));
 }
 
+// position of sample s along each gradient axis (the axes may be any of the
+// block's axes, in any order)
 __device__ __forceinline__ void sample_index(const GradGeom& g, int64_t s, int idx[3]) {
-  int64_t r = s;
-  for (int k = g.nd - 1; k >= 0; --k) {
-    idx[k] = static_cast<int>(r % g.dim[k]);
-    r /= g.dim[k];
-  }
+  for (int k = 0; k < g.nd; ++k) idx[k] = static_cast<int>((s / g.stride[k]) % g.dim[k]);
 }
 
 // Cyclic Jacobi on the symmetric nd x nd matrix a (overwritten by its
@@ -357,7 +355,7 @@ __global__ __launch_bounds__(kGrBlock) void grad_primal_kernel(float* __restrict
   }
 }
 
-int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims, int pad) {
+int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims, const int64_t* strides, int pad) {
   CHIP_CHECK_ARG(nd >= 1 && nd <= 3, "nd must be 1, 2 or 3");
   CHIP_CHECK_ARG(pad >= kPadEdge && pad <= kPadSymmetric, "pad_mode must be 0..4 (edge, constant, wrap, reflect, "
                  "symmetric)");
@@ -378,6 +376,17 @@ int make_geom(GradGeom& g, int n_chan, int64_t n_batch, int nd, const int* dims,
     g.dim[k] = dims[k];
     g.S *= dims[k];
   }
+  if (strides) {  // the gradient axes at these strides inside one block of n_batch samples
+    g.B = 1;
+    g.S = n_batch;
+    for (int k = 0; k < nd; ++k) {
+      CHIP_CHECK_ARG(strides[k] >= 1 && strides[k] * dims[k] <= n_batch && n_batch % (strides[k] * dims[k]) == 0,
+                     "each gradient axis must tile the block: stride * size dividing the block size");
+      for (int j = 0; j < k; ++j) CHIP_CHECK_ARG(strides[j] != strides[k], "gradient axes must be distinct");
+      g.stride[k] = strides[k];
+    }
+    return CHIP_OK;
+  }
   int64_t st = 1;
   for (int k = nd - 1; k >= 0; --k) {
     g.stride[k] = st;
@@ -393,8 +402,8 @@ using namespace chip;
 
 extern "C" {
 
-int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims, int pad_mode,
-                   int mode, float weight, const float* weights, int64_t weights_period, int der_norm,
+int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd, const int* dims,
+                   const int64_t* strides, int pad_mode, int mode, float weight, const float* weights, int64_t weights_period, int der_norm,
                    int chan_norm, const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
@@ -407,7 +416,7 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
   CHIP_CHECK_ARG(weights == nullptr || mode == kGrL12 || mode == kGrL1 || mode == kGrVTV,
                  "a weight array is taken by the l12, l1 and VTV modes");
   GradGeom g;
-  const int rc = make_geom(g, n_chan, n_batch, nd, dims, pad_mode);
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims, strides, pad_mode);
   if (rc != CHIP_OK) return rc;
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   CHIP_CHECK_ARG(weights == nullptr || (weights_period >= 1 && total % weights_period == 0),
@@ -422,12 +431,12 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
 }
 
 int chip_grad_update_primal(float* upd, const float* q, int n_chan, int64_t n_batch, int nd, const int* dims,
-                            int pad_mode, float scale, const float* scales, int64_t scales_period, int apply_scale,
+                            const int64_t* strides, int pad_mode, float scale, const float* scales, int64_t scales_period, int apply_scale,
                             int subtract, const float* mask, int64_t mask_period, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(upd != nullptr && q != nullptr, "NULL argument");
   GradGeom g;
-  const int rc = make_geom(g, n_chan, n_batch, nd, dims, pad_mode);
+  const int rc = make_geom(g, n_chan, n_batch, nd, dims, strides, pad_mode);
   if (rc != CHIP_OK) return rc;
   const int64_t total = static_cast<int64_t>(g.C) * g.B * g.S;
   CHIP_CHECK_ARG(mask == nullptr || (mask_period >= 1 && total % mask_period == 0),

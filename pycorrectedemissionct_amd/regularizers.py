@@ -122,8 +122,10 @@ class BaseRegularizer:
                          _native.ptr(cfg.get("weights")), cfg.get("weights_period", 1), _native.stream())
         else:
             zero = torch.zeros(cfg["q_shape"][1:], dtype=torch.float32, device=t.device)
+            strides = cfg.get("strides")
             _native.call("chip_grad_dual", t.data_ptr(), zero.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                         cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["mode"], cfg["weight"],
+                         cfg["dims"].ctypes.data, None if strides is None else strides.ctypes.data,
+                         cfg.get("pad", 0), cfg["mode"], cfg["weight"],
                          _native.ptr(cfg.get("weights")),
                          cfg.get("weights_period", 1), cfg["der"], cfg["chan"], _native.ptr(cfg.get("qref")),
                          _native.stream())
@@ -195,21 +197,34 @@ class Regularizer_Grad(BaseRegularizer):
         norm as in the reference), on the device."""
         self._prox_on_device(dual, self.device_config(tuple(np.shape(dual))[1:]))
 
-    def _check_trailing_axes(self, shape) -> None:
-        nd = len(shape)
-        axes = sorted(int(a) % nd for a in np.asarray(self.axes).reshape(-1))
-        if axes != list(range(nd - self.ndims, nd)) or self.ndims > 3:
-            raise NotImplementedError("the HIP gradient kernels take the gradient over the last 1 to 3 axes, in order")
-        if list(np.asarray(self.axes) % nd) != axes:
-            raise NotImplementedError("the gradient axes must be given in increasing order")
+    def _layout(self, shape, multichannel: bool = False, what: str = "") -> tuple:
+        """(channels, batch, gradient-axis sizes, strides or None) of a primal of ``shape``: the gradient
+        axes are ``self.axes`` in their given order; with ``multichannel`` axis 0 holds the coupled
+        channels (``_multichannel_layout``). Trailing axes in increasing order keep the batched layout
+        (strides None); any other choice runs over one block per channel at the axes' strides."""
+        nd_all = len(shape)
+        axes = [int(a) % nd_all for a in np.asarray(self.axes).reshape(-1)]
+        if len(set(axes)) != len(axes):
+            raise ValueError(f"the gradient axes {list(self.axes)} repeat an axis")
+        if not 1 <= len(axes) <= 3:
+            raise NotImplementedError("the HIP gradient kernels take 1 to 3 gradient axes")
         if self.pad_mode not in _PAD_CODE:
             raise ValueError(f"Unsupported padding mode: {self.pad_mode}")
+        n_chan, c0 = 1, 0
+        if multichannel and nd_all > len(axes):
+            n_chan, _ = _multichannel_layout(shape, len(axes), what)
+            c0 = 1
+            if 0 in axes:
+                raise NotImplementedError(f"{what} takes the channels on axis 0, which cannot be a gradient axis")
+        if axes == list(range(nd_all - len(axes), nd_all)):
+            return n_chan, int(np.prod(shape[c0: nd_all - len(axes)])), [shape[a] for a in axes], None
+        strides = [int(np.prod(shape[a + 1:], dtype=np.int64)) for a in axes]
+        return n_chan, int(np.prod(shape[c0:], dtype=np.int64)), [shape[a] for a in axes], strides
 
     def device_config(self, shape) -> dict:
         """Kernel parameters of ``chip_grad_dual`` / ``chip_grad_update_primal`` for a primal of ``shape``."""
         shape = tuple(int(v) for v in shape)
-        self._check_trailing_axes(shape)
-        n_batch = int(np.prod(shape[: len(shape) - self.ndims]))
+        layout = self._layout(shape)
         if isinstance(self.norm, dt.DataFidelity_l12):
             if getattr(self.norm, "l2_axis", 0) != 0:
                 raise NotImplementedError("the l12 norm is supported over the gradient axis (l2_axis=0)")
@@ -218,7 +233,7 @@ class Regularizer_Grad(BaseRegularizer):
             mode = 1
         elif type(self.norm) is dt.DataFidelity_l2:
             # assign_data(None, sigma=0.5) leaves the dual step q * 1 / (1 + 0.5) (data_terms.py:294-304)
-            cfg = self._config(shape, 4, 1, n_batch, 0, 0)
+            cfg = self._config(shape, 4, layout, 0, 0)
             cfg["weight"] = float(np.float32(1 / (1 + self.sigma)))
             return cfg
         elif type(self.norm) is dt.DataFidelity_Huber:
@@ -226,15 +241,16 @@ class Regularizer_Grad(BaseRegularizer):
                 raise NotImplementedError("the Huber norm is supported over the gradient axis (l2_axis=0)")
             if np.asarray(self.norm.local_error).size != 1:
                 raise NotImplementedError("the HIP gradient kernels take a scalar Huber size")
-            cfg = self._config(shape, 5, 1, n_batch, 0, 0)
+            cfg = self._config(shape, 5, layout, 0, 0)
             # one_sigma_error = 1 / (1 + sigma huber_size) (data_terms.py:482-483)
             cfg["weight"] = float(np.float32(1.0 / (1.0 + self.sigma * float(self.norm.local_error))))
             return cfg
         else:
             raise NotImplementedError(f"norm {self.norm.info()} is not supported by the HIP gradient kernels")
-        return self._config(shape, mode, 1, n_batch, 0, 0)
+        return self._config(shape, mode, layout, 0, 0)
 
-    def _config(self, shape, mode, n_chan, n_batch, der, chan) -> dict:
+    def _config(self, shape, mode, layout, der, chan) -> dict:
+        n_chan, n_batch, dims, strides = layout
         l1_family = isinstance(self.norm, dt.DataFidelity_l1)
         mask = None if self.upd_mask is None else self._device_mask(shape)
         w_arr = self._weight_dev(shape)
@@ -243,9 +259,11 @@ class Regularizer_Grad(BaseRegularizer):
         # l1 family; TNV, l2 and Huber do not read it in the prox
         dual_w = w_arr if mode in (0, 1, 2) else None
         scales = None if l1_family else w_arr
-        return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=int(self.ndims),
-                    dims=np.ascontiguousarray(shape[len(shape) - self.ndims:], dtype=np.int32),
-                    pad=_PAD_CODE[self.pad_mode], weight=w, weights=dual_w, weights_period=1 if dual_w is None else int(dual_w.numel()),
+        return dict(mode=mode, n_chan=int(n_chan), n_batch=int(n_batch), nd=len(dims),
+                    dims=np.ascontiguousarray(dims, dtype=np.int32),
+                    strides=None if strides is None else np.ascontiguousarray(strides, dtype=np.int64),
+                    pad=_PAD_CODE[self.pad_mode], weight=w, weights=dual_w,
+                    weights_period=1 if dual_w is None else int(dual_w.numel()),
                     der=der, chan=chan, scale=w, scales=scales,
                     scales_period=1 if scales is None else int(scales.numel()),
                     apply_scale=0 if l1_family else 1, q_shape=(self.ndims, *shape),
@@ -405,9 +423,7 @@ class Regularizer_VTV(Regularizer_Grad):
 
     def device_config(self, shape) -> dict:
         shape = tuple(int(v) for v in shape)
-        self._check_trailing_axes(shape)
-        n_chan, n_batch = _multichannel_layout(shape, self.ndims, "VTV")
-        cfg = self._config(shape, 2, n_chan, n_batch, _PWISE_CODE[self.pwise_der_norm],
+        cfg = self._config(shape, 2, self._layout(shape, True, "VTV"), _PWISE_CODE[self.pwise_der_norm],
                            _PWISE_CODE[self.pwise_chan_norm])
         if self.q_ref is not None:
             want = shape[1:] if len(shape) > self.ndims else shape  # the primal without its channel axis
@@ -438,13 +454,11 @@ class Regularizer_TNV(Regularizer_Grad):
 
     def device_config(self, shape) -> dict:
         shape = tuple(int(v) for v in shape)
-        self._check_trailing_axes(shape)
         if type(self.norm.spectral_norm) is not dt.DataFidelity_l1:
             raise NotImplementedError("the HIP TNV kernel implements the l1 spectral norm")
         if len(shape) <= self.ndims:
             raise NotImplementedError("TNV needs a channel axis in front of the gradient axes")
-        n_chan, n_batch = _multichannel_layout(shape, self.ndims, "TNV")
-        return self._config(shape, 3, n_chan, n_batch, 0, 0)
+        return self._config(shape, 3, self._layout(shape, True, "TNV"), 0, 0)
 
 
 class Regularizer_l1(BaseRegularizer):

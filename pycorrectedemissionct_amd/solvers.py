@@ -273,8 +273,10 @@ class Solver:
             _native.call("chip_l1_dual", q.data_ptr(), x.data_ptr(), cfg["n"], cfg["weight"],
                          _native.ptr(cfg["weights"]), cfg["weights_period"], st)
             return
+        strides = cfg.get("strides")
         _native.call("chip_grad_dual", q.data_ptr(), x.data_ptr(), cfg["n_chan"], cfg["n_batch"], cfg["nd"],
-                     cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["mode"], cfg["weight"],
+                     cfg["dims"].ctypes.data, None if strides is None else strides.ctypes.data, cfg.get("pad", 0),
+                     cfg["mode"], cfg["weight"],
                      _native.ptr(cfg.get("weights")), cfg.get("weights_period", 1), cfg["der"], cfg["chan"],
                      _native.ptr(cfg.get("qref")), st)
 
@@ -284,8 +286,10 @@ class Solver:
             _native.call("chip_l1_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n"], _native.ptr(cfg["mask"]),
                          cfg["mask_period"], int(subtract), st)
             return
+        strides = cfg.get("strides")
         _native.call("chip_grad_update_primal", upd.data_ptr(), q.data_ptr(), cfg["n_chan"], cfg["n_batch"],
-                     cfg["nd"], cfg["dims"].ctypes.data, cfg.get("pad", 0), cfg["scale"], _native.ptr(cfg.get("scales")),
+                     cfg["nd"], cfg["dims"].ctypes.data, None if strides is None else strides.ctypes.data,
+                     cfg.get("pad", 0), cfg["scale"], _native.ptr(cfg.get("scales")),
                      cfg.get("scales_period", 1), cfg["apply_scale"], int(subtract), _native.ptr(cfg.get("mask")),
                      cfg.get("mask_period", 1), st)
 
