@@ -2,7 +2,8 @@
 
 ``Regularizer_l1`` (``regularizers.py:531-585``): sparsity, identity transform;
 its primal proximal (soft thresholding at tau * weight) is fused into
-``chip_fista_update``.
+``chip_fista_update``, its dual steps in PDHG / SIRT run on ``chip_l1_dual`` /
+``chip_l1_update_primal``.
 
 ``Regularizer_TV2D`` (``regularizers.py:308-321``) over ``Regularizer_Grad``
 (``regularizers.py:218-289``): isotropic total variation with the l12 norm,
@@ -12,8 +13,9 @@ update and prox run fused in ``chip_tv_dual``; its primal contribution
 
 ``Regularizer_TV1D`` / ``TV3D`` (``regularizers.py:292-337``), ``HubTV2D`` /
 ``HubTV3D`` (Huber norm, ``regularizers.py:340-383``), ``smooth1D/2D/3D`` (l2
-norm, ``regularizers.py:386-431``), any other ``Regularizer_Grad`` over the
-trailing axes with the l12, l1, l2 or Huber (``l2_axis=0``) norm,
+norm, ``regularizers.py:386-431``), any other ``Regularizer_Grad`` over 1 to 3
+axes (in any order and position, with any ``np.pad`` mode) with the l12, l1,
+l2 or Huber (``l2_axis=0``) norm, and a scalar or array weight,
 ``Regularizer_VTV`` (``regularizers.py:1362-1430``) and ``Regularizer_TNV``
 (``regularizers.py:1324-1359``) run on ``chip_grad_dual`` (dual update and
 prox in one pass) and ``chip_grad_update_primal`` (``-div q``, weighted for
