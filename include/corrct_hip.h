@@ -370,8 +370,10 @@ int chip_psf_lines(const float* in, float* out, const float* kern, int k_len, in
  * 1 / (1 + 0.5): Regularizer_smooth1D/2D/3D, regularizers.py:386-433 with
  * data_terms.py:294-304), 5 = Huber over the gradient components (q *= weight,
  * the caller passing 1 / (1 + 0.5 huber_size); q /= max(1, |q|_2):
- * Regularizer_HubTV2D/3D, regularizers.py:340-384 with data_terms.py:482-501).
- * Modes 0, 1, 4 and 5 need n_chan == 1; n_chan <= 16.
+ * Regularizer_HubTV2D/3D, regularizers.py:340-384 with data_terms.py:482-501),
+ * 6 = Huber without an l2 axis (q *= weight as in 5, then each component
+ * q /= max(1, |q|), data_terms.py:497-498).
+ * Modes 0, 1, 4, 5 and 6 need n_chan == 1; n_chan <= 16.
  * weights (modes 0, 1, 2; else NULL): the regularizer's weight as an array
  * broadcast over the primal, weights[e % weights_period] for primal element e
  * (weights_period divides the primal's element count); it replaces `weight` in

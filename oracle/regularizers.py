@@ -118,14 +118,16 @@ def prox_tnv(q: np.ndarray) -> np.ndarray:
 
 
 class GradReg:
-    """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv", "l2", "hub"}.
+    """One gradient-family regularizer: kind in {"l12", "l1", "vtv", "tnv", "l2", "hub", "hub_el"}.
 
     "l2" is ``Regularizer_smooth1D/2D/3D`` (``regularizers.py:386-431``): the l2 norm assigned
     sigma = 1/2 without data scales the dual by ``1 / (1 + 1/2)`` (``data_terms.py:294-304``).
     "hub" is ``Regularizer_HubTV2D/3D`` (``regularizers.py:340-383``): Huber over the gradient
     axis, the dual scaled by ``1 / (1 + huber_size / 2)`` (a Python float, rounded to float32 when
     numpy multiplies, ``data_terms.py:482-495``) then divided by ``max(1, |q|_2)``
-    (``data_terms.py:500-501``). Both weight the primal update (outside the l1 family)."""
+    (``data_terms.py:500-501``); "hub_el" is the same norm without an l2 axis, each component divided by
+    ``max(1, |q|)`` (``data_terms.py:497-498``). All three weight the primal update (outside the l1
+    family)."""
 
     def __init__(self, nd: int, weight, kind: str = "l12", der=2, chan=np.inf, upd_mask=None, x_ref=None,
                  huber_size: float = 0.0, pad_mode: str = "edge", axes=None):
@@ -141,7 +143,7 @@ class GradReg:
 
     @property
     def weighted(self) -> bool:
-        return self.kind in ("tnv", "l2", "hub")  # outside the l1 family
+        return self.kind in ("tnv", "l2", "hub", "hub_el")  # outside the l1 family
 
     def tau(self):
         t = 2.0 * self.nd
@@ -158,9 +160,9 @@ class GradReg:
             return prox_vtv(q, self.weight, self.der, self.chan, self.nd, self.q_ref)
         if self.kind == "l2":
             return (q * F32(1 / (1 + 0.5))).astype(F32)
-        if self.kind == "hub":
+        if self.kind in ("hub", "hub_el"):
             q = (q * F32(1.0 / (1.0 + 0.5 * self.huber_size))).astype(F32)
-            nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=F32))
+            nrm = np.sqrt(np.sum(q * q, axis=0, keepdims=True, dtype=F32)) if self.kind == "hub" else np.abs(q)
             return (q / np.fmax(F32(1), nrm)).astype(F32)
         return prox_tnv(q)
 

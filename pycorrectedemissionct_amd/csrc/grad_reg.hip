@@ -17,7 +17,8 @@
 //   l1     q = (q / max(|q|, w)) * w
 //   l2     q = q * w,  w = 1 / (1 + 0.5)             (data_terms.py:294-304, no data)
 //   Huber  q = q * w,  w = 1 / (1 + 0.5 huber_size); q = q / max(1, |q|_2 over nd)
-//          (data_terms.py:482-501 with l2_axis 0)
+//          (data_terms.py:482-501 with l2_axis 0); without an l2 axis each
+//          component is divided by max(1, |q|) (data_terms.py:497-498)
 //   VTV    q = (q / max(dn, w)) * w with dn the point-wise derivative norm
 //          (1, 2, inf) followed by the channel norm (1, 2, inf); the dual
 //          norms of inf are 1, as in regularizers.py:1395-1411
@@ -38,7 +39,7 @@ namespace {
 constexpr int kGrBlock = 256;
 constexpr int kGrMaxChan = 16;
 
-enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3, kGrL2 = 4, kGrHuber = 5 };
+enum GradMode { kGrL12 = 0, kGrL1 = 1, kGrVTV = 2, kGrTNV = 3, kGrL2 = 4, kGrHuber = 5, kGrHuberEl = 6 };
 
 // np.pad modes of TransformGradient (operators.py:796-835)
 enum GradPad { kPadEdge = 0, kPadConstant = 1, kPadWrap = 2, kPadReflect = 3, kPadSymmetric = 4 };
@@ -202,6 +203,17 @@ __global__ __launch_bounds__(kGrBlock) void grad_dual_kernel(float* __restrict__
         const float den = fmaxf(1.f, sqrtf(acc));
 #pragma unroll
         for (int k = 0; k < ND; ++k) v[k][c] = v[k][c] / den;
+      }
+    } else if (mode == kGrHuberEl) {
+      // DataFidelity_Huber without an l2 axis: q = q * w, then q / max(1, |q|) per component
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= g.C) break;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+          const float t = v[k][c] * w;
+          v[k][c] = t / fmaxf(1.f, fabsf(t));
+        }
       }
     } else if (mode == kGrL1) {
 #pragma unroll
@@ -407,7 +419,7 @@ int chip_grad_dual(float* q, const float* x, int n_chan, int64_t n_batch, int nd
                    int chan_norm, const float* qref, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(q != nullptr && x != nullptr, "NULL argument");
-  CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrHuber, "unknown mode");
+  CHIP_CHECK_ARG(mode >= kGrL12 && mode <= kGrHuberEl, "unknown mode");
   CHIP_CHECK_ARG(mode != kGrVTV || ((der_norm == 0 || der_norm == 1 || der_norm == 2) &&
                                     (chan_norm == 0 || chan_norm == 1 || chan_norm == 2)),
                  "VTV point-wise norms must be 1, 2 or 0 (inf)");

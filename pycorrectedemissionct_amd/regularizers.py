@@ -239,11 +239,12 @@ class Regularizer_Grad(BaseRegularizer):
             cfg["weight"] = float(np.float32(1 / (1 + self.sigma)))
             return cfg
         elif type(self.norm) is dt.DataFidelity_Huber:
-            if self.norm.l2_axis != 0:
-                raise NotImplementedError("the Huber norm is supported over the gradient axis (l2_axis=0)")
+            if self.norm.l2_axis not in (0, None):
+                raise NotImplementedError("the Huber norm is supported over the gradient axis (l2_axis=0) or "
+                                          "per component (l2_axis=None)")
             if np.asarray(self.norm.local_error).size != 1:
                 raise NotImplementedError("the HIP gradient kernels take a scalar Huber size")
-            cfg = self._config(shape, 5, layout, 0, 0)
+            cfg = self._config(shape, 5 if self.norm.l2_axis == 0 else 6, layout, 0, 0)
             # one_sigma_error = 1 / (1 + sigma huber_size) (data_terms.py:482-483)
             cfg["weight"] = float(np.float32(1.0 / (1.0 + self.sigma * float(self.norm.local_error))))
             return cfg

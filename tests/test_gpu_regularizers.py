@@ -145,8 +145,13 @@ def test_smooth_and_huber_tv_regularizers(pkg):
         ref = osol.sirt(A, At, b, f.shape, 30, regs=[orc])
         print(f"SIRT-{name}: rel_max {rel_max(ref, x):.2e}")
         assert rel_max(ref, x) < TOL, name
+    x = pkg.PDHG(regularizer=R.Regularizer_Grad(5e-2, norm=pkg.data_terms.DataFidelity_Huber(0.1)))(
+        prj, b, iterations=30)[0]
+    ref = osol.pdhg(A, At, b, f.shape, 30, regs=[oreg.GradReg(2, 5e-2, "hub_el", huber_size=0.1)])
+    print(f"PDHG-l2-Huber per component: rel_max {rel_max(ref, x):.2e}")
+    assert rel_max(ref, x) < TOL
     with pytest.raises(NotImplementedError):
-        pkg.PDHG(regularizer=R.Regularizer_Grad(1e-2, norm=pkg.data_terms.DataFidelity_Huber(0.1, l2_axis=None)))(
+        pkg.PDHG(regularizer=R.Regularizer_Grad(1e-2, norm=pkg.data_terms.DataFidelity_Huber(0.1, l2_axis=1)))(
             prj, b, iterations=1)
 
 
