@@ -416,25 +416,44 @@ def run_otf(args) -> dict:
         log(f"otf fp with a {mib} MiB scratch budget: {budgets[mib]:.1f} ms")
     best = min(budgets, key=budgets.get)
     otf._plan().tune("otf_budget_mib", best)
-    y = otf.fp(x)
-    otf.bp(y)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    y = otf.fp(x)
-    torch.cuda.synchronize()
-    t_fp = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    v = otf.bp(y)
-    torch.cuda.synchronize()
-    t_bp = time.perf_counter() - t0
+
+    def timed(fn, reps):  # warm call, then the mean of `reps` calls
+        out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) / reps
+
+    y, t_fp = timed(lambda: otf.fp(x), 3)
+    v, t_bp = timed(lambda: otf.bp(y), 3)
     pre = pc.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, bp_mode="adjoint")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    y2 = pre.fp(x)
-    torch.cuda.synchronize()
-    t_fp_pre = time.perf_counter() - t0
+    y2, t_fp_pre = timed(lambda: pre.fp(x), 10)
+    _, t_bp_pre = timed(lambda: pre.bp(y2), 10)
     same = bool(torch.equal(y, y2))
     dv = float((pre.bp(y2) - v).abs().max() / v.abs().max())
+    # PDHG-KL-TV iterations at this size: precomputed maps, on-the-fly maps built separately for the fp
+    # and the bp, and on-the-fly with the fused per-angle-block data step (maps built once)
+    b = pc.testing.poisson_data(y2.cpu().numpy(), 1e3, rng)
+    nofuse = pc.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, bp_mode="adjoint",
+                                        maps_mode="otf")
+    nofuse._plan().tune("otf_budget_mib", best)
+    nofuse.supports_fused_data_step = lambda: False
+    iter_ms, xs_out = {}, {}
+    for name, prj in (("precomputed", pre), ("otf_separate", nofuse), ("otf_fused", otf)):
+        solver = pc.PDHG(data_term="kl", regularizer=pc.regularizers.Regularizer_TV2D(1e-2))
+        st = solver.prepare(prj, b)
+        solver.iterate(prj, st, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.iterate(prj, st, args.otf_iters)
+        torch.cuda.synchronize()
+        iter_ms[name] = 1e3 * (time.perf_counter() - t0) / args.otf_iters
+        xs_out[name] = st["x"].clone()
+        log(f"PDHG-KL-TV iteration, {name}: {iter_ms[name]:.1f} ms")
+    fused_equal = bool(torch.equal(xs_out["otf_fused"], xs_out["otf_separate"]))
+    del nofuse
     # the warm start's network at this slice size (8 slices per launch; its cost does not depend
     # on the weights, so they are the seeded initial ones)
     from pycorrectedemissionct_amd import warmstart as ws
@@ -454,13 +473,17 @@ def run_otf(args) -> dict:
         "config": {"workload": f"1024x1024, 1440 angles, 1 detector at 90 deg, maps rebuilt per angle block "
                                f"({best} MiB scratch) instead of 6 GB of precomputed maps",
                    "otf_fp_ms": 1e3 * t_fp, "otf_bp_ms": 1e3 * t_bp, "precomputed_fp_ms": 1e3 * t_fp_pre,
-                   "otf_over_precomputed_fp": t_fp / t_fp_pre, "otf_fp_ms_by_budget_mib": budgets,
+                   "precomputed_bp_ms": 1e3 * t_bp_pre, "otf_over_precomputed_fp": t_fp / t_fp_pre, "otf_fp_ms_by_budget_mib": budgets,
                    "otf_budget_mib": best,
                    # samples the on-the-fly fp computes: the map builder's rotated-frame scan (two beams of
                    # the padded side squared per angle), its composition (two beams per map pixel) and the
                    # projection's own samples (one per pixel and angle)
                    "otf_fp_gsamples_per_s": a * (2 * otf._plan().n_pad ** 2 + 2 * n * n + n * n) / t_fp / 1e9,
                    "otf_fp_equals_precomputed": same, "otf_bp_rel_diff": dv,
+                   "pdhg_kl_tv_iteration_ms": iter_ms, "pdhg_iterations_timed": args.otf_iters,
+                   "otf_fused_iteration_over_precomputed": iter_ms["otf_fused"] / iter_ms["precomputed"],
+                   "otf_separate_iteration_over_precomputed": iter_ms["otf_separate"] / iter_ms["precomputed"],
+                   "otf_fused_equals_separate": fused_equal,
                    "precomputed_maps_gb": 4.0 * a * n * n / 1e9, "unet_ms_per_slice": 1e3 * t_unet,
                    "unet_batch": 8},
     }
@@ -614,6 +637,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--fp-reps", type=int, default=20)
+    ap.add_argument("--otf-iters", type=int, default=3, help="otf workload: PDHG iterations timed per variant")
     ap.add_argument("--otf-budgets", type=int, nargs="+", default=[1024, 2048, 4096],
                     help="on-the-fly scratch budgets (MiB) to time; the fastest is kept")
     args = ap.parse_args()

@@ -49,6 +49,15 @@ enum chip_bp_mode {
   CHIP_BP_ADJOINT = 2
 };
 
+/* Data steps of the fused on-the-fly iteration (chip_otf_fp_row_bp), each the
+ * elementwise kernel of the same name applied to one angle block's rows. */
+enum chip_row_op {
+  CHIP_ROW_PDHG_L2 = 0, /* p = prox_l2(p + sigma (ax + bg)) * mask (chip_pdhg_dual_l2) */
+  CHIP_ROW_PDHG_KL = 1, /* p = prox_KL(p + sigma (ax + bg)) * mask (chip_pdhg_dual_kl) */
+  CHIP_ROW_SIRT = 2,    /* p = (b - (ax + bg)) * mask * sigma (chip_sirt_residual; sigma may be NULL) */
+  CHIP_ROW_MLEM = 3     /* p = b / max(ax + bg, eps) (chip_mlem_ratio; sigma and mask unused) */
+};
+
 /* Data-fidelity terms of chip_pdhg_dual_norm / chip_data_residual
  * (src/corrct/data_terms.py). `e` is the term's local_error, `w` its weights,
  * both read as param[i % param_period]. */
@@ -127,6 +136,23 @@ int chip_plan_set_subangles(chip_plan* plan, int n_sub_max, const int* n_sub_det
  */
 int chip_att_maps(chip_plan* plan, const float* mu_in, const float* mu_out, float* maps, int batch,
                   void* stream);
+
+/*
+ * One solver iteration's projector work with on-the-fly maps, angle block by
+ * angle block: build the block's maps once, fp x over the block into ax, apply
+ * row_op to the block's rows of p (enum chip_row_op; b, sigma, bg, b_mask as the
+ * op needs, NULL where absent), and back-project the block's rows of p into vol
+ * (bp_mode CHIP_BP_ADJOINT or CHIP_BP_SKIMAGE_SYM, accumulated over the blocks
+ * in order). The results equal chip_fp_otf, the elementwise kernel and
+ * chip_bp_otf called in turn, bit for bit, with half the map building: the data
+ * steps of SIRT, MLEM and PDHG (solvers.py:775-803, 951-982, 1165-1183) are
+ * elementwise on the sinogram, so each block's rows are final once its fp is.
+ * No reference counterpart (the reference always precomputes the maps).
+ * Layouts as chip_fp_otf / chip_bp_otf; p, ax and the data are [batch][D][A][N].
+ */
+int chip_otf_fp_row_bp(chip_plan* plan, const float* x, const float* mu_in, const float* mu_out, int batch,
+                       int row_op, float* p, const float* sigma, const float* b, const float* bg,
+                       const float* b_mask, float* ax, float* vol, int bp_mode, void* stream);
 
 /*
  * Binned maps: each of n_img images of nd (2 or 3) axes dims[nd] (slowest

@@ -24,6 +24,12 @@ CHIP_BP_SKIMAGE = 0
 CHIP_BP_SKIMAGE_SYM = 1
 CHIP_BP_ADJOINT = 2
 
+# enum chip_row_op
+CHIP_ROW_PDHG_L2 = 0
+CHIP_ROW_PDHG_KL = 1
+CHIP_ROW_SIRT = 2
+CHIP_ROW_MLEM = 3
+
 _lock = threading.Lock()
 _lib = None
 
@@ -52,6 +58,8 @@ _SIGNATURES = {
     "chip_rdiv": [c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p],
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_bin_images": [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p],
+    "chip_otf_fp_row_bp": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],
     "chip_bp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],
     "chip_bp_noalloc": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p],

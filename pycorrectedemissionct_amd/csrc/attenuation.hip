@@ -26,6 +26,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "solver_elem.hpp"
 
 // skimage's separate float32 / float64 products and sums: no contraction of
 // a * b + c into one rounding in this file (explicit fmaf / fma stay fused)
@@ -442,6 +443,68 @@ int otf_bp(const Plan& p, const float* sino, const float* mu_in, const float* mu
   return CHIP_OK;
 }
 
+// The data step of one angle block, between its fp and its bp: for slice `sb`'s rows
+// [a0, a0 + na) of every detector (sinogram [D][A][N] per slice), p = the row op of ax.
+__global__ __launch_bounds__(256) void row_op_kernel(float* __restrict__ p, const float* __restrict__ ax,
+                                                     const float* __restrict__ sigma,
+                                                     const float* __restrict__ b, const float* __restrict__ bg,
+                                                     const float* __restrict__ mask, int op, int64_t base,
+                                                     int n_det, int n_ang, int n, int a0, int na) {
+  const int64_t per_det = static_cast<int64_t>(na) * n;
+  const int64_t total = per_det * n_det;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int d = static_cast<int>(t / per_det);
+    const int64_t i = base + (static_cast<int64_t>(d) * n_ang + a0) * n + (t - d * per_det);
+    float a = ax[i];
+    if (bg) a = a + bg[i];
+    float v;
+    if (op == CHIP_ROW_PDHG_L2) {
+      v = elem_dual_l2(p[i], a, sigma[i], b[i]);
+      if (mask) v = v * mask[i];
+    } else if (op == CHIP_ROW_PDHG_KL) {
+      v = elem_dual_kl(p[i], a, sigma[i], b[i]);
+      if (mask) v = v * mask[i];
+    } else if (op == CHIP_ROW_SIRT) {
+      v = elem_residual(a, b[i]);
+      if (mask) v = v * mask[i];
+      if (sigma) v = v * sigma[i];
+    } else {
+      v = elem_mlem_ratio(a, b[i]);
+    }
+    p[i] = v;
+  }
+}
+
+int otf_fp_row_bp(const Plan& p, const float* x, const float* mu_in, const float* mu_out, int batch, int op,
+                  float* pd, const float* sigma, const float* b, const float* bg, const float* mask, float* ax,
+                  float* vol, int mode, int chunk, float* scan, float* block, hipStream_t st) {
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  const int64_t sino_n = static_cast<int64_t>(p.n_det) * p.n_angles * p.n;
+  BpPartGuard guard(p, 1, mode, chunk, st);
+  for (int sb = 0; sb < batch; ++sb) {
+    const float* mi = mu_in ? mu_in + sb * nn : nullptr;
+    const float* mo = mu_out ? mu_out + sb * nn : nullptr;
+    const int64_t base = sb * sino_n;
+    float* v = vol + sb * nn;
+    for (int a0 = 0; a0 < p.n_angles; a0 += chunk) {
+      const int na = std::min(chunk, p.n_angles - a0);
+      const int rc = build_maps_chunk(p, mi, mo, a0, na, block, a0, scan, st);
+      if (rc != CHIP_OK) return rc;
+      launch_fp_range(p, x + sb * nn, block, ax + base, a0, na, a0, st);
+      CHIP_LAUNCH_CHECK();
+      const int64_t total = static_cast<int64_t>(p.n_det) * na * p.n;
+      const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 65536));
+      hipLaunchKernelGGL(row_op_kernel, dim3(blocks), dim3(256), 0, st, pd, ax, sigma, b, bg, mask, op, base,
+                         p.n_det, p.n_angles, p.n, a0, na);
+      CHIP_LAUNCH_CHECK();
+      launch_bp_range(p, pd + base, block, v, mode, a0, na, a0, a0 > 0 ? 1 : 0, st, guard.scratch());
+      CHIP_LAUNCH_CHECK();
+    }
+  }
+  return CHIP_OK;
+}
+
 // out[i][z][y][x] = mean of the b^nd inputs of the block at (z, y, x) after
 // cropping the volume axes at (o0, o1, o2): summed in row-major order in
 // float32, divided by b^nd (skimage block_reduce with np.mean)
@@ -556,4 +619,27 @@ extern "C" int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_i
   const int rc = otf_prepare(p, mu_in, mu_out, st, chunk);
   if (rc != CHIP_OK) return rc;
   return otf_bp(p, sino, mu_in, mu_out, vol, batch, mode, chunk, p.d_scratch, p.d_mblock, st);
+}
+
+extern "C" int chip_otf_fp_row_bp(chip_plan* plan, const float* x, const float* mu_in, const float* mu_out, int batch,
+                                  int row_op, float* p, const float* sigma, const float* b, const float* bg,
+                                  const float* b_mask, float* ax, float* vol, int bp_mode, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr, "plan is NULL");
+  CHIP_CHECK_ARG(x != nullptr && p != nullptr && b != nullptr && ax != nullptr && vol != nullptr, "NULL argument");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(batch >= 1, "batch must be >= 1");
+  CHIP_CHECK_ARG(row_op >= CHIP_ROW_PDHG_L2 && row_op <= CHIP_ROW_MLEM, "unknown row_op");
+  CHIP_CHECK_ARG(sigma != nullptr || row_op == CHIP_ROW_SIRT || row_op == CHIP_ROW_MLEM,
+                 "the PDHG row ops need sigma");
+  CHIP_CHECK_ARG(bp_mode == CHIP_BP_SKIMAGE_SYM || bp_mode == CHIP_BP_ADJOINT,
+                 "bp_mode must be CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT");
+  CHIP_CHECK_ARG(p != ax, "p and ax must be different buffers");
+  Plan& pl = plan->p;
+  hipStream_t st = as_stream(stream);
+  int chunk = 0;
+  const int rc = otf_prepare(pl, mu_in, mu_out, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  return otf_fp_row_bp(pl, x, mu_in, mu_out, batch, row_op, p, sigma, b, bg, b_mask, ax, vol, bp_mode, chunk,
+                       pl.d_scratch, pl.d_mblock, st);
 }

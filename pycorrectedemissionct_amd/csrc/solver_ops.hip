@@ -11,6 +11,7 @@
 #include <cfloat>
 
 #include "common.hpp"
+#include "solver_elem.hpp"
 
 namespace chip {
 
@@ -102,12 +103,9 @@ __global__ __launch_bounds__(kEwBlock) void dual_l2_kernel(float* __restrict__ p
                                                            const float* __restrict__ b, const float* __restrict__ bg,
                                                            const float* __restrict__ mask, int64_t n) {
   GRID_STRIDE(i, n) {
-    const float s = sigma[i];
     float a = ax[i];
     if (bg) a = a + bg[i];
-    float v = p[i] + a * s;
-    v = v - s * b[i];
-    v = v * (1.f / (1.f + s));
+    float v = elem_dual_l2(p[i], a, sigma[i], b[i]);
     if (mask) v = v * mask[i];
     p[i] = v;
   }
@@ -118,13 +116,9 @@ __global__ __launch_bounds__(kEwBlock) void dual_kl_kernel(float* __restrict__ p
                                                            const float* __restrict__ b, const float* __restrict__ bg,
                                                            const float* __restrict__ mask, int64_t n) {
   GRID_STRIDE(i, n) {
-    const float s = sigma[i];
     float a = ax[i];
     if (bg) a = a + bg[i];
-    const float v = p[i] + a * s;
-    const float sd = (4.f * s) * fmaxf(b[i], 0.f);
-    const float dm = v - 1.f;
-    float r = ((1.f + v) - sqrtf(dm * dm + sd)) / 2.f;
+    float r = elem_dual_kl(p[i], a, sigma[i], b[i]);
     if (mask) r = r * mask[i];
     p[i] = r;
   }
@@ -335,7 +329,7 @@ __global__ __launch_bounds__(kEwBlock) void sirt_residual_kernel(float* __restri
   GRID_STRIDE(i, n) {
     float a = ax[i];
     if (bg) a = a + bg[i];
-    float r = b[i] - a;
+    float r = elem_residual(a, b[i]);
     if (mask) r = r * mask[i];
     if (sigma) r = r * sigma[i];
     res[i] = r;
@@ -476,8 +470,7 @@ __global__ __launch_bounds__(kEwBlock) void mlem_ratio_kernel(float* __restrict_
   GRID_STRIDE(i, n) {
     float a = ax[i];
     if (bg) a = a + bg[i];
-    a = fmaxf(a, FLT_EPSILON);
-    r[i] = b[i] / a;
+    r[i] = elem_mlem_ratio(a, b[i]);
   }
 }
 

@@ -361,6 +361,35 @@ class ProjectorAttenuationXRF(ProjectorUncorrected):
             out = out[0]
         return _arrays.from_dev(out.contiguous(), host)
 
+    def supports_fused_data_step(self) -> bool:
+        """Whether ``fp_row_bp`` applies: on-the-fly maps, one slice, no detector PSF, and a bp that reads
+        the maps (the exact adjoint or the symmetric bp)."""
+        return (self.maps_mode == "otf" and self._n_slices == 1 and self.psf_vu is None
+                and (self.bp_mode == "adjoint" or self.is_symmetric))
+
+    def fp_row_bp(self, x, row_op: int, p, b, sigma=None, bg=None, b_mask=None, ax=None):
+        """One solver iteration's projector work with on-the-fly maps (``chip_otf_fp_row_bp``): per angle
+        block, the block's maps are built once, ``ax = fp(x)`` over the block, the row op updates the
+        block's rows of ``p`` (``enum chip_row_op``: the PDHG l2 / KL dual step, the SIRT residual or the
+        MLEM ratio), and the block's rows of ``p`` are back-projected. Equals ``fp``, the elementwise
+        kernel and ``bp`` in turn, with half the map building. Device tensors in the sinogram shape the
+        projector returns; ``p`` is updated in place. Returns (bp of p, ax)."""
+        if not self.supports_fused_data_step():
+            raise ValueError("fp_row_bp needs maps_mode='otf', a single slice, no PSF and a bp reading the maps")
+        n = int(self.vol_shape[-1])
+        xt = _arrays.to_dev(x)[0].contiguous()
+        if ax is None:
+            ax = torch.empty_like(p)
+        for t in (p, b, sigma, bg, b_mask, ax):
+            if t is not None and (not t.is_contiguous() or t.numel() != p.numel()):
+                raise ValueError("fp_row_bp takes contiguous device arrays of the sinogram's size")
+        out = torch.empty((n, n), dtype=torch.float32, device=xt.device)
+        mode = _native.CHIP_BP_ADJOINT if self.bp_mode == "adjoint" else _native.CHIP_BP_SKIMAGE_SYM
+        _native.call("chip_otf_fp_row_bp", self._plan()._h, xt.data_ptr(), _native.ptr(self.mu_in_dev),
+                     _native.ptr(self.mu_out_dev), 1, int(row_op), p.data_ptr(), _native.ptr(sigma), b.data_ptr(),
+                     _native.ptr(bg), _native.ptr(b_mask), ax.data_ptr(), out.data_ptr(), mode, _native.stream())
+        return out.reshape(tuple(self.vol_shape)), ax
+
     def bp(self, sino):
         t, host = _arrays.to_dev(sino)
         n = int(self.vol_shape[-1])

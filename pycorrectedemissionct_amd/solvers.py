@@ -45,6 +45,16 @@ def _dev(x, like: torch.Tensor | None = None):
     return t
 
 
+def _fused_step(A, *arrays) -> bool:
+    """Whether a solver iteration can run its projector work and data step as one on-the-fly
+    ``fp_row_bp`` call: an on-the-fly XRF projector and contiguous, full-size data arrays."""
+    ok = getattr(A, "supports_fused_data_step", None)
+    if ok is None or not ok():
+        return False
+    ref = arrays[0]
+    return all(t is None or (t.is_contiguous() and t.numel() == ref.numel()) for t in arrays)
+
+
 def _residual(res, b, ax, bg, b_mask, asig, scale, term, st) -> None:
     """``res = data_term.compute_residual(ax, b_mask) * scale`` on the device, for the l2-based terms and
     KL (``asig``: the sigma the term was assigned, None = 1)."""
@@ -602,14 +612,21 @@ class MLEM(Solver):
         if track:
             self._init_residuals(info, A, x, b_t, b_mask, b_test_mask)
         ratio = torch.empty_like(b_t)
+        # on-the-fly maps: fp, ratio and bp per angle block with the maps built once
+        fused = _fused_step(A, b_t, bg)
+        ax_buf = torch.empty_like(b_t) if fused else None
         for ii in range(iterations):
             info.iterations += 1
-            Ax = _arrays.to_dev(A(x))[0]
+            if fused:
+                upd, Ax = A.fp_row_bp(x, _native.CHIP_ROW_MLEM, ratio, b_t, None, bg, None, ax_buf)
+            else:
+                Ax = _arrays.to_dev(A(x))[0]
             if track and self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
                 break
-            _native.call("chip_mlem_ratio", ratio.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
-                         b_t.numel(), st)
-            upd = _arrays.to_dev(A.T(ratio))[0]
+            if not fused:
+                _native.call("chip_mlem_ratio", ratio.data_ptr(), b_t.data_ptr(), Ax.data_ptr(), _native.ptr(bg),
+                             b_t.numel(), st)
+                upd = _arrays.to_dev(A.T(ratio))[0]
             _native.call("chip_mlem_update", x.data_ptr(), upd.data_ptr(), tau.data_ptr(), hl, lo, hu, hi,
                          _native.ptr(lim.kernel_mask(x_mask)), x.numel(), st)
             lim.finish(x, x_mask)
@@ -666,17 +683,25 @@ class SIRT(Solver):
         batch = x.numel() // (n * n) if x.ndim >= 2 else 1
         q = torch.empty((batch, 2, n, n), dtype=torch.float32, device=x.device) if tv_w is not None else None
         grad_regs = self._grad_regs(x) if (tv_w is None and self.regularizer) else []
+        # on-the-fly maps with the l2 term: fp, residual and bp per angle block with the maps built once
+        fused = term[0] == data_terms.TERM_L2 and _fused_step(A, b_t, sigma, bg, b_mask)
+        ax_buf = torch.empty_like(b_t) if fused else None
         for ii in range(iterations):
             info.iterations += 1
-            Ax = _arrays.to_dev(A(x))[0]
+            if fused:
+                upd, Ax = A.fp_row_bp(x, _native.CHIP_ROW_SIRT, res, b_t, sigma, bg, b_mask, ax_buf)
+            else:
+                Ax = _arrays.to_dev(A(x))[0]
             if track and self._track(info, ii, Ax, b_t, b_mask, b_test_mask):
                 break
-            _residual(res, b_t, Ax, bg, b_mask, sigma, sigma, term, st)
+            if not fused:
+                _residual(res, b_t, Ax, bg, b_mask, sigma, sigma, term, st)
             # the reference re-initialises the regularizer duals every iteration (solvers.py:950-958)
             for cfg, q_r in grad_regs:
                 q_r.zero_()
                 self._grad_dual(cfg, q_r, x, st)
-            upd = _arrays.to_dev(A.T(res))[0]
+            if not fused:
+                upd = _arrays.to_dev(A.T(res))[0]
             if q is not None:
                 q.zero_()
                 _native.call("chip_tv_dual", q.data_ptr(), x.data_ptr(), n, batch, tv_w, st)
@@ -770,22 +795,30 @@ class PDHG(Solver):
         code, prm, period, g_len, g_stride = s["term"]
         dual = {data_terms.TERM_L2: "chip_pdhg_dual_l2", data_terms.TERM_KL: "chip_pdhg_dual_kl"}.get(code)
         track = info is not None and (s["b_test_mask"] is not None or self.tolerance is not None)
+        # on-the-fly maps with the l2 / KL term: fp, dual step and bp per angle block, maps built once
+        fused = dual is not None and _fused_step(A, b, s["sigma"], s["bg"], s["b_mask"], p)
+        row_op = _native.CHIP_ROW_PDHG_KL if code == data_terms.TERM_KL else _native.CHIP_ROW_PDHG_L2
+        ax_buf = torch.empty_like(b) if fused else None
         for ii in range(iterations):
             if info is not None:
                 info.iterations += 1
-            Ax = _arrays.to_dev(A(xr))[0]
-            if dual is not None:
-                _native.call(dual, p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(),
-                             _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), st)
+            if fused:
+                upd, _ = A.fp_row_bp(xr, row_op, p, b, s["sigma"], s["bg"], s["b_mask"], ax_buf)
             else:
-                _native.call("chip_pdhg_dual_norm", p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(),
-                             _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), code, _native.ptr(prm),
-                             period, g_len, g_stride, st)
+                Ax = _arrays.to_dev(A(xr))[0]
+                if dual is not None:
+                    _native.call(dual, p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(), b.data_ptr(),
+                                 _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), st)
+                else:
+                    _native.call("chip_pdhg_dual_norm", p.data_ptr(), Ax.data_ptr(), s["sigma"].data_ptr(),
+                                 b.data_ptr(), _native.ptr(s["bg"]), _native.ptr(s["b_mask"]), p.numel(), code,
+                                 _native.ptr(prm), period, g_len, g_stride, st)
             if q is not None:
                 _native.call("chip_tv_dual", q.data_ptr(), xr.data_ptr(), s["n"], s["batch"], s["tv_w"], st)
             for cfg, q_r in s["grad_regs"]:
                 self._grad_dual(cfg, q_r, xr, st)
-            upd = _arrays.to_dev(A.T(p))[0]
+            if not fused:
+                upd = _arrays.to_dev(A.T(p))[0]
             for cfg, q_r in s["grad_regs"]:
                 self._grad_primal(cfg, upd, q_r, False, st)
             if x_old is not None:

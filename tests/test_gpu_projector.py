@@ -347,6 +347,80 @@ def test_otf_maps_match_precomputed(pkg, n):
         pkg.ProjectorAttenuationXRF([n, n], ang, att_maps=maps, maps_mode="otf")
 
 
+def test_fused_otf_iterations_equal_separate_steps(pkg):
+    """On-the-fly maps with the per-angle-block data step (chip_otf_fp_row_bp): one call equals fp, the
+    elementwise kernel and bp called in turn, bit for bit (several angle blocks; the first one split into
+    bp parts), and PDHG-KL-TV / SIRT (mask, background) / MLEM iterations on the fused path equal the same
+    solvers on the separate on-the-fly path bit for bit and the oracle solvers to 1e-4."""
+    import torch
+
+    from oracle import solvers as osol
+
+    n, a = 96, 160
+    rng = np.random.default_rng(460)
+    ang = _angles(a)
+    f = _phantom(n, rng)
+    mu_in = (rng.random((n, n)) * 0.03).astype(np.float32)
+    mu_out = (rng.random((n, n)) * 0.05).astype(np.float32)
+    for det in ([np.pi / 2], [np.pi / 2, -np.pi / 2]):
+        kw = dict(att_in=mu_in, att_out=mu_out, angles_detectors_rad=det, bp_mode="adjoint", maps_mode="otf")
+        fused = pkg.ProjectorAttenuationXRF([n, n], ang, **kw)
+        sep = pkg.ProjectorAttenuationXRF([n, n], ang, **kw)
+        for prj in (fused, sep):
+            prj._plan().tune("otf_budget_mib", 16)  # several angle blocks (the first of ~90 with one detector)
+        sep.supports_fused_data_step = lambda: False
+        assert fused.supports_fused_data_step()
+        x = torch.as_tensor(f, device="cuda")
+        y = sep.fp(x)
+        b = torch.as_tensor(pkg.testing.poisson_data(y.cpu().numpy(), 1e3, rng), device="cuda")
+        sigma = torch.rand_like(b) + 0.5
+        bg = torch.full_like(b, 0.01)
+        mask = (torch.rand_like(b) > 0.1).float()
+        p0 = torch.rand_like(b) * 0.1
+        for op in (pkg._native.CHIP_ROW_PDHG_L2, pkg._native.CHIP_ROW_PDHG_KL, pkg._native.CHIP_ROW_SIRT,
+                   pkg._native.CHIP_ROW_MLEM):
+            p_f = p0.clone()
+            upd_f, ax_f = fused.fp_row_bp(x, op, p_f, b, sigma, bg, mask)
+            ax_s = sep.fp(x)
+            p_s = p0.clone()
+            name = {0: "chip_pdhg_dual_l2", 1: "chip_pdhg_dual_kl"}.get(op)
+            st = pkg._native.stream()
+            if name:
+                pkg._native.call(name, p_s.data_ptr(), ax_s.data_ptr(), sigma.data_ptr(), b.data_ptr(), bg.data_ptr(),
+                                 mask.data_ptr(), p_s.numel(), st)
+            elif op == pkg._native.CHIP_ROW_SIRT:
+                pkg._native.call("chip_sirt_residual", p_s.data_ptr(), b.data_ptr(), ax_s.data_ptr(), bg.data_ptr(),
+                                 mask.data_ptr(), sigma.data_ptr(), p_s.numel(), st)
+            else:
+                pkg._native.call("chip_mlem_ratio", p_s.data_ptr(), b.data_ptr(), ax_s.data_ptr(), bg.data_ptr(),
+                                 p_s.numel(), st)
+            upd_s = sep.bp(p_s)
+            ok = torch.equal(ax_f, ax_s) and torch.equal(p_f, p_s) and torch.equal(upd_f, upd_s)
+            print(f"D={len(det)} row op {op}: fused == separate: {ok}")
+            assert ok, op
+        bn = b.cpu().numpy().reshape(len(det), a, n)  # the oracle keeps the detector axis
+        reg = pkg.regularizers.Regularizer_TV2D(1e-2)
+        xf = pkg.PDHG(data_term="kl", regularizer=reg)(fused, b, iterations=15)[0]
+        xs = pkg.PDHG(data_term="kl", regularizer=pkg.regularizers.Regularizer_TV2D(1e-2))(sep, b, iterations=15)[0]
+        assert torch.equal(xf, xs)
+        pre = pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in, att_out=mu_out, angles_detectors_rad=det)
+        mp = pre.att_vol_angles.cpu().numpy()
+        A = lambda v: oxrf.fp(v, ang, mp)  # noqa: E731
+        At = lambda w: oxrf.bp_adjoint(w, ang, n, mp)  # noqa: E731
+        ref = osol.pdhg(A, At, bn, (n, n), 15, data_term="kl", tv_weight=1e-2)
+        e = rel_max(ref, xf.cpu().numpy())
+        bgn, mn = np.full(bn.shape, 0.01, np.float32), mask.cpu().numpy().reshape(bn.shape)
+        sf = pkg.SIRT()(fused, b, iterations=15, b_mask=mask)[0]
+        ss = pkg.SIRT()(sep, b, iterations=15, b_mask=mask)[0]
+        mf = pkg.MLEM(data_term=pkg.data_terms.DataFidelity_KL(background=0.01))(fused, b, iterations=15)[0]
+        ms = pkg.MLEM(data_term=pkg.data_terms.DataFidelity_KL(background=0.01))(sep, b, iterations=15)[0]
+        e_s = rel_max(osol.sirt(A, At, bn, (n, n), 15, b_mask=mn), sf.cpu().numpy())
+        e_m = rel_max(osol.mlem(A, At, bn, (n, n), 15, background=bgn), mf.cpu().numpy())
+        print(f"D={len(det)}: PDHG-KL-TV fused vs oracle {e:.2e}, SIRT {e_s:.2e}, MLEM {e_m:.2e}")
+        assert torch.equal(sf, ss) and torch.equal(mf, ms)
+        assert e < 1e-4 and e_s < 1e-4 and e_m < 1e-4
+
+
 def test_band_fp_paths(pkg):
     """fp_kernel 6 (image-space bands) on the paths other than the whole two-detector sinogram:
     no maps (one detector), three detectors (a two- and a one-detector pass), and the
