@@ -516,6 +516,15 @@ int chip_batch_fp_otf(chip_batch_plan* plan, const float* vol, const float* mu_i
                       float* sino, void* stream);
 int chip_batch_bp_otf(chip_batch_plan* plan, const float* sino, const float* mu_in, const float* mu_out,
                       float* vol, int mode, void* stream);
+
+/* The fused on-the-fly iteration step of chip_otf_fp_row_bp on every slice of
+ * the batch (packed like chip_batch_fp / chip_batch_bp; no background or mask):
+ * per slice and angle block, the maps once, the fp of x into ax, row_op on p's
+ * rows, the bp of p's rows into vol. Equals chip_batch_fp_otf, the elementwise
+ * kernel and chip_batch_bp_otf in turn, bit for bit. No reference counterpart. */
+int chip_batch_otf_fp_row_bp(chip_batch_plan* plan, const float* x, const float* mu_in, const float* mu_out,
+                             int row_op, float* p, const float* sigma, const float* b, float* ax, float* vol,
+                             int bp_mode, void* stream);
 /* Spatial solver updates per slice (chip_tv_dual, chip_pdhg_primal,
  * chip_sirt_update semantics); x, tau, masks packed like volumes. */
 int chip_batch_tv_dual(const chip_batch_plan* plan, float* q, const float* x, float lam, void* stream);

@@ -58,6 +58,8 @@ _SIGNATURES = {
     "chip_rdiv": [c_void_p, ctypes.c_float, ctypes.c_int64, c_void_p],
     "chip_att_maps": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_bin_images": [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p],
+    "chip_batch_otf_fp_row_bp": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_otf_fp_row_bp": [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p],
     "chip_fp": [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int, c_void_p],

@@ -200,6 +200,19 @@ class BatchedProjectorXRF:
         maps = None if self.bp_mode == "skimage" else self.maps
         return self.plan.bp(sino, maps, out, _BP_MODES[self.bp_mode])
 
+    def fused_data_step(self) -> bool:
+        """Whether ``fp_row_bp`` applies: on-the-fly maps and a bp that reads them."""
+        return self.maps_mode == "otf" and self.bp_mode != "skimage"
+
+    def fp_row_bp(self, x: torch.Tensor, row_op: int, p: torch.Tensor, b: torch.Tensor, sigma, ax: torch.Tensor,
+                  out: torch.Tensor) -> torch.Tensor:
+        """One iteration's projector work per slice and angle block with the maps built once
+        (``chip_batch_otf_fp_row_bp``): ax = fp(x), the row op on p, out = bp(p)."""
+        _native.call("chip_batch_otf_fp_row_bp", self.plan._h, _native.ptr(x), _native.ptr(self.mu_in),
+                     _native.ptr(self.mu_out), int(row_op), _native.ptr(p), _native.ptr(sigma), _native.ptr(b),
+                     _native.ptr(ax), _native.ptr(out), _BP_MODES[self.bp_mode], _native.stream())
+        return out
+
 
 # fp tile shapes of the batched kernel that give the same bits: LDS staging through registers or
 # gfx950's direct-to-LDS loads, each with the whole chunk box or only its footprint
@@ -324,12 +337,17 @@ class BatchedSIRT:
         hl, lo = _limit(lower_limit)
         hu, hi = _limit(upper_limit)
 
+        fused = A.fused_data_step()
+
         def step():
             st = _native.stream()
-            A.fp(s["x"], s["ax"])
-            _native.call("chip_sirt_residual", s["res"].data_ptr(), s["b"].data_ptr(), s["ax"].data_ptr(), None,
-                         None, s["sigma"].data_ptr(), s["b"].numel(), st)
-            A.bp(s["res"], s["upd"])
+            if fused:  # on-the-fly maps: fp, residual and bp per angle block, the maps built once
+                A.fp_row_bp(s["x"], _native.CHIP_ROW_SIRT, s["res"], s["b"], s["sigma"], s["ax"], s["upd"])
+            else:
+                A.fp(s["x"], s["ax"])
+                _native.call("chip_sirt_residual", s["res"].data_ptr(), s["b"].data_ptr(), s["ax"].data_ptr(), None,
+                             None, s["sigma"].data_ptr(), s["b"].numel(), st)
+                A.bp(s["res"], s["upd"])
             _native.call("chip_batch_sirt_update", A.plan._h, s["x"].data_ptr(), s["upd"].data_ptr(), None,
                          s["tau"].data_ptr(), hl, lo, hu, hi, None, st)
 
@@ -430,19 +448,25 @@ class BatchedPDHG:
         hl, lo = _limit(lower_limit)
         hu, hi = _limit(upper_limit)
         dual = "chip_pdhg_dual_kl" if self.data_term == "kl" else "chip_pdhg_dual_l2"
+        row_op = _native.CHIP_ROW_PDHG_KL if self.data_term == "kl" else _native.CHIP_ROW_PDHG_L2
+        fused = A.fused_data_step()
 
         def step():
             st = _native.stream()
-            A.fp(s["xbar"], s["ax"])
-            _native.call(dual, s["p"].data_ptr(), s["ax"].data_ptr(), s["sigma"].data_ptr(), s["b"].data_ptr(), None,
-                         None, s["p"].numel(), st)
+            if fused:  # on-the-fly maps: fp, dual step and bp per angle block, the maps built once
+                A.fp_row_bp(s["xbar"], row_op, s["p"], s["b"], s["sigma"], s["ax"], s["upd"])
+            else:
+                A.fp(s["xbar"], s["ax"])
+                _native.call(dual, s["p"].data_ptr(), s["ax"].data_ptr(), s["sigma"].data_ptr(), s["b"].data_ptr(),
+                             None, None, s["p"].numel(), st)
             if s["q"] is not None:
                 _native.call("chip_batch_tv_dual", A.plan._h, s["q"].data_ptr(), s["xbar"].data_ptr(),
                              float(self.tv_lambda), st)
             if s["groups"] is not None:
                 for cfg, start, n_el, qg in s["groups"]:
                     Solver._grad_dual(cfg, qg, s["xbar"][start:start + n_el], st)
-            A.bp(s["p"], s["upd"])
+            if not fused:
+                A.bp(s["p"], s["upd"])
             if s["groups"] is not None:
                 for cfg, start, n_el, qg in s["groups"]:
                     Solver._grad_primal(cfg, s["upd"][start:start + n_el], qg, False, st)

@@ -430,6 +430,35 @@ int chip_batch_bp_otf(chip_batch_plan* plan, const float* sino, const float* mu_
   return CHIP_OK;
 }
 
+int chip_batch_otf_fp_row_bp(chip_batch_plan* plan, const float* x, const float* mu_in, const float* mu_out,
+                             int row_op, float* p, const float* sigma, const float* b, float* ax, float* vol,
+                             int bp_mode, void* stream) {
+  clear_error();
+  CHIP_CHECK_ARG(plan != nullptr && x != nullptr && p != nullptr && b != nullptr && ax != nullptr && vol != nullptr,
+                 "NULL argument");
+  CHIP_CHECK_ARG(mu_in != nullptr || mu_out != nullptr, "no attenuation volumes were given");
+  CHIP_CHECK_ARG(row_op >= CHIP_ROW_PDHG_L2 && row_op <= CHIP_ROW_MLEM, "unknown row_op");
+  CHIP_CHECK_ARG(sigma != nullptr || row_op == CHIP_ROW_SIRT || row_op == CHIP_ROW_MLEM,
+                 "the PDHG row ops need sigma");
+  CHIP_CHECK_ARG(bp_mode == CHIP_BP_SKIMAGE_SYM || bp_mode == CHIP_BP_ADJOINT,
+                 "bp_mode must be CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT");
+  CHIP_CHECK_ARG(p != ax, "p and ax must be different buffers");
+  BatchPlan& bp = plan->b;
+  hipStream_t st = as_stream(stream);
+  std::vector<int> chunk;
+  int rc = batch_otf_prepare(bp, mu_in != nullptr, mu_out != nullptr, st, chunk);
+  if (rc != CHIP_OK) return rc;
+  for (const BatchSlice& sl : bp.h_slices) {
+    const int64_t so = sl.sino_off;
+    rc = otf_fp_row_bp(bp.size_plans[sl.size_idx]->p, x + sl.vol_off, mu_in ? mu_in + sl.vol_off : nullptr,
+                       mu_out ? mu_out + sl.vol_off : nullptr, 1, row_op, p + so, sigma ? sigma + so : nullptr,
+                       b + so, nullptr, nullptr, ax + so, vol + sl.vol_off, bp_mode, chunk[sl.size_idx], bp.d_scan,
+                       bp.d_mblock, st);
+    if (rc != CHIP_OK) return rc;
+  }
+  return CHIP_OK;
+}
+
 int chip_batch_tv_dual(const chip_batch_plan* plan, float* q, const float* x, float lam, void* stream) {
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && q != nullptr && x != nullptr, "NULL argument");

@@ -266,6 +266,10 @@ int otf_fp(const Plan& p, const float* vol, const float* mu_in, const float* mu_
            int chunk, float* scan, float* block, hipStream_t st);
 int otf_bp(const Plan& p, const float* sino, const float* mu_in, const float* mu_out, float* vol, int batch,
            int mode, int chunk, float* scan, float* block, hipStream_t st);
+// per angle block: maps, fp of x into ax, the row op (enum chip_row_op) on p's rows, bp of p's rows
+int otf_fp_row_bp(const Plan& p, const float* x, const float* mu_in, const float* mu_out, int batch, int op,
+                  float* pd, const float* sigma, const float* b, const float* bg, const float* mask, float* ax,
+                  float* vol, int mode, int chunk, float* scan, float* block, hipStream_t st);
 
 }  // namespace chip
 

@@ -96,7 +96,7 @@ static const char* kSymbols[] = {
     "chip_pdhg_primal",   "chip_sirt_residual", "chip_sirt_update", "chip_mlem_ratio",
     "chip_mlem_update",   "chip_sumsq",        "chip_sumabs",       "chip_plan_tune",
     "chip_fp_otf",        "chip_bp_otf",       "chip_otf_fp_row_bp",       "chip_batch_plan_create", "chip_batch_plan_destroy",
-    "chip_batch_info",    "chip_batch_tune",   "chip_batch_att_maps", "chip_batch_fp",
+    "chip_batch_info",    "chip_batch_tune",   "chip_batch_att_maps", "chip_batch_otf_fp_row_bp", "chip_batch_fp",
     "chip_batch_bp",      "chip_batch_tv_dual", "chip_batch_pdhg_primal", "chip_batch_sirt_update",
     "chip_nn_conv3x3",    "chip_nn_conv3x3_wgrad", "chip_nn_bias_grad", "chip_nn_flip_weights",
     "chip_nn_relu_mask",  "chip_nn_avgpool2",  "chip_nn_upsample2", "chip_nn_head_fwd",

@@ -230,6 +230,36 @@ def test_batched_pdhg_channel_coupling_equals_volume_solves(pkg, reg_name):
         assert e < 1e-4
 
 
+def test_batched_fused_otf_step_equals_separate_steps(pkg):
+    """On-the-fly batches run the iteration's projector work as one fused step per slice and angle
+    block (chip_batch_otf_fp_row_bp, the maps built once): graphed batched SIRT and PDHG (l2, KL, TV) on
+    it equal the same solvers with the separate fp / elementwise / bp steps bit for bit, with several
+    angle blocks per slice."""
+    import torch
+
+    a = 72
+    rng = np.random.default_rng(706)
+    ang = _angles(a)
+    sizes = [96, 64, 128]
+    mi, mo = _mus(rng, sizes)
+    kw = dict(angles_detectors_rad=DET, weights_detectors=W_DET, bp_mode="adjoint", maps_mode="otf")
+    fused = pkg.batch.BatchedProjectorXRF(sizes, ang, mi, mo, **kw)
+    sep = pkg.batch.BatchedProjectorXRF(sizes, ang, mi, mo, **kw)
+    for prj in (fused, sep):
+        prj.plan.tune("otf_budget_mib", 16)
+    sep.fused_data_step = lambda: False
+    assert fused.fused_data_step()
+    b = fused.fp(fused.pack_vol([pkg.testing.phantom_ellipses(n, 4, rng) for n in sizes]))
+    xf = pkg.batch.BatchedSIRT(use_graph=True)(fused, b, 12)
+    xs = pkg.batch.BatchedSIRT(use_graph=True)(sep, b, 12)
+    assert torch.equal(xf, xs)
+    for term in ("l2", "kl"):
+        xf = pkg.batch.BatchedPDHG(data_term=term, tv_lambda=1e-2, use_graph=True)(fused, b, 12)
+        xs = pkg.batch.BatchedPDHG(data_term=term, tv_lambda=1e-2, use_graph=True)(sep, b, 12)
+        print(f"batched fused otf PDHG-{term}: equal to separate steps: {bool(torch.equal(xf, xs))}")
+        assert torch.equal(xf, xs), term
+
+
 def test_batched_otf_equals_precomputed(pkg):
     """maps_mode='otf' (each slice's maps rebuilt per angle block inside every fp / bp; a 16 MiB
     budget gives the 96^2 and 128^2 slices two blocks) equals the precomputed-maps batch: fp bit
