@@ -216,19 +216,31 @@ class BatchedProjectorXRF:
 
 # fp tile shapes of the batched kernel that give the same bits: LDS staging through registers or
 # gfx950's direct-to-LDS loads, each with the whole chunk box or only its footprint
-FP_SHAPES = {"stream": dict(fp_direct=0, fp_trim=0), "stream_trim": dict(fp_direct=0, fp_trim=1),
-             "direct": dict(fp_direct=1, fp_trim=0), "direct_trim": dict(fp_direct=1, fp_trim=1)}
+# The batched fp's kernel shapes. Image tiles (the default family): a workgroup owns a tile of one
+# or two slices and loops over a group of angles; its kinds differ only in when the map tiles are
+# fetched and in the samples per loop trip, so every tile shape gives the same bits. The angle x
+# bin-block stream shapes form a second family (the same bits among themselves, within 1e-6 of the
+# tile shapes: their line sums run in another order).
+TILE_SHAPES = {"tile": dict(fp_tile=1, fp_tile_kind=1), "tile_ahead2": dict(fp_tile=1, fp_tile_kind=0),
+               "tile_staged": dict(fp_tile=1, fp_tile_kind=3)}
+STREAM_SHAPES = {"stream": dict(fp_tile=0, fp_direct=0, fp_trim=0),
+                 "stream_trim": dict(fp_tile=0, fp_direct=0, fp_trim=1),
+                 "direct": dict(fp_tile=0, fp_direct=1, fp_trim=0),
+                 "direct_trim": dict(fp_tile=0, fp_direct=1, fp_trim=1)}
+FP_SHAPES = {**TILE_SHAPES, **STREAM_SHAPES}
 
 
 def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8),
-             fp_shapes=tuple(FP_SHAPES)) -> dict:
-    """Pick the batched fp's tile shape and grid size and the bp's grid size by timing them on
-    ``A``'s batch: k workgroups per CU of a persistent grid, or 0 for one workgroup per work item;
-    the fp shapes of ``FP_SHAPES``.
+             fp_shapes=tuple(TILE_SHAPES)) -> dict:
+    """Pick the batched fp's kernel shape (and, for the stream shapes, its grid size) and the bp's
+    grid size by timing them on ``A``'s batch: k workgroups per CU of a persistent grid, or 0 for one
+    workgroup per work item; the fp shapes named in ``fp_shapes`` (keys of ``FP_SHAPES``; the image
+    tiles by default, whose grid follows from the batch).
 
     Returns ``{"fp_shape": s, "fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``;
-    the chosen values are applied to ``A.plan``. Every candidate computes every work item the same
-    way, so results do not depend on the choice (bit for bit), only the time does.
+    the chosen values are applied to ``A.plan``. The candidates of one family compute every work
+    item the same way, so within the default candidates results do not depend on the choice (bit
+    for bit), only the time does.
     """
     x = torch.rand(A.vol_elems, dtype=torch.float32, device=A.device)
     y = torch.empty(A.sino_elems, dtype=torch.float32, device=A.device)
@@ -250,7 +262,7 @@ def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6
     for shape in fp_shapes:
         for key, val in FP_SHAPES[shape].items():
             A.plan.tune(key, val)
-        for k in fp_candidates:
+        for k in (fp_candidates if shape in STREAM_SHAPES else (0,)):
             A.plan.tune("fp_wg_per_cu", k)
             fp_us[(shape, k)] = timed(lambda: A.fp(x, y))
     for k in bp_candidates:

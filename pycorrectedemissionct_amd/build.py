@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "libcorrct_hip.so"
 ARCH = os.environ.get("CORRCT_HIP_ARCH", "gfx950")
 
-SOURCES = ["plan.cpp", "projector.hip", "fp_stream.hip", "bp_adjoint.hip", "attenuation.hip", "solver_ops.hip", "batch.hip", "unet.hip", "grad_reg.hip", "bp_skimage_tile.hip", "fp_band.hip", "noalloc.hip"]
+SOURCES = ["plan.cpp", "projector.hip", "fp_stream.hip", "bp_adjoint.hip", "attenuation.hip", "solver_ops.hip", "batch.hip", "unet.hip", "grad_reg.hip", "bp_skimage_tile.hip", "fp_band.hip", "fp_tile.hip", "noalloc.hip"]
 
 COMMON_FLAGS = [
     "-O3",

@@ -406,6 +406,9 @@ int maps_block_angles(const Plan& p, bool in, bool out, bool with_block, size_t 
 int otf_fp(const Plan& p, const float* vol, const float* mu_in, const float* mu_out, float* sino, int batch,
            int chunk, float* scan, float* block, hipStream_t st) {
   const size_t nn = static_cast<size_t>(p.n) * p.n;
+  ScratchUse use(*p.fp_state, st);
+  const int rc0 = fp_range_ready(p, chunk, st);
+  if (rc0 != CHIP_OK) return rc0;
   for (int b = 0; b < batch; ++b) {
     const float* mi = mu_in ? mu_in + b * nn : nullptr;
     const float* mo = mu_out ? mu_out + b * nn : nullptr;
@@ -481,6 +484,9 @@ int otf_fp_row_bp(const Plan& p, const float* x, const float* mu_in, const float
                   float* vol, int mode, int chunk, float* scan, float* block, hipStream_t st) {
   const size_t nn = static_cast<size_t>(p.n) * p.n;
   const int64_t sino_n = static_cast<int64_t>(p.n_det) * p.n_angles * p.n;
+  ScratchUse use(*p.fp_state, st);
+  const int rc0 = fp_range_ready(p, chunk, st);
+  if (rc0 != CHIP_OK) return rc0;
   BpPartGuard guard(p, 1, mode, chunk, st);
   for (int sb = 0; sb < batch; ++sb) {
     const float* mi = mu_in ? mu_in + sb * nn : nullptr;

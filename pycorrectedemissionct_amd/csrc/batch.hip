@@ -139,6 +139,9 @@ void release(BatchPlan& b) {
   if (b.d_geom) (void)hipFree(b.d_geom);
   if (b.d_scan) (void)hipFree(b.d_scan);
   if (b.d_mblock) (void)hipFree(b.d_mblock);
+  if (b.d_tile_slots) (void)hipFree(b.d_tile_slots);
+  if (b.d_fp_part) (void)hipFree(b.d_fp_part);
+  if (b.fp_state && b.fp_state->done) (void)hipEventDestroy(b.fp_state->done);
   for (chip_plan* p : b.size_plans) chip_plan_destroy(p);
   b = BatchPlan();
 }
@@ -200,7 +203,22 @@ int create(BatchPlan& b, int n_slices, const int* sizes, int n_angles, const flo
     b.h_sino_start.push_back(sl.sino_off + static_cast<int64_t>(n_det) * n_angles * n);
     b.h_maps_start.push_back(sl.maps_off + static_cast<int64_t>(n_angles) * n_det * nn);
   }
+  // image-tile fp slots: the slices of each size in plan order
+  std::vector<int64_t> offs;
+  b.tile_group_start.assign(1, 0);
+  for (size_t g = 0; g < b.size_plans.size(); ++g) {
+    for (int s = 0; s < n_slices; ++s) {
+      const BatchSlice& sl = b.h_slices[s];
+      if (sl.size_idx != static_cast<int>(g)) continue;
+      b.h_tile_slot_slice.push_back(s);
+      offs.push_back(sl.vol_off);
+      offs.push_back(sl.maps_off);
+      offs.push_back(sl.sino_off);
+    }
+    b.tile_group_start.push_back(static_cast<int>(b.h_tile_slot_slice.size()));
+  }
   int rc;
+  if ((rc = upload(b.d_tile_slots, offs)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_slices, b.h_slices)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_item_start_fp, b.h_item_start_fp)) != CHIP_OK) return rc;
   if ((rc = upload(b.d_item_start_bpa, b.h_item_start_bpa)) != CHIP_OK) return rc;
@@ -284,6 +302,13 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
+  if (k == "fp_tile") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_tile must be 0 or 1");
+    plan->b.fp_tile = value;
+    return CHIP_OK;
+  }
+  if (k == "fp_tile_kind" || k == "fp_tile_group" || k == "fp_tile_pairs")
+    return tune_tile_shape(plan->b.fp_tile_shape, k, value);
   if (k == "fp_wg_per_cu") {
     CHIP_CHECK_ARG(value >= 0 && value <= 8, "fp_wg_per_cu must be in 0..8 (0: one workgroup per work item)");
     plan->b.fp_wg_per_cu = value;
@@ -347,7 +372,38 @@ int chip_batch_fp(const chip_batch_plan* plan, const float* vol, const float* ma
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && vol != nullptr && sino != nullptr, "NULL argument");
   CHIP_CHECK_ARG(aligned16(vol) && (maps == nullptr || aligned16(maps)), "vol / maps must be 16-byte aligned");
-  launch_fp_stream_batch(plan->b, vol, maps, sino, as_stream(stream));
+  const BatchPlan& b = plan->b;
+  const hipStream_t st = as_stream(stream);
+  bool tile = b.fp_tile != 0;
+  for (chip_plan* sp : b.size_plans) tile = tile && fp_tile_count(sp->p.n) <= kFpTileMaxTiles;
+  if (tile) {
+    // one image-tile launch per slice size, through one partial-sum scratch
+    ScratchUse use(*b.fp_state, st);
+    size_t need = 0;
+    for (size_t g = 0; g < b.size_plans.size(); ++g) {
+      const Plan& sp = b.size_plans[g]->p;
+      const int rc = fp_tile_table(sp, st);
+      if (rc != CHIP_OK) return rc;
+      need = std::max(need, fp_tile_part_bytes(sp, b.tile_group_start[g + 1] - b.tile_group_start[g], b.n_angles));
+    }
+    if (b.fp_part_bytes < need) {
+      CHIP_CHECK_ARG(!stream_capturing(st), "the image-tile fp needs " + std::to_string(need) +
+                                                " bytes of partial sums: run it once outside stream capture first");
+      auto& bm = const_cast<BatchPlan&>(b);
+      const int rc = ensure_buffer(bm.d_fp_part, bm.fp_part_bytes, need, st, "image-tile fp partial sums");
+      if (rc != CHIP_OK) return rc;
+    }
+    for (size_t g = 0; g < b.size_plans.size(); ++g) {
+      const int g0 = b.tile_group_start[g], ng = b.tile_group_start[g + 1] - g0;
+      if (ng == 0) continue;
+      TileSlots sl;
+      sl.offs = b.d_tile_slots + 3 * static_cast<size_t>(g0);
+      launch_fp_tile(b.size_plans[g]->p, vol, maps, sl, sino, ng, 0, b.n_angles, 0, b.d_fp_part, b.fp_tile_shape, st);
+    }
+    CHIP_LAUNCH_CHECK();
+    return CHIP_OK;
+  }
+  launch_fp_stream_batch(b, vol, maps, sino, st);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }

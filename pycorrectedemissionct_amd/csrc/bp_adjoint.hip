@@ -416,10 +416,6 @@ int bp_adjoint_parts(const Plan& p, bool maps, int max_parts) {
              : bp_parts<1, false>(tiles, p.n_angles, max_parts, p.bp_parts_tune);
 }
 
-static bool stream_capturing(hipStream_t st) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
-}
 
 BpPartGuard::BpPartGuard(const Plan& p, int batch, int mode, int n_ang, hipStream_t st) : p_(p), st_(st) {
   scr_.tune = p.bp_parts_tune;

@@ -71,6 +71,32 @@ struct BpPartState {
   bool used = false;
 };
 
+// is `st` being captured into a graph (or cannot tell)?
+inline bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// Where the slices of an image-tile fp launch live: slot b at the element
+// offsets offs[3 b], offs[3 b + 1], offs[3 b + 2] of its volume, maps and
+// sinogram, or (offs NULL) at b times the strides.
+struct TileSlots {
+  const int64_t* offs = nullptr;
+  int64_t vol_stride = 0, maps_stride = 0, sino_stride = 0;
+  __device__ __forceinline__ int64_t vol(int b) const { return offs ? offs[3 * b] : b * vol_stride; }
+  __device__ __forceinline__ int64_t maps(int b) const { return offs ? offs[3 * b + 1] : b * maps_stride; }
+  __device__ __forceinline__ int64_t sino(int b) const { return offs ? offs[3 * b + 2] : b * sino_stride; }
+};
+
+// image-tile fp launch shape: kernel kind (0: map tiles fetched one angle ahead
+// into registers, two samples per lane and trip; 1: ahead, one; 2: at staging,
+// two; 3: at staging, one), angles per workgroup, two slices per workgroup
+struct FpTileShape {
+  int kind = 1;
+  int group = 8;
+  int pairs = 1;
+};
+
 struct Plan {
   int n = 0;          // slice side N (square), also detector pixels
   int n_angles = 0;   // A
@@ -114,8 +140,21 @@ struct Plan {
   // kernel selection (chip_plan_tune): fp 0 = gather, 1 = LDS-staged,
   // 2 = LDS-staged with register prefetch, 3 = streamed per-detector tiles,
   // 4 = 3 staging only the footprint of interior chunks, 5 = 3 with 32-bin tiles,
-  // 6 = image-space bands (2..6 fall back to 1 when N % 4 != 0)
-  int fp_variant = 5;
+  // 6 = image-space bands (2..6 fall back to 1 when N % 4 != 0), 10 = image tiles
+  // looping over angle groups (fp_tile.hip; 5 where a slice has more than
+  // kFpTileMaxTiles tiles or the buffers are not 16-byte aligned)
+  int fp_variant = 10;
+  // image-tile fp shape (chip_plan_tune "fp_tile_kind", "fp_tile_group", "fp_tile_pairs")
+  FpTileShape fp_tile;
+  // the image-tile fp's bin table ([A][tiles] records, geometry only) and its
+  // partial sums ([batch][angles][tiles][bins][<= 2]), allocated by the first
+  // whole-sinogram fp that needs them (never under stream capture)
+  mutable int* d_fp_tab = nullptr;
+  mutable size_t fp_tab_bytes = 0;
+  mutable bool fp_tab_ok = false;  // built for the current radon table
+  mutable float* d_fp_tile = nullptr;
+  mutable size_t fp_tile_bytes = 0;
+  std::unique_ptr<BpPartState> fp_state{new BpPartState};  // users of d_fp_tile across streams
 };
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -125,6 +164,14 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // ---------------------------------------------------------------------------
 // detector bins per workgroup of the batched fp (the kernel's tile width)
 constexpr int kFpBatchBins = 32;
+// image-tile fp: cells per tile row / column, bin slots per (tile, angle) and
+// ints per (angle, tile) record of its bin table
+constexpr int kFpTileCols = 64;
+constexpr int kFpTileRows = 32;
+constexpr int kFpTileBins = 80;
+constexpr int kFpTileTab = kFpTileBins + 4;
+// tiles per slice the image-tile fp handles (its per-angle tile sort runs in LDS)
+constexpr int kFpTileMaxTiles = 8192;
 
 struct BatchSlice {
   int n;         // slice side
@@ -165,6 +212,17 @@ struct BatchPlan {
   size_t mblock_bytes = 0;
   size_t otf_budget = size_t(4) << 30;  // chip_batch_tune "otf_budget_mib"
   int device = 0;
+  // image-tile fp (chip_batch_tune "fp_tile" and its shape keys): one launch per
+  // slice size over that size's slots (tile_slots: (vol, maps, sino) offsets of
+  // each size's slices in plan order, size group g at tile_group_start[g])
+  int fp_tile = 1;
+  FpTileShape fp_tile_shape;
+  std::vector<int> tile_group_start;  // [sizes + 1]
+  int64_t* d_tile_slots = nullptr;
+  std::vector<int> h_tile_slot_slice;  // slice of each slot
+  float* d_fp_part = nullptr;
+  size_t fp_part_bytes = 0;
+  std::unique_ptr<BpPartState> fp_state{new BpPartState};
 };
 
 // largest s in [0, n_slices) with start[s] <= k (start is non-decreasing)
@@ -193,6 +251,38 @@ void launch_fp_stream(const Plan& p, const float* vol, const float* maps, int64_
 // (fp_band.hip); same preconditions as launch_fp_stream
 void launch_fp_band(const Plan& p, const float* vol, const float* maps, int64_t stride, float* sino, int batch,
                     int det0, int nd, int angle0, int n_ang, int maps_a0, hipStream_t st);
+
+// the same fp by image tiles, each workgroup looping over a group of angles
+// (fp_tile.hip): partial sums per (slot, angle, tile) in `part` (at least
+// fp_tile_part_bytes), then a fixed-order sum over the tiles of each line;
+// every detector, two per pass; same preconditions as launch_fp_stream.
+// fp_tile_table builds the plan's bin table once per radon table (an error
+// under stream capture when it is missing).
+int fp_tile_count(int n);
+int fp_tile_table(const Plan& p, hipStream_t st);
+size_t fp_tile_part_bytes(const Plan& p, int slots, int n_ang);
+void launch_fp_tile(const Plan& p, const float* vol, const float* maps, const TileSlots& sl, float* sino, int slots,
+                    int angle0, int n_ang, int maps_a0, float* part, const FpTileShape& shape, hipStream_t st);
+
+// One call's use of scratch shared across streams (the state's mutex held while
+// it lives): waits on `st` for the previous user on another stream, then records
+// this use (no event work while `st` is being captured).
+// chip_plan_tune / chip_batch_tune keys "fp_tile_kind", "fp_tile_group", "fp_tile_pairs"
+int tune_tile_shape(FpTileShape& sh, const std::string& k, int value);
+
+class ScratchUse {
+ public:
+  ScratchUse(BpPartState& s, hipStream_t st);
+  ~ScratchUse();
+  ScratchUse(const ScratchUse&) = delete;
+  ScratchUse& operator=(const ScratchUse&) = delete;
+
+ private:
+  BpPartState& s_;
+  hipStream_t st_;
+  std::unique_lock<std::mutex> lk_;
+  bool record_ = false;
+};
 
 // exact adjoint of the fp over angles [angle0, angle0 + n_ang), all detectors;
 // maps[0] holds angle maps_a0; accumulate = 1 adds to vol (bp_adjoint.hip)
@@ -235,7 +325,11 @@ class BpPartGuard {
 };
 
 // fp / bp (mode CHIP_BP_SKIMAGE_SYM or CHIP_BP_ADJOINT) over an angle range
-// with a maps block whose first angle is maps_a0 (projector.hip)
+// with a maps block whose first angle is maps_a0 (projector.hip); the fp runs
+// the image-tile kernel when the plan selects it and fp_range_ready has set it
+// up for ranges of up to `chunk` angles (the callers hold the plan's fp_state)
+bool fp_range_tile(const Plan& p);
+int fp_range_ready(const Plan& p, int chunk, hipStream_t st);
 void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
                      int maps_a0, hipStream_t st);
 void launch_bp_range(const Plan& p, const float* sino, const float* maps, float* vol, int mode, int angle0,

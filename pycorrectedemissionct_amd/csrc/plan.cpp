@@ -207,7 +207,10 @@ int chip_plan_destroy(chip_plan* plan) {
   if (p.d_scratch) (void)hipFree(p.d_scratch);
   if (p.d_mblock) (void)hipFree(p.d_mblock);
   if (p.d_bp_part) (void)hipFree(p.d_bp_part);
+  if (p.d_fp_tile) (void)hipFree(p.d_fp_tile);
+  if (p.d_fp_tab) (void)hipFree(p.d_fp_tab);
   if (p.bp_state && p.bp_state->done) (void)hipEventDestroy(p.bp_state->done);
+  if (p.fp_state && p.fp_state->done) (void)hipEventDestroy(p.fp_state->done);
   delete plan;
   return CHIP_OK;
 }
@@ -224,6 +227,7 @@ int chip_plan_set_radon_trig(chip_plan* plan, const float* cos_sin) {
   CHIP_HIP(hipDeviceSynchronize());  // queued work may still read the old tables
   for (int a = 0; a < p.n_angles; ++a) fill_radon_geom(p, a, cos_sin[2 * a], cos_sin[2 * a + 1]);
   CHIP_HIP(upload_radon_geom(p));
+  p.fp_tab_ok = false;  // the image-tile fp's bin table follows the radon table
   return CHIP_OK;
 }
 

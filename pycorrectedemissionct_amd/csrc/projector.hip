@@ -762,10 +762,29 @@ void launch_bp_sinogram(const Plan& p, const float* sino, const float* maps, int
   launch_bp(p, sino, maps, stride, vol, batch, mode, 0, p.n_angles, 0, 0, st, scr);
 }
 
+bool fp_range_tile(const Plan& p) {
+  return p.fp_variant == 10 && p.n % 4 == 0 && fp_tile_count(p.n) <= kFpTileMaxTiles;
+}
+
+int fp_range_ready(const Plan& p, int chunk, hipStream_t st) {
+  if (!fp_range_tile(p)) return CHIP_OK;
+  const int rc = fp_tile_table(p, st);
+  if (rc != CHIP_OK) return rc;
+  const size_t need = fp_tile_part_bytes(p, 1, chunk);
+  if (p.fp_tile_bytes >= need) return CHIP_OK;
+  CHIP_CHECK_ARG(!stream_capturing(st), "the image-tile fp needs " + std::to_string(need) +
+                                            " bytes of partial sums: run it once outside stream capture first");
+  return ensure_buffer(p.d_fp_tile, p.fp_tile_bytes, need, st, "image-tile fp partial sums");
+}
+
 void launch_fp_range(const Plan& p, const float* vol, const float* maps, float* sino, int angle0, int n_ang,
                      int maps_a0, hipStream_t st) {
   const bool vec_ok = (p.n % 4 == 0) && (reinterpret_cast<uintptr_t>(vol) % 16 == 0) &&
                       (reinterpret_cast<uintptr_t>(maps) % 16 == 0);
+  if (vec_ok && fp_range_tile(p) && p.fp_tab_ok && p.fp_tile_bytes >= fp_tile_part_bytes(p, 1, n_ang)) {
+    launch_fp_tile(p, vol, maps, TileSlots{}, sino, 1, angle0, n_ang, maps_a0, p.d_fp_tile, p.fp_tile, st);
+    return;
+  }
   if (vec_ok && p.fp_variant >= 3) {
     for (int det0 = 0; det0 < p.n_det; det0 += 2) {
       const int nd = p.n_det - det0 >= 2 ? 2 : 1;
@@ -794,6 +813,20 @@ void launch_bp_range(const Plan& p, const float* sino, const float* maps, float*
   launch_bp_skimage_tiles(p, sino, maps, 0, vol, 1, 1, angle0, n_ang, maps_a0, accumulate, st);
 }
 
+int tune_tile_shape(FpTileShape& sh, const std::string& k, int value) {
+  if (k == "fp_tile_kind") {
+    CHIP_CHECK_ARG(value >= 0 && value <= 3, "fp_tile_kind must be in 0..3");
+    sh.kind = value;
+  } else if (k == "fp_tile_group") {
+    CHIP_CHECK_ARG(value >= 1 && value <= 4096, "fp_tile_group must be in 1..4096");
+    sh.group = value;
+  } else {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "fp_tile_pairs must be 0 or 1");
+    sh.pairs = value;
+  }
+  return CHIP_OK;
+}
+
 }  // namespace chip
 
 using namespace chip;
@@ -809,7 +842,29 @@ int chip_fp(const chip_plan* plan, const float* vol, const float* maps, int64_t 
   CHIP_CHECK_ARG(maps_slice_stride >= 0, "maps_slice_stride must be >= 0");
   const Plan& p = plan->p;
   CHIP_CHECK_ARG(p.n_angles <= 65535, "too many angles for one launch");
-  launch_fp(p, vol, maps, maps_slice_stride, sino, batch, 0, p.n_angles, 0, as_stream(stream));
+  const hipStream_t st = as_stream(stream);
+  const bool vec_ok = (p.n % 4 == 0) && (maps_slice_stride % 4 == 0) &&
+                      (reinterpret_cast<uintptr_t>(vol) % 16 == 0) && (reinterpret_cast<uintptr_t>(maps) % 16 == 0);
+  if (p.fp_variant == 10 && vec_ok && fp_tile_count(p.n) <= kFpTileMaxTiles) {
+    ScratchUse use(*p.fp_state, st);
+    int rc = fp_tile_table(p, st);
+    if (rc != CHIP_OK) return rc;
+    const size_t need = fp_tile_part_bytes(p, batch, p.n_angles);
+    if (p.fp_tile_bytes < need) {
+      CHIP_CHECK_ARG(!stream_capturing(st), "the image-tile fp needs " + std::to_string(need) +
+                                                " bytes of partial sums: run it once outside stream capture first");
+      rc = ensure_buffer(p.d_fp_tile, p.fp_tile_bytes, need, st, "image-tile fp partial sums");
+      if (rc != CHIP_OK) return rc;
+    }
+    TileSlots sl;
+    sl.vol_stride = static_cast<int64_t>(p.n) * p.n;
+    sl.maps_stride = maps_slice_stride;
+    sl.sino_stride = static_cast<int64_t>(p.n_det) * p.n_angles * p.n;
+    launch_fp_tile(p, vol, maps, sl, sino, batch, 0, p.n_angles, 0, p.d_fp_tile, p.fp_tile, st);
+    CHIP_LAUNCH_CHECK();
+    return CHIP_OK;
+  }
+  launch_fp(p, vol, maps, maps_slice_stride, sino, batch, 0, p.n_angles, 0, st);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;
 }
@@ -847,10 +902,12 @@ int chip_plan_tune(chip_plan* plan, const char* key, int value) {
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
   if (k == "fp_kernel") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 9, "fp_kernel must be in 0..9");
+    CHIP_CHECK_ARG(value >= 0 && value <= 10, "fp_kernel must be in 0..10");
     plan->p.fp_variant = value;
     return CHIP_OK;
   }
+  if (k == "fp_tile_kind" || k == "fp_tile_group" || k == "fp_tile_pairs")
+    return tune_tile_shape(plan->p.fp_tile, k, value);
   if (k == "maps_exact") {
     CHIP_CHECK_ARG(value == 0 || value == 1, "maps_exact must be 0 or 1");
     plan->p.maps_exact = value;

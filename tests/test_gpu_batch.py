@@ -93,10 +93,20 @@ def test_autotune_changes_time_not_results(pkg):
     print("autotune:", res)
     assert f"{res['fp_shape']}/{res['fp_wg_per_cu']}" in res["fp_us"] and res["bp_wg_per_cu"] in res["bp_us"]
     assert torch.equal(A.fp(x), f1) and torch.equal(A.bp(y), b1)
-    for shape, keys in pkg.batch.FP_SHAPES.items():  # every tile shape the tuner may pick: the same bits
+    for shape, keys in pkg.batch.TILE_SHAPES.items():  # every shape the tuner may pick: the same bits
         for key, val in keys.items():
             A.plan.tune(key, val)
-        assert torch.equal(A.fp(x), f1), shape
+        for group, pairs in ((8, 1), (3, 0), (64, 1)):  # angle groups and slice pairing: the same bits
+            A.plan.tune("fp_tile_group", group)
+            A.plan.tune("fp_tile_pairs", pairs)
+            assert torch.equal(A.fp(x), f1), (shape, group, pairs)
+    for shape, keys in pkg.batch.STREAM_SHAPES.items():  # the other family: the same bits among themselves
+        for key, val in keys.items():
+            A.plan.tune(key, val)
+        if shape == "stream":
+            f2 = A.fp(x).clone()
+        assert torch.equal(A.fp(x), f2), shape
+        assert float((A.fp(x) - f1).abs().max() / f1.abs().max()) < 1e-6
     for key, val in (("fp_patch", 1), ("fp_direct", 1), ("fp_wg_per_cu", 0), ("bp_wg_per_cu", 0)):
         A.plan.tune(key, val)  # other kernel shapes and flat grids: results equal to 2e-7 (patch: sum order)
         assert float((A.fp(x) - f1).abs().max() / f1.abs().max()) < 1e-6

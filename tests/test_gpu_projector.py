@@ -80,7 +80,7 @@ def _xrf_case(pkg, n, a, rng, symmetric=False, bp_mode="skimage"):
     return prj, ang, maps, w_det, w_ang
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6, 10])
 @pytest.mark.parametrize("n,a", [(48, 36), (130, 20), (300, 16)])
 def test_xrf_fp_parity(pkg, kernel, n, a):
     rng = np.random.default_rng(300 + n)
