@@ -37,6 +37,17 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the kernels of one batched two-detector fp per autotuned shape (the tile shapes: a pair of slices
+# per workgroup, then the fixed-order sum of the tiles' partial sums)
+FP_KERNEL_NAMES = {
+    "tile": "fp_tile_kernel<2, 2, true, true, 1> + fp_tile_reduce_kernel<2>",
+    "tile_ahead2": "fp_tile_kernel<2, 2, true, true, 2> + fp_tile_reduce_kernel<2>",
+    "tile_staged": "fp_tile_kernel<2, 2, true, false, 1> + fp_tile_reduce_kernel<2>",
+    "stream": "fp_stream_batch_kernel<2, true, false, false>",
+    "stream_trim": "fp_stream_batch_kernel<2, true, true, false>",
+    "direct": "fp_direct_batch_kernel<2, false>",
+    "direct_trim": "fp_direct_batch_kernel<2, true>",
+}
 WEIGHTS = os.path.join(ROOT, "pycorrectedemissionct_amd", "data", "unet_pdhg_kl_512x720x2.npz")
 
 
@@ -242,13 +253,11 @@ def run_headline(args) -> dict:
     t_mlem = sync_time(lambda: mlem(prj, b0, iterations=300))
     log(f"MLEM 300 iterations, one slice: {t_mlem * 1e3:.1f} ms")
 
+    tile = tune["fp_shape"] in B.TILE_SHAPES
     kernels = {
-        "fp": dict(kernel={"stream": "fp_stream_batch_kernel<2, true, false, false>",
-                           "stream_trim": "fp_stream_batch_kernel<2, true, true, false>",
-                           "direct": "fp_direct_batch_kernel<2, false>",
-                           "direct_trim": "fp_direct_batch_kernel<2, true>"}[tune["fp_shape"]], t=t_fp, bytes=fp_bytes,
-                   gbs=fp_gbs,
-                   traffic_file="fp_batch_2x512x720x2_traffic.json"),
+        "fp": dict(kernel=FP_KERNEL_NAMES[tune["fp_shape"]], t=t_fp, bytes=fp_bytes, gbs=fp_gbs,
+                   traffic_file="fp_tile_batch_2x512x720x2_traffic.json" if tile
+                   else "fp_batch_2x512x720x2_traffic.json"),
         "bp": dict(kernel="bp_adjoint_batch_kernel<2, true>", t=t_bp, bytes=bp_bytes, gbs=bp_gbs,
                    traffic_file="bp_batch_2x512x720x2_traffic.json"),
     }
