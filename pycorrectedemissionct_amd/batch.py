@@ -237,7 +237,8 @@ def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6
     workgroup per work item; the fp shapes named in ``fp_shapes`` (keys of ``FP_SHAPES``; the image
     tiles by default, whose grid follows from the batch).
 
-    Returns ``{"fp_shape": s, "fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``;
+    Returns ``{"fp_shape": s, "fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``
+    (``bp_wg_per_cu`` "pairs": the exact-adjoint bp runs two slices of a size per workgroup);
     the chosen values are applied to ``A.plan``. The candidates of one family compute every work
     item the same way, so within the default candidates results do not depend on the choice (bit
     for bit), only the time does.
@@ -265,6 +266,11 @@ def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6
         for k in (fp_candidates if shape in STREAM_SHAPES else (0,)):
             A.plan.tune("fp_wg_per_cu", k)
             fp_us[(shape, k)] = timed(lambda: A.fp(x, y))
+    # exact-adjoint bp: two slices of a size per workgroup (one launch per size), or one slice per
+    # workgroup in one launch over every slice with k workgroups per CU; the same bits either way
+    A.plan.tune("bp_pairs", 1)
+    bp_us["pairs"] = timed(lambda: A.bp(y, v))
+    A.plan.tune("bp_pairs", 0)
     for k in bp_candidates:
         A.plan.tune("bp_wg_per_cu", k)
         bp_us[k] = timed(lambda: A.bp(y, v))
@@ -273,7 +279,9 @@ def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6
     for key, val in FP_SHAPES[best_shape].items():
         A.plan.tune(key, val)
     A.plan.tune("fp_wg_per_cu", best_fp)
-    A.plan.tune("bp_wg_per_cu", best_bp)
+    A.plan.tune("bp_pairs", 1 if best_bp == "pairs" else 0)
+    if best_bp != "pairs":
+        A.plan.tune("bp_wg_per_cu", best_bp)
     return {"fp_shape": best_shape, "fp_wg_per_cu": best_fp, "bp_wg_per_cu": best_bp,
             "fp_us": {f"{s}/{k}": t for (s, k), t in fp_us.items()}, "bp_us": bp_us}
 

@@ -302,6 +302,11 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
+  if (k == "bp_pairs") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "bp_pairs must be 0 or 1");
+    plan->b.bp_pairs = value;
+    return CHIP_OK;
+  }
   if (k == "fp_tile") {
     CHIP_CHECK_ARG(value == 0 || value == 1, "fp_tile must be 0 or 1");
     plan->b.fp_tile = value;

@@ -32,15 +32,30 @@ constexpr int kAngBlk = 32;
 constexpr int kGrp = 8;  // angles whose map values are loaded together
 constexpr int kWin = 40;  // >= 31 |c| + 7 |s| + 2 (|c| + |s|) + 2
 
-// One workgroup's work: the 32 x 8 pixel tile (tx, ty) of one slice.
-struct BpItem {
-  const float* sino;  // slice sinogram [D_total][A][n]
-  const float* maps;  // slice maps [A - maps_a0 ...][D_total][n][n], or NULL
-  float* vol;         // slice [n][n]
-  const float4* geom; // [A] of this slice size
+// One workgroup's work: the 32 x 8 pixel tile (tx, ty) of S slices of one size
+// (their candidate weights are shared, each slice has its own lines, maps and sums).
+template <int S = 1>
+struct BpItemS {
+  const float* sino[S];  // slice sinograms [D_total][A][n]
+  const float* maps[S];  // slice maps [A - maps_a0 ...][D_total][n][n], or NULL
+  float* vol[S];         // slices [n][n]
+  const float4* geom;    // [A] of this slice size
   int n;
   int tx, ty;
 };
+using BpItem = BpItemS<1>;
+
+// element i < S of a per-slice array without a dynamic index (which would move the
+// whole item to scratch memory)
+template <int S, class T>
+__device__ __forceinline__ T pick(const T (&a)[S], int i) {
+  if constexpr (S == 1) {
+    return a[0];
+  } else {
+    static_assert(S == 2, "one or two slices per workgroup");
+    return i == 0 ? a[0] : a[1];
+  }
+}
 
 // hat(t) = max(0, 1 - |t|): 1 - |t| <= 1, so clamping it to [0, 1] is the same
 // value, which the hardware applies as an output modifier of the subtraction
@@ -56,18 +71,18 @@ struct __align__(16) BpAngle {
   int rows_inside;
 };
 
-template <int D, bool MAPS>
-__device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float* __restrict__ w, int n_angles,
+template <int D, bool MAPS, int S = 1>
+__device__ __forceinline__ void bp_adjoint_block(const BpItemS<S>& itm, const float* __restrict__ w, int n_angles,
                                                  int n_det_total, int det0, int angle0, int n_ang, int maps_a0,
-                                                 int accumulate, float (*win)[D][kWin], BpAngle* ang_s) {
+                                                 int accumulate, float (*win)[S * D][kWin], BpAngle* ang_s) {
+  constexpr int SD = S * D;  // channels (slice major)
+  constexpr int kG = S == 1 ? kGrp : kGrp / 2;  // angles per group of map loads
   const int n = itm.n;
   const float4* __restrict__ geom = itm.geom;
   const int tid = threadIdx.x;
   const int j = itm.tx * kTileW + (tid & (kTileW - 1));
   const int i = itm.ty * kTileH + (tid / kTileW);
   const size_t nn = static_cast<size_t>(n) * n;
-  const float* sb = itm.sino;
-  const float* mb = itm.maps;
   const bool inside = i < n && j < n;
   const size_t pix = static_cast<size_t>(min(i, n - 1)) * n + min(j, n - 1);
   const float fj = static_cast<float>(j), fi = static_cast<float>(i);
@@ -75,7 +90,9 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
   const float tj0 = static_cast<float>(itm.tx * kTileW), tj1 = tj0 + (kTileW - 1);
   const float ti0 = static_cast<float>(itm.ty * kTileH), ti1 = ti0 + (kTileH - 1);
 
-  float out = 0.f;
+  float out[S];
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) out[sl] = 0.f;
   for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kAngBlk) {
     const int na = min(kAngBlk, angle0 + n_ang - a0);
     __syncthreads();
@@ -93,7 +110,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       const float ymax = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
       BpAngle& ba = ang_s[tid];
       const int wlo = static_cast<int>(floorf(umin - r)) - 1;  // first window bin
-      ba.wbase = static_cast<int>(sizeof(float)) * (tid * (D * kWin) - wlo);
+      ba.wbase = static_cast<int>(sizeof(float)) * (tid * (SD * kWin) - wlo);
       const float omr = 1.f - r;
       ba.au = omr - cs * g.z + sn * g.w;
       ba.ay = omr - sn * g.z - cs * g.w;
@@ -101,14 +118,15 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
     }
     __syncthreads();
-    for (int k = tid; k < na * D * kWin; k += 256) {
-      const int al = k / (D * kWin);
-      const int rem = k - al * (D * kWin);
-      const int d = rem / kWin;
-      const int q = rem - d * kWin;
-      const int u = al * (D * kWin) - ang_s[al].wbase / static_cast<int>(sizeof(float)) + q;
+    for (int k = tid; k < na * SD * kWin; k += 256) {
+      const int al = k / (SD * kWin);
+      const int rem = k - al * (SD * kWin);
+      const int c = rem / kWin;
+      const int q = rem - c * kWin;
+      const int sl = c / D, d = c - sl * D;
+      const int u = al * (SD * kWin) - ang_s[al].wbase / static_cast<int>(sizeof(float)) + q;
       // the line times its weight w[a, d], applied once per bin instead of once per pixel
-      win[al][d][q] = (u >= 0 && u < n) ? sb[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
+      win[al][c][q] = (u >= 0 && u < n) ? pick<S>(itm.sino, sl)[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
                                               w[static_cast<size_t>(a0 + al) * n_det_total + det0 + d]
                                         : 0.f;
     }
@@ -117,20 +135,22 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
       // map values of the next group of kGrp angles are loaded while the current group is
       // computed: with one pixel per thread the per-angle loads would otherwise expose the
       // full memory latency once per angle
-      float ma[kGrp][D], mb2[kGrp][D];
-      auto load_maps = [&](float (*m)[D], int g0) {
+      float ma[kG][SD], mb2[kG][SD];
+      auto load_maps = [&](float (*m)[SD], int g0) {
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k)
+        for (int k = 0; k < kG; ++k)
 #pragma unroll
-          for (int d = 0; d < D; ++d)
-            m[k][d] = (MAPS && g0 + k < na)
-                          ? mb[(static_cast<size_t>(a0 + g0 + k - maps_a0) * n_det_total + det0 + d) * nn + pix]
+          for (int c = 0; c < SD; ++c)
+            m[k][c] = (MAPS && g0 + k < na)
+                          ? itm.maps[c / D][(static_cast<size_t>(a0 + g0 + k - maps_a0) * n_det_total + det0 + c % D) *
+                                                nn +
+                                            pix]
                           : 1.f;
       };
       const char* wbytes = reinterpret_cast<const char*>(&win[0][0][0]);
-      auto angle_group = [&](const float (*m)[D], int g0) {
+      auto angle_group = [&](const float (*m)[SD], int g0) {
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k) {
+        for (int k = 0; k < kG; ++k) {
           const int al = g0 + k;
           if (al >= na) break;
           const float4 g = geom[a0 + al];
@@ -176,27 +196,30 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItem& itm, const float*
           const float* wq =
               reinterpret_cast<const float*>(wbytes + (ba.wbase + static_cast<int>(sizeof(float)) * static_cast<int>(fu0)));
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const float gd = fmaf(K[0], wq[d * kWin], fmaf(K[1], wq[d * kWin + 1], K[2] * wq[d * kWin + 2]));
-            out = MAPS ? fmaf(m[k][d], gd, out) : out + gd;
+          for (int c = 0; c < SD; ++c) {
+            const float gd = fmaf(K[0], wq[c * kWin], fmaf(K[1], wq[c * kWin + 1], K[2] * wq[c * kWin + 2]));
+            out[c / D] = MAPS ? fmaf(m[k][c], gd, out[c / D]) : out[c / D] + gd;
           }
         }
       };
       load_maps(ma, 0);
-      for (int g0 = 0; g0 < na; g0 += kGrp) {
-        if (g0 + kGrp < na) load_maps(mb2, g0 + kGrp);
+      for (int g0 = 0; g0 < na; g0 += kG) {
+        if (g0 + kG < na) load_maps(mb2, g0 + kG);
         angle_group(ma, g0);
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k)
+        for (int k = 0; k < kG; ++k)
 #pragma unroll
-          for (int d = 0; d < D; ++d) ma[k][d] = mb2[k][d];
+          for (int c = 0; c < SD; ++c) ma[k][c] = mb2[k][c];
       }
     }
   }
   if (inside) {
-    float* o = itm.vol + pix;
-    if (accumulate > 1) unsafeAtomicAdd(o, out);  // angle halves of a zeroed slice (see launch)
-    else *o = accumulate ? *o + out : out;
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) {
+      float* o = itm.vol[sl] + pix;
+      if (accumulate > 1) unsafeAtomicAdd(o, out[sl]);  // angle halves of a zeroed slice (see launch)
+      else *o = accumulate ? *o + out[sl] : out[sl];
+    }
   }
 }
 
@@ -238,9 +261,9 @@ __global__ __launch_bounds__(256) void bp_adjoint_tile_kernel(const float* __res
   }
   const size_t nn = static_cast<size_t>(n) * n;
   BpItem itm;
-  itm.sino = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
-  itm.maps = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
-  itm.vol = vol + static_cast<size_t>(b) * nn;
+  itm.sino[0] = sino + static_cast<size_t>(b) * n_det_total * n_angles * n;
+  itm.maps[0] = MAPS ? maps + static_cast<size_t>(b) * maps_slice_stride : nullptr;
+  itm.vol[0] = vol + static_cast<size_t>(b) * nn;
   itm.geom = geom;
   itm.n = n;
   itm.tx = blockIdx.x;
@@ -268,9 +291,9 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
     const int local = static_cast<int>(k - item_start[s]);
     const int ntx = (sl.n + kTileW - 1) / kTileW;
     BpItem itm;
-    itm.sino = sino + sl.sino_off;
-    itm.maps = MAPS ? maps + sl.maps_off : nullptr;
-    itm.vol = vol + sl.vol_off;
+    itm.sino[0] = sino + sl.sino_off;
+    itm.maps[0] = MAPS ? maps + sl.maps_off : nullptr;
+    itm.vol[0] = vol + sl.vol_off;
     itm.geom = geom + static_cast<size_t>(sl.size_idx) * n_angles;
     itm.n = sl.n;
     itm.ty = local / ntx;
@@ -278,6 +301,35 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
     bp_adjoint_block<D, MAPS>(itm, w, n_angles, n_det_total, det0, 0, n_angles, 0, accumulate, win, ang_s);
     __syncthreads();  // the next item restages the windows
   }
+}
+
+// Slices of one size at the given slots: blockIdx.z = unit, unit u serves the S
+// slots [slot0 + S u, slot0 + S u + S) (with S = 2 one workgroup per slice pair:
+// the candidate weights are computed once for both; every slice gets the bits
+// of the one-slice kernels).
+template <int D, bool MAPS, int S>
+__global__ __launch_bounds__(256) void bp_adjoint_slots_kernel(const float* __restrict__ sino,
+                                                               const float* __restrict__ maps,
+                                                               float* __restrict__ vol, TileSlots slots,
+                                                               int slot0, const float4* __restrict__ geom,
+                                                               const float* __restrict__ w, int n, int n_angles,
+                                                               int n_det_total, int det0, int angle0, int n_ang,
+                                                               int maps_a0, int accumulate) {
+  __shared__ float win[kAngBlk][S * D][kWin];
+  __shared__ BpAngle ang_s[kAngBlk];
+  BpItemS<S> itm;
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    const int b = slot0 + S * static_cast<int>(blockIdx.z) + sl;
+    itm.sino[sl] = sino + slots.sino(b);
+    itm.maps[sl] = MAPS ? maps + slots.maps(b) : nullptr;
+    itm.vol[sl] = vol + slots.vol(b);
+  }
+  itm.geom = geom;
+  itm.n = n;
+  itm.tx = blockIdx.x;
+  itm.ty = blockIdx.y;
+  bp_adjoint_block<D, MAPS, S>(itm, w, n_angles, n_det_total, det0, angle0, n_ang, maps_a0, accumulate, win, ang_s);
 }
 
 // out = the parts' sums added in part order (the same bits on every run)
@@ -346,6 +398,22 @@ void launch(const Plan& p, const float* sino, const float* maps, int64_t stride,
                        static_cast<const float*>(p.d_bp_part), parts, nn);
     return;
   }
+  if (batch >= 2) {
+    // two slices per workgroup (their candidate weights shared), an odd last slice alone
+    TileSlots sl;
+    sl.vol_stride = nn;
+    sl.maps_stride = stride;
+    sl.sino_stride = static_cast<int64_t>(p.n_det) * p.n_angles * p.n;
+    const int pairs = batch / 2;
+    hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps, vol,
+                       sl, 0, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
+                       accumulate);
+    if (batch % 2)
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, 1), dim3(256), 0, st, sino, maps, vol,
+                         sl, batch - 1, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
+                         accumulate);
+    return;
+  }
   // slices too large for the scratch: two halves added atomically into the zeroed output
   const int split = (batch == 1 && !accumulate && det0 == 0 && p.n_det <= 2 && n_ang >= 2 * kAngBlk) ? 1 : 0;
   if (split) {
@@ -369,9 +437,44 @@ void launch_batch(const BatchPlan& bp, const float* sino, const float* maps, flo
                      accumulate);
 }
 
+template <int D, bool MAPS>
+void launch_slots(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int det0, int acc,
+                  hipStream_t st) {
+  for (size_t g = 0; g < bp.size_plans.size(); ++g) {
+    const Plan& sp = bp.size_plans[g]->p;
+    const int g0 = bp.tile_group_start[g], ng = bp.tile_group_start[g + 1] - g0;
+    const int pairs = ng / 2;
+    const unsigned tx = (sp.n + kTileW - 1) / kTileW, ty = (sp.n + kTileH - 1) / kTileH;
+    TileSlots sl;
+    sl.offs = bp.d_tile_slots;
+    if (pairs)
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps,
+                         vol, sl, g0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles, 0, acc);
+    if (ng % 2)
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, 1), dim3(256), 0, st, sino, maps, vol,
+                         sl, g0 + 2 * pairs, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles,
+                         0, acc);
+  }
+}
+
 }  // namespace
 
 void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st) {
+  if (bp.bp_pairs) {
+    // per slice size, two slices per workgroup
+    for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
+      const bool two = bp.n_det - det0 >= 2;
+      const int acc = det0 > 0 ? 1 : 0;
+      if (maps) {
+        if (two) launch_slots<2, true>(bp, sino, maps, vol, det0, acc, st);
+        else launch_slots<1, true>(bp, sino, maps, vol, det0, acc, st);
+      } else {
+        if (two) launch_slots<2, false>(bp, sino, nullptr, vol, det0, acc, st);
+        else launch_slots<1, false>(bp, sino, nullptr, vol, det0, acc, st);
+      }
+    }
+    return;
+  }
   for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
     const bool two = bp.n_det - det0 >= 2;
     const int acc = det0 > 0 ? 1 : 0;

@@ -217,6 +217,8 @@ struct BatchPlan {
   // each size's slices in plan order, size group g at tile_group_start[g])
   int fp_tile = 1;
   FpTileShape fp_tile_shape;
+  // exact-adjoint bp: two slices of a size per workgroup (chip_batch_tune "bp_pairs")
+  int bp_pairs = 1;
   std::vector<int> tile_group_start;  // [sizes + 1]
   int64_t* d_tile_slots = nullptr;
   std::vector<int> h_tile_slot_slice;  // slice of each slot

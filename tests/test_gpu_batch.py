@@ -312,3 +312,28 @@ def test_batched_otf_equals_precomputed(pkg):
     e = rel_max(xp.cpu().numpy(), xo.cpu().numpy())
     print(f"batched PDHG on otf vs precomputed: rel_max {e:.2e}")
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("det", [DET, [np.pi / 2]])
+def test_batched_bp_slice_pairs_same_bits(pkg, det):
+    """The exact-adjoint bp with two slices of a size per workgroup (odd counts leave one slice alone)
+    gives the bits of one slice per workgroup, with and without maps."""
+    import torch
+
+    a = 40
+    rng = np.random.default_rng(707)
+    sizes = [64, 32, 64, 96, 64, 32]
+    ang = _angles(a)
+    mu_in, mu_out = _mus(rng, sizes)
+    A = pkg.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=det, bp_mode="adjoint")
+    y = A.pack_sino([rng.random((len(det), a, n)).astype(np.float32) for n in sizes])
+    A.plan.tune("bp_pairs", 0)
+    ref = A.bp(y).clone()
+    A.plan.tune("bp_pairs", 1)
+    assert torch.equal(A.bp(y), ref)
+    from oracle import xrf as oxrf
+
+    maps = A.slice_maps(3).cpu().numpy()
+    got = A.unpack_vol(A.bp(y))[3].cpu().numpy()
+    want = oxrf.bp_adjoint(A.unpack_sino(y)[3].cpu().numpy(), ang, sizes[3], maps)
+    assert float(np.abs(got - want).max() / np.abs(want).max()) < 1e-5
