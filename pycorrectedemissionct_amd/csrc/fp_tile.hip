@@ -27,7 +27,8 @@
 //    counted by exactly one tile);
 //  * eight lanes walk a bin's interval (rows lo + lane % 8 + 8 k); the
 //    sample's position, floors and blend weights serve all C channels; the
-//    eight lanes' sums are added in a fixed butterfly order and each (slice,
+//    eight lanes' sums are added in a fixed butterfly order (DPP lane moves)
+//    and each (slice,
 //    tile, angle) writes the partial sums of its <= 80 bins. The bin groups
 //    rotate over the waves from angle to angle, so no SIMD always gets the
 //    longest rays;
@@ -215,6 +216,19 @@ __global__ __launch_bounds__(256) void fp_tile_sort_kernel(const int* __restrict
     }
   }
   for (int i = threadIdx.x; i < n_tiles; i += 256) sorted[static_cast<size_t>(a) * n_tiles + i] = keys[i];
+}
+
+// v of the lane paired with this one inside its group of eight, by DPP lane moves
+// (no LDS traffic): xor 1 and xor 2 within a quad, then the mirror of the
+// eight-lane half row (lane i <-> 7 - i)
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_half_mirror(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
 }
 
 template <int C>
@@ -447,12 +461,12 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
           }
         }
       }
-      // fixed-order sum over the eight lanes of the bin
+      // fixed-order sum over the eight lanes of the bin (pairs, quads, then the two quads)
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        acc[c] += __shfl_xor(acc[c], 1);
-        acc[c] += __shfl_xor(acc[c], 2);
-        acc[c] += __shfl_xor(acc[c], 4);
+        acc[c] += dpp_xor1(acc[c]);
+        acc[c] += dpp_xor2(acc[c]);
+        acc[c] += dpp_half_mirror(acc[c]);
       }
       if (sub == 0) {
 #pragma unroll
@@ -503,15 +517,16 @@ __global__ __launch_bounds__(256) void fp_tile_reduce_kernel(const float* __rest
   float s[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) s[d] = 0.f;
-#pragma unroll 4
+  // branch-free: tiles whose window misses u read their own first slot and add 0, so
+  // eight loads per trip are in flight together
+#pragma unroll 8
   for (int i = i0; i < i1; ++i) {
     const unsigned k = keys[i];
     const int j = u - static_cast<int>(k >> 16);
-    if (static_cast<unsigned>(j) < static_cast<unsigned>(kNb)) {
-      const float* q = pr + (static_cast<size_t>(k & 0xffffu) * kNb + j) * D;
+    const bool in = static_cast<unsigned>(j) < static_cast<unsigned>(kNb);
+    const float* q = pr + (static_cast<size_t>(k & 0xffffu) * kNb + (in ? j : 0)) * D;
 #pragma unroll
-      for (int d = 0; d < D; ++d) s[d] += q[d];
-    }
+    for (int d = 0; d < D; ++d) s[d] += in ? q[d] : 0.f;
   }
   if (u >= n) return;
   float* out = sino + slots.sino(b);
