@@ -35,8 +35,8 @@
 //  * fp_tile_reduce_kernel adds the partial sums of a line in a fixed order
 //    (the tiles sorted by first bin) and applies w[a, d].
 // The result is bitwise reproducible; it differs from the sequential sum over
-// y only by the order of the additions (and the blend is written with
-// weights).
+// y by the order of the additions and by the blend, written as four weighted
+// taps accumulated with fused multiply-adds.
 #include "common.hpp"
 
 // the sample coordinates are skimage's separate float32 products and sums:
@@ -405,6 +405,7 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
     const float4 gm = geom[a];
     const float cs = gm.x, sn = gm.y, m02 = gm.z, m12 = gm.w;
     const int ub = tab[(static_cast<size_t>(a) * n_tiles + t) * kTab];
+    const int nb = tab[(static_cast<size_t>(a) * n_tiles + t) * kTab + 1];
     if (MAPS && !PF) issue(la);
     __syncthreads();  // the previous angle's gathers are done with the tile
     if (MAPS) stage();
@@ -422,19 +423,32 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
       const int gg = w + 4 * k;
       if (gg >= kGroups) break;
       const int j = 8 * gg + (lane >> 3);  // the lane group's bin (slot)
+      if (8 * gg >= nb) {
+        // no bin of the group meets the tile: its partial sums are 0
+        if (sub == 0) {
+#pragma unroll
+          for (int sl = 0; sl < S; ++sl) {
+            float* pr = part + ((static_cast<size_t>(b0 + sl) * n_ang + la) * n_tiles + t) * (kNb * D);
+            if (D == 2) reinterpret_cast<float2*>(pr)[j] = make_float2(0.f, 0.f);
+            else pr[j] = 0.f;
+          }
+        }
+        continue;
+      }
       const int ylo = rcur[k] & 0xffff, cnt = rcur[k] >> 16;
       const float uf = static_cast<float>(ub + j);
       const float xu = mul_rn(cs, uf), yu = mul_rn(-sn, uf);
       float acc[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) acc[c] = 0.f;
-      for (int i0 = sub; i0 < cnt; i0 += 8 * U) {
+      // rows counted in float (exact below 2^24)
+      const float ylast = static_cast<float>(ylo + cnt - 1);
+      for (float yf0 = static_cast<float>(ylo + sub); yf0 <= ylast; yf0 += 8.f * U) {
 #pragma unroll
         for (int q = 0; q < U; ++q) {
           // samples past the interval repeat its last row with zero weights
-          const bool in = q == 0 || i0 + 8 * q < cnt;
-          const int y = ylo + min(i0 + 8 * q, cnt - 1);
-          const float yf = static_cast<float>(y);
+          const bool in = q == 0 || yf0 + 8.f * q <= ylast;
+          const float yf = q == 0 ? yf0 : fminf(yf0 + 8.f * q, ylast);
           const float xs = add_rn(add_rn(xu, mul_rn(sn, yf)), m02);
           const float ys = add_rn(add_rn(yu, mul_rn(cs, yf)), m12);
           const float c0f = floorf(xs);
@@ -453,11 +467,10 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
           const float w00 = (one - dcw) - w10;
 #pragma unroll
           for (int c = 0; c < C; ++c) {
-            float v = w00 * ch(t00, c);
-            v = fmaf(w01, ch(t01, c), v);
-            v = fmaf(w10, ch(t10, c), v);
-            v = fmaf(w11, ch(t11, c), v);
-            acc[c] += v;
+            acc[c] = fmaf(w00, ch(t00, c), acc[c]);
+            acc[c] = fmaf(w01, ch(t01, c), acc[c]);
+            acc[c] = fmaf(w10, ch(t10, c), acc[c]);
+            acc[c] = fmaf(w11, ch(t11, c), acc[c]);
           }
         }
       }
