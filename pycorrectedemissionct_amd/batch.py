@@ -230,7 +230,7 @@ STREAM_SHAPES = {"stream": dict(fp_tile=0, fp_direct=0, fp_trim=0),
 FP_SHAPES = {**TILE_SHAPES, **STREAM_SHAPES}
 
 
-def autotune(A: BatchedProjectorXRF, reps: int = 3, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8),
+def autotune(A: BatchedProjectorXRF, reps: int = 5, fp_candidates=(0, 2, 4, 5, 6), bp_candidates=(0, 2, 4, 8),
              fp_shapes=tuple(TILE_SHAPES)) -> dict:
     """Pick the batched fp's kernel shape (and, for the stream shapes, its grid size) and the bp's
     grid size by timing them on ``A``'s batch: k workgroups per CU of a persistent grid, or 0 for one
