@@ -248,14 +248,24 @@ int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const fl
  * prefetch), 3 (streamed LDS tiles of 64 bins x 32 rows, the detectors'
  * products interleaved), 4 (3 staging only the footprint of interior chunks:
  * less HBM traffic, no faster), 5 (3 with 32-bin tiles: a third of the LDS,
- * twice the workgroups per CU; the default) or 6 (workgroups walking
+ * twice the workgroups per CU) or 6 (workgroups walking
  * image-space bands of 16 rows or 32 columns that stage each map value once:
  * 27 % less HBM traffic than 5, slower at 512^2), 7 (5 with the slice and map
  * boxes staged by gfx950's direct-to-LDS buffer loads, the products formed at
  * the gathers; the same bits as 5), 8 (7 staging only the footprint of
  * interior chunks) or 9 (5 with each wave gathering 8 x 8 sample patches; sums
- * in another order); 2..9 need N % 4 == 0 and 16-byte aligned vol / maps,
- * else 1 runs. "bp_parts" = -1 (default: from
+ * in another order) or 10 (the default: workgroups owning a 64 x 32 image
+ * tile of one or two slices and a group of angles, each map value read once,
+ * partial line sums per tile added in a fixed order by a second kernel; a bin
+ * table per plan built by the first whole-sinogram fp, partial sums in plan
+ * scratch, so under stream capture it needs one earlier eager call; results
+ * within 1e-6 of 5); 2..10 need N % 4 == 0 and 16-byte aligned vol / maps,
+ * else 1 runs (10 with more than 8192 tiles per slice runs 5). The shape of
+ * 10: "fp_tile_kind" 0..3 (0: the next angle's map tiles fetched into
+ * registers, two samples per lane and trip; 1: the same, one sample, default;
+ * 2 / 3: map tiles loaded at staging, two / one), "fp_tile_group" (angles per
+ * workgroup, default 8), "fp_tile_pairs" (1: two slices of a batch per
+ * workgroup, default); every shape gives the same bits. "bp_parts" = -1 (default: from
  * the adjoint bp kernel's occupancy), 0 (never split into parts) or 2..8
  * (pinned: the same bits on every device). "otf_budget_mib" bounds the
  * on-the-fly scratch.
@@ -489,12 +499,20 @@ int chip_batch_plan_destroy(chip_batch_plan* plan);
 int chip_batch_set_radon_trig(chip_batch_plan* plan, const float* cos_sin);
 /* Total elements of the packed volume, sinogram and maps buffers (NULL outputs are skipped). */
 int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* sino_elems, int64_t* maps_elems);
-/* Grid sizes: "fp_wg_per_cu" (0..8, default 5) and "bp_wg_per_cu" (0..16,
- * default 4) workgroups per CU of a persistent grid, 0 = one workgroup per work
- * item; fp tile variants "fp_trim" (stage only the footprint of interior
+/* "fp_tile" (1, default: the image-tile fp of chip_plan_tune's fp_kernel 10,
+ * one launch per slice size over the batch-plan's slots with its partial sums
+ * in batch-plan scratch grown on first use; 0: the angle x bin-block kernels
+ * below) with its shape keys "fp_tile_kind", "fp_tile_group", "fp_tile_pairs";
+ * "bp_pairs" (1, default: the exact-adjoint bp runs two slices of a size per
+ * workgroup, one launch per size; 0: one slice per workgroup in one launch over
+ * every slice; the same bits). Grid sizes of the one-launch kernels:
+ * "fp_wg_per_cu" (0..8, default 5) and "bp_wg_per_cu" (0..16, default 4)
+ * workgroups per CU of a persistent grid, 0 = one workgroup per work item; fp
+ * variants of fp_tile 0: "fp_trim" (stage only the footprint of interior
  * chunks), "fp_direct" (direct-to-LDS staging) and "fp_patch" (8 x 8 sample
- * patches per wave), each 0 / 1, default 0; set by the autotuner. "otf_budget_mib" (>= 16,
- * default 4096) bounds the on-the-fly projectors' scratch. */
+ * patches per wave), each 0 / 1, default 0; set by the autotuner.
+ * "otf_budget_mib" (>= 16, default 4096) bounds the on-the-fly projectors'
+ * scratch. */
 int chip_batch_tune(chip_batch_plan* plan, const char* key, int value);
 /* Maps of every slice (chip_att_maps per slice; mu_* packed like volumes), through a prefix-sum
  * scratch of at most 1 GiB shared by all sizes. */
