@@ -141,6 +141,8 @@ void release(BatchPlan& b) {
   if (b.d_mblock) (void)hipFree(b.d_mblock);
   if (b.d_tile_slots) (void)hipFree(b.d_tile_slots);
   if (b.d_fp_part) (void)hipFree(b.d_fp_part);
+  if (b.d_bp_part) (void)hipFree(b.d_bp_part);
+  if (b.bp_state && b.bp_state->done) (void)hipEventDestroy(b.bp_state->done);
   if (b.fp_state && b.fp_state->done) (void)hipEventDestroy(b.fp_state->done);
   for (chip_plan* p : b.size_plans) chip_plan_destroy(p);
   b = BatchPlan();
@@ -302,6 +304,11 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
   clear_error();
   CHIP_CHECK_ARG(plan != nullptr && key != nullptr, "NULL argument");
   const std::string k(key);
+  if (k == "bp_pair_parts") {
+    CHIP_CHECK_ARG(value >= 0 && value <= kBpMaxParts, "bp_pair_parts must be in 0..8 (0: from occupancy)");
+    plan->b.bp_pair_parts = value;
+    return CHIP_OK;
+  }
   if (k == "bp_pairs") {
     CHIP_CHECK_ARG(value == 0 || value == 1, "bp_pairs must be 0 or 1");
     plan->b.bp_pairs = value;
@@ -419,7 +426,18 @@ int chip_batch_bp(const chip_batch_plan* plan, const float* sino, const float* m
   CHIP_CHECK_ARG(plan != nullptr && vol != nullptr && sino != nullptr, "NULL argument");
   CHIP_CHECK_ARG(mode >= CHIP_BP_SKIMAGE && mode <= CHIP_BP_ADJOINT, "unknown bp mode");
   hipStream_t st = as_stream(stream);
-  if (mode == CHIP_BP_ADJOINT) launch_bp_adjoint_batch(plan->b, sino, maps, vol, st);
+  if (mode == CHIP_BP_ADJOINT) {
+    const BatchPlan& b = plan->b;
+    if (b.bp_pairs) {
+      ScratchUse use(*b.bp_state, st);
+      int parts = 1;
+      const int rc = bp_pair_parts_ready(b, maps != nullptr, st, &parts);
+      if (rc != CHIP_OK) return rc;
+      launch_bp_adjoint_batch(b, sino, maps, vol, st, parts);
+    } else {
+      launch_bp_adjoint_batch(b, sino, maps, vol, st, 1);
+    }
+  }
   else launch_bp_skimage_batch(plan->b, sino, maps, vol, mode == CHIP_BP_SKIMAGE_SYM ? 1 : 0, st);
   CHIP_LAUNCH_CHECK();
   return CHIP_OK;

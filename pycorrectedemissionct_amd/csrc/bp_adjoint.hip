@@ -314,16 +314,26 @@ __global__ __launch_bounds__(256) void bp_adjoint_slots_kernel(const float* __re
                                                                int slot0, const float4* __restrict__ geom,
                                                                const float* __restrict__ w, int n, int n_angles,
                                                                int n_det_total, int det0, int angle0, int n_ang,
-                                                               int maps_a0, int accumulate) {
+                                                               int maps_a0, int accumulate, int parts,
+                                                               float* __restrict__ part, int64_t part_plane) {
   __shared__ float win[kAngBlk][S * D][kWin];
   __shared__ BpAngle ang_s[kAngBlk];
+  // blockIdx.z = unit x parts + part: with parts > 1 each part sums its share of the angle
+  // range into its own plane of `part` (laid out like the volume), added in part order later
+  const int unit = static_cast<int>(blockIdx.z) / parts, pi = static_cast<int>(blockIdx.z) - unit * parts;
+  if (parts > 1) {
+    const int lo = n_ang * pi / parts, hi = n_ang * (pi + 1) / parts;
+    angle0 += lo;
+    n_ang = hi - lo;
+    accumulate = 0;
+  }
   BpItemS<S> itm;
 #pragma unroll
   for (int sl = 0; sl < S; ++sl) {
-    const int b = slot0 + S * static_cast<int>(blockIdx.z) + sl;
+    const int b = slot0 + S * unit + sl;
     itm.sino[sl] = sino + slots.sino(b);
     itm.maps[sl] = MAPS ? maps + slots.maps(b) : nullptr;
-    itm.vol[sl] = vol + slots.vol(b);
+    itm.vol[sl] = parts > 1 ? part + pi * part_plane + slots.vol(b) : vol + slots.vol(b);
   }
   itm.geom = geom;
   itm.n = n;
@@ -339,6 +349,16 @@ __global__ void sum_parts_kernel(float* __restrict__ out, const float* __restric
   float acc = part[i];
   for (int s = 1; s < n_parts; ++s) acc += part[s * nn + i];
   out[i] = acc;
+}
+
+// out[i] = (accumulate ? out[i] : 0) + the parts' sums in part order, over a packed volume
+__global__ void sum_parts_acc_kernel(float* __restrict__ out, const float* __restrict__ part, int n_parts,
+                                     int64_t total, int accumulate) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  float acc = part[i];
+  for (int s = 1; s < n_parts; ++s) acc += part[s * total + i];
+  out[i] = accumulate ? out[i] + acc : acc;
 }
 
 // Parts of one slice's angle range (2 .. max_parts) that make its tiles x parts
@@ -407,11 +427,11 @@ void launch(const Plan& p, const float* sino, const float* maps, int64_t stride,
     const int pairs = batch / 2;
     hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps, vol,
                        sl, 0, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
-                       accumulate);
+                       accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
     if (batch % 2)
       hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, 1), dim3(256), 0, st, sino, maps, vol,
                          sl, batch - 1, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
-                         accumulate);
+                         accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
     return;
   }
   // slices too large for the scratch: two halves added atomically into the zeroed output
@@ -437,9 +457,38 @@ void launch_batch(const BatchPlan& bp, const float* sino, const float* maps, flo
                      accumulate);
 }
 
+// Angle parts of the paired batch bp: a paired workgroup walks every angle of its tile,
+// and with one workgroup per tile and slice pair too few loads are in flight to keep HBM
+// busy (a read-only kernel of that shape reaches 2.1 TB/s); splitting the angle range into
+// parts multiplies the workgroups. The count aims at about eight waves of what the GPU holds
+// at once (at most 8 parts of at least 16 angles each: 2 x 512^2 x 720 x 2 detectors 747 ->
+// 654 us with 6 parts), unless "bp_pair_parts" pins it; the count follows the kernel's
+// occupancy and the CU count, and with it the grouping of the sums.
+template <int D, bool MAPS>
+int pair_parts(const BatchPlan& bp) {
+  if (bp.bp_pair_parts >= 1) return bp.bp_pair_parts;
+  static const int per_cu = [] {
+    int k = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, bp_adjoint_slots_kernel<D, MAPS, 2>, 256, 0) == hipSuccess
+               ? k
+               : 0;
+  }();
+  const int64_t resident = static_cast<int64_t>(per_cu) * bp.n_cu;
+  int64_t wg = 0;
+  for (size_t g = 0; g < bp.size_plans.size(); ++g) {
+    const int n = bp.size_plans[g]->p.n;
+    const int ng = bp.tile_group_start[g + 1] - bp.tile_group_start[g];
+    wg += static_cast<int64_t>((n + kTileW - 1) / kTileW) * ((n + kTileH - 1) / kTileH) * ((ng + 1) / 2);
+  }
+  if (resident <= 0 || wg <= 0) return 1;
+  const int64_t want = (8 * resident + wg - 1) / wg;
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({want, kBpMaxParts, bp.n_angles / 16})));
+}
+
 template <int D, bool MAPS>
 void launch_slots(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int det0, int acc,
-                  hipStream_t st) {
+                  int parts, float* part, hipStream_t st) {
+  const int64_t plane = bp.h_vol_start.back();
   for (size_t g = 0; g < bp.size_plans.size(); ++g) {
     const Plan& sp = bp.size_plans[g]->p;
     const int g0 = bp.tile_group_start[g], ng = bp.tile_group_start[g + 1] - g0;
@@ -448,29 +497,51 @@ void launch_slots(const BatchPlan& bp, const float* sino, const float* maps, flo
     TileSlots sl;
     sl.offs = bp.d_tile_slots;
     if (pairs)
-      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps,
-                         vol, sl, g0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles, 0, acc);
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs * parts), dim3(256), 0, st, sino,
+                         maps, vol, sl, g0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles, 0,
+                         acc, parts, part, plane);
     if (ng % 2)
-      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, 1), dim3(256), 0, st, sino, maps, vol,
-                         sl, g0 + 2 * pairs, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles,
-                         0, acc);
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, parts), dim3(256), 0, st, sino, maps,
+                         vol, sl, g0 + 2 * pairs, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0,
+                         sp.n_angles, 0, acc, parts, part, plane);
   }
+  if (parts > 1)
+    hipLaunchKernelGGL(sum_parts_acc_kernel, dim3(static_cast<unsigned>((plane + 255) / 256)), dim3(256), 0, st, vol,
+                       static_cast<const float*>(part), parts, plane, acc);
 }
 
 }  // namespace
 
-void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st) {
+int bp_pair_parts_ready(const BatchPlan& bp, bool maps, hipStream_t st, int* parts_out) {
+  const bool two = bp.n_det >= 2;
+  int parts = maps ? (two ? pair_parts<2, true>(bp) : pair_parts<1, true>(bp))
+                   : (two ? pair_parts<2, false>(bp) : pair_parts<1, false>(bp));
+  *parts_out = parts;
+  if (parts <= 1) return CHIP_OK;
+  const size_t need = sizeof(float) * static_cast<size_t>(parts) * bp.h_vol_start.back();
+  if (bp.bp_part_bytes >= need) return CHIP_OK;
+  if (stream_capturing(st)) {
+    set_error("the paired batch bp needs " + std::to_string(need) +
+              " bytes of part sums: run it once outside stream capture first");
+    return CHIP_EINVAL;
+  }
+  auto& bm = const_cast<BatchPlan&>(bp);
+  return ensure_buffer(bm.d_bp_part, bm.bp_part_bytes, need, st, "paired bp part sums");
+}
+
+void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st,
+                             int parts) {
   if (bp.bp_pairs) {
-    // per slice size, two slices per workgroup
+    // per slice size, two slices per workgroup, the angle range in `parts` parts
     for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
       const bool two = bp.n_det - det0 >= 2;
       const int acc = det0 > 0 ? 1 : 0;
       if (maps) {
-        if (two) launch_slots<2, true>(bp, sino, maps, vol, det0, acc, st);
-        else launch_slots<1, true>(bp, sino, maps, vol, det0, acc, st);
+        if (two) launch_slots<2, true>(bp, sino, maps, vol, det0, acc, parts, bp.d_bp_part, st);
+        else launch_slots<1, true>(bp, sino, maps, vol, det0, acc, parts, bp.d_bp_part, st);
       } else {
-        if (two) launch_slots<2, false>(bp, sino, nullptr, vol, det0, acc, st);
-        else launch_slots<1, false>(bp, sino, nullptr, vol, det0, acc, st);
+        if (two) launch_slots<2, false>(bp, sino, nullptr, vol, det0, acc, parts, bp.d_bp_part, st);
+        else launch_slots<1, false>(bp, sino, nullptr, vol, det0, acc, parts, bp.d_bp_part, st);
       }
     }
     return;

@@ -217,8 +217,14 @@ struct BatchPlan {
   // each size's slices in plan order, size group g at tile_group_start[g])
   int fp_tile = 1;
   FpTileShape fp_tile_shape;
-  // exact-adjoint bp: two slices of a size per workgroup (chip_batch_tune "bp_pairs")
+  // exact-adjoint bp: two slices of a size per workgroup (chip_batch_tune "bp_pairs"),
+  // the angle range split into parts ("bp_pair_parts": 0 = from occupancy, 1..8 pinned)
+  // summed in order from part scratch (parts x the packed volume, grown on first use)
   int bp_pairs = 1;
+  int bp_pair_parts = 0;
+  float* d_bp_part = nullptr;
+  size_t bp_part_bytes = 0;
+  std::unique_ptr<BpPartState> bp_state{new BpPartState};  // users of d_bp_part across streams
   std::vector<int> tile_group_start;  // [sizes + 1]
   int64_t* d_tile_slots = nullptr;
   std::vector<int> h_tile_slot_slice;  // slice of each slot
@@ -239,7 +245,11 @@ __device__ __forceinline__ int find_slice(const int64_t* __restrict__ start, int
 }
 
 void launch_fp_stream_batch(const BatchPlan& bp, const float* vol, const float* maps, float* sino, hipStream_t st);
-void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st);
+// the batched exact-adjoint bp; with bp_pairs, bp_pair_parts_ready first (it picks the part
+// count and grows the part scratch; an error under stream capture when the scratch is missing)
+int bp_pair_parts_ready(const BatchPlan& bp, bool maps, hipStream_t st, int* parts);
+void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st,
+                             int parts);
 void launch_bp_skimage_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, int sym,
                              hipStream_t st);
 

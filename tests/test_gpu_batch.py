@@ -64,7 +64,13 @@ def test_batched_projector_equals_single_slices(pkg, bp_mode):
     xs = [rng.random((n, n)).astype(np.float32) for n in SIZES]
     ys = [rng.random((2, a, n)).astype(np.float32) for n in SIZES]
     fx = A.unpack_sino(A.fp(A.pack_vol(xs)))
+    by_auto = A.unpack_vol(A.bp(A.pack_sino(ys)))
+    # the paired adjoint bp splits the angle range into parts by occupancy (another grouping of
+    # the sum); with one part it adds in the single-slice order
+    A.plan.tune("bp_pair_parts", 1)
     by = A.unpack_vol(A.bp(A.pack_sino(ys)))
+    for s in range(len(SIZES)):
+        assert float((by_auto[s] - by[s]).abs().max() / by[s].abs().max()) < 1e-6, f"bp parts of slice {s}"
     for s, n in enumerate(SIZES):
         single = _single(pkg, n, ang, mu_in[s], mu_out[s], bp_mode)
         assert torch.equal(A.slice_maps(s), single.att_vol_angles), f"maps of slice {s}"
@@ -330,7 +336,11 @@ def test_batched_bp_slice_pairs_same_bits(pkg, det):
     A.plan.tune("bp_pairs", 0)
     ref = A.bp(y).clone()
     A.plan.tune("bp_pairs", 1)
+    A.plan.tune("bp_pair_parts", 1)  # the whole angle range per workgroup: the same bits
     assert torch.equal(A.bp(y), ref)
+    for parts in (2, 3, 0):  # angle parts summed in part order (0: from occupancy): the sum's grouping only
+        A.plan.tune("bp_pair_parts", parts)
+        assert float((A.bp(y) - ref).abs().max() / ref.abs().max()) < 1e-6, parts
     from oracle import xrf as oxrf
 
     maps = A.slice_maps(3).cpu().numpy()
