@@ -258,7 +258,8 @@ def run_headline(args) -> dict:
         "fp": dict(kernel=FP_KERNEL_NAMES[tune["fp_shape"]], t=t_fp, bytes=fp_bytes, gbs=fp_gbs,
                    traffic_file="fp_tile_batch_2x512x720x2_traffic.json" if tile
                    else "fp_batch_2x512x720x2_traffic.json"),
-        "bp": dict(kernel="bp_adjoint_slots_kernel<2, true, 2>" if tune["bp_wg_per_cu"] == "pairs"
+        "bp": dict(kernel="bp_adjoint_slots_kernel<2, true, 2> + sum_parts_acc_kernel"
+                   if tune["bp_wg_per_cu"] == "pairs"
                    else "bp_adjoint_batch_kernel<2, true>", t=t_bp, bytes=bp_bytes, gbs=bp_gbs,
                    traffic_file="bp_batch_2x512x720x2_traffic.json"),
     }
