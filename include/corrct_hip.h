@@ -505,7 +505,9 @@ int chip_batch_info(const chip_batch_plan* plan, int64_t* vol_elems, int64_t* si
  * below) with its shape keys "fp_tile_kind", "fp_tile_group", "fp_tile_pairs";
  * "bp_pairs" (1, default: the exact-adjoint bp runs two slices of a size per
  * workgroup, one launch per size; 0: one slice per workgroup in one launch over
- * every slice; the same bits) and "bp_pair_parts" (0, default: the angle
+ * every slice; the same bits), "bp_quads" (1, default: in one-detector passes
+ * four slices of a size per workgroup first; the same bits) and
+ * "bp_pair_parts" (0, default: the angle
  * range of a pair split into parts by the kernel's occupancy, each part summed
  * into batch-plan scratch grown on first use, then added in part order, which
  * changes the grouping of the sum; 1..8 pins the count, 1 gives the bits of

@@ -238,7 +238,8 @@ def autotune(A: BatchedProjectorXRF, reps: int = 5, fp_candidates=(0, 2, 4, 5, 6
     tiles by default, whose grid follows from the batch).
 
     Returns ``{"fp_shape": s, "fp_wg_per_cu": k, "bp_wg_per_cu": k, "fp_us": {...}, "bp_us": {...}}``
-    (``bp_wg_per_cu`` "pairs": the exact-adjoint bp runs two slices of a size per workgroup);
+    (``bp_wg_per_cu`` "pairs": the exact-adjoint bp runs two slices of a size per workgroup, four in
+    one-detector passes);
     the chosen values are applied to ``A.plan``. The candidates of one family compute every work
     item the same way, so within the default candidates results do not depend on the choice (bit
     for bit), only the time does.

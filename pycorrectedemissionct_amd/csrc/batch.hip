@@ -309,6 +309,11 @@ int chip_batch_tune(chip_batch_plan* plan, const char* key, int value) {
     plan->b.bp_pair_parts = value;
     return CHIP_OK;
   }
+  if (k == "bp_quads") {
+    CHIP_CHECK_ARG(value == 0 || value == 1, "bp_quads must be 0 or 1");
+    plan->b.bp_quads = value;
+    return CHIP_OK;
+  }
   if (k == "bp_pairs") {
     CHIP_CHECK_ARG(value == 0 || value == 1, "bp_pairs must be 0 or 1");
     plan->b.bp_pairs = value;

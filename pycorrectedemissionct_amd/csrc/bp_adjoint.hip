@@ -45,18 +45,6 @@ struct BpItemS {
 };
 using BpItem = BpItemS<1>;
 
-// element i < S of a per-slice array without a dynamic index (which would move the
-// whole item to scratch memory)
-template <int S, class T>
-__device__ __forceinline__ T pick(const T (&a)[S], int i) {
-  if constexpr (S == 1) {
-    return a[0];
-  } else {
-    static_assert(S == 2, "one or two slices per workgroup");
-    return i == 0 ? a[0] : a[1];
-  }
-}
-
 // hat(t) = max(0, 1 - |t|): 1 - |t| <= 1, so clamping it to [0, 1] is the same
 // value, which the hardware applies as an output modifier of the subtraction
 __device__ __forceinline__ float tent(float t) { return __builtin_amdgcn_fmed3f(1.f - fabsf(t), 0.f, 1.f); }
@@ -76,7 +64,8 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItemS<S>& itm, const fl
                                                  int n_det_total, int det0, int angle0, int n_ang, int maps_a0,
                                                  int accumulate, float (*win)[S * D][kWin], BpAngle* ang_s) {
   constexpr int SD = S * D;  // channels (slice major)
-  constexpr int kG = S == 1 ? kGrp : kGrp / 2;  // angles per group of map loads
+  // angles per group of map loads (the prefetched group holds kG x S x D values)
+  constexpr int kG = S == 1 ? kGrp : (S == 2 ? kGrp / 2 : kGrp / (2 * D));
   const int n = itm.n;
   const float4* __restrict__ geom = itm.geom;
   const int tid = threadIdx.x;
@@ -118,17 +107,22 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItemS<S>& itm, const fl
       ba.rows_inside = (floorf(ymin - r) >= 0.f && floorf(ymax - r) + 4.f <= static_cast<float>(n - 1)) ? 1 : 0;
     }
     __syncthreads();
-    for (int k = tid; k < na * SD * kWin; k += 256) {
-      const int al = k / (SD * kWin);
-      const int rem = k - al * (SD * kWin);
-      const int c = rem / kWin;
-      const int q = rem - c * kWin;
-      const int sl = c / D, d = c - sl * D;
-      const int u = al * (SD * kWin) - ang_s[al].wbase / static_cast<int>(sizeof(float)) + q;
-      // the line times its weight w[a, d], applied once per bin instead of once per pixel
-      win[al][c][q] = (u >= 0 && u < n) ? pick<S>(itm.sino, sl)[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
-                                              w[static_cast<size_t>(a0 + al) * n_det_total + det0 + d]
-                                        : 0.f;
+    // one slice at a time, so that its line pointer is selected at compile time
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) {
+      const float* __restrict__ line = itm.sino[sl];
+      for (int k = tid; k < na * D * kWin; k += 256) {
+        const int al = k / (D * kWin);
+        const int rem = k - al * (D * kWin);
+        const int d = rem / kWin;
+        const int q = rem - d * kWin;
+        const int c = sl * D + d;
+        const int u = al * (SD * kWin) - ang_s[al].wbase / static_cast<int>(sizeof(float)) + q;
+        // the line times its weight w[a, d], applied once per bin instead of once per pixel
+        win[al][c][q] = (u >= 0 && u < n) ? line[(static_cast<size_t>(det0 + d) * n_angles + a0 + al) * n + u] *
+                                                w[static_cast<size_t>(a0 + al) * n_det_total + det0 + d]
+                                          : 0.f;
+      }
     }
     __syncthreads();
     if (inside) {
@@ -304,9 +298,9 @@ __global__ __launch_bounds__(256) void bp_adjoint_batch_kernel(const BatchSlice*
 }
 
 // Slices of one size at the given slots: blockIdx.z = unit, unit u serves the S
-// slots [slot0 + S u, slot0 + S u + S) (with S = 2 one workgroup per slice pair:
-// the candidate weights are computed once for both; every slice gets the bits
-// of the one-slice kernels).
+// slots [slot0 + S u, slot0 + S u + S) (with S = 2 or 4 one workgroup per two or
+// four slices: the candidate weights are computed once for all of them; every
+// slice gets the bits of the one-slice kernels).
 template <int D, bool MAPS, int S>
 __global__ __launch_bounds__(256) void bp_adjoint_slots_kernel(const float* __restrict__ sino,
                                                                const float* __restrict__ maps,
@@ -419,18 +413,27 @@ void launch(const Plan& p, const float* sino, const float* maps, int64_t stride,
     return;
   }
   if (batch >= 2) {
-    // two slices per workgroup (their candidate weights shared), an odd last slice alone
+    // four or two slices per workgroup (their candidate weights shared), an odd last slice alone
     TileSlots sl;
     sl.vol_stride = nn;
     sl.maps_stride = stride;
     sl.sino_stride = static_cast<int64_t>(p.n_det) * p.n_angles * p.n;
-    const int pairs = batch / 2;
-    hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps, vol,
-                       sl, 0, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
-                       accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
-    if (batch % 2)
+    // (four slices only in one-detector passes: with two detectors the eight channels'
+    // windows and map loads cost more occupancy than the shared weights save)
+    const int quads = D == 1 ? batch / 4 : 0, pairs = (batch - 4 * quads) / 2, b1 = 4 * quads + 2 * pairs;
+    if constexpr (D == 1) {
+      if (quads)
+        hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 4>), dim3(tx, ty, quads), dim3(256), 0, st, sino, maps,
+                           vol, sl, 0, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
+                           accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
+    }
+    if (pairs)
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs), dim3(256), 0, st, sino, maps,
+                         vol, sl, 4 * quads, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang,
+                         maps_a0, accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
+    if (b1 < batch)
       hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, 1), dim3(256), 0, st, sino, maps, vol,
-                         sl, batch - 1, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
+                         sl, b1, p.d_fp_geom, p.d_w, p.n, p.n_angles, p.n_det, det0, angle0, n_ang, maps_a0,
                          accumulate, 1, static_cast<float*>(nullptr), int64_t(0));
     return;
   }
@@ -457,29 +460,60 @@ void launch_batch(const BatchPlan& bp, const float* sino, const float* maps, flo
                      accumulate);
 }
 
+// Workgroups of one size group of the paired batch bp: ng slices of a size as
+// groups of four (with "bp_quads", in one-detector passes), then pairs, then one
+// alone. Four slices per workgroup share the candidate weights twice as far: at one
+// detector 772 -> 722 us for 32 x 256^2 x 360, 8.36 -> 7.84 ms for 128 slices of
+// 128..768 at 360 angles; with two detectors (eight channels) the kernel holds too
+// many registers and too much LDS per workgroup and was 15 % slower.
+struct SlotUnits {
+  int quads, pairs, single;
+};
+inline SlotUnits slot_units(int ng, int quads_on) {
+  SlotUnits u;
+  u.quads = quads_on ? ng / 4 : 0;
+  const int rest = ng - 4 * u.quads;
+  u.pairs = rest / 2;
+  u.single = rest % 2;
+  return u;
+}
+
+template <int D, bool MAPS, int S>
+int slots_per_cu() {
+  static const int per_cu = [] {
+    int k = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, bp_adjoint_slots_kernel<D, MAPS, S>, 256, 0) == hipSuccess
+               ? k
+               : 0;
+  }();
+  return per_cu;
+}
+
 // Angle parts of the paired batch bp: a paired workgroup walks every angle of its tile,
 // and with one workgroup per tile and slice pair too few loads are in flight to keep HBM
 // busy (a read-only kernel of that shape reaches 2.1 TB/s); splitting the angle range into
 // parts multiplies the workgroups. The count aims at about eight waves of what the GPU holds
 // at once (at most 8 parts of at least 16 angles each: 2 x 512^2 x 720 x 2 detectors 747 ->
-// 654 us with 6 parts), unless "bp_pair_parts" pins it; the count follows the kernel's
+// 654 us with 6 parts), unless "bp_pair_parts" pins it; the count follows the kernels'
 // occupancy and the CU count, and with it the grouping of the sums.
 template <int D, bool MAPS>
 int pair_parts(const BatchPlan& bp) {
   if (bp.bp_pair_parts >= 1) return bp.bp_pair_parts;
-  static const int per_cu = [] {
-    int k = 0;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, bp_adjoint_slots_kernel<D, MAPS, 2>, 256, 0) == hipSuccess
-               ? k
-               : 0;
-  }();
-  const int64_t resident = static_cast<int64_t>(per_cu) * bp.n_cu;
+  bool any_quads = false;
   int64_t wg = 0;
   for (size_t g = 0; g < bp.size_plans.size(); ++g) {
     const int n = bp.size_plans[g]->p.n;
-    const int ng = bp.tile_group_start[g + 1] - bp.tile_group_start[g];
-    wg += static_cast<int64_t>((n + kTileW - 1) / kTileW) * ((n + kTileH - 1) / kTileH) * ((ng + 1) / 2);
+    const SlotUnits u = slot_units(bp.tile_group_start[g + 1] - bp.tile_group_start[g], D == 1 && bp.bp_quads);
+    any_quads = any_quads || u.quads > 0;
+    wg += static_cast<int64_t>((n + kTileW - 1) / kTileW) * ((n + kTileH - 1) / kTileH) *
+          (u.quads + u.pairs + u.single);
   }
+  // the residency of the widest kernel in use (four slices hold the most registers and LDS)
+  int per_cu = slots_per_cu<D, MAPS, 2>();
+  if constexpr (D == 1) {
+    if (any_quads) per_cu = slots_per_cu<D, MAPS, 4>();
+  }
+  const int64_t resident = static_cast<int64_t>(per_cu) * bp.n_cu;
   if (resident <= 0 || wg <= 0) return 1;
   const int64_t want = (8 * resident + wg - 1) / wg;
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({want, kBpMaxParts, bp.n_angles / 16})));
@@ -492,17 +526,24 @@ void launch_slots(const BatchPlan& bp, const float* sino, const float* maps, flo
   for (size_t g = 0; g < bp.size_plans.size(); ++g) {
     const Plan& sp = bp.size_plans[g]->p;
     const int g0 = bp.tile_group_start[g], ng = bp.tile_group_start[g + 1] - g0;
-    const int pairs = ng / 2;
+    const SlotUnits u = slot_units(ng, D == 1 && bp.bp_quads);
     const unsigned tx = (sp.n + kTileW - 1) / kTileW, ty = (sp.n + kTileH - 1) / kTileH;
     TileSlots sl;
     sl.offs = bp.d_tile_slots;
-    if (pairs)
-      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, pairs * parts), dim3(256), 0, st, sino,
-                         maps, vol, sl, g0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0, sp.n_angles, 0,
-                         acc, parts, part, plane);
-    if (ng % 2)
+    if constexpr (D == 1) {
+      if (u.quads)
+        hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 4>), dim3(tx, ty, u.quads * parts), dim3(256), 0, st,
+                           sino, maps, vol, sl, g0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0,
+                           sp.n_angles, 0, acc, parts, part, plane);
+    }
+    const int p0 = g0 + 4 * u.quads;
+    if (u.pairs)
+      hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 2>), dim3(tx, ty, u.pairs * parts), dim3(256), 0, st,
+                         sino, maps, vol, sl, p0, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0,
+                         sp.n_angles, 0, acc, parts, part, plane);
+    if (u.single)
       hipLaunchKernelGGL((bp_adjoint_slots_kernel<D, MAPS, 1>), dim3(tx, ty, parts), dim3(256), 0, st, sino, maps,
-                         vol, sl, g0 + 2 * pairs, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0,
+                         vol, sl, p0 + 2 * u.pairs, sp.d_fp_geom, sp.d_w, sp.n, sp.n_angles, sp.n_det, det0, 0,
                          sp.n_angles, 0, acc, parts, part, plane);
   }
   if (parts > 1)
@@ -532,7 +573,7 @@ int bp_pair_parts_ready(const BatchPlan& bp, bool maps, hipStream_t st, int* par
 void launch_bp_adjoint_batch(const BatchPlan& bp, const float* sino, const float* maps, float* vol, hipStream_t st,
                              int parts) {
   if (bp.bp_pairs) {
-    // per slice size, two slices per workgroup, the angle range in `parts` parts
+    // per slice size, four or two slices per workgroup, the angle range in `parts` parts
     for (int det0 = 0; det0 < bp.n_det; det0 += 2) {
       const bool two = bp.n_det - det0 >= 2;
       const int acc = det0 > 0 ? 1 : 0;

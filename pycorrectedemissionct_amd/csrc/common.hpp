@@ -221,6 +221,7 @@ struct BatchPlan {
   // the angle range split into parts ("bp_pair_parts": 0 = from occupancy, 1..8 pinned)
   // summed in order from part scratch (parts x the packed volume, grown on first use)
   int bp_pairs = 1;
+  int bp_quads = 1;  // with bp_pairs: four slices of a size per workgroup first ("bp_quads")
   int bp_pair_parts = 0;
   float* d_bp_part = nullptr;
   size_t bp_part_bytes = 0;

@@ -322,13 +322,13 @@ def test_batched_otf_equals_precomputed(pkg):
 
 @pytest.mark.parametrize("det", [DET, [np.pi / 2]])
 def test_batched_bp_slice_pairs_same_bits(pkg, det):
-    """The exact-adjoint bp with two slices of a size per workgroup (odd counts leave one slice alone)
-    gives the bits of one slice per workgroup, with and without maps."""
+    """The exact-adjoint bp with four (one-detector passes) or two slices of a size per workgroup (the
+    rest in pairs, an odd one alone) gives the bits of one slice per workgroup, with and without maps."""
     import torch
 
     a = 40
     rng = np.random.default_rng(707)
-    sizes = [64, 32, 64, 96, 64, 32]
+    sizes = [64, 32, 64, 96, 64, 32, 64, 64, 32, 32]  # 64: four + one, 32: four, 96: one
     ang = _angles(a)
     mu_in, mu_out = _mus(rng, sizes)
     A = pkg.batch.BatchedProjectorXRF(sizes, ang, mu_in, mu_out, angles_detectors_rad=det, bp_mode="adjoint")
@@ -337,13 +337,39 @@ def test_batched_bp_slice_pairs_same_bits(pkg, det):
     ref = A.bp(y).clone()
     A.plan.tune("bp_pairs", 1)
     A.plan.tune("bp_pair_parts", 1)  # the whole angle range per workgroup: the same bits
-    assert torch.equal(A.bp(y), ref)
+    for quads in (1, 0):
+        A.plan.tune("bp_quads", quads)
+        assert torch.equal(A.bp(y), ref), quads
+    A.plan.tune("bp_quads", 1)
     for parts in (2, 3, 0):  # angle parts summed in part order (0: from occupancy): the sum's grouping only
         A.plan.tune("bp_pair_parts", parts)
         assert float((A.bp(y) - ref).abs().max() / ref.abs().max()) < 1e-6, parts
     from oracle import xrf as oxrf
 
-    maps = A.slice_maps(3).cpu().numpy()
-    got = A.unpack_vol(A.bp(y))[3].cpu().numpy()
-    want = oxrf.bp_adjoint(A.unpack_sino(y)[3].cpu().numpy(), ang, sizes[3], maps)
-    assert float(np.abs(got - want).max() / np.abs(want).max()) < 1e-5
+    for s in (3, 9):  # a slice alone and one of a group of four
+        maps = A.slice_maps(s).cpu().numpy()
+        got = A.unpack_vol(A.bp(y))[s].cpu().numpy()
+        want = oxrf.bp_adjoint(A.unpack_sino(y)[s].cpu().numpy(), ang, sizes[s], maps)
+        assert float(np.abs(got - want).max() / np.abs(want).max()) < 1e-5, s
+    with pytest.raises(pkg._native.NativeError):
+        A.plan.tune("bp_quads", 2)
+
+
+@pytest.mark.parametrize("d", [1, 2])
+def test_volume_adjoint_bp_slice_groups_equal_slices(pkg, d):
+    """A [Z, N, N] volume's exact-adjoint bp runs its slices four (one detector) or two per workgroup,
+    an odd last one alone: every slice has the bits of the single-slice projector."""
+    n, a, z = 48, 40, 7
+    rng = np.random.default_rng(708)
+    ang = _angles(a)
+    det = DET[:d]
+    mu_in = (rng.random((z, n, n)) * 0.04).astype(np.float32)
+    mu_out = (rng.random((z, n, n)) * 0.06).astype(np.float32)
+    y = rng.random((d, z, a, n) if d > 1 else (z, a, n)).astype(np.float32)
+    kw = dict(angles_detectors_rad=det, bp_mode="adjoint")
+    p3 = pkg.ProjectorAttenuationXRF([n, n, z], ang, att_in=mu_in, att_out=mu_out, **kw)
+    b3 = p3.bp(y)
+    for i in range(z):
+        p2 = pkg.ProjectorAttenuationXRF([n, n], ang, att_in=mu_in[i], att_out=mu_out[i], **kw)
+        yi = np.ascontiguousarray(y[:, i] if d > 1 else y[i])
+        assert np.array_equal(b3[i], p2.bp(yi)), i
