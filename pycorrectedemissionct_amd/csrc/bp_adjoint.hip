@@ -85,6 +85,9 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItemS<S>& itm, const fl
   for (int a0 = angle0; a0 < angle0 + n_ang; a0 += kAngBlk) {
     const int na = min(kAngBlk, angle0 + n_ang - a0);
     __syncthreads();
+    // the block's staging outranks the other workgroups' pixel loops on their SIMDs (as in
+    // the image-tile fp; within 1 % here)
+    __builtin_amdgcn_s_setprio(3);
     if (tid < na) {
       const float4 g = geom[a0 + tid];
       const float cs = g.x, sn = g.y;
@@ -124,6 +127,7 @@ __device__ __forceinline__ void bp_adjoint_block(const BpItemS<S>& itm, const fl
                                           : 0.f;
       }
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     if (inside) {
       // map values of the next group of kGrp angles are loaded while the current group is

@@ -408,7 +408,11 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
     const int nb = tab[(static_cast<size_t>(a) * n_tiles + t) * kTab + 1];
     if (MAPS && !PF) issue(la);
     __syncthreads();  // the previous angle's gathers are done with the tile
+    // the staging waves outrank the other workgroups' gathers on their SIMDs: the tile's
+    // four waves then leave the barrier sooner (2 % at 2 x 512^2 x 720 x 2)
+    __builtin_amdgcn_s_setprio(3);
     if (MAPS) stage();
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     int rcur[kGPerWave];
 #pragma unroll
