@@ -265,6 +265,12 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
   constexpr int kPitch = pitch_of<C>();
   using PixT = typename Pix<C>::T;
   __shared__ float tile[C * kRowsL * kPitch];
+  // U = 1: the partial sums of the last two angles (by parity) are gathered in LDS and
+  // stored to `part` after the next angle's staging, so that the staging's wait for the
+  // prefetched map tiles does not also wait for the stores just issued (-2 % at 2 x 512^2
+  // x 720 x 2, -4 % at 32 x 256^2 x 360); with U = 2 the extra registers cost occupancy
+  constexpr bool kDefer = U == 1;
+  __shared__ float psum[kDefer ? 2 : 1][kDefer ? S * kNb * D : 1];
   const int tid = threadIdx.x;
   const int lb = xcd_logical(blockIdx.x, gridDim.x);
   const int gi = lb / n_tiles;
@@ -400,6 +406,14 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
     stage();
   }
 
+  // angle l's partial sums from LDS to `part` (after a barrier that follows their writes)
+  auto store_part = [&](int l) {
+    const float* src = psum[l & 1];
+    for (int i = tid; i < S * kNb * D; i += 256) {
+      const int sl = i / (kNb * D), r = i - sl * (kNb * D);
+      part[((static_cast<size_t>(b0 + sl) * n_ang + l) * n_tiles + t) * (kNb * D) + r] = src[i];
+    }
+  };
   for (int la = la_begin; la < la_end; ++la) {
     const int a = angle0 + la;
     const float4 gm = geom[a];
@@ -414,6 +428,7 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
     if (MAPS) stage();
     __builtin_amdgcn_s_setprio(0);
     __syncthreads();
+    if (kDefer && la > la_begin) store_part(la - 1);
     int rcur[kGPerWave];
 #pragma unroll
     for (int k = 0; k < kGPerWave; ++k) rcur[k] = rw[k];
@@ -432,7 +447,8 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
         if (sub == 0) {
 #pragma unroll
           for (int sl = 0; sl < S; ++sl) {
-            float* pr = part + ((static_cast<size_t>(b0 + sl) * n_ang + la) * n_tiles + t) * (kNb * D);
+            float* pr = kDefer ? psum[la & 1] + sl * (kNb * D)
+                               : part + ((static_cast<size_t>(b0 + sl) * n_ang + la) * n_tiles + t) * (kNb * D);
             if (D == 2) reinterpret_cast<float2*>(pr)[j] = make_float2(0.f, 0.f);
             else pr[j] = 0.f;
           }
@@ -488,12 +504,17 @@ __global__ __launch_bounds__(256) void fp_tile_kernel(const float* __restrict__ 
       if (sub == 0) {
 #pragma unroll
         for (int sl = 0; sl < S; ++sl) {
-          float* pr = part + ((static_cast<size_t>(b0 + sl) * n_ang + la) * n_tiles + t) * (kNb * D);
+          float* pr = kDefer ? psum[la & 1] + sl * (kNb * D)
+                             : part + ((static_cast<size_t>(b0 + sl) * n_ang + la) * n_tiles + t) * (kNb * D);
           if (D == 2) reinterpret_cast<float2*>(pr)[j] = make_float2(acc[sl * D], acc[sl * D + D - 1]);
           else pr[j] = acc[sl * D];
         }
       }
     }
+  }
+  if (kDefer && la_end > la_begin) {
+    __syncthreads();
+    store_part(la_end - 1);
   }
 }
 
