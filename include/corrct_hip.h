@@ -261,9 +261,10 @@ int chip_bp_otf(chip_plan* plan, const float* sino, const float* mu_in, const fl
  * scratch, so under stream capture it needs one earlier eager call; results
  * within 1e-6 of 5); 2..10 need N % 4 == 0 and 16-byte aligned vol / maps,
  * else 1 runs (10 with more than 8192 tiles per slice runs 5). The shape of
- * 10: "fp_tile_kind" 0..3 (0: the next angle's map tiles fetched into
- * registers, two samples per lane and trip; 1: the same, one sample, default;
- * 2 / 3: map tiles loaded at staging, two / one), "fp_tile_group" (angles per
+ * 10: "fp_tile_kind" -1..3 (0: the next angle's map tiles fetched into
+ * registers, two samples per lane and trip; 1: the same, one sample; 2 / 3: map
+ * tiles loaded at staging, two / one; -1, default: 1 in two-detector passes, 3
+ * in one-detector passes), "fp_tile_group" (angles per
  * workgroup, default 8), "fp_tile_pairs" (1: two slices of a batch per
  * workgroup, default); every shape gives the same bits. "bp_parts" = -1 (default: from
  * the adjoint bp kernel's occupancy), 0 (never split into parts) or 2..8

@@ -90,9 +90,10 @@ struct TileSlots {
 
 // image-tile fp launch shape: kernel kind (0: map tiles fetched one angle ahead
 // into registers, two samples per lane and trip; 1: ahead, one; 2: at staging,
-// two; 3: at staging, one), angles per workgroup, two slices per workgroup
+// two; 3: at staging, one; -1: 1 for two-detector passes, 3 for one-detector
+// passes), angles per workgroup, two slices per workgroup
 struct FpTileShape {
-  int kind = 1;
+  int kind = -1;
   int group = 8;
   int pairs = 1;
 };

@@ -644,7 +644,11 @@ void launch_fp_tile(const Plan& p, const float* vol, const float* maps, const Ti
   const int singles = slots - 2 * pairs;
   const dim3 grid2(n_groups * n_tiles, pairs), grid1(n_groups * n_tiles, singles);
   const dim3 rgrid(n_ang, slots, (p.n + 255) / 256);
-  const int kind = shape.kind;
+  // kind -1 (default): map tiles fetched one angle ahead with two detectors (four
+  // channels); loaded at staging in one-detector passes, where the registers of the
+  // prefetch cost more than it hides (32 x 256^2 x 360: -2 %)
+  const int kind = shape.kind >= 0 ? shape.kind : 1;
+  const int kind1 = shape.kind >= 0 ? shape.kind : 3;
   if (!maps) {
     if (pairs)
       launch_kind<1, 2, false>(kind, grid2, st, vol, maps, sl, part, tab, p.d_fp_geom, p.n, p.n_det, 0, angle0, n_ang,
@@ -669,10 +673,10 @@ void launch_fp_tile(const Plan& p, const float* vol, const float* maps, const Ti
                          p.n_angles, p.n_det, det0, 2, angle0, n_ang, n_tiles);
     } else {
       if (pairs)
-        launch_kind<1, 2, true>(kind, grid2, st, vol, maps, sl, part, tab, p.d_fp_geom, p.n, p.n_det, det0, angle0,
+        launch_kind<1, 2, true>(kind1, grid2, st, vol, maps, sl, part, tab, p.d_fp_geom, p.n, p.n_det, det0, angle0,
                                 n_ang, maps_a0, group, tiles_x, n_tiles, 0);
       if (singles)
-        launch_kind<1, 1, true>(kind, grid1, st, vol, maps, sl, part, tab, p.d_fp_geom, p.n, p.n_det, det0, angle0,
+        launch_kind<1, 1, true>(kind1, grid1, st, vol, maps, sl, part, tab, p.d_fp_geom, p.n, p.n_det, det0, angle0,
                                 n_ang, maps_a0, group, tiles_x, n_tiles, 2 * pairs);
       hipLaunchKernelGGL((fp_tile_reduce_kernel<1>), rgrid, dim3(256), red_lds, st, part, sorted, sino, sl, p.d_w, p.n,
                          p.n_angles, p.n_det, det0, 1, angle0, n_ang, n_tiles);

@@ -815,7 +815,7 @@ void launch_bp_range(const Plan& p, const float* sino, const float* maps, float*
 
 int tune_tile_shape(FpTileShape& sh, const std::string& k, int value) {
   if (k == "fp_tile_kind") {
-    CHIP_CHECK_ARG(value >= 0 && value <= 3, "fp_tile_kind must be in 0..3");
+    CHIP_CHECK_ARG(value >= -1 && value <= 3, "fp_tile_kind must be in -1..3 (-1: by detector count)");
     sh.kind = value;
   } else if (k == "fp_tile_group") {
     CHIP_CHECK_ARG(value >= 1 && value <= 4096, "fp_tile_group must be in 1..4096");
