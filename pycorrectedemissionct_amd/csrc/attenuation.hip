@@ -145,12 +145,14 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
   const int x = (blockIdx.x * kScanWaves + wave) * kScanCols + col;
   const int beam = blockIdx.y;
   if (beam >= n_beams) return;  // uniform over the workgroup
-  const bool live = x < np;
+  // a band of the workgroup's 32 columns goes through LDS so that its rows are stored as
+  // whole 128-byte segments (a wave's own 8 columns are 32 bytes per row)
+  __shared__ float band[kScanBand][kScanWaves * kScanCols];
+  __shared__ int rows_lo[kScanWaves], rows_hi[kScanWaves];
   const bool inb = beam < n_in;
   const float* mu = inb ? mu_in : mu_out;
   const Affine m = inb ? rot_in[beam] : rot_out[beam - n_in];
   const float xf = static_cast<float>(min(x, np - 1));
-  float* out = cum + static_cast<size_t>(beam) * np * np + x;
   float xs, ys;
   // a sample reads the slice only inside (pad - 1, pad + n) in both coordinates
   const float lo_f = static_cast<float>(pad - 1), hi_f = static_cast<float>(pad + n);
@@ -207,10 +209,22 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
       lo_r = min(lo_r, __shfl_xor(lo_r, sh, 64));
       hi_r = max(hi_r, __shfl_xor(hi_r, sh, 64));
     }
+    // union over the workgroup's waves: every wave walks the same bands (rows outside a
+    // wave's own interval have R = 0 there and are computed like any other)
+    if (lane == 0) {
+      rows_lo[wave] = lo_r;
+      rows_hi[wave] = hi_r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanWaves; ++k) {
+      lo_r = min(lo_r, rows_lo[k]);
+      hi_r = max(hi_r, rows_hi[k]);
+    }
     r_first = max(lo_r, 0) & ~(kScanBand - 1);
     r_last = min(hi_r, np - 1);
   }
-  if (r_last < r_first) return;  // no column of this wave meets the slice (uniform over the wave)
+  if (r_last < r_first) return;  // no column of the workgroup meets the slice (uniform)
   if (r_first > 0) {
     carry = (0.f + last) * 0.5f;  // C[0] with R[0] = 0; rows 1 .. r_first - 1 add 0
     last = 0.f;                   // R[r_first - 1]
@@ -222,7 +236,6 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
 #pragma unroll
   for (int k = 0; k < kAhead; ++k) ahead[k] = band_sample(r_first + k * kScanBand);
   for (int r0 = r_first; r0 <= r_last; r0 += kScanBand) {
-    const int r = r0 + row;
     const float cur = ahead[0];
 #pragma unroll
     for (int k = 0; k + 1 < kAhead; ++k) ahead[k] = ahead[k + 1];
@@ -237,9 +250,16 @@ __global__ __launch_bounds__(256) void att_scan_kernel(const float* __restrict__
       const float before = from_prev(run);
       if (row == k) run = before + v;
     }
-    if (live && r <= r_last) out[static_cast<size_t>(r) * np] = run;
     carry = from_row7(run);  // used by row 0 only
     last = from_row7(cur);
+    band[row][wave * kScanCols + col] = run;
+    __syncthreads();
+    {
+      const int br = threadIdx.x / (kScanWaves * kScanCols), bc = threadIdx.x % (kScanWaves * kScanCols);
+      const int xs0 = blockIdx.x * kScanWaves * kScanCols + bc;
+      if (xs0 < np && r0 + br <= r_last) cum[static_cast<size_t>(beam) * np * np + static_cast<size_t>(r0 + br) * np + xs0] = band[br][bc];
+    }
+    __syncthreads();
   }
 }
 
