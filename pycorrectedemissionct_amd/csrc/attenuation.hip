@@ -275,14 +275,16 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
                                                           int has_out, int n_in, int a0, int n_det, int n, int pad,
                                                           int np, float* __restrict__ maps, int out_a0, int n_sub,
                                                           const int* __restrict__ nsub) {
-  // a workgroup owns a 16 x 32 tile, each thread two pixels 16 rows apart (their gathers
-  // are in flight together); each wave takes an 8 x 8 quadrant of both 16 x 16 halves,
-  // so its back-rotated samples form compact patches of the prefix-sum planes
+  // a workgroup owns a 32 x 16 tile, each thread two pixels 8 rows apart (their gathers
+  // are in flight together); each wave takes 8 columns of both 8-row halves, so its
+  // back-rotated samples form compact patches of the prefix-sum planes. The tile's maps
+  // go out through LDS as 128-byte rows (a wave's own columns are 32 bytes per row).
+  __shared__ float tile_out[16][32];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  const int i0 = blockIdx.y * 32 + (wave >> 1) * 8 + (lane >> 3);
+  const int j_raw = blockIdx.x * 32 + wave * 8 + (lane & 7);
+  const int i0_raw = blockIdx.y * 16 + (lane >> 3);
+  const int j = min(j_raw, n - 1), i0 = min(i0_raw, n - 1);
   const int ad = blockIdx.z;  // chunk-local (la, d)
-  if (j >= n || i0 >= n) return;
   const int la = ad / n_det;
   const int d = ad - la * n_det;
   const int a = a0 + la;
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
 #pragma unroll
     for (int h = 0; h < kPix; ++h) {
       float xs, ys;
-      affine_apply(m, xf, static_cast<float>(min(i0 + 16 * h, n - 1) + pad), xs, ys);
+      affine_apply(m, xf, static_cast<float>(min(i0 + 8 * h, n - 1) + pad), xs, ys);
       v[h] = warp_any<EXACT>(cum + static_cast<size_t>(la) * npp, np, 0, xs, ys);
     }
 #pragma unroll
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
 #pragma unroll
       for (int h = 0; h < kPix; ++h) {
         float xs, ys;
-        affine_apply(m, xf, static_cast<float>(min(i0 + 16 * h, n - 1) + pad), xs, ys);
+        affine_apply(m, xf, static_cast<float>(min(i0 + 8 * h, n - 1) + pad), xs, ys);
         v[h] = warp_any<EXACT>(plane, np, 0, xs, ys);
       }
 #pragma unroll
@@ -325,11 +327,16 @@ __global__ __launch_bounds__(256) void att_compose_kernel(const float* __restric
 #pragma unroll
     for (int h = 0; h < kPix; ++h) e_out[h] = acc[h] / static_cast<float>(cnt);
   }
-  float* dst = maps + (static_cast<size_t>(a - out_a0) * n_det + d) * n * n + j;
 #pragma unroll
-  for (int h = 0; h < kPix; ++h) {
-    const int i = i0 + 16 * h;
-    if (i < n) dst[static_cast<size_t>(i) * n] = e_in[h] * e_out[h];
+  for (int h = 0; h < kPix; ++h) tile_out[(lane >> 3) + 8 * h][wave * 8 + (lane & 7)] = e_in[h] * e_out[h];
+  __syncthreads();
+  float* dst = maps + (static_cast<size_t>(a - out_a0) * n_det + d) * n * n;
+  const int oc = blockIdx.x * 32 + (threadIdx.x & 31);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int tr = (threadIdx.x >> 5) + 8 * h;
+    const int i = blockIdx.y * 16 + tr;
+    if (i < n && oc < n) dst[static_cast<size_t>(i) * n + oc] = tile_out[tr][threadIdx.x & 31];
   }
 }
 
@@ -380,7 +387,7 @@ static int build_maps_chunk(const Plan& p, const float* mi, const float* mo, int
   const int n_in = mi ? na : 0;
   const int n_beams = n_in + (mo ? na * D * S : 0);
   dim3 g1((p.np + kScanWaves * kScanCols - 1) / (kScanWaves * kScanCols), n_beams);
-  dim3 g2((p.n + 15) / 16, (p.n + 31) / 32, na * D);
+  dim3 g2((p.n + 31) / 32, (p.n + 15) / 16, na * D);
   if (p.maps_exact) {
     hipLaunchKernelGGL(att_scan_kernel<true>, g1, dim3(256), 0, st, mi, mo, rin_f + a0,
                        rout_f + static_cast<size_t>(a0) * D * S, n_in, n_beams, p.n, p.pad, p.np, scan);
